@@ -1,0 +1,305 @@
+// C ABI of the gfx950 AFQMC step library: context lifetime, error reporting and the
+// entry points declared in include/cs_afqmc_amd.h.
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+
+static thread_local char g_err[512] = "";
+
+int afqmc_set_error(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+extern "C" const char* afqmc_last_error(void) { return g_err; }
+
+namespace {
+
+template <typename T>
+int dalloc(afqmc_ctx* c, T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return AFQMC_OK;
+  const size_t bytes = count * sizeof(T);
+  HIP_TRY(hipMalloc((void**)p, bytes));
+  c->device_bytes += (int64_t)bytes;
+  return AFQMC_OK;
+}
+
+template <typename T>
+int upload(afqmc_ctx* c, T** p, const T* host, size_t count) {
+  int rc = dalloc(c, p, count);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(*p, host, count * sizeof(T), hipMemcpyHostToDevice));
+  return AFQMC_OK;
+}
+
+void free_ctx(afqmc_ctx* c) {
+  if (!c) return;
+  void* ptrs[] = {c->trial[0], c->trial[1], c->rot_h1[0], c->rot_h1[1], c->rot_chol[0],
+                  c->rot_chol[1], c->chol_t, c->exp_h1[0], c->exp_h1[1], c->mf_shifts,
+                  c->chol_t_f32, c->rot_chol_f32[0], c->rot_chol_f32[1], c->chol_t_bf[0],
+                  c->chol_t_bf[1], c->rot_chol_bf[0][0], c->rot_chol_bf[0][1],
+                  c->rot_chol_bf[1][0], c->rot_chol_bf[1][1], c->green[0], c->green[1], c->fb,
+                  c->xs, c->vhs, c->terms, c->dets, c->energy_f, c->energy_acc, c->walk_tmp,
+                  c->sr_cum, c->sr_idx};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete c;
+}
+
+int check_w(const afqmc_ctx* c, int W) {
+  if (!c) return afqmc_set_error(AFQMC_ERR_ARG, "null context");
+  if (W <= 0 || W > c->d.n_walkers)
+    return afqmc_set_error(AFQMC_ERR_ARG, "n_walkers=%d outside [1, %d]", W, c->d.n_walkers);
+  return AFQMC_OK;
+}
+
+int check_walkers(const afqmc_ctx* c, const void* up, const void* dn) {
+  if (!up) return afqmc_set_error(AFQMC_ERR_ARG, "walkers_up is NULL");
+  if (c->nspin == 2 && !dn)
+    return afqmc_set_error(AFQMC_ERR_ARG, "unrestricted walkers need walkers_dn");
+  return AFQMC_OK;
+}
+
+int set_device(const afqmc_ctx* c) {
+  HIP_TRY(hipSetDevice(c->device));
+  return AFQMC_OK;
+}
+
+// fb[w][c][g] = mult * sum_s green_s[w][c][:] . rot_chol_s[g][:]
+int force_bias_gemm(afqmc_ctx* c, int W) {
+  for (int s = 0; s < c->nspin; ++s) {
+    const int kd = c->kdim[s];
+    int rc = launch_gemm_nt_f64(c->stream, 2 * W, c->d.nchol, kd, (double)c->mult, c->green[s],
+                                kd, c->rot_chol[s], kd, s == 0 ? 0.0 : 1.0, c->fb, c->d.nchol);
+    if (rc) return rc;
+  }
+  return AFQMC_OK;
+}
+
+}  // namespace
+
+extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
+                                const afqmc_ham_host* ham, void* stream) {
+  if (!out || !dims || !ham) return afqmc_set_error(AFQMC_ERR_ARG, "null argument");
+  *out = nullptr;
+  const afqmc_dims d = *dims;
+  if (d.walker_type != AFQMC_RHF && d.walker_type != AFQMC_UHF)
+    return afqmc_set_error(AFQMC_ERR_ARG, "walker_type=%d", d.walker_type);
+  if (d.norb < 1 || d.norb > AFQMC_MAX_NORB)
+    return afqmc_set_error(AFQMC_ERR_SIZE, "norb=%d outside [1, %d]", d.norb, AFQMC_MAX_NORB);
+  if (d.nocc_up < 1 || d.nocc_up > AFQMC_MAX_NOCC || d.nocc_up > d.norb)
+    return afqmc_set_error(AFQMC_ERR_SIZE, "nocc_up=%d outside [1, min(norb, %d)]", d.nocc_up,
+                           AFQMC_MAX_NOCC);
+  if (d.walker_type == AFQMC_UHF &&
+      (d.nocc_dn < 1 || d.nocc_dn > AFQMC_MAX_NOCC || d.nocc_dn > d.norb))
+    return afqmc_set_error(AFQMC_ERR_SIZE, "nocc_dn=%d outside [1, min(norb, %d)]", d.nocc_dn,
+                           AFQMC_MAX_NOCC);
+  if (d.walker_type == AFQMC_RHF && d.nocc_dn != d.nocc_up)
+    return afqmc_set_error(AFQMC_ERR_ARG, "restricted walkers need nocc_dn == nocc_up");
+  if (d.nchol < 1) return afqmc_set_error(AFQMC_ERR_ARG, "nchol=%d", d.nchol);
+  if (d.n_walkers < 1) return afqmc_set_error(AFQMC_ERR_ARG, "n_walkers=%d", d.n_walkers);
+  if (d.n_exp_terms < 1 || d.n_exp_terms > 18)
+    return afqmc_set_error(AFQMC_ERR_ARG, "n_exp_terms=%d outside [1, 18]", d.n_exp_terms);
+  if (!(d.dt > 0.0)) return afqmc_set_error(AFQMC_ERR_ARG, "dt must be > 0");
+  if (d.precision != AFQMC_PREC_FP64)
+    return afqmc_set_error(AFQMC_ERR_ARG, "precision mode %d not built yet", d.precision);
+  if (!ham->chol || !ham->mo_coeff || !ham->rot_h1 || !ham->rot_chol || !ham->mf_shifts ||
+      !ham->exp_h1)
+    return afqmc_set_error(AFQMC_ERR_ARG, "null intermediate array");
+
+  afqmc_ctx* c = new afqmc_ctx();
+  std::memset(c, 0, sizeof(*c));
+  c->d = d;
+  c->stream = (hipStream_t)stream;
+  hipError_t e = hipGetDevice(&c->device);
+  if (e != hipSuccess) {
+    delete c;
+    return afqmc_set_error(AFQMC_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+  }
+  c->nspin = d.walker_type == AFQMC_RHF ? 1 : 2;
+  c->mult = d.walker_type == AFQMC_RHF ? 2 : 1;
+  c->nocc[0] = d.nocc_up;
+  c->nocc[1] = d.walker_type == AFQMC_RHF ? d.nocc_up : d.nocc_dn;
+  const int N = d.norb, G = d.nchol, W = d.n_walkers;
+  c->kdim[0] = c->nocc[0] * N;
+  c->kdim[1] = c->nocc[1] * N;
+  c->h0 = ham->h0;
+  c->h0_prop = ham->h0_prop;
+
+  int rc = AFQMC_OK;
+#define TRY(x)          \
+  do {                  \
+    rc = (x);           \
+    if (rc) goto fail;  \
+  } while (0)
+  {
+    size_t off_mo = 0, off_h1 = 0, off_rc = 0;
+    for (int s = 0; s < c->nspin; ++s) {
+      const int n = c->nocc[s];
+      TRY(upload(c, &c->trial[s], ham->mo_coeff + off_mo, (size_t)N * n));
+      TRY(upload(c, &c->rot_h1[s], ham->rot_h1 + off_h1, (size_t)n * N));
+      TRY(upload(c, &c->rot_chol[s], ham->rot_chol + off_rc, (size_t)G * n * N));
+      TRY(upload(c, &c->exp_h1[s], ham->exp_h1 + (size_t)s * N * N, (size_t)N * N));
+      off_mo += (size_t)N * n;
+      off_h1 += (size_t)n * N;
+      off_rc += (size_t)G * n * N;
+    }
+    // transposed Cholesky vectors (N*N, G): K-contiguous B operand of the VHS GEMM
+    std::vector<double> ct((size_t)N * N * G);
+    for (int g = 0; g < G; ++g)
+      for (size_t pq = 0; pq < (size_t)N * N; ++pq) ct[pq * G + g] = ham->chol[(size_t)g * N * N + pq];
+    TRY(upload(c, &c->chol_t, ct.data(), ct.size()));
+    TRY(upload(c, &c->mf_shifts, reinterpret_cast<const cplx*>(ham->mf_shifts), (size_t)G));
+
+    // workspace
+    for (int s = 0; s < c->nspin; ++s) TRY(dalloc(c, &c->green[s], (size_t)W * 2 * c->kdim[s]));
+    TRY(dalloc(c, &c->fb, (size_t)W * 2 * G));
+    TRY(dalloc(c, &c->xs, (size_t)W * 2 * G));
+    TRY(dalloc(c, &c->vhs, (size_t)W * 2 * N * N));
+    TRY(dalloc(c, &c->terms, (size_t)W * 2));
+    TRY(dalloc(c, &c->dets, (size_t)W * 2));
+    {
+      // energy f_g chunk: per walker G * n_s * 2 n_s doubles per spin; cap at 512 MiB
+      size_t per_w = 0;
+      for (int s = 0; s < c->nspin; ++s) per_w += (size_t)G * c->nocc[s] * 2 * c->nocc[s];
+      size_t wc = (size_t)(512u << 20) / (per_w * sizeof(double));
+      if (wc < 1) wc = 1;
+      if (wc > (size_t)W) wc = W;
+      c->energy_wchunk = (int)wc;
+      TRY(dalloc(c, &c->energy_f, per_w * wc));
+    }
+    TRY(dalloc(c, &c->walk_tmp, (size_t)W * N * (c->nocc[0] > c->nocc[1] ? c->nocc[0] : c->nocc[1])));
+    TRY(dalloc(c, &c->sr_cum, (size_t)W));
+    TRY(dalloc(c, &c->sr_idx, (size_t)W));
+  }
+#undef TRY
+  *out = c;
+  return AFQMC_OK;
+fail:
+  free_ctx(c);
+  return rc;
+}
+
+extern "C" int afqmc_ctx_destroy(afqmc_ctx* ctx) {
+  if (!ctx) return AFQMC_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  free_ctx(ctx);
+  return AFQMC_OK;
+}
+
+extern "C" int afqmc_set_stream(afqmc_ctx* ctx, void* stream) {
+  if (!ctx) return afqmc_set_error(AFQMC_ERR_ARG, "null context");
+  ctx->stream = (hipStream_t)stream;
+  return AFQMC_OK;
+}
+
+extern "C" int64_t afqmc_ctx_device_bytes(const afqmc_ctx* ctx) {
+  return ctx ? ctx->device_bytes : 0;
+}
+
+extern "C" int afqmc_propagate(afqmc_ctx* c, int32_t W, void* walkers_up, void* walkers_dn,
+                               double* weights, void* overlaps, const double* fields,
+                               double* scalars, void* force_bias_out) {
+  int rc = check_w(c, W);
+  if (rc) return rc;
+  if ((rc = check_walkers(c, walkers_up, walkers_dn))) return rc;
+  if (!weights || !overlaps || !fields || !scalars)
+    return afqmc_set_error(AFQMC_ERR_ARG, "null weights/overlaps/fields/scalars");
+  if ((rc = set_device(c))) return rc;
+  cplx* wu = static_cast<cplx*>(walkers_up);
+  cplx* wd = static_cast<cplx*>(walkers_dn);
+  const int N = c->d.norb;
+  // force bias: Green of the current walkers, then the rot_chol contraction
+  if ((rc = launch_measure(c, W, wu, wd, true))) return rc;
+  if ((rc = force_bias_gemm(c, W))) return rc;
+  if ((rc = launch_fields(c, W, fields, static_cast<cplx*>(force_bias_out)))) return rc;
+  // VHS[w] = sum_g (i sqrt(dt) x~_wg) L_g  : (2W, G) x (N*N, G)^T
+  if ((rc = launch_gemm_nt_f64(c->stream, 2 * W, N * N, c->d.nchol, 1.0, c->xs, c->d.nchol,
+                               c->chol_t, c->d.nchol, 0.0, c->vhs, N * N)))
+    return rc;
+  if ((rc = launch_taylor(c, W, wu, wd))) return rc;
+  // new overlaps and the weight update
+  if ((rc = launch_measure(c, W, wu, wd, false))) return rc;
+  return launch_weights(c, W, weights, static_cast<cplx*>(overlaps), scalars);
+}
+
+extern "C" int afqmc_overlap(afqmc_ctx* c, int32_t W, const void* walkers_up,
+                             const void* walkers_dn, void* overlaps) {
+  int rc = check_w(c, W);
+  if (rc) return rc;
+  if ((rc = check_walkers(c, walkers_up, walkers_dn))) return rc;
+  if (!overlaps) return afqmc_set_error(AFQMC_ERR_ARG, "null overlaps");
+  if ((rc = set_device(c))) return rc;
+  if ((rc = launch_measure(c, W, static_cast<const cplx*>(walkers_up),
+                           static_cast<const cplx*>(walkers_dn), false)))
+    return rc;
+  return launch_overlap_combine(c, W, static_cast<cplx*>(overlaps));
+}
+
+extern "C" int afqmc_force_bias(afqmc_ctx* c, int32_t W, const void* walkers_up,
+                                const void* walkers_dn, void* force_bias) {
+  int rc = check_w(c, W);
+  if (rc) return rc;
+  if ((rc = check_walkers(c, walkers_up, walkers_dn))) return rc;
+  if (!force_bias) return afqmc_set_error(AFQMC_ERR_ARG, "null force_bias");
+  if ((rc = set_device(c))) return rc;
+  if ((rc = launch_measure(c, W, static_cast<const cplx*>(walkers_up),
+                           static_cast<const cplx*>(walkers_dn), true)))
+    return rc;
+  if ((rc = force_bias_gemm(c, W))) return rc;
+  return launch_fb_out(c, W, static_cast<cplx*>(force_bias));
+}
+
+extern "C" int afqmc_energy(afqmc_ctx* c, int32_t W, const void* walkers_up,
+                            const void* walkers_dn, void* energies) {
+  int rc = check_w(c, W);
+  if (rc) return rc;
+  if ((rc = check_walkers(c, walkers_up, walkers_dn))) return rc;
+  if (!energies) return afqmc_set_error(AFQMC_ERR_ARG, "null energies");
+  if ((rc = set_device(c))) return rc;
+  if ((rc = launch_measure(c, W, static_cast<const cplx*>(walkers_up),
+                           static_cast<const cplx*>(walkers_dn), true)))
+    return rc;
+  return launch_energy(c, W, static_cast<cplx*>(energies));
+}
+
+extern "C" int afqmc_orthonormalize(afqmc_ctx* c, int32_t W, void* walkers_up,
+                                    void* walkers_dn) {
+  int rc = check_w(c, W);
+  if (rc) return rc;
+  if ((rc = check_walkers(c, walkers_up, walkers_dn))) return rc;
+  if ((rc = set_device(c))) return rc;
+  return launch_qr(c, W, static_cast<cplx*>(walkers_up), static_cast<cplx*>(walkers_dn));
+}
+
+extern "C" int afqmc_sr(afqmc_ctx* c, int32_t W, void* walkers_up, void* walkers_dn,
+                        double* weights, double zeta, int32_t* idx_out) {
+  int rc = check_w(c, W);
+  if (rc) return rc;
+  if ((rc = check_walkers(c, walkers_up, walkers_dn))) return rc;
+  if (!weights) return afqmc_set_error(AFQMC_ERR_ARG, "null weights");
+  if (!(zeta >= 0.0 && zeta < 1.0))
+    return afqmc_set_error(AFQMC_ERR_ARG, "zeta must lie in [0, 1)");
+  if ((rc = set_device(c))) return rc;
+  return launch_sr(c, W, static_cast<cplx*>(walkers_up), static_cast<cplx*>(walkers_dn), weights,
+                   zeta, idx_out);
+}
+
+extern "C" int afqmc_fields_normal(afqmc_ctx* c, uint64_t seed, uint64_t counter,
+                                   double* fields, int64_t count) {
+  if (!c) return afqmc_set_error(AFQMC_ERR_ARG, "null context");
+  if (!fields || count < 0) return afqmc_set_error(AFQMC_ERR_ARG, "bad fields buffer");
+  if (count == 0) return AFQMC_OK;
+  int rc = set_device(c);
+  if (rc) return rc;
+  return launch_fields_normal(c, seed, counter, fields, count);
+}

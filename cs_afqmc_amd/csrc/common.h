@@ -1,0 +1,146 @@
+// Shared device helpers and the context layout of the gfx950 AFQMC step library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/cs_afqmc_amd.h"
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+#define AFQMC_MAX_NORB 256
+#define AFQMC_MAX_NOCC 64
+
+// ---------------------------------------------------------------- complex float64
+struct cplx {
+  double re, im;
+};
+
+__host__ __device__ inline cplx cmake(double r, double i) { return cplx{r, i}; }
+__host__ __device__ inline cplx cadd(cplx a, cplx b) { return cplx{a.re + b.re, a.im + b.im}; }
+__host__ __device__ inline cplx csub(cplx a, cplx b) { return cplx{a.re - b.re, a.im - b.im}; }
+__host__ __device__ inline cplx cmul(cplx a, cplx b) {
+  return cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__host__ __device__ inline cplx cscale(cplx a, double s) { return cplx{a.re * s, a.im * s}; }
+__host__ __device__ inline cplx cconj(cplx a) { return cplx{a.re, -a.im}; }
+// a * conj(b)
+__host__ __device__ inline cplx cmulc(cplx a, cplx b) {
+  return cplx{a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
+}
+// Smith's algorithm for a / b (robust against overflow of |b|^2)
+__host__ __device__ inline cplx cdiv(cplx a, cplx b) {
+  if (fabs(b.re) >= fabs(b.im)) {
+    double r = b.im / b.re, d = b.re + b.im * r;
+    return cplx{(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+  } else {
+    double r = b.re / b.im, d = b.re * r + b.im;
+    return cplx{(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+  }
+}
+__host__ __device__ inline double cabs1(cplx a) { return fabs(a.re) + fabs(a.im); }
+
+// ---------------------------------------------------------------- reductions
+template <int NT>
+__device__ inline double block_sum(double v, double* red) {
+  // deterministic tree: wave shuffle, then one wave over the per-wave partials
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x < 64) {
+    s = (lane < NT / 64) ? red[lane] : 0.0;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) red[0] = s;
+  }
+  __syncthreads();
+  s = red[0];
+  __syncthreads();
+  return s;
+}
+
+// ---------------------------------------------------------------- context
+struct afqmc_ctx {
+  int device;
+  hipStream_t stream;
+  afqmc_dims d;
+  int nspin;          // 1 (restricted) or 2
+  int nocc[2];
+  int mult;           // 2 for restricted (both spins share the walker), 1 otherwise
+  int kdim[2];        // n_sigma * N
+  double h0, h0_prop;
+
+  // intermediates (device, float64)
+  double* trial[2];      // (N, n_sigma)
+  double* rot_h1[2];     // (n_sigma, N)
+  double* rot_chol[2];   // (G, n_sigma, N)
+  double* chol_t;        // (N*N, G): transposed Cholesky vectors
+  double* exp_h1[2];     // (N, N)
+  cplx* mf_shifts;       // (G)
+
+  // reduced-precision copies (mixed modes)
+  float* chol_t_f32;         // (N*N, G)
+  float* rot_chol_f32[2];    // (G, n_sigma, N)
+  uint16_t* chol_t_bf[2];    // hi / lo bf16 planes of chol_t
+  uint16_t* rot_chol_bf[2][2];
+
+  // workspace (device)
+  double* green[2];      // (W, 2, n_sigma*N): split re / im planes per walker
+  double* fb;            // (W, 2, G)
+  double* xs;            // (W, 2, G): i sqrt(dt) * shifted fields, split
+  double* vhs;           // (W, 2, N*N)
+  cplx* terms;           // (W, 2): shift_term, fb_term
+  cplx* dets;            // (W, 2): det(C_sigma^H w_sigma)
+  double* energy_f;      // (G*n, Wc*2*n) chunk of f_g matrices
+  cplx* energy_acc;      // (W): partial energies
+  cplx* walk_tmp;        // gather target for SR (W * N * n_max)
+  double* sr_cum;        // (W) cumulative |w|
+  int32_t* sr_idx;       // (W)
+  int energy_wchunk;
+  int64_t device_bytes;
+};
+
+// status helpers (capi.hip)
+int afqmc_set_error(int code, const char* fmt, ...);
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess)                                                          \
+      return afqmc_set_error(AFQMC_ERR_HIP, "%s failed: %s (%s:%d)", #expr,        \
+                             hipGetErrorString(_e), __FILE__, __LINE__);           \
+  } while (0)
+
+#define LAUNCH_CHECK()                                                             \
+  do {                                                                             \
+    hipError_t _e = hipGetLastError();                                             \
+    if (_e != hipSuccess)                                                          \
+      return afqmc_set_error(AFQMC_ERR_HIP, "kernel launch failed: %s (%s:%d)",   \
+                             hipGetErrorString(_e), __FILE__, __LINE__);           \
+  } while (0)
+
+// ---------------------------------------------------------------- kernel launchers
+// gemm_f64.hip: C[m][n] = alpha * sum_k A[m][k] * B[n][k] (+ beta * C), row-major
+int launch_gemm_nt_f64(hipStream_t s, int M, int N, int K, double alpha, const double* A,
+                       int lda, const double* B, int ldb, double beta, double* C, int ldc);
+// measure.hip
+int launch_measure(afqmc_ctx* c, int W, const cplx* wup, const cplx* wdn, bool want_green);
+// taylor.hip
+int launch_taylor(afqmc_ctx* c, int W, cplx* wup, cplx* wdn);
+// step.hip
+int launch_fields(afqmc_ctx* c, int W, const double* fields, cplx* fb_out);
+int launch_weights(afqmc_ctx* c, int W, double* weights, cplx* overlaps, double* scalars);
+int launch_overlap_combine(afqmc_ctx* c, int W, cplx* overlaps);
+int launch_fb_out(afqmc_ctx* c, int W, cplx* fb_out);
+// energy.hip
+int launch_energy(afqmc_ctx* c, int W, cplx* energies);
+// qr.hip
+int launch_qr(afqmc_ctx* c, int W, cplx* wup, cplx* wdn);
+// sr.hip
+int launch_sr(afqmc_ctx* c, int W, cplx* wup, cplx* wdn, double* weights, double zeta,
+              int32_t* idx_out);
+int launch_fields_normal(afqmc_ctx* c, uint64_t seed, uint64_t counter, double* out,
+                         int64_t count);

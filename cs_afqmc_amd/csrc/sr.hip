@@ -1,0 +1,134 @@
+// Stochastic reconfiguration (sr.py:24-48) and the auxiliary-field generator.
+//
+// SR: c = cumsum |w| (sequential, left to right, as numpy.cumsum), total = c[W-1],
+// z_i = total * (i + zeta) / W, idx_i = searchsorted(c, z_i, side='left') clamped to
+// W - 1 (jax gathers clamp out-of-range indices), walkers <- walkers[idx],
+// weights <- total / W.  Indices are integers, so they are reproduced bit for bit.
+//
+// Fields: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) keyed by the 64-bit seed,
+// counter = (pair index, stream counter); two 53-bit uniforms per draw feed a
+// Box-Muller transform (float64).
+#include "common.h"
+
+namespace {
+
+constexpr int SR_NT = 1024;
+
+__global__ __launch_bounds__(SR_NT) void sr_index_kernel(int W, double zeta,
+                                                         double* __restrict__ weights,
+                                                         double* __restrict__ cum,
+                                                         int32_t* __restrict__ idx,
+                                                         int32_t* __restrict__ idx_out) {
+  __shared__ double total_s;
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < W; ++i) {
+      s += fabs(weights[i]);
+      cum[i] = s;
+    }
+    total_s = s;
+  }
+  __syncthreads();
+  const double total = total_s;
+  const double avg = total / (double)W;
+  for (int i = threadIdx.x; i < W; i += SR_NT) {
+    const double z = total * ((double)i + zeta) / (double)W;
+    // first j with cum[j] >= z
+    int lo = 0, hi = W;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (cum[mid] < z)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    const int j = lo < W ? lo : W - 1;
+    idx[i] = j;
+    if (idx_out) idx_out[i] = j;
+    weights[i] = avg;
+  }
+}
+
+// dst[i] = src[idx[i]] for walkers of `elems` complex numbers each
+__global__ void gather_kernel(int W, int elems, const int32_t* __restrict__ idx,
+                              const cplx* __restrict__ src, cplx* __restrict__ dst) {
+  const int i = blockIdx.y;
+  const cplx* s = src + (int64_t)idx[i] * elems;
+  cplx* d = dst + (int64_t)i * elems;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < elems; e += gridDim.x * blockDim.x)
+    d[e] = s[e];
+}
+
+// ---- Philox4x32-10
+struct u32x4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ inline void mulhilo(uint32_t a, uint32_t b, uint32_t& hi, uint32_t& lo) {
+  const uint64_t p = (uint64_t)a * (uint64_t)b;
+  hi = (uint32_t)(p >> 32);
+  lo = (uint32_t)p;
+}
+
+__device__ inline u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi0, lo0, hi1, lo1;
+    mulhilo(M0, c.x, hi0, lo0);
+    mulhilo(M1, c.z, hi1, lo1);
+    c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    k0 += W0;
+    k1 += W1;
+  }
+  return c;
+}
+
+__global__ void normal_kernel(uint64_t seed, uint64_t counter, double* __restrict__ out,
+                              int64_t count) {
+  const int64_t pair = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (2 * pair >= count) return;
+  const u32x4 c = u32x4{(uint32_t)pair, (uint32_t)((uint64_t)pair >> 32), (uint32_t)counter,
+                        (uint32_t)(counter >> 32)};
+  const u32x4 r = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const uint64_t a = ((uint64_t)r.y << 32) | r.x;
+  const uint64_t b = ((uint64_t)r.w << 32) | r.z;
+  const double u1 = ((double)(a >> 11) + 1.0) * 0x1.0p-53;  // (0, 1]
+  const double u2 = (double)(b >> 11) * 0x1.0p-53;          // [0, 1)
+  const double rad = sqrt(-2.0 * log(u1));
+  double s, co;
+  sincos(2.0 * M_PI * u2, &s, &co);
+  out[2 * pair] = rad * co;
+  if (2 * pair + 1 < count) out[2 * pair + 1] = rad * s;
+}
+
+}  // namespace
+
+int launch_sr(afqmc_ctx* c, int W, cplx* wup, cplx* wdn, double* weights, double zeta,
+              int32_t* idx_out) {
+  hipLaunchKernelGGL(sr_index_kernel, dim3(1), dim3(SR_NT), 0, c->stream, W, zeta, weights,
+                     c->sr_cum, c->sr_idx, idx_out);
+  LAUNCH_CHECK();
+  const int N = c->d.norb;
+  for (int s = 0; s < c->nspin; ++s) {
+    cplx* wk = (s == 0) ? wup : wdn;
+    const int elems = N * c->nocc[s];
+    const int bx = (elems + 255) / 256 < 8 ? (elems + 255) / 256 : 8;
+    hipLaunchKernelGGL(gather_kernel, dim3(bx, W), dim3(256), 0, c->stream, W, elems,
+                       c->sr_idx, wk, c->walk_tmp);
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(wk, c->walk_tmp, (size_t)W * elems * sizeof(cplx),
+                           hipMemcpyDeviceToDevice, c->stream));
+  }
+  return AFQMC_OK;
+}
+
+int launch_fields_normal(afqmc_ctx* c, uint64_t seed, uint64_t counter, double* out,
+                         int64_t count) {
+  const int64_t pairs = (count + 1) / 2;
+  hipLaunchKernelGGL(normal_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0,
+                     c->stream, seed, counter, out, count);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}

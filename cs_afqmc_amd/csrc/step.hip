@@ -1,0 +1,142 @@
+// Element-wise parts of the phaseless step (propagator.propagate, propagation.py:100-165):
+//   fields_kernel  : field shifts, shifted fields, shift / force-bias terms
+//                    (propagation.py:121-127) and the VHS GEMM operand i sqrt(dt) x~
+//   weights_kernel : importance function, phaseless projection, clipping, weight cap and
+//                    the population-control shift (propagation.py:133-164)
+#include "common.h"
+
+namespace {
+
+constexpr int FIELDS_NT = 256;
+constexpr int WEIGHTS_NT = 1024;
+
+__device__ inline cplx overlap_of(const cplx* dets, int w, int nspin) {
+  const cplx d0 = dets[2 * w];
+  return nspin == 1 ? cmul(d0, d0) : cmul(d0, dets[2 * w + 1]);
+}
+
+__device__ inline cplx cexp_(cplx a) {
+  const double e = exp(a.re);
+  double s, c;
+  sincos(a.im, &s, &c);
+  return cplx{e * c, e * s};
+}
+
+// one workgroup per walker
+__global__ __launch_bounds__(FIELDS_NT) void fields_kernel(
+    int G, double sqdt, const double* __restrict__ fb, const cplx* __restrict__ mf,
+    const double* __restrict__ fields, double* __restrict__ xs, cplx* __restrict__ terms,
+    cplx* __restrict__ fb_out) {
+  __shared__ double red[FIELDS_NT / 64];
+  const int w = blockIdx.x;
+  const double* fbr = fb + (int64_t)w * 2 * G;
+  const double* fbi = fbr + G;
+  const double* x = fields + (int64_t)w * G;
+  double* xr = xs + (int64_t)w * 2 * G;
+  double* xi = xr + G;
+  cplx st = cplx{0.0, 0.0}, ft = cplx{0.0, 0.0};
+  for (int g = threadIdx.x; g < G; g += FIELDS_NT) {
+    const cplx f = cplx{fbr[g], fbi[g]};
+    const cplx m = mf[g];
+    // field_shifts = -sqrt(dt) * (1j * fb - mf_shifts)
+    const cplx ifb_m = cplx{-f.im - m.re, f.re - m.im};
+    const cplx fs = cplx{-sqdt * ifb_m.re, -sqdt * ifb_m.im};
+    const double xv = x[g];
+    // shifted_fields = fields - field_shifts
+    const cplx xt = cplx{xv - fs.re, -fs.im};
+    st = cadd(st, cmul(xt, m));
+    // fields * field_shifts - field_shifts * field_shifts / 2
+    const cplx fs2 = cmul(fs, fs);
+    ft = cadd(ft, cplx{xv * fs.re - fs2.re / 2.0, xv * fs.im - fs2.im / 2.0});
+    // VHS operand: 1j * sqrt(dt) * shifted_fields
+    xr[g] = -sqdt * xt.im;
+    xi[g] = sqdt * xt.re;
+    if (fb_out) fb_out[(int64_t)w * G + g] = f;
+  }
+  const double s0 = block_sum<FIELDS_NT>(st.re, red);
+  const double s1 = block_sum<FIELDS_NT>(st.im, red);
+  const double s2 = block_sum<FIELDS_NT>(ft.re, red);
+  const double s3 = block_sum<FIELDS_NT>(ft.im, red);
+  if (threadIdx.x == 0) {
+    terms[2 * w] = cplx{s0, s1};
+    terms[2 * w + 1] = cplx{s2, s3};
+  }
+}
+
+// single workgroup over all walkers (the weight sum needs every walker)
+__global__ __launch_bounds__(WEIGHTS_NT) void weights_kernel(
+    int W, int nspin, double dt, double sqdt, double h0_prop, const cplx* __restrict__ dets,
+    const cplx* __restrict__ terms, double* __restrict__ weights, cplx* __restrict__ overlaps,
+    double* __restrict__ scalars) {
+  __shared__ double red[WEIGHTS_NT / 64];
+  const double e_estimate = scalars[0];
+  const double pcs = scalars[1];
+  double part = 0.0;
+  for (int w = threadIdx.x; w < W; w += WEIGHTS_NT) {
+    const cplx onew = overlap_of(dets, w, nspin);
+    const cplx oold = overlaps[w];
+    const cplx st = terms[2 * w], ft = terms[2 * w + 1];
+    const cplx a = cplx{-sqdt * st.re + ft.re + dt * (pcs + h0_prop), -sqdt * st.im + ft.im};
+    const cplx imp = cdiv(cmul(cexp_(a), onew), oold);
+    const cplx rb = cdiv(cmul(cexp_(cplx{-sqdt * st.re, -sqdt * st.im}), onew), oold);
+    const double theta = atan2(rb.im, rb.re);
+    double ipl = hypot(imp.re, imp.im) * cos(theta);
+    if (isnan(ipl)) ipl = 0.0;
+    if (ipl < 1.0e-3) ipl = 0.0;
+    if (ipl > 100.0) ipl = 0.0;
+    double wt = ipl * weights[w];
+    if (wt > 100.0) wt = 0.0;
+    weights[w] = wt;
+    overlaps[w] = onew;
+    part += wt;
+  }
+  const double total = block_sum<WEIGHTS_NT>(part, red);
+  if (threadIdx.x == 0) scalars[1] = e_estimate - 0.1 * (log(total / (double)W) / dt);
+}
+
+__global__ void overlap_combine_kernel(int W, int nspin, const cplx* __restrict__ dets,
+                                       cplx* __restrict__ overlaps) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < W) overlaps[w] = overlap_of(dets, w, nspin);
+}
+
+__global__ void fb_out_kernel(int W, int G, const double* __restrict__ fb,
+                              cplx* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < (int64_t)W * G) {
+    const int64_t w = e / G, g = e - w * G;
+    out[e] = cplx{fb[w * 2 * G + g], fb[w * 2 * G + G + g]};
+  }
+}
+
+}  // namespace
+
+int launch_fields(afqmc_ctx* c, int W, const double* fields, cplx* fb_out) {
+  hipLaunchKernelGGL(fields_kernel, dim3(W), dim3(FIELDS_NT), 0, c->stream, c->d.nchol,
+                     sqrt(c->d.dt), c->fb, c->mf_shifts, fields, c->xs, c->terms, fb_out);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+int launch_weights(afqmc_ctx* c, int W, double* weights, cplx* overlaps, double* scalars) {
+  hipLaunchKernelGGL(weights_kernel, dim3(1), dim3(WEIGHTS_NT), 0, c->stream, W, c->nspin,
+                     c->d.dt, sqrt(c->d.dt), c->h0_prop, c->dets, c->terms, weights, overlaps,
+                     scalars);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+int launch_overlap_combine(afqmc_ctx* c, int W, cplx* overlaps) {
+  hipLaunchKernelGGL(overlap_combine_kernel, dim3((W + 255) / 256), dim3(256), 0, c->stream, W,
+                     c->nspin, c->dets, overlaps);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+int launch_fb_out(afqmc_ctx* c, int W, cplx* fb_out) {
+  const int64_t total = (int64_t)W * c->d.nchol;
+  hipLaunchKernelGGL(fb_out_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     c->stream, W, c->d.nchol, c->fb, fb_out);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}

@@ -1,0 +1,197 @@
+"""Device step context: one afqmc_ctx of the HIP library plus its torch plumbing.
+
+torch provides device memory and the current HIP stream; every computation of the
+step runs in the library's gfx950 kernels (see include/cs_afqmc_amd.h).  There is no
+CPU fallback: without a GPU or without the built library the calls raise.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+_f64 = np.float64
+
+
+def _cptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _dptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class DeviceStep:
+    """Owns the device copies of the intermediates and the step workspace.
+
+    Args mirror the reference's ham_data / wave_data entries (host float64 arrays):
+      walker_type: "rhf" (propagator_restricted + rhf trial) or "uhf"
+      mo_coeff:    (N, n) for rhf; [(N, n_up), (N, n_dn)] for uhf
+      rot_h1:      (n, N) / [(n_up, N), (n_dn, N)]
+      rot_chol:    (G, n, N) / [(G, n_up, N), (G, n_dn, N)]
+      chol:        (G, N*N)
+      mf_shifts:   (G,) complex
+      exp_h1:      (N, N) / (2, N, N)
+    """
+
+    def __init__(self, walker_type, norb, nelec, n_walkers, dt, n_exp_terms, h0, h0_prop,
+                 chol, mo_coeff, rot_h1, rot_chol, mf_shifts, exp_h1, precision="fp64"):
+        if not torch.cuda.is_available():
+            raise _lib.AfqmcError("cs_afqmc_amd needs a ROCm GPU (torch.cuda is unavailable)")
+        self.walker_type = walker_type
+        self.norb = int(norb)
+        self.nelec = (int(nelec[0]), int(nelec[1]))
+        self.n_walkers = int(n_walkers)
+        self.dt = float(dt)
+        self.n_exp_terms = int(n_exp_terms)
+        self.precision = precision
+        chol = np.ascontiguousarray(np.asarray(chol, dtype=_f64).reshape(-1, norb * norb))
+        self.nchol = chol.shape[0]
+        if walker_type == "rhf":
+            mo = np.ascontiguousarray(np.asarray(mo_coeff, dtype=_f64))
+            rh1 = np.ascontiguousarray(np.asarray(rot_h1, dtype=_f64))
+            rch = np.ascontiguousarray(np.asarray(rot_chol, dtype=_f64))
+            eh1 = np.ascontiguousarray(np.asarray(exp_h1, dtype=_f64).reshape(norb, norb))
+            wt = _lib.AFQMC_RHF
+        elif walker_type == "uhf":
+            mo = np.ascontiguousarray(np.concatenate(
+                [np.asarray(m, dtype=_f64).ravel() for m in mo_coeff]))
+            rh1 = np.ascontiguousarray(np.concatenate(
+                [np.asarray(m, dtype=_f64).ravel() for m in rot_h1]))
+            rch = np.ascontiguousarray(np.concatenate(
+                [np.asarray(m, dtype=_f64).ravel() for m in rot_chol]))
+            eh1 = np.ascontiguousarray(np.asarray(exp_h1, dtype=_f64).reshape(2, norb, norb))
+            wt = _lib.AFQMC_UHF
+        else:
+            raise ValueError(f"walker_type {walker_type!r}")
+        for a in (mo, rh1, rch):
+            if np.iscomplexobj(a):
+                raise NotImplementedError("complex trial coefficients are not supported")
+        mf = np.ascontiguousarray(np.asarray(mf_shifts, dtype=np.complex128).reshape(-1))
+        if mf.shape[0] != self.nchol:
+            raise ValueError("mf_shifts length != nchol")
+        mf_ri = np.ascontiguousarray(mf.view(_f64))
+        self._keep = (chol, mo, rh1, rch, eh1, mf_ri)
+        dims = _lib.Dims(wt, self.norb, self.nelec[0], self.nelec[1], self.nchol,
+                         self.n_walkers, self.n_exp_terms, _lib.PRECISIONS[precision], self.dt)
+        ham = _lib.HamHost(float(np.real(h0)), float(np.real(h0_prop)), _cptr(chol), _cptr(mo),
+                           _cptr(rh1), _cptr(rch), _cptr(mf_ri), _cptr(eh1))
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        handle = ctypes.c_void_p()
+        L = _lib.lib()
+        _lib.check(L.afqmc_ctx_create(ctypes.byref(handle), ctypes.byref(dims),
+                                      ctypes.byref(ham), _stream()))
+        self._h = handle
+        self._lib = L
+
+    # ------------------------------------------------------------------ plumbing
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.afqmc_ctx_destroy(h)
+            self._h = None
+
+    def device_bytes(self):
+        return int(self._lib.afqmc_ctx_device_bytes(self._h))
+
+    def _prep(self):
+        _lib.check(self._lib.afqmc_set_stream(self._h, _stream()))
+
+    def _walkers(self, walkers):
+        if self.walker_type == "rhf":
+            up, dn = walkers, None
+            self._check_walker_tensor(up, self.nelec[0])
+            return up, None, up.shape[0]
+        up, dn = walkers
+        self._check_walker_tensor(up, self.nelec[0])
+        self._check_walker_tensor(dn, self.nelec[1])
+        if up.shape[0] != dn.shape[0]:
+            raise ValueError("up/dn walker counts differ")
+        return up, dn, up.shape[0]
+
+    def _check_walker_tensor(self, t, n):
+        if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+            raise TypeError("walkers must be cuda tensors (complex128)")
+        if t.dtype != torch.complex128 or not t.is_contiguous():
+            raise TypeError("walkers must be contiguous complex128")
+        if t.dim() != 3 or t.shape[1] != self.norb or t.shape[2] != n:
+            raise ValueError(f"walker shape {tuple(t.shape)} != (W, {self.norb}, {n})")
+        if t.shape[0] > self.n_walkers:
+            raise ValueError(f"{t.shape[0]} walkers > context capacity {self.n_walkers}")
+
+    def _vec(self, t, W, dtype, name, cols=None):
+        shape = (W,) if cols is None else (W, cols)
+        if (not isinstance(t, torch.Tensor) or t.device.type != "cuda" or t.dtype != dtype
+                or not t.is_contiguous() or tuple(t.shape) != shape):
+            raise TypeError(f"{name} must be a contiguous cuda {dtype} tensor of shape {shape}")
+
+    # ------------------------------------------------------------------ entry points
+    def propagate(self, walkers, weights, overlaps, fields, scalars, force_bias_out=None):
+        up, dn, W = self._walkers(walkers)
+        self._vec(weights, W, torch.float64, "weights")
+        self._vec(overlaps, W, torch.complex128, "overlaps")
+        self._vec(fields, W, torch.float64, "fields", self.nchol)
+        if (scalars.dtype != torch.float64 or scalars.numel() != 2 or not scalars.is_cuda
+                or not scalars.is_contiguous()):
+            raise TypeError("scalars must be a contiguous cuda float64 tensor of 2 elements")
+        if force_bias_out is not None:
+            self._vec(force_bias_out, W, torch.complex128, "force_bias_out", self.nchol)
+        self._prep()
+        _lib.check(self._lib.afqmc_propagate(
+            self._h, W, _dptr(up), _dptr(dn), _dptr(weights), _dptr(overlaps), _dptr(fields),
+            _dptr(scalars), _dptr(force_bias_out)))
+
+    def overlap(self, walkers, out=None):
+        up, dn, W = self._walkers(walkers)
+        if out is None:
+            out = torch.empty(W, dtype=torch.complex128, device=up.device)
+        self._prep()
+        _lib.check(self._lib.afqmc_overlap(self._h, W, _dptr(up), _dptr(dn), _dptr(out)))
+        return out
+
+    def force_bias(self, walkers, out=None):
+        up, dn, W = self._walkers(walkers)
+        if out is None:
+            out = torch.empty((W, self.nchol), dtype=torch.complex128, device=up.device)
+        self._prep()
+        _lib.check(self._lib.afqmc_force_bias(self._h, W, _dptr(up), _dptr(dn), _dptr(out)))
+        return out
+
+    def energy(self, walkers, out=None):
+        up, dn, W = self._walkers(walkers)
+        if out is None:
+            out = torch.empty(W, dtype=torch.complex128, device=up.device)
+        self._prep()
+        _lib.check(self._lib.afqmc_energy(self._h, W, _dptr(up), _dptr(dn), _dptr(out)))
+        return out
+
+    def orthonormalize(self, walkers):
+        up, dn, W = self._walkers(walkers)
+        self._prep()
+        _lib.check(self._lib.afqmc_orthonormalize(self._h, W, _dptr(up), _dptr(dn)))
+        return walkers
+
+    def sr(self, walkers, weights, zeta, idx_out=None):
+        up, dn, W = self._walkers(walkers)
+        self._vec(weights, W, torch.float64, "weights")
+        if idx_out is not None:
+            self._vec(idx_out, W, torch.int32, "idx_out")
+        self._prep()
+        _lib.check(self._lib.afqmc_sr(self._h, W, _dptr(up), _dptr(dn), _dptr(weights),
+                                      float(zeta), _dptr(idx_out)))
+        return idx_out
+
+    def fields_normal(self, seed, counter, shape, out=None):
+        if out is None:
+            out = torch.empty(shape, dtype=torch.float64, device=self.device)
+        self._prep()
+        _lib.check(self._lib.afqmc_fields_normal(
+            self._h, int(seed) & (2**64 - 1), int(counter) & (2**64 - 1), _dptr(out),
+            out.numel()))
+        return out

@@ -1,0 +1,75 @@
+"""Seeded synthetic AFQMC problems (host float64 arrays in the reference's layout).
+
+``synthetic_rhf`` / ``synthetic_uhf`` build the workloads used for measurement:
+  L_g = sym(A_g) exp(-g / 80), A_g entries N(0, 1/norb), sym(A) = (A + A^T) / sqrt(2)
+  h1  = sym(B), B entries N(0, 0.1)
+  trial = first nocc columns of a Haar-random orthogonal matrix (QR of a Gaussian
+          matrix with the signs of diag(R) folded into Q); separate alpha / beta
+          matrices for the unrestricted trial
+  h0 = 0.
+Returned as (ham_data, wave_data, nelec) with ham_data = {h0, h1 (2,N,N), chol (G,N*N),
+ene0} and wave_data = {mo_coeff, rdm1}, the dicts mpi_jax.py:107-140 builds from
+FCIDUMP_chol / mo_coeff.npz.
+"""
+
+import numpy as np
+
+WORKLOADS = {
+    # name: (walker_type, norb, nocc, nchol, n_walkers)
+    "rhf_n100_o20_g400_w1024": ("rhf", 100, 20, 400, 1024),
+    "uhf_n200_o40_g1000_w2048": ("uhf", 200, 40, 1000, 2048),
+}
+
+
+def _sym(a):
+    return (a + a.T) / np.sqrt(2.0)
+
+
+def haar_orthogonal(n, rng):
+    z = rng.standard_normal((n, n))
+    q, r = np.linalg.qr(z)
+    return q * np.sign(np.diag(r))[None, :]
+
+
+def _chol_h1(norb, nchol, rng, decay=80.0):
+    chol = np.empty((nchol, norb, norb))
+    for g in range(nchol):
+        a = rng.normal(0.0, np.sqrt(1.0 / norb), (norb, norb))
+        chol[g] = _sym(a) * np.exp(-g / decay)
+    h1 = _sym(rng.normal(0.0, np.sqrt(0.1), (norb, norb)))
+    return chol.reshape(nchol, norb * norb), h1
+
+
+def synthetic_rhf(norb, nocc, nchol, seed=0, decay=80.0):
+    rng = np.random.default_rng(seed)
+    chol, h1 = _chol_h1(norb, nchol, rng, decay)
+    mo = haar_orthogonal(norb, rng)[:, :nocc]
+    ham_data = {"h0": 0.0, "h1": np.array([h1, h1]), "chol": chol, "ene0": 0.0}
+    wave_data = {"mo_coeff": mo, "rdm1": np.array([mo @ mo.T] * 2)}
+    return ham_data, wave_data, (nocc, nocc)
+
+
+def synthetic_uhf(norb, nocc_up, nocc_dn, nchol, seed=0, decay=80.0):
+    rng = np.random.default_rng(seed)
+    chol, h1 = _chol_h1(norb, nchol, rng, decay)
+    mo_a = haar_orthogonal(norb, rng)[:, :nocc_up]
+    mo_b = haar_orthogonal(norb, rng)[:, :nocc_dn]
+    ham_data = {"h0": 0.0, "h1": np.array([h1, h1]), "chol": chol, "ene0": 0.0}
+    wave_data = {"mo_coeff": [mo_a, mo_b],
+                 "rdm1": np.array([mo_a @ mo_a.T, mo_b @ mo_b.T])}
+    return ham_data, wave_data, (nocc_up, nocc_dn)
+
+
+def random_walkers(walker_type, norb, nelec, n_walkers, rng, near=None, noise=0.1):
+    """Complex walkers: ``near`` (trial coefficients) plus complex Gaussian noise, or
+    purely random ones.  Used for parity tests away from the trial state."""
+    def one(n, base):
+        w = noise * (rng.standard_normal((n_walkers, norb, n))
+                     + 1j * rng.standard_normal((n_walkers, norb, n)))
+        if base is not None:
+            w = w + base[None, :, :]
+        return np.ascontiguousarray(w)
+    if walker_type == "rhf":
+        return one(nelec[0], near)
+    base = near if near is not None else [None, None]
+    return [one(nelec[0], base[0]), one(nelec[1], base[1])]

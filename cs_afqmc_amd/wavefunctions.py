@@ -1,0 +1,209 @@
+"""Trial wave functions with the reference's measurement API, evaluated on the GPU.
+
+Mirrors ad_afqmc/wavefunctions.py: ``rhf`` (:453-635) and ``uhf`` (:638-844) with
+``calc_overlap`` / ``calc_force_bias`` / ``calc_energy`` (:45-241), ``get_rdm1``
+(:268-282), ``get_init_walkers`` (:298-355) and ``_build_measurement_intermediates``
+(:602-619 / :817-841).  Walkers are cuda complex128 tensors: (W, N, n) for
+restricted walkers, [up, dn] for unrestricted ones.  Measurements run in the HIP
+library (afqmc_overlap / afqmc_force_bias / afqmc_energy); intermediates are built
+once on the host in float64, as the reference builds them once per run.
+
+Supported pairs (mpi_jax.py:314-352): restricted walkers with an rhf trial, and
+unrestricted walkers with a uhf trial.  The trial coefficients must be real.
+"""
+
+from dataclasses import dataclass
+from typing import Tuple
+
+import numpy as np
+import torch
+
+from .device import DeviceStep
+
+
+def _as_np(x):
+    if isinstance(x, torch.Tensor):
+        return x.detach().cpu().numpy()
+    return np.asarray(x)
+
+
+def _device_for(trial, ham_data, wave_data, walker_type, n_walkers):
+    """Return a DeviceStep for measurements on ``n_walkers`` walkers.
+
+    Reuses the propagator's context stored in ham_data when it fits; otherwise builds
+    a measurement-only context (identity exp_h1, zero mf_shifts: unused by the
+    measurement kernels)."""
+    holders = [ham_data] if ham_data is not None else [wave_data]
+    for holder in holders:
+        for key in ("_afqmc_device", "_afqmc_measure_device"):
+            dev = holder.get(key)
+            if dev is not None and dev.walker_type == walker_type and dev.n_walkers >= n_walkers:
+                return dev
+    norb = trial.norb
+    eye = np.eye(norb) if walker_type == "rhf" else np.array([np.eye(norb)] * 2)
+    if ham_data is not None:
+        nchol = np.asarray(_as_np(ham_data["chol"])).reshape(-1, norb * norb).shape[0]
+        dev = DeviceStep(
+            walker_type, norb, trial.nelec, n_walkers, 0.01, 2, _as_np(ham_data["h0"]), 0.0,
+            _as_np(ham_data["chol"]), wave_data["mo_coeff"], ham_data["rot_h1"],
+            ham_data["rot_chol"], np.zeros(nchol, dtype=np.complex128), eye)
+        ham_data["_afqmc_measure_device"] = dev
+        return dev
+    # overlap only: the Hamiltonian arrays are placeholders the overlap kernel never reads
+    if walker_type == "rhf":
+        n = trial.nelec[0]
+        rh1, rch = np.zeros((n, norb)), np.zeros((1, n, norb))
+    else:
+        rh1 = [np.zeros((trial.nelec[s], norb)) for s in range(2)]
+        rch = [np.zeros((1, trial.nelec[s], norb)) for s in range(2)]
+    dev = DeviceStep(walker_type, norb, trial.nelec, n_walkers, 0.01, 2, 0.0, 0.0,
+                     np.zeros((1, norb * norb)), wave_data["mo_coeff"], rh1, rch,
+                     np.zeros(1, dtype=np.complex128), eye)
+    wave_data["_afqmc_measure_device"] = dev
+    return dev
+
+
+def _nwalkers(walkers):
+    return walkers.shape[0] if isinstance(walkers, torch.Tensor) else walkers[0].shape[0]
+
+
+@dataclass
+class rhf:
+    """Restricted Hartree-Fock trial (wavefunctions.py:453-635).  wave_data["mo_coeff"]
+    is (norb, nocc); ham_data holds rot_h1 (nocc, norb) and rot_chol (nchol, nocc, norb)."""
+
+    norb: int
+    nelec: Tuple[int, int]
+    n_opt_iter: int = 30
+    n_batch: int = 1
+
+    walker_type = "rhf"
+
+    def __post_init__(self):
+        assert self.nelec[0] == self.nelec[1], "RHF requires equal up and down electrons."
+
+    def calc_overlap(self, walkers, wave_data):
+        dev = _device_for(self, None, wave_data, "rhf", _nwalkers(walkers))
+        return dev.overlap(walkers)
+
+    def calc_force_bias(self, walkers, ham_data, wave_data):
+        dev = _device_for(self, ham_data, wave_data, "rhf", _nwalkers(walkers))
+        return dev.force_bias(walkers)
+
+    def calc_energy(self, walkers, ham_data, wave_data):
+        dev = _device_for(self, ham_data, wave_data, "rhf", _nwalkers(walkers))
+        return dev.energy(walkers)
+
+    def _calc_rdm1(self, wave_data):
+        mo = _as_np(wave_data["mo_coeff"])
+        return np.array([mo @ mo.T] * 2)
+
+    def get_rdm1(self, wave_data):
+        if "rdm1" in wave_data:
+            return _as_np(wave_data["rdm1"])
+        return self._calc_rdm1(wave_data)
+
+    def get_init_walkers(self, wave_data, n_walkers, restricted=True):
+        return _init_walkers(self, wave_data, n_walkers, restricted)
+
+    def _build_measurement_intermediates(self, ham_data, wave_data):
+        """wavefunctions.py:602-619: symmetrised h1, rot_h1 = C^T (h1a + h1b)/2,
+        rot_chol[g] = C^T L_g."""
+        norb = self.norb
+        h1 = np.asarray(_as_np(ham_data["h1"]), dtype=np.float64)
+        h1 = np.array([(h1[0] + h1[0].T) / 2.0, (h1[1] + h1[1].T) / 2.0])
+        ham_data["h1"] = h1
+        mo = np.asarray(_as_np(wave_data["mo_coeff"]), dtype=np.float64)
+        chol3 = np.asarray(_as_np(ham_data["chol"]), dtype=np.float64).reshape(-1, norb, norb)
+        ham_data["rot_h1"] = mo.T @ ((h1[0] + h1[1]) / 2.0)
+        ham_data["rot_chol"] = np.einsum("pi,gij->gpj", mo.T, chol3, optimize=True)
+        _invalidate(ham_data)
+        return ham_data
+
+
+@dataclass
+class uhf:
+    """Unrestricted Hartree-Fock trial (wavefunctions.py:638-844).  wave_data["mo_coeff"]
+    is [C_up (norb, n_up), C_dn (norb, n_dn)]."""
+
+    norb: int
+    nelec: Tuple[int, int]
+    n_opt_iter: int = 30
+    n_batch: int = 1
+
+    walker_type = "uhf"
+
+    def calc_overlap(self, walkers, wave_data):
+        dev = _device_for(self, None, wave_data, "uhf", _nwalkers(walkers))
+        return dev.overlap(walkers)
+
+    def calc_force_bias(self, walkers, ham_data, wave_data):
+        dev = _device_for(self, ham_data, wave_data, "uhf", _nwalkers(walkers))
+        return dev.force_bias(walkers)
+
+    def calc_energy(self, walkers, ham_data, wave_data):
+        dev = _device_for(self, ham_data, wave_data, "uhf", _nwalkers(walkers))
+        return dev.energy(walkers)
+
+    def _calc_rdm1(self, wave_data):
+        mo = [_as_np(m) for m in wave_data["mo_coeff"]]
+        return np.array([mo[0] @ mo[0].T, mo[1] @ mo[1].T])
+
+    def get_rdm1(self, wave_data):
+        if "rdm1" in wave_data:
+            return _as_np(wave_data["rdm1"])
+        return self._calc_rdm1(wave_data)
+
+    def get_init_walkers(self, wave_data, n_walkers, restricted=False):
+        return _init_walkers(self, wave_data, n_walkers, restricted)
+
+    def _build_measurement_intermediates(self, ham_data, wave_data):
+        """wavefunctions.py:817-841."""
+        norb = self.norb
+        h1 = np.asarray(_as_np(ham_data["h1"]), dtype=np.float64)
+        h1 = np.array([(h1[0] + h1[0].T) / 2.0, (h1[1] + h1[1].T) / 2.0])
+        ham_data["h1"] = h1
+        mo = [np.asarray(_as_np(m), dtype=np.float64) for m in wave_data["mo_coeff"]]
+        chol3 = np.asarray(_as_np(ham_data["chol"]), dtype=np.float64).reshape(-1, norb, norb)
+        ham_data["rot_h1"] = [mo[0].T @ h1[0], mo[1].T @ h1[1]]
+        ham_data["rot_chol"] = [
+            np.einsum("pi,gij->gpj", mo[0].T, chol3, optimize=True),
+            np.einsum("pi,gij->gpj", mo[1].T, chol3, optimize=True),
+        ]
+        _invalidate(ham_data)
+        return ham_data
+
+
+def _invalidate(ham_data):
+    ham_data.pop("_afqmc_device", None)
+    ham_data.pop("_afqmc_measure_device", None)
+
+
+def _init_walkers(trial, wave_data, n_walkers, restricted):
+    """wavefunctions.py:298-355: natural orbitals (descending occupation) of each spin
+    rdm1, copied n_walkers times, as cuda complex128 tensors."""
+    rdm1 = trial.get_rdm1(wave_data)
+    natorbs_up = np.linalg.eigh(rdm1[0])[1][:, ::-1][:, : trial.nelec[0]]
+    natorbs_dn = np.linalg.eigh(rdm1[1])[1][:, ::-1][:, : trial.nelec[1]]
+    dev = torch.device("cuda", torch.cuda.current_device())
+
+    def rep(m):
+        t = torch.as_tensor(np.ascontiguousarray(m + 0.0j), device=dev)
+        return t.unsqueeze(0).repeat(n_walkers, 1, 1).contiguous()
+
+    if restricted:
+        if trial.nelec[0] != trial.nelec[1]:
+            raise NotImplementedError("restricted walkers need nelec[0] == nelec[1]")
+        det_overlap = np.linalg.det(natorbs_up.T @ natorbs_dn)
+        if np.abs(det_overlap) > 1e-3:
+            return rep(natorbs_up)
+        overlaps = np.array(
+            [natorbs_up[:, i].T @ natorbs_dn[:, i] for i in range(trial.nelec[0])])
+        new_vecs = natorbs_up + np.einsum("ij,j->ij", natorbs_dn, np.sign(overlaps))
+        new_vecs = np.linalg.qr(new_vecs)[0]
+        det_overlap = np.linalg.det(new_vecs.T @ natorbs_up) * np.linalg.det(
+            new_vecs.T @ natorbs_dn)
+        if np.abs(det_overlap) > 1e-3:
+            return rep(new_vecs)
+        raise ValueError("Cannot find a set of RHF orbitals with good trial overlap.")
+    return [rep(natorbs_up), rep(natorbs_dn)]

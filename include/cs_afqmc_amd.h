@@ -1,0 +1,149 @@
+/*
+ * cs_afqmc_amd -- C ABI of the MI355X (gfx950) phaseless-AFQMC propagation step.
+ *
+ * The step replaces ad_afqmc's propagator / trial measurement pytree functions
+ * (reference files cited per entry point, paths relative to the reference root).
+ * Plain pointers and sizes only; no torch or jax types cross this boundary.
+ *
+ * Conventions
+ *   - Every entry point returns AFQMC_OK (0) or a nonzero status; the message of the
+ *     last failure on the calling thread is returned by afqmc_last_error().
+ *     Numerical pathologies (NaN overlaps, collapsed weights) are not errors: they
+ *     follow the reference's zeroing rules (propagation.py:149-160).
+ *   - "dev" pointers are device memory on the context's device; they are owned by
+ *     the caller.  Host pointers are read during the call only.
+ *   - Complex numbers are interleaved (re, im) float64 pairs (complex128).
+ *   - Work is enqueued on the context's stream; no entry point except
+ *     afqmc_ctx_create / afqmc_ctx_destroy synchronises the host.
+ *   - Walkers: restricted  -> one array (W, norb, nocc) complex128 (pass it as
+ *     walkers_up, walkers_dn = NULL); unrestricted -> up (W, norb, nocc_up) and
+ *     dn (W, norb, nocc_dn).
+ *   - Calls that share one context must not overlap (the context owns one workspace).
+ */
+#ifndef CS_AFQMC_AMD_H
+#define CS_AFQMC_AMD_H
+
+#include <stdint.h>
+
+#if defined(__GNUC__) || defined(__clang__)
+#define AFQMC_API __attribute__((visibility("default")))
+#else
+#define AFQMC_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct afqmc_ctx afqmc_ctx;
+
+enum {
+  AFQMC_OK = 0,
+  AFQMC_ERR_ARG = 1,     /* bad argument (null pointer, size out of range) */
+  AFQMC_ERR_HIP = 2,     /* HIP runtime error */
+  AFQMC_ERR_SIZE = 3,    /* problem larger than the kernels support */
+  AFQMC_ERR_STATE = 4    /* call not valid in the context's state */
+};
+
+/* walker_type: propagator_restricted + rhf trial, or propagator_unrestricted + uhf
+ * trial (mpi_jax.py:314-352 picks the pair from options["walker_type"/"trial"]). */
+enum { AFQMC_RHF = 1, AFQMC_UHF = 2 };
+
+/* Precision of the Cholesky contractions (force bias, VHS, energy) and of the
+ * Taylor/one-body products.  Overlaps, determinants, QR, weights and energy
+ * accumulation are float64 in every mode. */
+enum { AFQMC_PREC_FP64 = 0, AFQMC_PREC_FP32 = 1, AFQMC_PREC_BF16X3 = 2 };
+
+typedef struct afqmc_dims {
+  int32_t walker_type;  /* AFQMC_RHF or AFQMC_UHF */
+  int32_t norb;         /* N  (<= 256) */
+  int32_t nocc_up;      /* n_up (<= 64) */
+  int32_t nocc_dn;      /* n_dn (<= 64; == nocc_up for AFQMC_RHF) */
+  int32_t nchol;        /* G */
+  int32_t n_walkers;    /* W: workspace is sized for this many walkers */
+  int32_t n_exp_terms;  /* Taylor terms incl. identity (propagation.py:208, default 6) */
+  int32_t precision;    /* AFQMC_PREC_* */
+  double dt;            /* time step */
+} afqmc_dims;
+
+/* Host float64 intermediates, laid out as the reference's ham_data / wave_data
+ * after build_measurement_intermediates (wavefunctions.py:602-619, :817-841) and
+ * build_propagation_intermediates (propagation.py:292-325, :464-509). */
+typedef struct afqmc_ham_host {
+  double h0;               /* ham_data["h0"]                                        */
+  double h0_prop;          /* Re ham_data["h0_prop"]                                */
+  const double* chol;      /* ham_data["chol"]      (G, N*N)                         */
+  const double* mo_coeff;  /* wave_data["mo_coeff"] rhf (N,n); uhf up (N,n_up)
+                              followed by dn (N,n_dn)                               */
+  const double* rot_h1;    /* ham_data["rot_h1"]    rhf (n,N); uhf up then dn       */
+  const double* rot_chol;  /* ham_data["rot_chol"]  rhf (G,n,N); uhf up (G,n_up,N)
+                              followed by dn (G,n_dn,N)                             */
+  const double* mf_shifts; /* ham_data["mf_shifts"] (G) complex128                   */
+  const double* exp_h1;    /* ham_data["exp_h1"]    rhf (N,N); uhf (2,N,N)           */
+} afqmc_ham_host;
+
+/* Create a context: copies the intermediates to the device (plus the transposed and
+ * reduced-precision copies the kernels use) and allocates the step workspace for
+ * dims->n_walkers walkers.  stream: a hipStream_t (NULL = default stream).
+ * Replaces the jit-time setup of propagator.propagate's closure over ham_data
+ * (propagation.py:100-165, static args propagation.py:327-328). */
+AFQMC_API int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
+                     const afqmc_ham_host* ham, void* stream);
+AFQMC_API int afqmc_ctx_destroy(afqmc_ctx* ctx);
+AFQMC_API int afqmc_set_stream(afqmc_ctx* ctx, void* stream);
+/* Bytes of device memory the context holds (intermediates + workspace). */
+AFQMC_API int64_t afqmc_ctx_device_bytes(const afqmc_ctx* ctx);
+AFQMC_API const char* afqmc_last_error(void);
+
+/* One phaseless step, propagator.propagate (propagation.py:100-165) with
+ * trial.calc_force_bias (wavefunctions.py:151-165 -> :503-511 / :694-709),
+ * _apply_trotprop (propagation.py:255-290 / :359-400) and trial.calc_overlap
+ * (wavefunctions.py:82-96 -> :478-480 / :658-667).
+ *   walkers_up/dn  dev  updated in place
+ *   weights        dev  (W) float64, updated in place
+ *   overlaps       dev  (W) complex128, in: old overlaps, out: new overlaps
+ *   fields         dev  (W, G) float64 auxiliary fields
+ *   scalars        dev  float64[2] = {e_estimate, pop_control_ene_shift};
+ *                       [1] is updated as in propagation.py:161-163
+ *   force_bias_out dev  (W, G) complex128 or NULL (debug / parity output)
+ * n_walkers may be smaller than the context's workspace size. */
+AFQMC_API int afqmc_propagate(afqmc_ctx* ctx, int32_t n_walkers, void* walkers_up,
+                    void* walkers_dn, double* weights, void* overlaps,
+                    const double* fields, double* scalars, void* force_bias_out);
+
+/* trial.calc_overlap (wavefunctions.py:82-96 / :61-80): overlaps (W) complex128. */
+AFQMC_API int afqmc_overlap(afqmc_ctx* ctx, int32_t n_walkers, const void* walkers_up,
+                  const void* walkers_dn, void* overlaps);
+
+/* trial.calc_force_bias (wavefunctions.py:151-165 / :127-149): fb (W, G) complex128. */
+AFQMC_API int afqmc_force_bias(afqmc_ctx* ctx, int32_t n_walkers, const void* walkers_up,
+                     const void* walkers_dn, void* force_bias);
+
+/* trial.calc_energy (wavefunctions.py:225-241 / :203-223) -> rhf._calc_energy_restricted
+ * (wavefunctions.py:529-541) / uhf._calc_energy (:711-743): energies (W) complex128. */
+AFQMC_API int afqmc_energy(afqmc_ctx* ctx, int32_t n_walkers, const void* walkers_up,
+                 const void* walkers_dn, void* energies);
+
+/* propagator.orthonormalize_walkers (propagation.py:251-253 / :422-424 ->
+ * linalg_utils.qr_vmap :51-55 / qr_vmap_uhf :58-65): walkers <- Q of the reduced
+ * Householder QR (LAPACK zgeqr2 + zung2r convention), in place. */
+AFQMC_API int afqmc_orthonormalize(afqmc_ctx* ctx, int32_t n_walkers, void* walkers_up,
+                         void* walkers_dn);
+
+/* stochastic_reconfiguration (sr.py:24-34 / :37-48): systematic resampling with
+ * the uniform draw zeta.  walkers gathered in place, weights <- sum|w| / W.
+ * idx_out dev (W) int32 or NULL: the resampled indices. */
+AFQMC_API int afqmc_sr(afqmc_ctx* ctx, int32_t n_walkers, void* walkers_up, void* walkers_dn,
+             double* weights, double zeta, int32_t* idx_out);
+
+/* Auxiliary fields for one step: count float64 standard normals written to fields
+ * (dev), from a counter-based generator (Philox4x32-10 + Box-Muller) keyed by
+ * (seed, counter).  Stands in for jax.random.normal in sampling.py:61-69. */
+AFQMC_API int afqmc_fields_normal(afqmc_ctx* ctx, uint64_t seed, uint64_t counter,
+                        double* fields, int64_t count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CS_AFQMC_AMD_H */

@@ -1,0 +1,303 @@
+"""Benchmark: walker-steps/s of the phaseless AFQMC step on one MI355X.
+
+Workload (default): synthetic closed-shell RHF, norb=100, nocc=20, nchol=400, 1024
+complex128 walkers, dt=0.005; a "step" is one phaseless propagation of all walkers
+(force bias, VHS, exp(VHS) Taylor + one-body half steps, overlaps, weights).  Steps
+run in blocks of n_prop_steps=50 as sampler.propagate_phaseless does: auxiliary fields
+are drawn on the device per block, and each block ends with QR re-orthonormalisation,
+overlaps, local energies and block statistics; stochastic reconfiguration follows
+every 4 blocks (one 200-step SR block).  Block work is inside the timed region and
+amortised into the per-step figure.
+
+Prints ONE JSON line.  `value` = walker-steps/s over the timed region (inputs already
+resident in HBM).  `roofline` describes the dominant kernel, timed with HIP events on
+the library's stream during the timed region.  `cpu_baseline` times the numpy
+restatement of the reference algorithm (the oracle) on a bounded sample on the host
+cores, one single-threaded process per core (the reference's own model: one MPI rank
+per core with one intra-op thread).
+"""
+
+import argparse
+import json
+import math
+import multiprocessing as mp
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+HBM_PEAK_GBS = 8000.0
+
+N_PROP_STEPS = 50
+N_ENE_BLOCKS = 4
+
+
+def flops_per_walker(norb, nelec, nchol, n_exp_terms, walker_type):
+    """Algorithmic real flops per walker per phase (real x complex MAC = 4 flop,
+    complex x complex MAC = 8 flop)."""
+    K = n_exp_terms - 1
+    N, G = norb, nchol
+    ns = [nelec[0]] if walker_type == "rhf" else [nelec[0], nelec[1]]
+    f = {
+        "vhs_gemm": 4.0 * G * N * N,
+        "fb_gemm": sum(4.0 * G * N * n for n in ns),
+        "green": sum(16.0 * N * n * n + 8.0 * n**3 for n in ns),
+        "taylor": sum((8.0 + 8.0 * K) * N * N * n for n in ns),
+        "overlap": sum(4.0 * N * n * n + (8.0 / 3.0) * n**3 for n in ns),
+        "energy": sum(4.0 * G * N * n * n + 8.0 * G * n * n + 12.0 * N * n * n + 8.0 * n**3
+                      for n in ns),
+        "qr": sum(16.0 * N * n * n for n in ns),
+    }
+    return f
+
+
+# ------------------------------------------------------------------ CPU baseline
+
+
+def _cpu_worker(args):
+    """One single-threaded process: oracle steps on a slice of the workload."""
+    (walker_type, norb, nocc, nchol, n_walk, budget_s, seed, dt) = args
+    import numpy as np
+
+    from cs_afqmc_amd import problems
+    from oracle import afqmc_oracle as o
+
+    hd, wd, nelec = problems.synthetic_rhf(norb, nocc, nchol, seed=seed)
+    oh = o.build_measurement_intermediates("rhf", hd, wd, norb)
+    oh = o.build_propagation_intermediates("rhf", oh, wd["rdm1"], norb, nelec, dt)
+    rng = np.random.default_rng(seed + 17)
+    walkers = o.get_init_walkers(wd["rdm1"], nelec, n_walk, restricted=True)
+    prop = {"walkers": walkers, "weights": np.ones(n_walk), "e_estimate": 0.0,
+            "pop_control_ene_shift": 0.0,
+            "overlaps": o.calc_overlap("rhf", walkers, wd)}
+    steps = 0
+    t_prop = 0.0
+    while True:
+        t0 = time.perf_counter()
+        fields = rng.standard_normal((n_walk, nchol))
+        prop = o.propagate("rhf", "rhf", oh, prop, fields, wd, dt, 6, norb,
+                           n_walkers=n_walk)
+        t_prop += time.perf_counter() - t0
+        steps += 1
+        if steps % N_PROP_STEPS == 0:  # keep the walkers orthonormal as the blocks do
+            prop["walkers"] = o.orthonormalize_walkers("rhf", prop["walkers"])
+            prop["overlaps"] = o.calc_overlap("rhf", prop["walkers"], wd)
+        if t_prop > budget_s:
+            break
+    t_step = t_prop / steps
+    t1 = time.perf_counter()
+    prop["walkers"] = o.orthonormalize_walkers("rhf", prop["walkers"])
+    o.calc_overlap("rhf", prop["walkers"], wd)
+    o.calc_energy("rhf", prop["walkers"], oh, wd)
+    t_block = time.perf_counter() - t1
+    return n_walk, steps, t_step, t_block
+
+
+def cpu_baseline(workload, budget_s):
+    walker_type, norb, nocc, nchol, W = workload
+    cores = len(os.sched_getaffinity(0))
+    procs = max(1, min(cores, 16))
+    n_walk = max(1, min(16, W // procs))
+    env_before = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS",
+                                                 "MKL_NUM_THREADS")}
+    for k in env_before:
+        os.environ[k] = "1"
+    try:
+        ctx = mp.get_context("spawn")
+        with ctx.Pool(procs) as pool:
+            res = pool.map(_cpu_worker, [(walker_type, norb, nocc, nchol, n_walk, budget_s,
+                                          11 + i, 0.005) for i in range(procs)])
+    finally:
+        for k, v in env_before.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    # per-process walker-steps/s with block work (QR, overlap, energy) amortised over
+    # N_PROP_STEPS steps; processes run concurrently, so rates add
+    rate = 0.0
+    for n_w, steps, t_step, t_block in res:
+        rate += n_w / (t_step + t_block / N_PROP_STEPS)
+    steps_done = sum(r[1] for r in res)
+    return {
+        "value": rate,
+        "unit": "walker-steps/s",
+        "cores": procs,
+        "kind": "port",
+        "sample": (f"{procs} single-threaded processes x {n_walk} walkers of the same workload, "
+                   f"{steps_done} oracle steps total (~{budget_s:.0f} s each) + one "
+                   f"QR/overlap/energy per process amortised over {N_PROP_STEPS} steps"),
+    }
+
+
+# ------------------------------------------------------------------ GPU run
+
+
+def run(args):
+    import numpy as np
+    import torch
+
+    from cs_afqmc_amd import hamiltonian, problems, propagation, rng, sampling, wavefunctions
+
+    workload = problems.WORKLOADS[args.workload]
+    walker_type, norb, nocc, nchol, W = workload
+    dt = 0.005
+    torch.cuda.set_device(0)
+    if walker_type == "rhf":
+        hd, wd, nelec = problems.synthetic_rhf(norb, nocc, nchol, seed=0)
+        trial = wavefunctions.rhf(norb, nelec)
+        prop = propagation.propagator_restricted(dt=dt, n_walkers=W, precision=args.precision)
+    else:
+        hd, wd, nelec = problems.synthetic_uhf(norb, nocc, nocc, nchol, seed=0)
+        trial = wavefunctions.uhf(norb, nelec)
+        prop = propagation.propagator_unrestricted(dt=dt, n_walkers=W,
+                                                   precision=args.precision)
+    ham = hamiltonian.hamiltonian(norb)
+    ham_data = ham.build_measurement_intermediates(hd, trial, wd)
+    ham_data = ham.build_propagation_intermediates(ham_data, prop, trial, wd)
+    prop_data = prop.init_prop_data(trial, wd, ham_data)
+    prop_data["key"] = rng.PRNGKey(0)
+    dev = ham_data["_afqmc_device"]
+    smp = sampling.sampler(n_prop_steps=N_PROP_STEPS, n_ene_blocks=N_ENE_BLOCKS, n_sr_blocks=1)
+
+    # device state as in sampler.propagate_phaseless
+    ddev = dev.device
+    e0 = prop_data["e_estimate"].reshape(())
+    scalars = torch.stack([e0, e0]).contiguous()
+    prop_data["_scalars"] = scalars
+    prop_data["_energies"] = torch.empty(W, dtype=torch.complex128, device=ddev)
+    fields_buf = torch.empty((N_PROP_STEPS, W, nchol), dtype=torch.float64, device=ddev)
+    stats = torch.zeros((max(1, math.ceil(args.steps / N_PROP_STEPS)) + 64, 3),
+                        dtype=torch.float64, device=ddev)
+
+    state = {"block": 0, "in_block": 0, "blocks_since_sr": 0, "fields": None}
+
+    def one_step():
+        """Advance one propagation step; block and SR work happen at the same points
+        as in sampler.propagate_phaseless."""
+        if state["in_block"] == 0:
+            prop_data["key"], subkey = rng.split(prop_data["key"])
+            state["fields"] = dev.fields_normal(subkey[0], subkey[1], fields_buf.shape,
+                                                out=fields_buf)
+        dev.propagate(prop_data["walkers"], prop_data["weights"], prop_data["overlaps"],
+                      state["fields"][state["in_block"]], scalars,
+                      reuse_green=state["in_block"] > 0)
+        state["in_block"] += 1
+        if state["in_block"] == N_PROP_STEPS:
+            state["in_block"] = 0
+            dev.orthonormalize(prop_data["walkers"])
+            dev.overlap(prop_data["walkers"], out=prop_data["overlaps"])
+            energies = dev.energy(prop_data["walkers"], out=prop_data["_energies"])
+            slot = state["block"] % stats.shape[0]
+            dev.block_stats(energies, prop_data["weights"], scalars, stats[slot])
+            state["block"] += 1
+            state["blocks_since_sr"] += 1
+            if state["blocks_since_sr"] == N_ENE_BLOCKS:
+                state["blocks_since_sr"] = 0
+                prop.stochastic_reconfiguration_local(prop_data)
+                dev.overlap(prop_data["walkers"], out=prop_data["overlaps"])
+
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+    print(f"# warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+
+    dev.prof_collect()  # drop anything recorded before
+    dev.prof_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one_step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    dev.prof_enable(False)
+    phases = dev.prof_collect()
+    print(f"# timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    # energy sanity (mixed estimator of the timed blocks)
+    nb = min(state["block"], stats.shape[0])
+    st = stats[:nb].cpu().numpy() if nb else np.zeros((0, 3))
+    e_mixed = float(np.sum(st[:, 0] * st[:, 1]) / np.sum(st[:, 1])) if nb else float("nan")
+
+    walker_steps = W * args.steps
+    value = walker_steps / elapsed
+    fl = flops_per_walker(norb, nelec, nchol, prop.n_exp_terms, walker_type)
+    # dominant kernel = phase with the largest device time in the timed region
+    per_phase = {p: {"ms_total": round(ms, 4), "launches": n,
+                     "ms_avg": round(ms / n, 5) if n else None}
+                 for p, (ms, n) in phases.items() if n}
+    dom = max((p for p in per_phase if p in fl), key=lambda p: phases[p][0])
+    avg_s = phases[dom][0] / phases[dom][1] / 1e3
+    flop_launch = fl[dom] * W
+    achieved = flop_launch / avg_s / 1e12
+    peak = PEAK_TFLOPS[args.precision]
+    roofline = {
+        "kernel": dom,
+        "bound": "mfma",
+        "achieved": round(achieved, 3),
+        "peak": peak,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / peak, 4),
+        "traffic": None,
+        "flop_per_launch": flop_launch,
+        "avg_launch_ms": round(avg_s * 1e3, 5),
+    }
+    step_flops = sum(v for k, v in fl.items() if k not in ("energy", "qr")) + (
+        fl["energy"] + fl["qr"]) / N_PROP_STEPS
+    out = {
+        "metric": "walker-steps/s (phaseless AFQMC step, one MI355X)",
+        "value": round(value, 1),
+        "unit": "walker-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else args.precision,
+        "data": "synthetic (seeded Cholesky vectors, h1, Haar trial; device Philox fields)",
+        "config": {
+            "workload": args.workload,
+            "walker_type": walker_type, "norb": norb, "nocc": nocc, "nchol": nchol,
+            "n_walkers": W, "dt": dt, "n_exp_terms": prop.n_exp_terms,
+            "n_prop_steps": N_PROP_STEPS, "n_ene_blocks": N_ENE_BLOCKS,
+            "precision": args.precision,
+        },
+        "achieved_tflops_step": round(step_flops * value / 1e12, 3),
+        "mixed_energy": e_mixed,
+        "phases": per_phase,
+        "roofline": roofline,
+    }
+    return out, workload
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="rhf_n100_o20_g400_w1024")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs the walker population on one GPU; --gpus {args.gpus} "
+              "is not supported (multi-GPU sharding is a design note only).", file=sys.stderr)
+        sys.exit(2)
+    out, workload = run(args)
+    if not args.no_cpu_baseline:
+        print("# cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(workload, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

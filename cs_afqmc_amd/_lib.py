@@ -11,6 +11,7 @@ from . import _build
 
 AFQMC_RHF = 1
 AFQMC_UHF = 2
+AFQMC_STEP_REUSE_GREEN = 1
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 
 # every symbol declared in include/cs_afqmc_amd.h
@@ -26,8 +27,15 @@ EXPORTS = (
     "afqmc_energy",
     "afqmc_orthonormalize",
     "afqmc_sr",
+    "afqmc_block_stats",
+    "afqmc_prof_enable",
+    "afqmc_prof_collect",
     "afqmc_fields_normal",
 )
+
+# phase indices of afqmc_prof_collect (AFQMC_PH_* in the header)
+PHASES = ("green", "fb_gemm", "fields", "vhs_gemm", "taylor", "overlap", "weights", "qr",
+          "energy", "sr", "rng")
 
 
 class AfqmcError(RuntimeError):
@@ -78,12 +86,15 @@ def _declare(lib):
     lib.afqmc_ctx_device_bytes.restype = ctypes.c_int64
     lib.afqmc_last_error.argtypes = []
     lib.afqmc_last_error.restype = ctypes.c_char_p
-    lib.afqmc_propagate.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    lib.afqmc_propagate.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32]
     lib.afqmc_overlap.argtypes = [_vp, _i32, _vp, _vp, _vp]
     lib.afqmc_force_bias.argtypes = [_vp, _i32, _vp, _vp, _vp]
     lib.afqmc_energy.argtypes = [_vp, _i32, _vp, _vp, _vp]
     lib.afqmc_orthonormalize.argtypes = [_vp, _i32, _vp, _vp]
     lib.afqmc_sr.argtypes = [_vp, _i32, _vp, _vp, _vp, ctypes.c_double, _vp]
+    lib.afqmc_block_stats.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp]
+    lib.afqmc_prof_enable.argtypes = [_vp, _i32]
+    lib.afqmc_prof_collect.argtypes = [_vp, _vp, _vp]
     lib.afqmc_fields_normal.argtypes = [
         _vp, ctypes.c_uint64, ctypes.c_uint64, _vp, ctypes.c_int64]
     for name in EXPORTS:

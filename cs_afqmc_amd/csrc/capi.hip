@@ -49,6 +49,11 @@ void free_ctx(afqmc_ctx* c) {
                   c->rot_chol_bf[1][0], c->rot_chol_bf[1][1], c->green[0], c->green[1], c->fb,
                   c->xs, c->vhs, c->terms, c->dets, c->energy_f, c->energy_acc, c->walk_tmp,
                   c->sr_cum, c->sr_idx};
+  if (c->prof_ev) {
+    for (int i = 0; i < 2 * c->prof_cap; ++i) (void)hipEventDestroy(c->prof_ev[i]);
+    delete[] c->prof_ev;
+    delete[] c->prof_phase;
+  }
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete c;
@@ -73,14 +78,23 @@ int set_device(const afqmc_ctx* c) {
   return AFQMC_OK;
 }
 
-// fb[w][c][g] = mult * sum_s green_s[w][c][:] . rot_chol_s[g][:]
+// fb[w][c][g] = mult * sum_s green_s[w][c][:] . rot_chol_s[g][:], as c->fb_parts split-K
+// slices of (2W, G) that the consumers add in slice order
 int force_bias_gemm(afqmc_ctx* c, int W) {
+  const int G = c->d.nchol;
+  const int64_t slice = (int64_t)2 * W * G;
+  int part = 0;
   for (int s = 0; s < c->nspin; ++s) {
     const int kd = c->kdim[s];
-    int rc = launch_gemm_nt_f64(c->stream, 2 * W, c->d.nchol, kd, (double)c->mult, c->green[s],
-                                kd, c->rot_chol[s], kd, s == 0 ? 0.0 : 1.0, c->fb, c->d.nchol);
+    const int sp = gemm_f64_splits(2 * W, G, kd);
+    if (part + sp > c->fb_cap)
+      return afqmc_set_error(AFQMC_ERR_STATE, "force-bias split slices exceed workspace");
+    int rc = launch_gemm_nt_f64_split(c->stream, 2 * W, G, kd, (double)c->mult, c->green[s], kd,
+                                      c->rot_chol[s], kd, c->fb + part * slice, G, sp, slice);
     if (rc) return rc;
+    part += sp;
   }
+  c->fb_parts = part;
   return AFQMC_OK;
 }
 
@@ -161,7 +175,12 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
 
     // workspace
     for (int s = 0; s < c->nspin; ++s) TRY(dalloc(c, &c->green[s], (size_t)W * 2 * c->kdim[s]));
-    TRY(dalloc(c, &c->fb, (size_t)W * 2 * G));
+    c->fb_cap = 0;
+    for (int s = 0; s < c->nspin; ++s) {
+      int cap = c->kdim[s] / 256;
+      c->fb_cap += cap < 1 ? 1 : (cap > 8 ? 8 : cap);
+    }
+    TRY(dalloc(c, &c->fb, (size_t)c->fb_cap * W * 2 * G));
     TRY(dalloc(c, &c->xs, (size_t)W * 2 * G));
     TRY(dalloc(c, &c->vhs, (size_t)W * 2 * N * N));
     TRY(dalloc(c, &c->terms, (size_t)W * 2));
@@ -208,7 +227,7 @@ extern "C" int64_t afqmc_ctx_device_bytes(const afqmc_ctx* ctx) {
 
 extern "C" int afqmc_propagate(afqmc_ctx* c, int32_t W, void* walkers_up, void* walkers_dn,
                                double* weights, void* overlaps, const double* fields,
-                               double* scalars, void* force_bias_out) {
+                               double* scalars, void* force_bias_out, int32_t flags) {
   int rc = check_w(c, W);
   if (rc) return rc;
   if ((rc = check_walkers(c, walkers_up, walkers_dn))) return rc;
@@ -218,18 +237,81 @@ extern "C" int afqmc_propagate(afqmc_ctx* c, int32_t W, void* walkers_up, void* 
   cplx* wu = static_cast<cplx*>(walkers_up);
   cplx* wd = static_cast<cplx*>(walkers_dn);
   const int N = c->d.norb;
-  // force bias: Green of the current walkers, then the rot_chol contraction
-  if ((rc = launch_measure(c, W, wu, wd, true))) return rc;
-  if ((rc = force_bias_gemm(c, W))) return rc;
-  if ((rc = launch_fields(c, W, fields, static_cast<cplx*>(force_bias_out)))) return rc;
-  // VHS[w] = sum_g (i sqrt(dt) x~_wg) L_g  : (2W, G) x (N*N, G)^T
-  if ((rc = launch_gemm_nt_f64(c->stream, 2 * W, N * N, c->d.nchol, 1.0, c->xs, c->d.nchol,
-                               c->chol_t, c->d.nchol, 0.0, c->vhs, N * N)))
-    return rc;
-  if ((rc = launch_taylor(c, W, wu, wd))) return rc;
-  // new overlaps and the weight update
-  if ((rc = launch_measure(c, W, wu, wd, false))) return rc;
+  // force bias: Green of the current walkers (or the one the previous step left),
+  // then the rot_chol contraction
+  const bool reuse = (flags & AFQMC_STEP_REUSE_GREEN) && c->green_src == walkers_up &&
+                     c->green_w == W;
+  c->green_src = nullptr;
+  if (!reuse) {
+    PhaseTimer t(c, AFQMC_PH_GREEN);
+    if ((rc = launch_measure(c, W, wu, wd, true))) return rc;
+  }
+  {
+    PhaseTimer t(c, AFQMC_PH_FB_GEMM);
+    if ((rc = force_bias_gemm(c, W))) return rc;
+  }
+  {
+    PhaseTimer t(c, AFQMC_PH_FIELDS);
+    if ((rc = launch_fields(c, W, fields, static_cast<cplx*>(force_bias_out)))) return rc;
+  }
+  {
+    // VHS[w] = sum_g (i sqrt(dt) x~_wg) L_g  : (2W, G) x (N*N, G)^T
+    PhaseTimer t(c, AFQMC_PH_VHS_GEMM);
+    if ((rc = launch_gemm_nt_f64(c->stream, 2 * W, N * N, c->d.nchol, 1.0, c->xs, c->d.nchol,
+                                 c->chol_t, c->d.nchol, 0.0, c->vhs, N * N)))
+      return rc;
+  }
+  {
+    PhaseTimer t(c, AFQMC_PH_TAYLOR);
+    if ((rc = launch_taylor(c, W, wu, wd))) return rc;
+  }
+  {
+    // new overlaps and the weight update
+    // new overlaps; the Green's function of the new walkers is kept for the next step
+    PhaseTimer t(c, AFQMC_PH_OVERLAP);
+    if ((rc = launch_measure(c, W, wu, wd, true))) return rc;
+    c->green_src = walkers_up;
+    c->green_w = W;
+  }
+  PhaseTimer t(c, AFQMC_PH_WEIGHTS);
   return launch_weights(c, W, weights, static_cast<cplx*>(overlaps), scalars);
+}
+
+extern "C" int afqmc_block_stats(afqmc_ctx* c, int32_t W, const void* energies,
+                                 const double* weights, double* scalars, double* out) {
+  int rc = check_w(c, W);
+  if (rc) return rc;
+  if (!energies || !weights || !scalars || !out)
+    return afqmc_set_error(AFQMC_ERR_ARG, "null energies/weights/scalars/out");
+  if ((rc = set_device(c))) return rc;
+  return launch_block_stats(c, W, static_cast<const cplx*>(energies), weights, scalars, out);
+}
+
+extern "C" int afqmc_prof_enable(afqmc_ctx* c, int32_t on) {
+  if (!c) return afqmc_set_error(AFQMC_ERR_ARG, "null context");
+  if (on && !c->prof_ev) {
+    const int cap = 1 << 14;
+    c->prof_ev = new hipEvent_t[2 * cap];
+    c->prof_phase = new int[cap];
+    for (int i = 0; i < 2 * cap; ++i) HIP_TRY(hipEventCreate(&c->prof_ev[i]));
+    c->prof_cap = cap;
+    c->prof_n = 0;
+  }
+  c->prof_on = on ? 1 : 0;
+  return AFQMC_OK;
+}
+
+extern "C" int afqmc_prof_collect(afqmc_ctx* c, double* ms, int64_t* counts) {
+  if (!c || !ms || !counts) return afqmc_set_error(AFQMC_ERR_ARG, "null argument");
+  for (int i = 0; i < c->prof_n; ++i) {
+    HIP_TRY(hipEventSynchronize(c->prof_ev[2 * i + 1]));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, c->prof_ev[2 * i], c->prof_ev[2 * i + 1]));
+    ms[c->prof_phase[i]] += (double)t;
+    counts[c->prof_phase[i]] += 1;
+  }
+  c->prof_n = 0;
+  return AFQMC_OK;
 }
 
 extern "C" int afqmc_overlap(afqmc_ctx* c, int32_t W, const void* walkers_up,
@@ -252,6 +334,7 @@ extern "C" int afqmc_force_bias(afqmc_ctx* c, int32_t W, const void* walkers_up,
   if ((rc = check_walkers(c, walkers_up, walkers_dn))) return rc;
   if (!force_bias) return afqmc_set_error(AFQMC_ERR_ARG, "null force_bias");
   if ((rc = set_device(c))) return rc;
+  c->green_src = nullptr;
   if ((rc = launch_measure(c, W, static_cast<const cplx*>(walkers_up),
                            static_cast<const cplx*>(walkers_dn), true)))
     return rc;
@@ -266,9 +349,11 @@ extern "C" int afqmc_energy(afqmc_ctx* c, int32_t W, const void* walkers_up,
   if ((rc = check_walkers(c, walkers_up, walkers_dn))) return rc;
   if (!energies) return afqmc_set_error(AFQMC_ERR_ARG, "null energies");
   if ((rc = set_device(c))) return rc;
+  c->green_src = nullptr;
   if ((rc = launch_measure(c, W, static_cast<const cplx*>(walkers_up),
                            static_cast<const cplx*>(walkers_dn), true)))
     return rc;
+  PhaseTimer t(c, AFQMC_PH_ENERGY);
   return launch_energy(c, W, static_cast<cplx*>(energies));
 }
 
@@ -278,6 +363,8 @@ extern "C" int afqmc_orthonormalize(afqmc_ctx* c, int32_t W, void* walkers_up,
   if (rc) return rc;
   if ((rc = check_walkers(c, walkers_up, walkers_dn))) return rc;
   if ((rc = set_device(c))) return rc;
+  c->green_src = nullptr;
+  PhaseTimer t(c, AFQMC_PH_QR);
   return launch_qr(c, W, static_cast<cplx*>(walkers_up), static_cast<cplx*>(walkers_dn));
 }
 
@@ -290,6 +377,8 @@ extern "C" int afqmc_sr(afqmc_ctx* c, int32_t W, void* walkers_up, void* walkers
   if (!(zeta >= 0.0 && zeta < 1.0))
     return afqmc_set_error(AFQMC_ERR_ARG, "zeta must lie in [0, 1)");
   if ((rc = set_device(c))) return rc;
+  c->green_src = nullptr;
+  PhaseTimer t(c, AFQMC_PH_SR);
   return launch_sr(c, W, static_cast<cplx*>(walkers_up), static_cast<cplx*>(walkers_dn), weights,
                    zeta, idx_out);
 }
@@ -301,5 +390,6 @@ extern "C" int afqmc_fields_normal(afqmc_ctx* c, uint64_t seed, uint64_t counter
   if (count == 0) return AFQMC_OK;
   int rc = set_device(c);
   if (rc) return rc;
+  PhaseTimer t(c, AFQMC_PH_RNG);
   return launch_fields_normal(c, seed, counter, fields, count);
 }

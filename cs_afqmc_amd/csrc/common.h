@@ -89,7 +89,9 @@ struct afqmc_ctx {
 
   // workspace (device)
   double* green[2];      // (W, 2, n_sigma*N): split re / im planes per walker
-  double* fb;            // (W, 2, G)
+  double* fb;            // fb_cap slices of (W, 2, G) split-K partial force biases
+  int fb_cap;            // slices allocated
+  int fb_parts;          // slices written by the last force-bias contraction
   double* xs;            // (W, 2, G): i sqrt(dt) * shifted fields, split
   double* vhs;           // (W, 2, N*N)
   cplx* terms;           // (W, 2): shift_term, fb_term
@@ -101,6 +103,32 @@ struct afqmc_ctx {
   int32_t* sr_idx;       // (W)
   int energy_wchunk;
   int64_t device_bytes;
+  // the Green's function in `green` belongs to these walkers (set by afqmc_propagate)
+  const void* green_src;
+  int green_w;
+
+  // phase timing (afqmc_prof_*)
+  int prof_on;
+  int prof_n;                 // recorded event pairs
+  int prof_cap;
+  hipEvent_t* prof_ev;        // 2 * prof_cap events
+  int* prof_phase;            // phase of each pair
+};
+
+// phase timer: records an event pair around a phase when profiling is on
+struct PhaseTimer {
+  afqmc_ctx* c;
+  int slot;
+  PhaseTimer(afqmc_ctx* ctx, int phase) : c(ctx), slot(-1) {
+    if (c->prof_on && c->prof_n < c->prof_cap) {
+      slot = c->prof_n++;
+      c->prof_phase[slot] = phase;
+      (void)hipEventRecord(c->prof_ev[2 * slot], c->stream);
+    }
+  }
+  ~PhaseTimer() {
+    if (slot >= 0) (void)hipEventRecord(c->prof_ev[2 * slot + 1], c->stream);
+  }
 };
 
 // status helpers (capi.hip)
@@ -122,10 +150,24 @@ int afqmc_set_error(int code, const char* fmt, ...);
                              hipGetErrorString(_e), __FILE__, __LINE__);           \
   } while (0)
 
+// Raise a kernel's dynamic-LDS limit to `bytes` when it exceeds the 64 KiB default and
+// the limit already set (`current`, one static per kernel).
+inline int ensure_dyn_lds(const void* fn, size_t bytes, int* current) {
+  if (bytes <= 64 * 1024 || (int)bytes <= *current) return AFQMC_OK;
+  HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  *current = (int)bytes;
+  return AFQMC_OK;
+}
+
 // ---------------------------------------------------------------- kernel launchers
 // gemm_f64.hip: C[m][n] = alpha * sum_k A[m][k] * B[n][k] (+ beta * C), row-major
 int launch_gemm_nt_f64(hipStream_t s, int M, int N, int K, double alpha, const double* A,
                        int lda, const double* B, int ldb, double beta, double* C, int ldc);
+// split-K form: slice z of C (C + z * c_split_stride) gets alpha * A[:, Kz] B[:, Kz]^T
+int launch_gemm_nt_f64_split(hipStream_t s, int M, int N, int K, double alpha, const double* A,
+                             int lda, const double* B, int ldb, double* C, int ldc, int splits,
+                             int64_t c_split_stride);
+int gemm_f64_splits(int M, int N, int K);
 // measure.hip
 int launch_measure(afqmc_ctx* c, int W, const cplx* wup, const cplx* wdn, bool want_green);
 // taylor.hip
@@ -135,6 +177,8 @@ int launch_fields(afqmc_ctx* c, int W, const double* fields, cplx* fb_out);
 int launch_weights(afqmc_ctx* c, int W, double* weights, cplx* overlaps, double* scalars);
 int launch_overlap_combine(afqmc_ctx* c, int W, cplx* overlaps);
 int launch_fb_out(afqmc_ctx* c, int W, cplx* fb_out);
+int launch_block_stats(afqmc_ctx* c, int W, const cplx* energies, const double* weights,
+                       double* scalars, double* out);
 // energy.hip
 int launch_energy(afqmc_ctx* c, int W, cplx* energies);
 // qr.hip

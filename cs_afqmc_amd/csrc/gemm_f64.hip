@@ -1,11 +1,17 @@
 // FP64 MFMA GEMM for gfx950:  C[m][n] = alpha * sum_k A[m][k] * B[n][k] + beta * C[m][n]
 // (row-major, both operands K-contiguous).  Used for the Cholesky contractions of the
-// step: force bias (split Green x rot_chol^T) and VHS (i sqrt(dt) x~ x chol).
+// step: force bias (split Green x rot_chol^T), VHS (i sqrt(dt) x~ x chol) and the
+// local-energy f_g matrices (rot_chol x Green^T).
 //
 // v_mfma_f64_16x16x4f64 lane layout (measured on gfx950):
 //   A operand: lane l holds A[l % 16][l / 16]
 //   B operand: lane l holds B[l / 16][l % 16]
 //   D (double4): lane l, element r holds D[l / 16 + 4 r][l % 16]
+//
+// Tiles of BM x BN with BK = 16, four waves (2 x 2), operands staged k-major in LDS
+// (double buffered, register prefetch of the next k-tile).  Optional split-K: grid.y
+// splits the K range and each split writes alpha * partial product to its own slice
+// C + split * c_split_stride (beta must be 0); consumers sum the slices in order.
 #include "common.h"
 
 namespace {
@@ -13,19 +19,21 @@ namespace {
 constexpr int GEMM_BK = 16;
 constexpr int GEMM_PAD = 16;  // row stride == 16 (mod 32) doubles: k and k+1 rows hit disjoint bank halves
 
-template <int BM, int BN, int WM, int WN>
+template <int BM, int BN, int WM, int WN, int VEC>
 __global__ __launch_bounds__(256) void gemm_nt_f64_kernel(
     int M, int N, int K, double alpha, const double* __restrict__ A, int lda,
     const double* __restrict__ B, int ldb, double beta, double* __restrict__ C, int ldc,
-    int tiles_m, int tiles_n) {
+    int tiles_m, int tiles_n, int k_per_split, int64_t c_split_stride) {
   constexpr int NT = 256;
   constexpr int BK = GEMM_BK;
   constexpr int TM = BM / WM / 16;
   constexpr int TN = BN / WN / 16;
-  constexpr int A_PER_T = BM * BK / NT;  // doubles of A loaded per thread per k-tile
-  constexpr int B_PER_T = BN * BK / NT;
+  constexpr int KT = BK / VEC;             // threads along k per row
+  constexpr int RSTEP = NT / KT;           // rows covered per load pass
+  constexpr int A_LD = BM / RSTEP;         // vector loads of A per thread per k-tile
+  constexpr int B_LD = BN / RSTEP;
   static_assert(WM * WN == 4, "four waves");
-  static_assert(A_PER_T >= 1 && B_PER_T >= 1, "tile too small");
+  static_assert(A_LD >= 1 && B_LD >= 1, "tile too small");
 
   __shared__ double As[2][BK][BM + GEMM_PAD];
   __shared__ double Bs[2][BK][BN + GEMM_PAD];
@@ -37,33 +45,52 @@ __global__ __launch_bounds__(256) void gemm_nt_f64_kernel(
   if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
   const int tile_m = b % tiles_m, tile_n = b / tiles_m;
   const int m0 = tile_m * BM, n0 = tile_n * BN;
+  const int kbeg = blockIdx.y * k_per_split;
+  const int kend = (kbeg + k_per_split < K) ? kbeg + k_per_split : K;
+  C += (int64_t)blockIdx.y * c_split_stride;
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave % WM, wn = wave / WM;
+  const int lk = (t % KT) * VEC, lr = t / KT;
 
-  // global load mapping: 16 consecutive threads read 16 consecutive k of one row
-  const int lk = t % BK, lr = t / BK;  // lr in [0, NT/BK)
-  constexpr int RSTEP = NT / BK;
-
-  double ra[A_PER_T], rb[B_PER_T];
+  double ra[A_LD][VEC], rb[B_LD][VEC];
   auto load_tile = [&](int k0) {
     const int k = k0 + lk;
+    const bool kok = k < kend;  // VEC == 2 needs even k ranges (checked at launch)
 #pragma unroll
-    for (int i = 0; i < A_PER_T; ++i) {
+    for (int i = 0; i < A_LD; ++i) {
       const int r = m0 + lr + i * RSTEP;
-      ra[i] = (r < M && k < K) ? A[(int64_t)r * lda + k] : 0.0;
+      if (VEC == 2) {
+        double2 v = make_double2(0.0, 0.0);
+        if (r < M && kok) v = *reinterpret_cast<const double2*>(A + (int64_t)r * lda + k);
+        ra[i][0] = v.x;
+        ra[i][VEC - 1] = v.y;
+      } else {
+        ra[i][0] = (r < M && kok) ? A[(int64_t)r * lda + k] : 0.0;
+      }
     }
 #pragma unroll
-    for (int i = 0; i < B_PER_T; ++i) {
+    for (int i = 0; i < B_LD; ++i) {
       const int r = n0 + lr + i * RSTEP;
-      rb[i] = (r < N && k < K) ? B[(int64_t)r * ldb + k] : 0.0;
+      if (VEC == 2) {
+        double2 v = make_double2(0.0, 0.0);
+        if (r < N && kok) v = *reinterpret_cast<const double2*>(B + (int64_t)r * ldb + k);
+        rb[i][0] = v.x;
+        rb[i][VEC - 1] = v.y;
+      } else {
+        rb[i][0] = (r < N && kok) ? B[(int64_t)r * ldb + k] : 0.0;
+      }
     }
   };
   auto store_tile = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < A_PER_T; ++i) As[buf][lk][lr + i * RSTEP] = ra[i];
+    for (int i = 0; i < A_LD; ++i)
 #pragma unroll
-    for (int i = 0; i < B_PER_T; ++i) Bs[buf][lk][lr + i * RSTEP] = rb[i];
+      for (int v = 0; v < VEC; ++v) As[buf][lk + v][lr + i * RSTEP] = ra[i][v];
+#pragma unroll
+    for (int i = 0; i < B_LD; ++i)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) Bs[buf][lk + v][lr + i * RSTEP] = rb[i][v];
   };
 
   double4_t acc[TM][TN];
@@ -72,16 +99,18 @@ __global__ __launch_bounds__(256) void gemm_nt_f64_kernel(
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
 
-  const int nk = (K + BK - 1) / BK;
-  load_tile(0);
-  store_tile(0);
+  const int nk = (kend - kbeg + BK - 1) / BK;
+  if (nk > 0) {
+    load_tile(kbeg);
+    store_tile(0);
+  }
   __syncthreads();
   const int am = wm * (BM / WM) + (lane & 15);
   const int bn = wn * (BN / WN) + (lane & 15);
   const int kq = lane >> 4;
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
-    if (kt + 1 < nk) load_tile((kt + 1) * BK);
+    if (kt + 1 < nk) load_tile(kbeg + (kt + 1) * BK);
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       double af[TM], bf[TN];
@@ -117,23 +146,64 @@ __global__ __launch_bounds__(256) void gemm_nt_f64_kernel(
     }
 }
 
+template <int BM, int BN>
+int launch_tiles(hipStream_t s, int M, int N, int K, double alpha, const double* A, int lda,
+                 const double* B, int ldb, double beta, double* C, int ldc, int splits,
+                 int k_per_split, int64_t c_split_stride, bool vec2) {
+  const int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
+  dim3 grid(tm * tn, splits);
+  if (vec2)
+    hipLaunchKernelGGL((gemm_nt_f64_kernel<BM, BN, 2, 2, 2>), grid, dim3(256), 0, s, M, N, K,
+                       alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, k_per_split,
+                       c_split_stride);
+  else
+    hipLaunchKernelGGL((gemm_nt_f64_kernel<BM, BN, 2, 2, 1>), grid, dim3(256), 0, s, M, N, K,
+                       alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, k_per_split,
+                       c_split_stride);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
 }  // namespace
+
+int gemm_f64_splits(int M, int N, int K) {
+  // split K only when the 64x64 tile grid cannot fill the 256 CUs twice over
+  const int64_t tiles = (int64_t)((M + 63) / 64) * ((N + 63) / 64);
+  if (tiles >= 512) return 1;
+  int sp = (int)((512 + tiles - 1) / tiles);
+  const int max_sp = K / 256;  // keep >= 256 k per split
+  if (sp > max_sp) sp = max_sp;
+  if (sp > 8) sp = 8;
+  return sp < 1 ? 1 : sp;
+}
+
+int launch_gemm_nt_f64_split(hipStream_t s, int M, int N, int K, double alpha, const double* A,
+                             int lda, const double* B, int ldb, double* C, int ldc, int splits,
+                             int64_t c_split_stride) {
+  if (M <= 0 || N <= 0) return AFQMC_OK;
+  if (K <= 0) return afqmc_set_error(AFQMC_ERR_ARG, "gemm: K=%d", K);
+  if (splits < 1) splits = 1;
+  int kps = (K + splits - 1) / splits;
+  kps = (kps + GEMM_BK - 1) / GEMM_BK * GEMM_BK;
+  const bool vec2 = ((lda | ldb | K) % 2 == 0) && ((uintptr_t)A % 16 == 0) &&
+                    ((uintptr_t)B % 16 == 0);
+  const int64_t big_tiles = (int64_t)((M + 127) / 128) * ((N + 127) / 128);
+  if (big_tiles * splits >= 512)
+    return launch_tiles<128, 128>(s, M, N, K, alpha, A, lda, B, ldb, 0.0, C, ldc, splits, kps,
+                                  c_split_stride, vec2);
+  return launch_tiles<64, 64>(s, M, N, K, alpha, A, lda, B, ldb, 0.0, C, ldc, splits, kps,
+                              c_split_stride, vec2);
+}
 
 int launch_gemm_nt_f64(hipStream_t s, int M, int N, int K, double alpha, const double* A,
                        int lda, const double* B, int ldb, double beta, double* C, int ldc) {
   if (M <= 0 || N <= 0) return AFQMC_OK;
   if (K <= 0) return afqmc_set_error(AFQMC_ERR_ARG, "gemm: K=%d", K);
-  // large problems: 128x128 tiles; otherwise 64x64 so that the grid covers the 256 CUs
+  const bool vec2 = ((lda | ldb | K) % 2 == 0) && ((uintptr_t)A % 16 == 0) &&
+                    ((uintptr_t)B % 16 == 0);
   const int64_t big_tiles = (int64_t)((M + 127) / 128) * ((N + 127) / 128);
-  if (big_tiles >= 512) {
-    const int tm = (M + 127) / 128, tn = (N + 127) / 128;
-    hipLaunchKernelGGL((gemm_nt_f64_kernel<128, 128, 2, 2>), dim3(tm * tn), dim3(256), 0, s,
-                       M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn);
-  } else {
-    const int tm = (M + 63) / 64, tn = (N + 63) / 64;
-    hipLaunchKernelGGL((gemm_nt_f64_kernel<64, 64, 2, 2>), dim3(tm * tn), dim3(256), 0, s, M,
-                       N, K, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn);
-  }
-  LAUNCH_CHECK();
-  return AFQMC_OK;
+  if (big_tiles >= 512)
+    return launch_tiles<128, 128>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, 1, K, 0,
+                                  vec2);
+  return launch_tiles<64, 64>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, 1, K, 0, vec2);
 }

@@ -136,15 +136,9 @@ int launch_qr(afqmc_ctx* c, int W, cplx* wup, cplx* wdn) {
     const size_t bytes = (size_t)N * n * sizeof(cplx);
     const int use_lds = bytes <= QR_LDS_LIMIT ? 1 : 0;
     const size_t shm = use_lds ? bytes : 0;
-    if (shm > 64 * 1024) {
-      static bool attr_set = false;
-      if (!attr_set) {
-        HIP_TRY(hipFuncSetAttribute((const void*)qr_kernel,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)QR_LDS_LIMIT));
-        attr_set = true;
-      }
-    }
+    static int lds_limit = 0;
+    const int rc = ensure_dyn_lds((const void*)qr_kernel, shm, &lds_limit);
+    if (rc) return rc;
     hipLaunchKernelGGL(qr_kernel, dim3(W), dim3(QR_NT), shm, c->stream, N, n,
                        (s == 0) ? wup : wdn, use_lds);
     LAUNCH_CHECK();

@@ -23,20 +23,30 @@ __device__ inline cplx cexp_(cplx a) {
 }
 
 // one workgroup per walker
+// force bias of walker w, entry g: sum of the split-K slices in slice order
+__device__ inline cplx fb_sum(const double* __restrict__ fb, int nparts, int64_t part_stride,
+                              int w, int G, int g) {
+  const double* p = fb + (int64_t)w * 2 * G + g;
+  double re = p[0], im = p[G];
+  for (int s = 1; s < nparts; ++s) {
+    re += p[s * part_stride];
+    im += p[s * part_stride + G];
+  }
+  return cplx{re, im};
+}
+
 __global__ __launch_bounds__(FIELDS_NT) void fields_kernel(
-    int G, double sqdt, const double* __restrict__ fb, const cplx* __restrict__ mf,
-    const double* __restrict__ fields, double* __restrict__ xs, cplx* __restrict__ terms,
-    cplx* __restrict__ fb_out) {
+    int G, double sqdt, const double* __restrict__ fb, int nparts, int64_t part_stride,
+    const cplx* __restrict__ mf, const double* __restrict__ fields, double* __restrict__ xs,
+    cplx* __restrict__ terms, cplx* __restrict__ fb_out) {
   __shared__ double red[FIELDS_NT / 64];
   const int w = blockIdx.x;
-  const double* fbr = fb + (int64_t)w * 2 * G;
-  const double* fbi = fbr + G;
   const double* x = fields + (int64_t)w * G;
   double* xr = xs + (int64_t)w * 2 * G;
   double* xi = xr + G;
   cplx st = cplx{0.0, 0.0}, ft = cplx{0.0, 0.0};
   for (int g = threadIdx.x; g < G; g += FIELDS_NT) {
-    const cplx f = cplx{fbr[g], fbi[g]};
+    const cplx f = fb_sum(fb, nparts, part_stride, w, G, g);
     const cplx m = mf[g];
     // field_shifts = -sqrt(dt) * (1j * fb - mf_shifts)
     const cplx ifb_m = cplx{-f.im - m.re, f.re - m.im};
@@ -94,18 +104,46 @@ __global__ __launch_bounds__(WEIGHTS_NT) void weights_kernel(
   if (threadIdx.x == 0) scalars[1] = e_estimate - 0.1 * (log(total / (double)W) / dt);
 }
 
+// sampling.py:79-91: clamp, weighted block energy, population-control mix
+__global__ __launch_bounds__(WEIGHTS_NT) void block_stats_kernel(
+    int W, double dt, const cplx* __restrict__ energies, const double* __restrict__ weights,
+    double* __restrict__ scalars, double* __restrict__ out) {
+  __shared__ double red[WEIGHTS_NT / 64];
+  const double e_est = scalars[0];
+  const double cap = sqrt(2.0 / dt);
+  double sw = 0.0, sew = 0.0, nz = 0.0;
+  for (int w = threadIdx.x; w < W; w += WEIGHTS_NT) {
+    double e = energies[w].re;
+    if (fabs(e - e_est) > cap) e = e_est;
+    const double wt = weights[w];
+    sw += wt;
+    sew += e * wt;
+    nz += (wt == 0.0) ? 1.0 : 0.0;
+  }
+  const double bw = block_sum<WEIGHTS_NT>(sw, red);
+  const double bew = block_sum<WEIGHTS_NT>(sew, red);
+  const double bz = block_sum<WEIGHTS_NT>(nz, red);
+  if (threadIdx.x == 0) {
+    const double be = bew / bw;
+    out[0] = be;
+    out[1] = bw;
+    out[2] = bz;
+    scalars[1] = 0.9 * scalars[1] + 0.1 * be;
+  }
+}
+
 __global__ void overlap_combine_kernel(int W, int nspin, const cplx* __restrict__ dets,
                                        cplx* __restrict__ overlaps) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w < W) overlaps[w] = overlap_of(dets, w, nspin);
 }
 
-__global__ void fb_out_kernel(int W, int G, const double* __restrict__ fb,
-                              cplx* __restrict__ out) {
+__global__ void fb_out_kernel(int W, int G, const double* __restrict__ fb, int nparts,
+                              int64_t part_stride, cplx* __restrict__ out) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e < (int64_t)W * G) {
-    const int64_t w = e / G, g = e - w * G;
-    out[e] = cplx{fb[w * 2 * G + g], fb[w * 2 * G + G + g]};
+    const int w = (int)(e / G), g = (int)(e - (int64_t)w * G);
+    out[e] = fb_sum(fb, nparts, part_stride, w, G, g);
   }
 }
 
@@ -113,7 +151,8 @@ __global__ void fb_out_kernel(int W, int G, const double* __restrict__ fb,
 
 int launch_fields(afqmc_ctx* c, int W, const double* fields, cplx* fb_out) {
   hipLaunchKernelGGL(fields_kernel, dim3(W), dim3(FIELDS_NT), 0, c->stream, c->d.nchol,
-                     sqrt(c->d.dt), c->fb, c->mf_shifts, fields, c->xs, c->terms, fb_out);
+                     sqrt(c->d.dt), c->fb, c->fb_parts, (int64_t)2 * W * c->d.nchol, c->mf_shifts,
+                     fields, c->xs, c->terms, fb_out);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }
@@ -122,6 +161,14 @@ int launch_weights(afqmc_ctx* c, int W, double* weights, cplx* overlaps, double*
   hipLaunchKernelGGL(weights_kernel, dim3(1), dim3(WEIGHTS_NT), 0, c->stream, W, c->nspin,
                      c->d.dt, sqrt(c->d.dt), c->h0_prop, c->dets, c->terms, weights, overlaps,
                      scalars);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+int launch_block_stats(afqmc_ctx* c, int W, const cplx* energies, const double* weights,
+                       double* scalars, double* out) {
+  hipLaunchKernelGGL(block_stats_kernel, dim3(1), dim3(WEIGHTS_NT), 0, c->stream, W, c->d.dt,
+                     energies, weights, scalars, out);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }
@@ -136,7 +183,8 @@ int launch_overlap_combine(afqmc_ctx* c, int W, cplx* overlaps) {
 int launch_fb_out(afqmc_ctx* c, int W, cplx* fb_out) {
   const int64_t total = (int64_t)W * c->d.nchol;
   hipLaunchKernelGGL(fb_out_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                     c->stream, W, c->d.nchol, c->fb, fb_out);
+                     c->stream, W, c->d.nchol, c->fb, c->fb_parts, (int64_t)2 * W * c->d.nchol,
+                     fb_out);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }

@@ -133,21 +133,206 @@ __global__ __launch_bounds__(1024) void taylor_kernel(
   }
 }
 
+// ------------------------------------------------------------------------------------
+// Register-resident form (N <= 128, n <= 32).  One workgroup per (walker, spin); wave r
+// keeps rows [16 r, 16 r + 16) of A = [Vr | Vi] (N x 2Np, Np = N rounded up to 4) in
+// registers for the whole series, so V is read from HBM once.  The complex product is
+// one real product  [ur | ui] = [Vr | Vi] [[tr, ti], [-ti, tr]]  whose B operand lives
+// in LDS as 2Np rows: rows [0, Np) = [tr | ti], rows [Np, 2Np) = [-ti | tr] (columns
+// padded to CTP*16), so k-group kg reads rows 4 kg .. 4 kg + 3 of one uniform array.
+// The series is evaluated in Horner form  y <- t0 + V y / k  (k = K..1), which equals
+// t0 + sum_k V^k t0 / k! of propagation.py:67-92.
+template <int CTP>
+struct RegTile {
+  static constexpr int RS = 16 * CTP + ((CTP % 2 == 0) ? 16 : 0);  // == 16 (mod 32) doubles
+};
+
+template <int NKGV, int CTP>
+__global__ __launch_bounds__(512) void taylor_reg_kernel(
+    int N, int n, int n_terms_k, cplx* __restrict__ walkers, const double* __restrict__ expE,
+    const double* __restrict__ vhs) {
+  constexpr int RS = RegTile<CTP>::RS;
+  extern __shared__ double TT[];  // [2 Np][RS]
+  const int w = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rb = wave * 16;
+  const int Np = (N + 3) & ~3;
+  const int nkg = Np >> 2;         // k-groups of one half (E products)
+  const int nkgv = 2 * nkg;        // k-groups of [Vr | Vi]
+  const int s = lane >> 4, col = lane & 15;
+  const int p = rb + col;          // A-operand row of this lane
+  const bool prow = p < N;
+  const int n2 = 2 * n;
+  cplx* phi = walkers + (int64_t)w * N * n;
+  const double* Vr = vhs + (int64_t)w * 2 * N * N;
+  const double* Vi = Vr + (int64_t)N * N;
+
+  // A = [Vr | Vi] rows in registers: k-group kg, slot s -> column q = 4 kg + s
+  double va[NKGV];
+#pragma unroll
+  for (int kg = 0; kg < NKGV; ++kg) {
+    const int q = 4 * kg + s;
+    double v = 0.0;
+    if (kg < nkgv && prow) {
+      if (q < Np) {
+        if (q < N) v = Vr[(int64_t)p * N + q];
+      } else if (q - Np < N) {
+        v = Vi[(int64_t)p * N + q - Np];
+      }
+    }
+    va[kg] = v;
+  }
+
+  // walker tile: [tr | ti] and [-ti | tr], zero padded
+  for (int e = threadIdx.x; e < Np * RS; e += blockDim.x) {
+    const int q = e / RS, c = e - (e / RS) * RS;
+    double up = 0.0, lo = 0.0;
+    if (q < N && c < n2) {
+      const int j = c < n ? c : c - n;
+      const cplx z = phi[(int64_t)q * n + j];
+      up = c < n ? z.re : z.im;
+      lo = c < n ? -z.im : z.re;
+    }
+    TT[e] = up;
+    TT[Np * RS + e] = lo;
+  }
+  __syncthreads();
+
+  const double* bbase = TT + s * RS + col;  // + kg * 4 RS + ct * 16
+
+  // y rows -> both halves (rows >= Np never written: they belong to the lower half)
+  auto put = [&](const double4_t* v) {
+#pragma unroll
+    for (int ct = 0; ct < CTP; ++ct)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rb + s + 4 * r;
+        const int cc = ct * 16 + col;
+        if (row < Np) {
+          TT[row * RS + cc] = v[ct][r];
+          if (cc < n)
+            TT[(Np + row) * RS + cc + n] = v[ct][r];
+          else if (cc < n2)
+            TT[(Np + row) * RS + cc - n] = -v[ct][r];
+        }
+      }
+  };
+
+  auto e_product = [&](double4_t* acc) {
+#pragma unroll
+    for (int ct = 0; ct < CTP; ++ct) acc[ct] = double4_t{0.0, 0.0, 0.0, 0.0};
+    double a = (prow && s < N) ? expE[(int64_t)p * N + s] : 0.0;
+    for (int kg = 0; kg < nkg; ++kg) {
+      const int qn = 4 * (kg + 1) + s;
+      const double an = (kg + 1 < nkg && prow && qn < N) ? expE[(int64_t)p * N + qn] : 0.0;
+#pragma unroll
+      for (int ct = 0; ct < CTP; ++ct)
+        acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bbase[kg * 4 * RS + ct * 16], acc[ct],
+                                                       0, 0, 0);
+      a = an;
+    }
+  };
+
+  double4_t t0[CTP], u[CTP];
+  e_product(t0);
+  __syncthreads();
+  put(t0);
+  __syncthreads();
+  for (int k = n_terms_k; k >= 1; --k) {
+#pragma unroll
+    for (int ct = 0; ct < CTP; ++ct) u[ct] = double4_t{0.0, 0.0, 0.0, 0.0};
+    double b[CTP];
+#pragma unroll
+    for (int ct = 0; ct < CTP; ++ct) b[ct] = bbase[ct * 16];
+#pragma unroll
+    for (int kg = 0; kg < NKGV; ++kg) {
+      if (kg < nkgv) {
+        // software pipeline: B of k-group kg + 1 is read from LDS under the MFMAs of kg
+        double bn[CTP];
+#pragma unroll
+        for (int ct = 0; ct < CTP; ++ct)
+          bn[ct] = (kg + 1 < nkgv) ? bbase[(kg + 1) * 4 * RS + ct * 16] : 0.0;
+#pragma unroll
+        for (int ct = 0; ct < CTP; ++ct)
+          u[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[kg], b[ct], u[ct], 0, 0, 0);
+#pragma unroll
+        for (int ct = 0; ct < CTP; ++ct) b[ct] = bn[ct];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+    const double rk = (double)k;
+#pragma unroll
+    for (int ct = 0; ct < CTP; ++ct)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) u[ct][r] = t0[ct][r] + u[ct][r] / rk;
+    put(u);
+    __syncthreads();
+  }
+  e_product(u);
+#pragma unroll
+  for (int ct = 0; ct < CTP; ++ct)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = rb + s + 4 * r;
+      const int cc = ct * 16 + col;
+      if (row < N && cc < n2) {
+        double* dst = reinterpret_cast<double*>(phi + (int64_t)row * n + (cc < n ? cc : cc - n));
+        dst[cc < n ? 0 : 1] = u[ct][r];
+      }
+    }
+}
+
+template <int NKGV, int CTP>
+int launch_reg(afqmc_ctx* c, int W, int n, cplx* wk, const double* E) {
+  const int N = c->d.norb;
+  const int nwaves = (N + 15) / 16;
+  const int Np = (N + 3) & ~3;
+  const size_t shm = (size_t)2 * Np * RegTile<CTP>::RS * sizeof(double);
+  static int lds_limit = 0;
+  const int rc = ensure_dyn_lds((const void*)taylor_reg_kernel<NKGV, CTP>, shm, &lds_limit);
+  if (rc) return rc;
+  hipLaunchKernelGGL((taylor_reg_kernel<NKGV, CTP>), dim3(W), dim3(nwaves * 64), shm, c->stream,
+                     N, n, c->d.n_exp_terms - 1, wk, E, c->vhs);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+template <int NKGV>
+int launch_reg_ct(afqmc_ctx* c, int W, int n, cplx* wk, const double* E) {
+  const int ctp = (2 * n + 15) / 16;
+  switch (ctp) {
+    case 1: return launch_reg<NKGV, 1>(c, W, n, wk, E);
+    case 2: return launch_reg<NKGV, 2>(c, W, n, wk, E);
+    case 3: return launch_reg<NKGV, 3>(c, W, n, wk, E);
+    default: return launch_reg<NKGV, 4>(c, W, n, wk, E);
+  }
+}
+
 }  // namespace
 
 int launch_taylor(afqmc_ctx* c, int W, cplx* wup, cplx* wdn) {
   const int N = c->d.norb;
+  if (N <= 128 && c->nocc[0] <= 32 && c->nocc[1] <= 32) {
+    for (int s = 0; s < c->nspin; ++s) {
+      const int n = c->nocc[s];
+      cplx* wk = (s == 0) ? wup : wdn;
+      // register budget: 2 ceil(N/4) k-groups of [Vr | Vi] per lane, bucketed
+      const int nkgv = 2 * ((N + 3) / 4);
+      const int rc = nkgv <= 32   ? launch_reg_ct<32>(c, W, n, wk, c->exp_h1[s])
+                     : nkgv <= 48 ? launch_reg_ct<48>(c, W, n, wk, c->exp_h1[s])
+                     : nkgv <= 56 ? launch_reg_ct<56>(c, W, n, wk, c->exp_h1[s])
+                                  : launch_reg_ct<64>(c, W, n, wk, c->exp_h1[s]);
+      if (rc) return rc;
+    }
+    return AFQMC_OK;
+  }
   const int nwaves = (N + 15) / 16;
   const int rows_pad = nwaves * 16;
   const size_t shm = (size_t)rows_pad * 16 * 4 * sizeof(double);
-  if (shm > 64 * 1024) {
-    static bool attr_set = false;
-    if (!attr_set) {
-      HIP_TRY(hipFuncSetAttribute((const void*)taylor_kernel,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-      attr_set = true;
-    }
-  }
+  static int lds_limit = 0;
+  const int lrc = ensure_dyn_lds((const void*)taylor_kernel, shm, &lds_limit);
+  if (lrc) return lrc;
   for (int s = 0; s < c->nspin; ++s) {
     const int n = c->nocc[s];
     cplx* wk = (s == 0) ? wup : wdn;

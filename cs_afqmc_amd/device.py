@@ -132,7 +132,10 @@ class DeviceStep:
             raise TypeError(f"{name} must be a contiguous cuda {dtype} tensor of shape {shape}")
 
     # ------------------------------------------------------------------ entry points
-    def propagate(self, walkers, weights, overlaps, fields, scalars, force_bias_out=None):
+    def propagate(self, walkers, weights, overlaps, fields, scalars, force_bias_out=None,
+                  reuse_green=False):
+        """reuse_green: the walkers are those the previous propagate on this context
+        returned (no QR / SR / edits since), so its Green's function is reused."""
         up, dn, W = self._walkers(walkers)
         self._vec(weights, W, torch.float64, "weights")
         self._vec(overlaps, W, torch.complex128, "overlaps")
@@ -145,7 +148,8 @@ class DeviceStep:
         self._prep()
         _lib.check(self._lib.afqmc_propagate(
             self._h, W, _dptr(up), _dptr(dn), _dptr(weights), _dptr(overlaps), _dptr(fields),
-            _dptr(scalars), _dptr(force_bias_out)))
+            _dptr(scalars), _dptr(force_bias_out),
+            _lib.AFQMC_STEP_REUSE_GREEN if reuse_green else 0))
 
     def overlap(self, walkers, out=None):
         up, dn, W = self._walkers(walkers)
@@ -195,3 +199,24 @@ class DeviceStep:
             self._h, int(seed) & (2**64 - 1), int(counter) & (2**64 - 1), _dptr(out),
             out.numel()))
         return out
+
+    def block_stats(self, energies, weights, scalars, out):
+        W = weights.shape[0]
+        self._vec(energies, W, torch.complex128, "energies")
+        self._vec(weights, W, torch.float64, "weights")
+        self._prep()
+        _lib.check(self._lib.afqmc_block_stats(self._h, W, _dptr(energies), _dptr(weights),
+                                               _dptr(scalars), _dptr(out)))
+        return out
+
+    # ------------------------------------------------------------------ phase timing
+    def prof_enable(self, on=True):
+        _lib.check(self._lib.afqmc_prof_enable(self._h, 1 if on else 0))
+
+    def prof_collect(self):
+        """{phase: (total ms, launches)} since the last collect (synchronises)."""
+        ms = np.zeros(len(_lib.PHASES), dtype=np.float64)
+        cnt = np.zeros(len(_lib.PHASES), dtype=np.int64)
+        _lib.check(self._lib.afqmc_prof_collect(
+            self._h, ms.ctypes.data_as(ctypes.c_void_p), cnt.ctypes.data_as(ctypes.c_void_p)))
+        return {p: (float(ms[i]), int(cnt[i])) for i, p in enumerate(_lib.PHASES)}

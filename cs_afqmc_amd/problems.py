@@ -1,8 +1,13 @@
 """Seeded synthetic AFQMC problems (host float64 arrays in the reference's layout).
 
 ``synthetic_rhf`` / ``synthetic_uhf`` build the workloads used for measurement:
-  L_g = sym(A_g) exp(-g / 80), A_g entries N(0, 1/norb), sym(A) = (A + A^T) / sqrt(2)
-  h1  = sym(B), B entries N(0, 0.1)
+  L_g = sym(A_g) exp(-g / 80), A_g entries N(0, sigma = 1/norb), sym(A) = (A + A^T) / sqrt(2)
+  h1  = sym(B), B entries N(0, sigma = 0.1)
+N(0, s) is read as standard deviation s.  Read as a variance instead (sigma =
+1/sqrt(norb)), the VHS of one dt = 0.005 step has spectral norm ~0.9 at norb = 100 and
+the phaseless weights of the whole population are clipped to zero within ~10 steps
+(the CPU oracle shows the same), so no energy can be measured; ``chol_std`` /
+``h1_std`` select other scales.
   trial = first nocc columns of a Haar-random orthogonal matrix (QR of a Gaussian
           matrix with the signs of diag(R) folded into Q); separate alpha / beta
           matrices for the unrestricted trial
@@ -31,27 +36,29 @@ def haar_orthogonal(n, rng):
     return q * np.sign(np.diag(r))[None, :]
 
 
-def _chol_h1(norb, nchol, rng, decay=80.0):
+def _chol_h1(norb, nchol, rng, decay=80.0, chol_std=None, h1_std=0.1):
+    chol_std = 1.0 / norb if chol_std is None else chol_std
     chol = np.empty((nchol, norb, norb))
     for g in range(nchol):
-        a = rng.normal(0.0, np.sqrt(1.0 / norb), (norb, norb))
+        a = rng.normal(0.0, chol_std, (norb, norb))
         chol[g] = _sym(a) * np.exp(-g / decay)
-    h1 = _sym(rng.normal(0.0, np.sqrt(0.1), (norb, norb)))
+    h1 = _sym(rng.normal(0.0, h1_std, (norb, norb)))
     return chol.reshape(nchol, norb * norb), h1
 
 
-def synthetic_rhf(norb, nocc, nchol, seed=0, decay=80.0):
+def synthetic_rhf(norb, nocc, nchol, seed=0, decay=80.0, chol_std=None, h1_std=0.1):
     rng = np.random.default_rng(seed)
-    chol, h1 = _chol_h1(norb, nchol, rng, decay)
+    chol, h1 = _chol_h1(norb, nchol, rng, decay, chol_std, h1_std)
     mo = haar_orthogonal(norb, rng)[:, :nocc]
     ham_data = {"h0": 0.0, "h1": np.array([h1, h1]), "chol": chol, "ene0": 0.0}
     wave_data = {"mo_coeff": mo, "rdm1": np.array([mo @ mo.T] * 2)}
     return ham_data, wave_data, (nocc, nocc)
 
 
-def synthetic_uhf(norb, nocc_up, nocc_dn, nchol, seed=0, decay=80.0):
+def synthetic_uhf(norb, nocc_up, nocc_dn, nchol, seed=0, decay=80.0, chol_std=None,
+                  h1_std=0.1):
     rng = np.random.default_rng(seed)
-    chol, h1 = _chol_h1(norb, nchol, rng, decay)
+    chol, h1 = _chol_h1(norb, nchol, rng, decay, chol_std, h1_std)
     mo_a = haar_orthogonal(norb, rng)[:, :nocc_up]
     mo_b = haar_orthogonal(norb, rng)[:, :nocc_dn]
     ham_data = {"h0": 0.0, "h1": np.array([h1, h1]), "chol": chol, "ene0": 0.0}

@@ -106,10 +106,16 @@ AFQMC_API const char* afqmc_last_error(void);
  *   scalars        dev  float64[2] = {e_estimate, pop_control_ene_shift};
  *                       [1] is updated as in propagation.py:161-163
  *   force_bias_out dev  (W, G) complex128 or NULL (debug / parity output)
+ *   flags          AFQMC_STEP_REUSE_GREEN: the walkers are exactly those the previous
+ *                  afqmc_propagate on this context returned, so the Green's function
+ *                  it left in the workspace is reused for the force bias (the library
+ *                  also requires the same walker pointer and count, else recomputes)
  * n_walkers may be smaller than the context's workspace size. */
+enum { AFQMC_STEP_REUSE_GREEN = 1 };
 AFQMC_API int afqmc_propagate(afqmc_ctx* ctx, int32_t n_walkers, void* walkers_up,
                     void* walkers_dn, double* weights, void* overlaps,
-                    const double* fields, double* scalars, void* force_bias_out);
+                    const double* fields, double* scalars, void* force_bias_out,
+                    int32_t flags);
 
 /* trial.calc_overlap (wavefunctions.py:82-96 / :61-80): overlaps (W) complex128. */
 AFQMC_API int afqmc_overlap(afqmc_ctx* ctx, int32_t n_walkers, const void* walkers_up,
@@ -135,6 +141,38 @@ AFQMC_API int afqmc_orthonormalize(afqmc_ctx* ctx, int32_t n_walkers, void* walk
  * idx_out dev (W) int32 or NULL: the resampled indices. */
 AFQMC_API int afqmc_sr(afqmc_ctx* ctx, int32_t n_walkers, void* walkers_up, void* walkers_dn,
              double* weights, double zeta, int32_t* idx_out);
+
+/* Block statistics of sampler._block_scan (sampling.py:74-91), on the device:
+ *   energies dev (W) complex128 local energies; the real part is clamped to
+ *            e_estimate when |E - e_estimate| > sqrt(2 / dt)
+ *   weights  dev (W) float64
+ *   scalars  dev float64[2] = {e_estimate, pop_control_ene_shift};
+ *            [1] <- 0.9 [1] + 0.1 block_energy
+ *   out      dev float64[3] = {block_energy, block_weight, number of zero weights} */
+AFQMC_API int afqmc_block_stats(afqmc_ctx* ctx, int32_t n_walkers, const void* energies,
+                                const double* weights, double* scalars, double* out);
+
+/* Per-phase device timing with HIP events on the context's stream (rocprof-free
+ * measurement for benchmarks).  While enabled, each phase of afqmc_propagate and the
+ * block entry points records an event pair; afqmc_prof_collect (after the stream
+ * has drained) adds the elapsed milliseconds and launch counts per phase into
+ * ms[AFQMC_NPHASE] / counts[AFQMC_NPHASE] and clears the record. */
+enum {
+  AFQMC_PH_GREEN = 0,    /* Green's function of the current walkers   */
+  AFQMC_PH_FB_GEMM = 1,  /* force-bias contraction                    */
+  AFQMC_PH_FIELDS = 2,   /* field shifts, shifted fields              */
+  AFQMC_PH_VHS_GEMM = 3, /* VHS contraction                           */
+  AFQMC_PH_TAYLOR = 4,   /* exp(VHS) Taylor + one-body half steps     */
+  AFQMC_PH_OVERLAP = 5,  /* new overlaps                              */
+  AFQMC_PH_WEIGHTS = 6,  /* importance function, weights, E shift     */
+  AFQMC_PH_QR = 7,
+  AFQMC_PH_ENERGY = 8,
+  AFQMC_PH_SR = 9,
+  AFQMC_PH_RNG = 10,
+  AFQMC_NPHASE = 11
+};
+AFQMC_API int afqmc_prof_enable(afqmc_ctx* ctx, int32_t on);
+AFQMC_API int afqmc_prof_collect(afqmc_ctx* ctx, double* ms, int64_t* counts);
 
 /* Auxiliary fields for one step: count float64 standard normals written to fields
  * (dev), from a counter-based generator (Philox4x32-10 + Box-Muller) keyed by

@@ -40,4 +40,4 @@ def test_error_path_without_gpu_work():
     dims = _lib.Dims(1, 300, 2, 2, 5, 4, 6, 0, 0.01)  # norb above the kernels' limit
     rc = L.afqmc_ctx_create(ctypes.byref(h), ctypes.byref(dims), ctypes.byref(ham), None)
     assert rc == 3 and b"norb" in L.afqmc_last_error()
-    assert L.afqmc_propagate(None, 4, None, None, None, None, None, None, None) == 1
+    assert L.afqmc_propagate(None, 4, None, None, None, None, None, None, None, 0) == 1

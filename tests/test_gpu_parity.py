@@ -23,6 +23,8 @@ CASES = [
     ("uhf", 36, (7, 5), 90, 48),
     ("rhf", 100, (20, 20), 400, 32),
     ("uhf", 70, (17, 13), 150, 24),
+    # beyond the register-resident Taylor kernel (N > 128, n > 32): streaming form
+    ("uhf", 136, (34, 30), 160, 8),
 ]
 IDS = [f"{c[0]}-N{c[1]}-n{c[2][0]}.{c[2][1]}-G{c[3]}-W{c[4]}" for c in CASES]
 
@@ -54,12 +56,15 @@ class Case:
         print(f"case {walker_type} N={norb} nelec={nelec} G={nchol} W={W}", flush=True)
         sys.stdout.flush()
         self.wt, self.norb, self.W, self.dt = walker_type, norb, W, dt
+        # Cholesky scale large enough that exp(VHS) is far from the identity
+        cs = 0.5 / np.sqrt(norb)
         if walker_type == "rhf":
-            hd, wd, nelec = problems.synthetic_rhf(norb, nelec[0], nchol, seed=seed)
+            hd, wd, nelec = problems.synthetic_rhf(norb, nelec[0], nchol, seed=seed, chol_std=cs)
             self.trial = wavefunctions.rhf(norb, nelec)
             self.prop = propagation.propagator_restricted(dt=dt, n_walkers=W)
         else:
-            hd, wd, nelec = problems.synthetic_uhf(norb, nelec[0], nelec[1], nchol, seed=seed)
+            hd, wd, nelec = problems.synthetic_uhf(norb, nelec[0], nelec[1], nchol, seed=seed,
+                                                   chol_std=cs)
             self.trial = wavefunctions.uhf(norb, nelec)
             self.prop = propagation.propagator_unrestricted(dt=dt, n_walkers=W)
         hd["h0"] = 0.37
@@ -199,3 +204,22 @@ def test_sr_indices_bit_exact(case):
     else:
         assert np.array_equal(got[0], ref_walkers[0]) and np.array_equal(got[1], ref_walkers[1])
     assert np.array_equal(to_np(w_t), ref_w)
+
+
+def test_propagate_reuse_green(case):
+    """Steps 2..5 reuse the Green's function the previous step left in the workspace."""
+    rng = np.random.default_rng(5)
+    oprop, gprop = _prop_pair(case, rng)
+    dev = case.ham["_afqmc_device"]
+    scalars = torch.tensor([oprop["e_estimate"], oprop["pop_control_ene_shift"]],
+                           dtype=torch.float64, device=case.dev)
+    for s in range(5):
+        fields = rng.standard_normal((case.W, case.nchol))
+        dev.propagate(gprop["walkers"], gprop["weights"], gprop["overlaps"],
+                      to_dev(fields, case.dev), scalars, reuse_green=s > 0)
+        oprop = o.propagate(case.wt, case.wt, case.oham, oprop, fields, case.owave, case.dt,
+                            6, case.norb)
+    assert rel(to_np(gprop["walkers"]), oprop["walkers"]) < TOL
+    assert rel(to_np(gprop["weights"]), oprop["weights"]) < TOL
+    assert rel(to_np(gprop["overlaps"]), oprop["overlaps"]) < TOL
+    assert abs(float(scalars[1]) - oprop["pop_control_ene_shift"]) < 1e-9

@@ -1,93 +1,97 @@
 // Trial measurements per walker and spin: O = C^T w (n x n), det(O) by Gauss-Jordan
 // elimination with partial pivoting (pivots as in LAPACK getrf: first max of
-// |re| + |im|), O^-1, and the half-rotated Green's function G = (w O^-1)^T (n x N).
+// |re| + |im|), X = O^-1, and the half-rotated Green's function G = (w X)^T (n x N).
 //
 // Follows rhf._calc_overlap_restricted / _calc_green (wavefunctions.py:478-501) and
 // uhf._calc_overlap / _calc_green (wavefunctions.py:658-692).  The trial C is real.
-// One workgroup per (walker, spin).  Rows of the walker and of the trial are staged
-// through LDS in chunks of R rows (coalesced global reads); O and its inverse live in
-// LDS as the augmented matrix [O | I].
 //
-// Green output layout: green[w][0][i*N + p] = Re G[i][p], green[w][1][...] = Im,
-// i.e. a row-major (2W, n*N) real matrix -- the A operand of the force-bias GEMM.
+// One wave (a 64-lane workgroup) per (walker, spin), so many walkers are in flight on
+// each CU and the elimination needs no cross-wave barriers.  Both products run on
+// v_mfma_f64_16x16x4f64 in real block form:
+//   [Or | Oi]  = C^T [wr | wi]                         (n x 2n,  K = N)
+//   [Gr ; Gi]  = [[Xr^T, -Xi^T], [Xi^T, Xr^T]] [wr^T ; wi^T]   (2n x N, K = 2n)
+// The second product's rows are exactly the rows of the split Green layout
+// green[w][c][i][p] (c = re / im), so results go straight to global memory.
+// [O | I] lives in LDS (n x 2n complex).
 #include "common.h"
 
 namespace {
 
-constexpr int MEAS_NT = 256;
-constexpr int MEAS_STAGE_BYTES = 48 * 1024;  // walker + trial row chunk
-constexpr int MEAS_MAX_OUT = (AFQMC_MAX_NOCC * AFQMC_MAX_NOCC + MEAS_NT - 1) / MEAS_NT;
+constexpr int MEAS_KC = 4;  // k-groups whose operands are loaded ahead of their MFMAs
 
-// rows per staged chunk: at most 48 KiB of walker + trial rows, and the whole LDS
-// footprint (augmented matrix included) within 150 KiB
-__host__ __device__ inline int meas_rows(int N, int n) {
-  int budget = 150 * 1024 - (2 * n * n + n) * 16;
-  if (budget > MEAS_STAGE_BYTES) budget = MEAS_STAGE_BYTES;
-  int r = budget / (n * 24);
-  if (r < 1) r = 1;
-  return r < N ? r : N;
-}
-
-__global__ __launch_bounds__(MEAS_NT) void measure_kernel(
+// T2 = ceil(2n / 16): column tiles of [Or | Oi] and row tiles of [Gr ; Gi]
+template <int T2>
+__global__ __launch_bounds__(64) void measure_kernel(
     int N, int n, const cplx* __restrict__ walkers, const double* __restrict__ trial,
     cplx* __restrict__ dets, int spin, double* __restrict__ green, int want_green) {
   extern __shared__ double smem[];
   const int n2 = 2 * n;
-  const int R = meas_rows(N, n);
   cplx* M = reinterpret_cast<cplx*>(smem);  // n x 2n augmented matrix
   cplx* fcol = M + n * n2;                  // pivot column copy (n)
-  cplx* ph = fcol + n;                      // staged walker rows (R x n)
-  double* tr = reinterpret_cast<double*>(ph + R * n);  // staged trial rows (R x n)
   __shared__ int piv_s;
 
-  const int w = blockIdx.x, tid = threadIdx.x;
+  const int w = blockIdx.x, lane = threadIdx.x;
+  const int s = lane >> 4, col = lane & 15;
   const cplx* phi = walkers + (int64_t)w * N * n;
+  const double* phid = reinterpret_cast<const double*>(phi);
 
-  // ---- O = C^T phi, accumulated over row chunks; thread owns entries e = tid + k*NT
-  double accr[MEAS_MAX_OUT], acci[MEAS_MAX_OUT];
+  // ---- [Or | Oi] = C^T [wr | wi]: rows i (tiles of 16), columns c < 2n, K = N
+  {
+    const int ti_n = (n + 15) >> 4;
+    const int nkg = (N + 3) >> 2;
+    // per-lane B column offsets (walker element, re / im) of each column tile
+    int boff[T2];
 #pragma unroll
-  for (int k = 0; k < MEAS_MAX_OUT; ++k) accr[k] = acci[k] = 0.0;
-  for (int p0 = 0; p0 < N; p0 += R) {
-    const int rows = (N - p0 < R) ? N - p0 : R;
-    for (int e = tid; e < rows * n; e += MEAS_NT) {
-      ph[e] = phi[(int64_t)p0 * n + e];
-      tr[e] = trial[(int64_t)p0 * n + e];
+    for (int tc = 0; tc < T2; ++tc) {
+      const int c = tc * 16 + col;
+      boff[tc] = c < n2 ? 2 * (c < n ? c : c - n) + (c < n ? 0 : 1) : -1;
     }
-    __syncthreads();
+    for (int ti = 0; ti < ti_n; ++ti) {
+      const int i = ti * 16 + col;  // A row
+      double4_t acc[T2];
 #pragma unroll
-    for (int k = 0; k < MEAS_MAX_OUT; ++k) {
-      const int e = tid + k * MEAS_NT;
-      if (e < n * n) {
-        const int i = e / n, j = e - (e / n) * n;
-        double sr = accr[k], si = acci[k];
-        for (int p = 0; p < rows; ++p) {
-          const double a = tr[p * n + i];
-          const cplx b = ph[p * n + j];
-          sr = fma(a, b.re, sr);
-          si = fma(a, b.im, si);
+      for (int tc = 0; tc < T2; ++tc) acc[tc] = double4_t{0.0, 0.0, 0.0, 0.0};
+      for (int kg0 = 0; kg0 < nkg; kg0 += MEAS_KC) {
+        double a[MEAS_KC], b[MEAS_KC][T2];
+#pragma unroll
+        for (int u = 0; u < MEAS_KC; ++u) {
+          const int p = 4 * (kg0 + u) + s;
+          const bool pok = p < N;
+          a[u] = (pok && i < n) ? trial[p * n + i] : 0.0;
+#pragma unroll
+          for (int tc = 0; tc < T2; ++tc)
+            b[u][tc] = (pok && boff[tc] >= 0) ? phid[(int64_t)2 * p * n + boff[tc]] : 0.0;
         }
-        accr[k] = sr;
-        acci[k] = si;
+#pragma unroll
+        for (int u = 0; u < MEAS_KC; ++u)
+#pragma unroll
+          for (int tc = 0; tc < T2; ++tc)
+            acc[tc] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u][tc], acc[tc], 0, 0, 0);
+      }
+#pragma unroll
+      for (int tc = 0; tc < T2; ++tc) {
+        if (boff[tc] < 0) continue;
+        const int c = tc * 16 + col;
+        const int cj = c < n ? c : c - n, ci = c < n ? 0 : 1;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = ti * 16 + s + 4 * r;
+          if (row < n) reinterpret_cast<double*>(&M[row * n2 + cj])[ci] = acc[tc][r];
+        }
       }
     }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int k = 0; k < MEAS_MAX_OUT; ++k) {
-    const int e = tid + k * MEAS_NT;
-    if (e < n * n) {
+    for (int e = lane; e < n * n; e += 64) {
       const int i = e / n, j = e - (e / n) * n;
-      M[i * n2 + j] = cplx{accr[k], acci[k]};
       M[i * n2 + n + j] = cplx{i == j ? 1.0 : 0.0, 0.0};
     }
   }
   __syncthreads();
 
-  // ---- Gauss-Jordan with partial pivoting: [O | I] -> [I | O^-1], det from pivots
+  // ---- Gauss-Jordan with partial pivoting: [O | I] -> [* | X], det from pivots.
   cplx det = cplx{1.0, 0.0};
   for (int k = 0; k < n; ++k) {
-    if (tid < 64) {
-      const int r = k + tid;
+    {
+      const int r = k + lane;
       double v = (r < n) ? cabs1(M[r * n2 + k]) : -1.0;
       int idx = r;
       for (int off = 32; off > 0; off >>= 1) {
@@ -98,30 +102,26 @@ __global__ __launch_bounds__(MEAS_NT) void measure_kernel(
           idx = i2;
         }
       }
-      if (tid == 0) piv_s = idx;
+      if (lane == 0) piv_s = idx;
     }
     __syncthreads();
     const int pr = piv_s;
-    if (pr != k) {
-      for (int c = k + tid; c < n2; c += MEAS_NT) {
-        const cplx a = M[k * n2 + c];
-        M[k * n2 + c] = M[pr * n2 + c];
-        M[pr * n2 + c] = a;
-      }
-    }
-    __syncthreads();
-    const cplx pivot = M[k * n2 + k];
-    if (tid == 0) {
+    const cplx pivot = M[pr * n2 + k];
+    const cplx inv = cdiv(cplx{1.0, 0.0}, pivot);
+    if (lane == 0) {
       det = cmul(det, pivot);
       if (pr != k) det = cplx{-det.re, -det.im};
     }
-    for (int r = tid; r < n; r += MEAS_NT) fcol[r] = M[r * n2 + k];
-    __syncthreads();
-    const cplx inv = cdiv(cplx{1.0, 0.0}, pivot);
-    for (int c = k + 1 + tid; c < n2; c += MEAS_NT) M[k * n2 + c] = cmul(M[k * n2 + c], inv);
+    // pivot column as seen after the row exchange; exchange + scale the pivot row
+    for (int r = lane; r < n; r += 64) fcol[r] = M[(r == pr ? k : r) * n2 + k];
+    for (int c = k + 1 + lane; c < n2; c += 64) {
+      const cplx a = M[k * n2 + c];
+      M[k * n2 + c] = cmul(M[pr * n2 + c], inv);
+      if (pr != k) M[pr * n2 + c] = a;
+    }
     __syncthreads();
     const int width = n2 - k - 1;
-    for (int e = tid; e < n * width; e += MEAS_NT) {
+    for (int e = lane; e < n * width; e += 64) {
       const int r = e / width, c = k + 1 + (e - (e / width) * width);
       if (r != k) {
         const cplx f = fcol[r], m = M[k * n2 + c];
@@ -133,47 +133,86 @@ __global__ __launch_bounds__(MEAS_NT) void measure_kernel(
     }
     __syncthreads();
   }
-  if (tid == 0) dets[(int64_t)w * 2 + spin] = det;
+  if (lane == 0) dets[(int64_t)w * 2 + spin] = det;
   if (!want_green) return;
 
-  // ---- G[i][p] = sum_j phi[p][j] * Oinv[j][i], walker rows staged again
-  double* gre = green + (int64_t)w * 2 * n * N;
-  double* gim = gre + (int64_t)n * N;
-  for (int p0 = 0; p0 < N; p0 += R) {
-    const int rows = (N - p0 < R) ? N - p0 : R;
-    for (int e = tid; e < rows * n; e += MEAS_NT) ph[e] = phi[(int64_t)p0 * n + e];
-    __syncthreads();
-    for (int e = tid; e < n * rows; e += MEAS_NT) {
-      const int i = e / rows, p = e - (e / rows) * rows;
-      double sr = 0.0, si = 0.0;
-      for (int j = 0; j < n; ++j) {
-        const cplx a = ph[p * n + j];
-        const cplx b = M[j * n2 + n + i];
-        sr += a.re * b.re - a.im * b.im;
-        si += a.re * b.im + a.im * b.re;
+  // ---- [Gr ; Gi] = [[Xr^T, -Xi^T], [Xi^T, Xr^T]] [wr^T ; wi^T], K = 2n, columns p
+  {
+    const int tp_n = (N + 15) >> 4;
+    const int nkg = (n2 + 3) >> 2;
+    double* gw = green + (int64_t)w * n2 * N;
+    for (int tp = 0; tp < tp_n; ++tp) {
+      const int p = tp * 16 + col;  // B column = orbital p
+      double4_t acc[T2];
+#pragma unroll
+      for (int tr = 0; tr < T2; ++tr) acc[tr] = double4_t{0.0, 0.0, 0.0, 0.0};
+      for (int kg0 = 0; kg0 < nkg; kg0 += MEAS_KC) {
+        double a[MEAS_KC][T2], b[MEAS_KC];
+#pragma unroll
+        for (int u = 0; u < MEAS_KC; ++u) {
+          const int k = 4 * (kg0 + u) + s;
+          const bool kok = k < n2;
+          const int kj = k < n ? k : k - n;
+          b[u] = (kok && p < N) ? phid[2 * ((int64_t)p * n + kj) + (k < n ? 0 : 1)] : 0.0;
+#pragma unroll
+          for (int tr = 0; tr < T2; ++tr) {
+            const int r = tr * 16 + col;  // A row
+            double v = 0.0;
+            if (kok && r < n2) {
+              const cplx x = M[kj * n2 + n + (r < n ? r : r - n)];  // X[kj][ri]
+              // (r < n, k < n): Xr; (r < n, k >= n): -Xi; (r >= n, k < n): Xi; else Xr
+              v = (r < n) == (k < n) ? x.re : ((r < n) ? -x.im : x.im);
+            }
+            a[u][tr] = v;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < MEAS_KC; ++u)
+#pragma unroll
+          for (int tr = 0; tr < T2; ++tr)
+            acc[tr] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][tr], b[u], acc[tr], 0, 0, 0);
       }
-      gre[(int64_t)i * N + p0 + p] = sr;
-      gim[(int64_t)i * N + p0 + p] = si;
+#pragma unroll
+      for (int tr = 0; tr < T2; ++tr)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int row = tr * 16 + s + 4 * q;
+          if (row < n2 && p < N) gw[(int64_t)row * N + p] = acc[tr][q];
+        }
     }
-    __syncthreads();
   }
+}
+
+template <int T2>
+int launch_measure_t(afqmc_ctx* c, int W, int n, const cplx* wk, int s, bool want_green) {
+  const size_t shm = (size_t)(n * 2 * n + n) * sizeof(cplx);
+  static int lds_limit = 0;
+  const int rc = ensure_dyn_lds((const void*)measure_kernel<T2>, shm, &lds_limit);
+  if (rc) return rc;
+  hipLaunchKernelGGL((measure_kernel<T2>), dim3(W), dim3(64), shm, c->stream, c->d.norb, n, wk,
+                     c->trial[s], c->dets, s, c->green[s], want_green ? 1 : 0);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
 }
 
 }  // namespace
 
 int launch_measure(afqmc_ctx* c, int W, const cplx* wup, const cplx* wdn, bool want_green) {
-  const int N = c->d.norb;
-  for (int s = 0; s < c->nspin; ++s) {
-    const int n = c->nocc[s];
-    const int R = meas_rows(N, n);
-    const size_t shm = (size_t)(n * 2 * n + n + R * n) * sizeof(cplx) + (size_t)R * n * 8;
-    const cplx* wk = (s == 0) ? wup : wdn;
-    static int lds_limit = 0;
-    const int rc = ensure_dyn_lds((const void*)measure_kernel, shm, &lds_limit);
+  for (int sp = 0; sp < c->nspin; ++sp) {
+    const int n = c->nocc[sp];
+    const cplx* wk = (sp == 0) ? wup : wdn;
+    int rc;
+    switch ((2 * n + 15) / 16) {
+      case 1: rc = launch_measure_t<1>(c, W, n, wk, sp, want_green); break;
+      case 2: rc = launch_measure_t<2>(c, W, n, wk, sp, want_green); break;
+      case 3: rc = launch_measure_t<3>(c, W, n, wk, sp, want_green); break;
+      case 4: rc = launch_measure_t<4>(c, W, n, wk, sp, want_green); break;
+      case 5: rc = launch_measure_t<5>(c, W, n, wk, sp, want_green); break;
+      case 6: rc = launch_measure_t<6>(c, W, n, wk, sp, want_green); break;
+      case 7: rc = launch_measure_t<7>(c, W, n, wk, sp, want_green); break;
+      default: rc = launch_measure_t<8>(c, W, n, wk, sp, want_green); break;
+    }
     if (rc) return rc;
-    hipLaunchKernelGGL(measure_kernel, dim3(W), dim3(MEAS_NT), shm, c->stream, N, n, wk,
-                       c->trial[s], c->dets, s, c->green[s], want_green ? 1 : 0);
-    LAUNCH_CHECK();
   }
   return AFQMC_OK;
 }

@@ -40,6 +40,16 @@ int upload(afqmc_ctx* c, T** p, const T* host, size_t count) {
   return AFQMC_OK;
 }
 
+int zero_async(void* p, size_t bytes, hipStream_t s) {
+  HIP_TRY(hipMemsetAsync(p, 0, bytes, s));
+  return AFQMC_OK;
+}
+
+int sync_stream(hipStream_t s) {
+  HIP_TRY(hipStreamSynchronize(s));
+  return AFQMC_OK;
+}
+
 void free_ctx(afqmc_ctx* c) {
   if (!c) return;
   void* ptrs[] = {c->trial[0], c->trial[1], c->rot_h1[0], c->rot_h1[1], c->rot_chol[0],
@@ -48,7 +58,9 @@ void free_ctx(afqmc_ctx* c) {
                   c->chol_t_bf[1], c->rot_chol_bf[0][0], c->rot_chol_bf[0][1],
                   c->rot_chol_bf[1][0], c->rot_chol_bf[1][1], c->green[0], c->green[1], c->fb,
                   c->xs, c->vhs, c->terms, c->dets, c->energy_f, c->energy_acc, c->walk_tmp,
-                  c->sr_cum, c->sr_idx};
+                  c->sr_cum, c->sr_idx, c->xs_bf[0], c->xs_bf[1], c->green_bf[0][0],
+                  c->green_bf[0][1], c->green_bf[1][0], c->green_bf[1][1], c->vhs_f32,
+                  c->exp_h1_f32[0], c->exp_h1_f32[1]};
   if (c->prof_ev) {
     for (int i = 0; i < 2 * c->prof_cap; ++i) (void)hipEventDestroy(c->prof_ev[i]);
     delete[] c->prof_ev;
@@ -85,12 +97,25 @@ int force_bias_gemm(afqmc_ctx* c, int W) {
   const int64_t slice = (int64_t)2 * W * G;
   int part = 0;
   for (int s = 0; s < c->nspin; ++s) {
-    const int kd = c->kdim[s];
-    const int sp = gemm_f64_splits(2 * W, G, kd);
-    if (part + sp > c->fb_cap)
-      return afqmc_set_error(AFQMC_ERR_STATE, "force-bias split slices exceed workspace");
-    int rc = launch_gemm_nt_f64_split(c->stream, 2 * W, G, kd, (double)c->mult, c->green[s], kd,
-                                      c->rot_chol[s], kd, c->fb + part * slice, G, sp, slice);
+    int rc;
+    int sp;
+    if (c->mixed) {
+      // split-bf16: rows (w, re/im) of the padded Green planes, K = n_s * Np32
+      const int kd = c->nocc[s] * c->Np32;
+      sp = gemm_f64_splits(2 * W, G, kd);
+      if (part + sp > c->fb_cap)
+        return afqmc_set_error(AFQMC_ERR_STATE, "force-bias split slices exceed workspace");
+      rc = launch_gemm_nt_bf16x3_f64(c->stream, 2 * W, G, kd, (float)c->mult, c->green_bf[s][0],
+                                     c->green_bf[s][1], kd, c->rot_chol_bf[s][0],
+                                     c->rot_chol_bf[s][1], kd, c->fb + part * slice, G, sp, slice);
+    } else {
+      const int kd = c->kdim[s];
+      sp = gemm_f64_splits(2 * W, G, kd);
+      if (part + sp > c->fb_cap)
+        return afqmc_set_error(AFQMC_ERR_STATE, "force-bias split slices exceed workspace");
+      rc = launch_gemm_nt_f64_split(c->stream, 2 * W, G, kd, (double)c->mult, c->green[s], kd,
+                                    c->rot_chol[s], kd, c->fb + part * slice, G, sp, slice);
+    }
     if (rc) return rc;
     part += sp;
   }
@@ -123,8 +148,8 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
   if (d.n_exp_terms < 1 || d.n_exp_terms > 18)
     return afqmc_set_error(AFQMC_ERR_ARG, "n_exp_terms=%d outside [1, 18]", d.n_exp_terms);
   if (!(d.dt > 0.0)) return afqmc_set_error(AFQMC_ERR_ARG, "dt must be > 0");
-  if (d.precision != AFQMC_PREC_FP64)
-    return afqmc_set_error(AFQMC_ERR_ARG, "precision mode %d not built yet", d.precision);
+  if (d.precision != AFQMC_PREC_FP64 && d.precision != AFQMC_PREC_BF16X3)
+    return afqmc_set_error(AFQMC_ERR_ARG, "precision mode %d not supported", d.precision);
   if (!ham->chol || !ham->mo_coeff || !ham->rot_h1 || !ham->rot_chol || !ham->mf_shifts ||
       !ham->exp_h1)
     return afqmc_set_error(AFQMC_ERR_ARG, "null intermediate array");
@@ -173,16 +198,53 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
     TRY(upload(c, &c->chol_t, ct.data(), ct.size()));
     TRY(upload(c, &c->mf_shifts, reinterpret_cast<const cplx*>(ham->mf_shifts), (size_t)G));
 
+    // mixed mode: split-bf16 planes of the contraction operands (zero padded to 32)
+    c->mixed = d.precision == AFQMC_PREC_BF16X3;
+    c->Gp = (G + 31) / 32 * 32;
+    c->Np32 = (N + 31) / 32 * 32;
+    c->taylor_f32 = c->mixed && N <= 128 && c->nocc[0] <= 32 && c->nocc[1] <= 32;
+    if (c->mixed) {
+      const size_t nn = (size_t)N * N;
+      for (int h = 0; h < 2; ++h) TRY(dalloc(c, &c->chol_t_bf[h], nn * c->Gp));
+      TRY(launch_split_bf16(c->stream, (int)nn, G, c->chol_t, G, c->chol_t_bf[0], c->chol_t_bf[1],
+                            c->Gp));
+      for (int s = 0; s < c->nspin; ++s) {
+        const size_t rows = (size_t)G * c->nocc[s];
+        for (int h = 0; h < 2; ++h) {
+          TRY(dalloc(c, &c->rot_chol_bf[s][h], rows * c->Np32));
+          TRY(dalloc(c, &c->green_bf[s][h], (size_t)W * 2 * c->nocc[s] * c->Np32));
+          TRY(zero_async(c->green_bf[s][h],
+                         (size_t)W * 2 * c->nocc[s] * c->Np32 * sizeof(uint16_t), c->stream));
+        }
+        TRY(launch_split_bf16(c->stream, (int)rows, N, c->rot_chol[s], N, c->rot_chol_bf[s][0],
+                              c->rot_chol_bf[s][1], c->Np32));
+      }
+      for (int h = 0; h < 2; ++h) {
+        TRY(dalloc(c, &c->xs_bf[h], (size_t)W * 2 * c->Gp));
+        TRY(zero_async(c->xs_bf[h], (size_t)W * 2 * c->Gp * sizeof(uint16_t), c->stream));
+      }
+      if (c->taylor_f32) {
+        TRY(dalloc(c, &c->vhs_f32, (size_t)W * 2 * nn));
+        std::vector<float> ef(nn);
+        for (int s = 0; s < c->nspin; ++s) {
+          for (size_t e = 0; e < nn; ++e) ef[e] = (float)ham->exp_h1[(size_t)s * nn + e];
+          TRY(upload(c, &c->exp_h1_f32[s], ef.data(), nn));
+        }
+      }
+      TRY(sync_stream(c->stream));
+    }
+
     // workspace
     for (int s = 0; s < c->nspin; ++s) TRY(dalloc(c, &c->green[s], (size_t)W * 2 * c->kdim[s]));
     c->fb_cap = 0;
     for (int s = 0; s < c->nspin; ++s) {
-      int cap = c->kdim[s] / 256;
+      const int kd = c->mixed ? c->nocc[s] * c->Np32 : c->kdim[s];
+      int cap = kd / 256;
       c->fb_cap += cap < 1 ? 1 : (cap > 8 ? 8 : cap);
     }
     TRY(dalloc(c, &c->fb, (size_t)c->fb_cap * W * 2 * G));
     TRY(dalloc(c, &c->xs, (size_t)W * 2 * G));
-    TRY(dalloc(c, &c->vhs, (size_t)W * 2 * N * N));
+    if (!c->taylor_f32) TRY(dalloc(c, &c->vhs, (size_t)W * 2 * N * N));
     TRY(dalloc(c, &c->terms, (size_t)W * 2));
     TRY(dalloc(c, &c->dets, (size_t)W * 2));
     {
@@ -244,7 +306,7 @@ extern "C" int afqmc_propagate(afqmc_ctx* c, int32_t W, void* walkers_up, void* 
   c->green_src = nullptr;
   if (!reuse) {
     PhaseTimer t(c, AFQMC_PH_GREEN);
-    if ((rc = launch_measure(c, W, wu, wd, true))) return rc;
+    if ((rc = launch_measure(c, W, wu, wd, !c->mixed, c->mixed))) return rc;
   }
   {
     PhaseTimer t(c, AFQMC_PH_FB_GEMM);
@@ -257,9 +319,18 @@ extern "C" int afqmc_propagate(afqmc_ctx* c, int32_t W, void* walkers_up, void* 
   {
     // VHS[w] = sum_g (i sqrt(dt) x~_wg) L_g  : (2W, G) x (N*N, G)^T
     PhaseTimer t(c, AFQMC_PH_VHS_GEMM);
-    if ((rc = launch_gemm_nt_f64(c->stream, 2 * W, N * N, c->d.nchol, 1.0, c->xs, c->d.nchol,
-                                 c->chol_t, c->d.nchol, 0.0, c->vhs, N * N)))
-      return rc;
+    if (c->mixed && c->taylor_f32)
+      rc = launch_gemm_nt_bf16x3_f32(c->stream, 2 * W, N * N, c->Gp, 1.0f, c->xs_bf[0],
+                                     c->xs_bf[1], c->Gp, c->chol_t_bf[0], c->chol_t_bf[1], c->Gp,
+                                     c->vhs_f32, N * N, 1, 0);
+    else if (c->mixed)
+      rc = launch_gemm_nt_bf16x3_f64(c->stream, 2 * W, N * N, c->Gp, 1.0f, c->xs_bf[0],
+                                     c->xs_bf[1], c->Gp, c->chol_t_bf[0], c->chol_t_bf[1], c->Gp,
+                                     c->vhs, N * N, 1, 0);
+    else
+      rc = launch_gemm_nt_f64(c->stream, 2 * W, N * N, c->d.nchol, 1.0, c->xs, c->d.nchol,
+                              c->chol_t, c->d.nchol, 0.0, c->vhs, N * N);
+    if (rc) return rc;
   }
   {
     PhaseTimer t(c, AFQMC_PH_TAYLOR);
@@ -269,7 +340,7 @@ extern "C" int afqmc_propagate(afqmc_ctx* c, int32_t W, void* walkers_up, void* 
     // new overlaps and the weight update
     // new overlaps; the Green's function of the new walkers is kept for the next step
     PhaseTimer t(c, AFQMC_PH_OVERLAP);
-    if ((rc = launch_measure(c, W, wu, wd, true))) return rc;
+    if ((rc = launch_measure(c, W, wu, wd, !c->mixed, c->mixed))) return rc;
     c->green_src = walkers_up;
     c->green_w = W;
   }
@@ -335,8 +406,9 @@ extern "C" int afqmc_force_bias(afqmc_ctx* c, int32_t W, const void* walkers_up,
   if (!force_bias) return afqmc_set_error(AFQMC_ERR_ARG, "null force_bias");
   if ((rc = set_device(c))) return rc;
   c->green_src = nullptr;
+  // the same contraction the step uses: float64, or split-bf16 in mixed mode
   if ((rc = launch_measure(c, W, static_cast<const cplx*>(walkers_up),
-                           static_cast<const cplx*>(walkers_dn), true)))
+                           static_cast<const cplx*>(walkers_dn), !c->mixed, c->mixed)))
     return rc;
   if ((rc = force_bias_gemm(c, W))) return rc;
   return launch_fb_out(c, W, static_cast<cplx*>(force_bias));
@@ -350,8 +422,9 @@ extern "C" int afqmc_energy(afqmc_ctx* c, int32_t W, const void* walkers_up,
   if (!energies) return afqmc_set_error(AFQMC_ERR_ARG, "null energies");
   if ((rc = set_device(c))) return rc;
   c->green_src = nullptr;
+  // local energies are contracted in float64 in every precision mode
   if ((rc = launch_measure(c, W, static_cast<const cplx*>(walkers_up),
-                           static_cast<const cplx*>(walkers_dn), true)))
+                           static_cast<const cplx*>(walkers_dn), true, false)))
     return rc;
   PhaseTimer t(c, AFQMC_PH_ENERGY);
   return launch_energy(c, W, static_cast<cplx*>(energies));

@@ -41,6 +41,19 @@ __host__ __device__ inline cplx cdiv(cplx a, cplx b) {
 }
 __host__ __device__ inline double cabs1(cplx a) { return fabs(a.re) + fabs(a.im); }
 
+// ---------------------------------------------------------------- bf16 split
+// x ~= hi + lo with hi = bf16(x), lo = bf16(x - hi), both rounded to nearest even
+__device__ inline uint16_t f32_to_bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline void split_bf16(double x, uint16_t* hi, uint16_t* lo) {
+  const uint16_t h = f32_to_bf16_rne((float)x);
+  *hi = h;
+  *lo = f32_to_bf16_rne((float)(x - (double)__uint_as_float((uint32_t)h << 16)));
+}
+
 // ---------------------------------------------------------------- reductions
 template <int NT>
 __device__ inline double block_sum(double v, double* red) {
@@ -81,11 +94,19 @@ struct afqmc_ctx {
   double* exp_h1[2];     // (N, N)
   cplx* mf_shifts;       // (G)
 
-  // reduced-precision copies (mixed modes)
-  float* chol_t_f32;         // (N*N, G)
-  float* rot_chol_f32[2];    // (G, n_sigma, N)
-  uint16_t* chol_t_bf[2];    // hi / lo bf16 planes of chol_t
-  uint16_t* rot_chol_bf[2][2];
+  // reduced-precision copies (mixed mode, AFQMC_PREC_BF16X3)
+  int mixed;                 // 1 when the contractions run split-bf16
+  int taylor_f32;            // 1 when the Taylor kernel runs in float32 (N <= 128, n <= 32)
+  int Gp;                    // nchol rounded up to 32 (bf16 plane leading dimension)
+  int Np32;                  // norb rounded up to 32
+  float* chol_t_f32;         // unused
+  float* rot_chol_f32[2];    // unused
+  uint16_t* chol_t_bf[2];    // hi / lo planes of chol_t, (N*N, Gp)
+  uint16_t* rot_chol_bf[2][2];  // [spin][hi/lo], (G, n_sigma, Np32)
+  uint16_t* xs_bf[2];        // [hi/lo] (2W, Gp): i sqrt(dt) x~
+  uint16_t* green_bf[2][2];  // [spin][hi/lo] (W, 2, n_sigma, Np32)
+  float* vhs_f32;            // (W, 2, N*N)
+  float* exp_h1_f32[2];      // (N, N)
 
   // workspace (device)
   double* green[2];      // (W, 2, n_sigma*N): split re / im planes per walker
@@ -168,8 +189,20 @@ int launch_gemm_nt_f64_split(hipStream_t s, int M, int N, int K, double alpha, c
                              int lda, const double* B, int ldb, double* C, int ldc, int splits,
                              int64_t c_split_stride);
 int gemm_f64_splits(int M, int N, int K);
-// measure.hip
-int launch_measure(afqmc_ctx* c, int W, const cplx* wup, const cplx* wdn, bool want_green);
+// gemm_bf16x3.hip: split-bf16 NT GEMM (K, lda, ldb multiples of 32 / 8), float / double out
+int launch_gemm_nt_bf16x3_f32(hipStream_t s, int M, int N, int K, float alpha,
+                              const uint16_t* Ah, const uint16_t* Al, int64_t lda,
+                              const uint16_t* Bh, const uint16_t* Bl, int64_t ldb, float* C,
+                              int64_t ldc, int splits, int64_t c_split_stride);
+int launch_gemm_nt_bf16x3_f64(hipStream_t s, int M, int N, int K, float alpha,
+                              const uint16_t* Ah, const uint16_t* Al, int64_t lda,
+                              const uint16_t* Bh, const uint16_t* Bl, int64_t ldb, double* C,
+                              int64_t ldc, int splits, int64_t c_split_stride);
+int launch_split_bf16(hipStream_t s, int rows, int cols, const double* x, int64_t ld_src,
+                      uint16_t* hi, uint16_t* lo, int64_t ld_dst);
+// measure.hip: green_f64 -> c->green (float64 split layout); green_bf -> c->green_bf
+int launch_measure(afqmc_ctx* c, int W, const cplx* wup, const cplx* wdn, bool green_f64,
+                   bool green_bf = false);
 // taylor.hip
 int launch_taylor(afqmc_ctx* c, int W, cplx* wup, cplx* wdn);
 // step.hip

@@ -1,0 +1,231 @@
+// Split-bf16 ("bf16x3") GEMM for gfx950 on v_mfma_f32_16x16x32_bf16:
+//   C[m][n] = alpha * sum_k A[m][k] * B[n][k]   (row-major, both operands K-contiguous)
+// Each operand x is stored as two bf16 planes, x = hi + lo with hi = bf16(x) and
+// lo = bf16(x - hi) (round to nearest even), so the pair carries 16 mantissa bits.  The
+// product keeps the three leading terms  Ah Bh + Ah Bl + Al Bh  (the dropped Al Bl is
+// ~2^-16 relative) with float32 accumulation in the MFMA.  Used for the Cholesky
+// contractions of the mixed-precision step (VHS, force bias, local-energy f_g).
+//
+// v_mfma_f32_16x16x32_bf16 lane layout (measured on gfx950):
+//   A: lane l holds A[l % 16][8 (l / 16) + e], e = 0..7      (16 bytes, K-contiguous)
+//   B: lane l holds B[8 (l / 16) + e][l % 16]                  (= row l%16 of an NT B)
+//   D (float4): lane l, element r holds D[4 (l / 16) + r][l % 16]
+// Operand planes have leading dimensions that are multiples of 32 elements, zero padded
+// in k, so K (a multiple of 32) needs no tail handling.
+#include "common.h"
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int BK = 32;
+constexpr int LDSW = BK + 8;  // LDS row stride in bf16 (80 B): 16 rows x 16 B are conflict free
+
+__device__ inline uint16_t f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// x (float64, rows x cols, ld_src) -> hi / lo bf16 planes (rows x ld_dst, zero padded)
+__global__ void split_bf16_kernel(int rows, int cols, const double* __restrict__ x, int64_t ld_src,
+                                  uint16_t* __restrict__ hi, uint16_t* __restrict__ lo,
+                                  int64_t ld_dst) {
+  const int64_t total = (int64_t)rows * ld_dst;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e / ld_dst, c = e - r * ld_dst;
+    uint16_t h = 0, l = 0;
+    if (c < cols) {
+      const double v = x[r * ld_src + c];
+      h = f32_to_bf16((float)v);
+      l = f32_to_bf16((float)(v - (double)bf16_to_f32(h)));
+    }
+    hi[e] = h;
+    lo[e] = l;
+  }
+}
+
+template <int BM, int BN, typename OutT>
+__global__ __launch_bounds__(256) void gemm_nt_bf16x3_kernel(
+    int M, int N, int K, float alpha, const uint16_t* __restrict__ Ah,
+    const uint16_t* __restrict__ Al, int64_t lda, const uint16_t* __restrict__ Bh,
+    const uint16_t* __restrict__ Bl, int64_t ldb, OutT* __restrict__ C, int64_t ldc, int tiles_m,
+    int tiles_n, int k_per_split, int64_t c_split_stride) {
+  constexpr int TM = BM / 2 / 16, TN = BN / 2 / 16;  // 2 x 2 waves
+  constexpr int A_LD = BM * 4 / 256, B_LD = BN * 4 / 256;  // 16-byte chunks per thread
+  __shared__ __attribute__((aligned(16))) uint16_t sA[2][2][BM * LDSW];  // [buf][hi/lo]
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][BN * LDSW];
+
+  const int nblk = tiles_m * tiles_n;
+  int b = blockIdx.x;
+  if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);  // XCD-contiguous tile runs
+  const int tile_m = b % tiles_m, tile_n = b / tiles_m;
+  const int m0 = tile_m * BM, n0 = tile_n * BN;
+  const int kbeg = blockIdx.y * k_per_split;
+  const int kend = (kbeg + k_per_split < K) ? kbeg + k_per_split : K;
+  C += (int64_t)blockIdx.y * c_split_stride;
+
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  u32x4_t ra[2][A_LD], rb[2][B_LD];
+  auto load_tile = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < A_LD; ++j) {
+      const int q = t + 256 * j, row = q >> 2, ch = q & 3;
+      const int r = m0 + row;
+      u32x4_t vh = {0, 0, 0, 0}, vl = {0, 0, 0, 0};
+      if (r < M) {
+        vh = *reinterpret_cast<const u32x4_t*>(Ah + r * lda + k0 + ch * 8);
+        vl = *reinterpret_cast<const u32x4_t*>(Al + r * lda + k0 + ch * 8);
+      }
+      ra[0][j] = vh;
+      ra[1][j] = vl;
+    }
+#pragma unroll
+    for (int j = 0; j < B_LD; ++j) {
+      const int q = t + 256 * j, row = q >> 2, ch = q & 3;
+      const int r = n0 + row;
+      u32x4_t vh = {0, 0, 0, 0}, vl = {0, 0, 0, 0};
+      if (r < N) {
+        vh = *reinterpret_cast<const u32x4_t*>(Bh + r * ldb + k0 + ch * 8);
+        vl = *reinterpret_cast<const u32x4_t*>(Bl + r * ldb + k0 + ch * 8);
+      }
+      rb[0][j] = vh;
+      rb[1][j] = vl;
+    }
+  };
+  auto store_tile = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < A_LD; ++j) {
+      const int q = t + 256 * j, row = q >> 2, ch = q & 3;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        *reinterpret_cast<u32x4_t*>(&sA[buf][h][row * LDSW + ch * 8]) = ra[h][j];
+    }
+#pragma unroll
+    for (int j = 0; j < B_LD; ++j) {
+      const int q = t + 256 * j, row = q >> 2, ch = q & 3;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        *reinterpret_cast<u32x4_t*>(&sB[buf][h][row * LDSW + ch * 8]) = rb[h][j];
+    }
+  };
+
+  float4_t acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+
+  const int nk = (kend - kbeg) / BK;
+  if (nk > 0) {
+    load_tile(kbeg);
+    store_tile(0);
+  }
+  __syncthreads();
+  const int arow = wm * (BM / 2) + (lane & 15);
+  const int brow = wn * (BN / 2) + (lane & 15);
+  const int koff = (lane >> 4) * 8;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load_tile(kbeg + (kt + 1) * BK);
+    bf16x8_t ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int o = (arow + i * 16) * LDSW + koff;
+      ah[i] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t*>(&sA[cur][0][o]));
+      al[i] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t*>(&sA[cur][1][o]));
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int o = (brow + j * 16) * LDSW + koff;
+      bh[j] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t*>(&sB[cur][0][o]));
+      bl[j] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t*>(&sB[cur][1][o]));
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      }
+    if (kt + 1 < nk) store_tile(cur ^ 1);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int col = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * (BM / 2) + i * 16 + 4 * (lane >> 4) + r;
+        if (row < M && col < N) C[(int64_t)row * ldc + col] = (OutT)(alpha * acc[i][j][r]);
+      }
+    }
+}
+
+template <int BM, int BN, typename OutT>
+int launch_bf(hipStream_t s, int M, int N, int K, float alpha, const uint16_t* Ah,
+              const uint16_t* Al, int64_t lda, const uint16_t* Bh, const uint16_t* Bl, int64_t ldb,
+              OutT* C, int64_t ldc, int splits, int kps, int64_t c_split_stride) {
+  const int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
+  hipLaunchKernelGGL((gemm_nt_bf16x3_kernel<BM, BN, OutT>), dim3(tm * tn, splits), dim3(256), 0,
+                     s, M, N, K, alpha, Ah, Al, lda, Bh, Bl, ldb, C, ldc, tm, tn, kps,
+                     c_split_stride);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+template <typename OutT>
+int gemm_bf16x3(hipStream_t s, int M, int N, int K, float alpha, const uint16_t* Ah,
+                const uint16_t* Al, int64_t lda, const uint16_t* Bh, const uint16_t* Bl,
+                int64_t ldb, OutT* C, int64_t ldc, int splits, int64_t c_split_stride) {
+  if (M <= 0 || N <= 0) return AFQMC_OK;
+  if (K <= 0 || K % BK || lda % 8 || ldb % 8)
+    return afqmc_set_error(AFQMC_ERR_ARG, "bf16x3 gemm: K=%d lda=%lld ldb=%lld", K,
+                           (long long)lda, (long long)ldb);
+  if (splits < 1) splits = 1;
+  int kps = (K / BK + splits - 1) / splits * BK;
+  const int64_t big = (int64_t)((M + 127) / 128) * ((N + 127) / 128);
+  if (big * splits >= 512)
+    return launch_bf<128, 128, OutT>(s, M, N, K, alpha, Ah, Al, lda, Bh, Bl, ldb, C, ldc, splits,
+                                     kps, c_split_stride);
+  return launch_bf<64, 64, OutT>(s, M, N, K, alpha, Ah, Al, lda, Bh, Bl, ldb, C, ldc, splits, kps,
+                                 c_split_stride);
+}
+
+}  // namespace
+
+int launch_split_bf16(hipStream_t s, int rows, int cols, const double* x, int64_t ld_src,
+                      uint16_t* hi, uint16_t* lo, int64_t ld_dst) {
+  const int64_t total = (int64_t)rows * ld_dst;
+  if (total <= 0) return AFQMC_OK;
+  const int64_t blocks = (total + 255) / 256 < 65536 ? (total + 255) / 256 : 65536;
+  hipLaunchKernelGGL(split_bf16_kernel, dim3((unsigned)blocks), dim3(256), 0, s, rows, cols, x,
+                     ld_src, hi, lo, ld_dst);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+int launch_gemm_nt_bf16x3_f32(hipStream_t s, int M, int N, int K, float alpha,
+                              const uint16_t* Ah, const uint16_t* Al, int64_t lda,
+                              const uint16_t* Bh, const uint16_t* Bl, int64_t ldb, float* C,
+                              int64_t ldc, int splits, int64_t c_split_stride) {
+  return gemm_bf16x3<float>(s, M, N, K, alpha, Ah, Al, lda, Bh, Bl, ldb, C, ldc, splits,
+                            c_split_stride);
+}
+
+int launch_gemm_nt_bf16x3_f64(hipStream_t s, int M, int N, int K, float alpha,
+                              const uint16_t* Ah, const uint16_t* Al, int64_t lda,
+                              const uint16_t* Bh, const uint16_t* Bl, int64_t ldb, double* C,
+                              int64_t ldc, int splits, int64_t c_split_stride) {
+  return gemm_bf16x3<double>(s, M, N, K, alpha, Ah, Al, lda, Bh, Bl, ldb, C, ldc, splits,
+                             c_split_stride);
+}

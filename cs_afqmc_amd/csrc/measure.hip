@@ -23,7 +23,8 @@ constexpr int MEAS_KC = 4;  // k-groups whose operands are loaded ahead of their
 template <int T2>
 __global__ __launch_bounds__(64) void measure_kernel(
     int N, int n, const cplx* __restrict__ walkers, const double* __restrict__ trial,
-    cplx* __restrict__ dets, int spin, double* __restrict__ green, int want_green) {
+    cplx* __restrict__ dets, int spin, double* __restrict__ green, uint16_t* __restrict__ ghi,
+    uint16_t* __restrict__ glo, int ld16) {
   extern __shared__ double smem[];
   const int n2 = 2 * n;
   cplx* M = reinterpret_cast<cplx*>(smem);  // n x 2n augmented matrix
@@ -134,13 +135,15 @@ __global__ __launch_bounds__(64) void measure_kernel(
     __syncthreads();
   }
   if (lane == 0) dets[(int64_t)w * 2 + spin] = det;
-  if (!want_green) return;
+  if (!green && !ghi) return;
 
   // ---- [Gr ; Gi] = [[Xr^T, -Xi^T], [Xi^T, Xr^T]] [wr^T ; wi^T], K = 2n, columns p
   {
     const int tp_n = (N + 15) >> 4;
     const int nkg = (n2 + 3) >> 2;
-    double* gw = green + (int64_t)w * n2 * N;
+    double* gw = green ? green + (int64_t)w * n2 * N : nullptr;
+    uint16_t* gh = ghi ? ghi + (int64_t)w * n2 * ld16 : nullptr;
+    uint16_t* gl = ghi ? glo + (int64_t)w * n2 * ld16 : nullptr;
     for (int tp = 0; tp < tp_n; ++tp) {
       const int p = tp * 16 + col;  // B column = orbital p
       double4_t acc[T2];
@@ -177,40 +180,44 @@ __global__ __launch_bounds__(64) void measure_kernel(
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int row = tr * 16 + s + 4 * q;
-          if (row < n2 && p < N) gw[(int64_t)row * N + p] = acc[tr][q];
+          if (row < n2 && p < N) {
+            if (gw) gw[(int64_t)row * N + p] = acc[tr][q];
+            if (gh) split_bf16(acc[tr][q], &gh[(int64_t)row * ld16 + p], &gl[(int64_t)row * ld16 + p]);
+          }
         }
     }
   }
 }
 
 template <int T2>
-int launch_measure_t(afqmc_ctx* c, int W, int n, const cplx* wk, int s, bool want_green) {
+int launch_measure_t(afqmc_ctx* c, int W, int n, const cplx* wk, int s, bool g64, bool gbf) {
   const size_t shm = (size_t)(n * 2 * n + n) * sizeof(cplx);
   static int lds_limit = 0;
   const int rc = ensure_dyn_lds((const void*)measure_kernel<T2>, shm, &lds_limit);
   if (rc) return rc;
   hipLaunchKernelGGL((measure_kernel<T2>), dim3(W), dim3(64), shm, c->stream, c->d.norb, n, wk,
-                     c->trial[s], c->dets, s, c->green[s], want_green ? 1 : 0);
+                     c->trial[s], c->dets, s, g64 ? c->green[s] : nullptr,
+                     gbf ? c->green_bf[s][0] : nullptr, gbf ? c->green_bf[s][1] : nullptr, c->Np32);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }
 
 }  // namespace
 
-int launch_measure(afqmc_ctx* c, int W, const cplx* wup, const cplx* wdn, bool want_green) {
+int launch_measure(afqmc_ctx* c, int W, const cplx* wup, const cplx* wdn, bool g64, bool gbf) {
   for (int sp = 0; sp < c->nspin; ++sp) {
     const int n = c->nocc[sp];
     const cplx* wk = (sp == 0) ? wup : wdn;
     int rc;
     switch ((2 * n + 15) / 16) {
-      case 1: rc = launch_measure_t<1>(c, W, n, wk, sp, want_green); break;
-      case 2: rc = launch_measure_t<2>(c, W, n, wk, sp, want_green); break;
-      case 3: rc = launch_measure_t<3>(c, W, n, wk, sp, want_green); break;
-      case 4: rc = launch_measure_t<4>(c, W, n, wk, sp, want_green); break;
-      case 5: rc = launch_measure_t<5>(c, W, n, wk, sp, want_green); break;
-      case 6: rc = launch_measure_t<6>(c, W, n, wk, sp, want_green); break;
-      case 7: rc = launch_measure_t<7>(c, W, n, wk, sp, want_green); break;
-      default: rc = launch_measure_t<8>(c, W, n, wk, sp, want_green); break;
+      case 1: rc = launch_measure_t<1>(c, W, n, wk, sp, g64, gbf); break;
+      case 2: rc = launch_measure_t<2>(c, W, n, wk, sp, g64, gbf); break;
+      case 3: rc = launch_measure_t<3>(c, W, n, wk, sp, g64, gbf); break;
+      case 4: rc = launch_measure_t<4>(c, W, n, wk, sp, g64, gbf); break;
+      case 5: rc = launch_measure_t<5>(c, W, n, wk, sp, g64, gbf); break;
+      case 6: rc = launch_measure_t<6>(c, W, n, wk, sp, g64, gbf); break;
+      case 7: rc = launch_measure_t<7>(c, W, n, wk, sp, g64, gbf); break;
+      default: rc = launch_measure_t<8>(c, W, n, wk, sp, g64, gbf); break;
     }
     if (rc) return rc;
   }

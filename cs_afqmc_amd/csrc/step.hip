@@ -38,6 +38,7 @@ __device__ inline cplx fb_sum(const double* __restrict__ fb, int nparts, int64_t
 __global__ __launch_bounds__(FIELDS_NT) void fields_kernel(
     int G, double sqdt, const double* __restrict__ fb, int nparts, int64_t part_stride,
     const cplx* __restrict__ mf, const double* __restrict__ fields, double* __restrict__ xs,
+    uint16_t* __restrict__ xs_hi, uint16_t* __restrict__ xs_lo, int Gp,
     cplx* __restrict__ terms, cplx* __restrict__ fb_out) {
   __shared__ double red[FIELDS_NT / 64];
   const int w = blockIdx.x;
@@ -58,9 +59,15 @@ __global__ __launch_bounds__(FIELDS_NT) void fields_kernel(
     // fields * field_shifts - field_shifts * field_shifts / 2
     const cplx fs2 = cmul(fs, fs);
     ft = cadd(ft, cplx{xv * fs.re - fs2.re / 2.0, xv * fs.im - fs2.im / 2.0});
-    // VHS operand: 1j * sqrt(dt) * shifted_fields
-    xr[g] = -sqdt * xt.im;
-    xi[g] = sqdt * xt.re;
+    // VHS operand: 1j * sqrt(dt) * shifted_fields (float64, or split-bf16 planes)
+    if (xs_hi) {
+      const int64_t o = (int64_t)w * 2 * Gp + g;
+      split_bf16(-sqdt * xt.im, &xs_hi[o], &xs_lo[o]);
+      split_bf16(sqdt * xt.re, &xs_hi[o + Gp], &xs_lo[o + Gp]);
+    } else {
+      xr[g] = -sqdt * xt.im;
+      xi[g] = sqdt * xt.re;
+    }
     if (fb_out) fb_out[(int64_t)w * G + g] = f;
   }
   const double s0 = block_sum<FIELDS_NT>(st.re, red);
@@ -152,7 +159,8 @@ __global__ void fb_out_kernel(int W, int G, const double* __restrict__ fb, int n
 int launch_fields(afqmc_ctx* c, int W, const double* fields, cplx* fb_out) {
   hipLaunchKernelGGL(fields_kernel, dim3(W), dim3(FIELDS_NT), 0, c->stream, c->d.nchol,
                      sqrt(c->d.dt), c->fb, c->fb_parts, (int64_t)2 * W * c->d.nchol, c->mf_shifts,
-                     fields, c->xs, c->terms, fb_out);
+                     fields, c->xs, c->mixed ? c->xs_bf[0] : nullptr,
+                     c->mixed ? c->xs_bf[1] : nullptr, c->Gp, c->terms, fb_out);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }

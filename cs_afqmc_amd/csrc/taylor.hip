@@ -298,6 +298,173 @@ int launch_reg(afqmc_ctx* c, int W, int n, cplx* wk, const double* E) {
   return AFQMC_OK;
 }
 
+// ------------------------------------------------------------------------------------
+// Float32 form of the register-resident kernel for the mixed-precision step: V comes
+// from the bf16x3 VHS contraction in float32, E is float32, products run on
+// v_mfma_f32_16x16x4f32 (same A / B lane layout as the f64 MFMA; D element r of lane l
+// is row 4 (l / 16) + r).  The walker is read and written in float64.
+template <int CTP>
+struct RegTileF {
+  static constexpr int RS = (CTP == 2) ? 48 : (CTP == 4 ? 80 : 16 * CTP);  // 4 RS == 64 (mod 128)
+};
+
+template <int NKGV, int CTP>
+__global__ __launch_bounds__(512) void taylor_reg_f32_kernel(
+    int N, int n, int n_terms_k, cplx* __restrict__ walkers, const float* __restrict__ expE,
+    const float* __restrict__ vhs) {
+  constexpr int RS = RegTileF<CTP>::RS;
+  extern __shared__ float TF[];  // [2 Np][RS]
+  const int w = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rb = wave * 16;
+  const int Np = (N + 3) & ~3;
+  const int nkg = Np >> 2;
+  const int nkgv = 2 * nkg;
+  const int s = lane >> 4, col = lane & 15;
+  const int p = rb + col;
+  const bool prow = p < N;
+  const int n2 = 2 * n;
+  cplx* phi = walkers + (int64_t)w * N * n;
+  const float* Vr = vhs + (int64_t)w * 2 * N * N;
+  const float* Vi = Vr + (int64_t)N * N;
+
+  float va[NKGV];
+#pragma unroll
+  for (int kg = 0; kg < NKGV; ++kg) {
+    const int q = 4 * kg + s;
+    float v = 0.f;
+    if (kg < nkgv && prow) {
+      if (q < Np) {
+        if (q < N) v = Vr[(int64_t)p * N + q];
+      } else if (q - Np < N) {
+        v = Vi[(int64_t)p * N + q - Np];
+      }
+    }
+    va[kg] = v;
+  }
+
+  for (int e = threadIdx.x; e < Np * RS; e += blockDim.x) {
+    const int q = e / RS, c = e - (e / RS) * RS;
+    float up = 0.f, lo = 0.f;
+    if (q < N && c < n2) {
+      const int j = c < n ? c : c - n;
+      const cplx z = phi[(int64_t)q * n + j];
+      up = (float)(c < n ? z.re : z.im);
+      lo = (float)(c < n ? -z.im : z.re);
+    }
+    TF[e] = up;
+    TF[Np * RS + e] = lo;
+  }
+  __syncthreads();
+
+  const float* bbase = TF + s * RS + col;
+
+  auto put = [&](const float4_t* v) {
+#pragma unroll
+    for (int ct = 0; ct < CTP; ++ct)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rb + 4 * s + r;
+        const int cc = ct * 16 + col;
+        if (row < Np) {
+          TF[row * RS + cc] = v[ct][r];
+          if (cc < n)
+            TF[(Np + row) * RS + cc + n] = v[ct][r];
+          else if (cc < n2)
+            TF[(Np + row) * RS + cc - n] = -v[ct][r];
+        }
+      }
+  };
+
+  auto e_product = [&](float4_t* acc) {
+#pragma unroll
+    for (int ct = 0; ct < CTP; ++ct) acc[ct] = float4_t{0.f, 0.f, 0.f, 0.f};
+    float a = (prow && s < N) ? expE[(int64_t)p * N + s] : 0.f;
+    for (int kg = 0; kg < nkg; ++kg) {
+      const int qn = 4 * (kg + 1) + s;
+      const float an = (kg + 1 < nkg && prow && qn < N) ? expE[(int64_t)p * N + qn] : 0.f;
+#pragma unroll
+      for (int ct = 0; ct < CTP; ++ct)
+        acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bbase[kg * 4 * RS + ct * 16], acc[ct],
+                                                       0, 0, 0);
+      a = an;
+    }
+  };
+
+  float4_t t0[CTP], u[CTP];
+  e_product(t0);
+  __syncthreads();
+  put(t0);
+  __syncthreads();
+  for (int k = n_terms_k; k >= 1; --k) {
+#pragma unroll
+    for (int ct = 0; ct < CTP; ++ct) u[ct] = float4_t{0.f, 0.f, 0.f, 0.f};
+    float b[CTP];
+#pragma unroll
+    for (int ct = 0; ct < CTP; ++ct) b[ct] = bbase[ct * 16];
+#pragma unroll
+    for (int kg = 0; kg < NKGV; ++kg) {
+      if (kg < nkgv) {
+        float bn[CTP];
+#pragma unroll
+        for (int ct = 0; ct < CTP; ++ct)
+          bn[ct] = (kg + 1 < nkgv) ? bbase[(kg + 1) * 4 * RS + ct * 16] : 0.f;
+#pragma unroll
+        for (int ct = 0; ct < CTP; ++ct)
+          u[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(va[kg], b[ct], u[ct], 0, 0, 0);
+#pragma unroll
+        for (int ct = 0; ct < CTP; ++ct) b[ct] = bn[ct];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    __syncthreads();
+    const float rk = 1.0f / (float)k;
+#pragma unroll
+    for (int ct = 0; ct < CTP; ++ct)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) u[ct][r] = t0[ct][r] + u[ct][r] * rk;
+    put(u);
+    __syncthreads();
+  }
+  e_product(u);
+#pragma unroll
+  for (int ct = 0; ct < CTP; ++ct)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = rb + 4 * s + r;
+      const int cc = ct * 16 + col;
+      if (row < N && cc < n2) {
+        double* dst = reinterpret_cast<double*>(phi + (int64_t)row * n + (cc < n ? cc : cc - n));
+        dst[cc < n ? 0 : 1] = (double)u[ct][r];
+      }
+    }
+}
+
+template <int NKGV, int CTP>
+int launch_reg_f32(afqmc_ctx* c, int W, int n, cplx* wk, const float* E) {
+  const int N = c->d.norb;
+  const int nwaves = (N + 15) / 16;
+  const int Np = (N + 3) & ~3;
+  const size_t shm = (size_t)2 * Np * RegTileF<CTP>::RS * sizeof(float);
+  static int lds_limit = 0;
+  const int rc = ensure_dyn_lds((const void*)taylor_reg_f32_kernel<NKGV, CTP>, shm, &lds_limit);
+  if (rc) return rc;
+  hipLaunchKernelGGL((taylor_reg_f32_kernel<NKGV, CTP>), dim3(W), dim3(nwaves * 64), shm,
+                     c->stream, N, n, c->d.n_exp_terms - 1, wk, E, c->vhs_f32);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+template <int NKGV>
+int launch_reg_f32_ct(afqmc_ctx* c, int W, int n, cplx* wk, const float* E) {
+  switch ((2 * n + 15) / 16) {
+    case 1: return launch_reg_f32<NKGV, 1>(c, W, n, wk, E);
+    case 2: return launch_reg_f32<NKGV, 2>(c, W, n, wk, E);
+    case 3: return launch_reg_f32<NKGV, 3>(c, W, n, wk, E);
+    default: return launch_reg_f32<NKGV, 4>(c, W, n, wk, E);
+  }
+}
+
 template <int NKGV>
 int launch_reg_ct(afqmc_ctx* c, int W, int n, cplx* wk, const double* E) {
   const int ctp = (2 * n + 15) / 16;
@@ -313,6 +480,19 @@ int launch_reg_ct(afqmc_ctx* c, int W, int n, cplx* wk, const double* E) {
 
 int launch_taylor(afqmc_ctx* c, int W, cplx* wup, cplx* wdn) {
   const int N = c->d.norb;
+  if (c->taylor_f32) {
+    for (int s = 0; s < c->nspin; ++s) {
+      const int n = c->nocc[s];
+      cplx* wk = (s == 0) ? wup : wdn;
+      const int nkgv = 2 * ((N + 3) / 4);
+      const int rc = nkgv <= 32   ? launch_reg_f32_ct<32>(c, W, n, wk, c->exp_h1_f32[s])
+                     : nkgv <= 48 ? launch_reg_f32_ct<48>(c, W, n, wk, c->exp_h1_f32[s])
+                     : nkgv <= 56 ? launch_reg_f32_ct<56>(c, W, n, wk, c->exp_h1_f32[s])
+                                  : launch_reg_f32_ct<64>(c, W, n, wk, c->exp_h1_f32[s]);
+      if (rc) return rc;
+    }
+    return AFQMC_OK;
+  }
   if (N <= 128 && c->nocc[0] <= 32 && c->nocc[1] <= 32) {
     for (int s = 0; s < c->nspin; ++s) {
       const int n = c->nocc[s];

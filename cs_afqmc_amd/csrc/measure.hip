@@ -15,12 +15,14 @@
 // [O | I] lives in LDS (n x 2n complex).
 #include "common.h"
 
+extern "C" __device__ unsigned long long __ockl_wfred_max_u64(unsigned long long);
+
 namespace {
 
 constexpr int MEAS_KC = 4;  // k-groups whose operands are loaded ahead of their MFMAs
 
 // T2 = ceil(2n / 16): column tiles of [Or | Oi] and row tiles of [Gr ; Gi]
-template <int T2>
+template <int T2, int TI, bool STAGE>
 __global__ __launch_bounds__(64) void measure_kernel(
     int N, int n, const cplx* __restrict__ walkers, const double* __restrict__ trial,
     cplx* __restrict__ dets, int spin, double* __restrict__ green, uint16_t* __restrict__ ghi,
@@ -29,57 +31,88 @@ __global__ __launch_bounds__(64) void measure_kernel(
   const int n2 = 2 * n;
   cplx* M = reinterpret_cast<cplx*>(smem);  // n x 2n augmented matrix
   cplx* fcol = M + n * n2;                  // pivot column copy (n)
-  __shared__ int piv_s;
 
   const int w = blockIdx.x, lane = threadIdx.x;
   const int s = lane >> 4, col = lane & 15;
   const cplx* phi = walkers + (int64_t)w * N * n;
+  // walker operand: staged once into LDS with 16-byte loads (all in flight together)
+  // when it fits, else read from global memory (L2)
   const double* phid = reinterpret_cast<const double*>(phi);
+  if (STAGE) {
+    cplx* ph = fcol + n;
+    for (int e = lane; e < N * n; e += 64) ph[e] = phi[e];
+    phid = reinterpret_cast<const double*>(ph);
+  }
+  __syncthreads();
 
-  // ---- [Or | Oi] = C^T [wr | wi]: rows i (tiles of 16), columns c < 2n, K = N
+  // ---- [Or | Oi] = C^T [wr | wi]: rows i (TI tiles of 16), columns c < 2n, K = N.
+  // All TI x T2 accumulators are live, so each B value feeds TI MFMAs; the operands of
+  // k-group chunk j + 1 are loaded while chunk j runs.
   {
-    const int ti_n = (n + 15) >> 4;
     const int nkg = (N + 3) >> 2;
-    // per-lane B column offsets (walker element, re / im) of each column tile
     int boff[T2];
 #pragma unroll
     for (int tc = 0; tc < T2; ++tc) {
       const int c = tc * 16 + col;
       boff[tc] = c < n2 ? 2 * (c < n ? c : c - n) + (c < n ? 0 : 1) : -1;
     }
-    for (int ti = 0; ti < ti_n; ++ti) {
-      const int i = ti * 16 + col;  // A row
-      double4_t acc[T2];
+    // row tiles in passes of TI (all of them when the accumulators fit in registers)
+    const int ti_total = (n + 15) >> 4;
+    for (int tb = 0; tb < ti_total; tb += TI) {
+    double4_t acc[TI][T2];
 #pragma unroll
-      for (int tc = 0; tc < T2; ++tc) acc[tc] = double4_t{0.0, 0.0, 0.0, 0.0};
-      for (int kg0 = 0; kg0 < nkg; kg0 += MEAS_KC) {
-        double a[MEAS_KC], b[MEAS_KC][T2];
+    for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
-        for (int u = 0; u < MEAS_KC; ++u) {
-          const int p = 4 * (kg0 + u) + s;
-          const bool pok = p < N;
-          a[u] = (pok && i < n) ? trial[p * n + i] : 0.0;
+      for (int tc = 0; tc < T2; ++tc) acc[ti][tc] = double4_t{0.0, 0.0, 0.0, 0.0};
+    // two operand buffers with compile-time names (a runtime buffer index would put
+    // them in scratch memory)
+    double a0[MEAS_KC][TI], b0[MEAS_KC][T2], a1[MEAS_KC][TI], b1[MEAS_KC][T2];
+    auto load = [&](double (&a)[MEAS_KC][TI], double (&b)[MEAS_KC][T2], int kg0) {
 #pragma unroll
-          for (int tc = 0; tc < T2; ++tc)
-            b[u][tc] = (pok && boff[tc] >= 0) ? phid[(int64_t)2 * p * n + boff[tc]] : 0.0;
+      for (int u = 0; u < MEAS_KC; ++u) {
+        const int p = 4 * (kg0 + u) + s;
+        const bool pok = p < N;
+#pragma unroll
+        for (int ti = 0; ti < TI; ++ti) {
+          const int i = (tb + ti) * 16 + col;
+          a[u][ti] = (pok && i < n) ? trial[p * n + i] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < MEAS_KC; ++u)
+        for (int tc = 0; tc < T2; ++tc)
+          b[u][tc] = (pok && boff[tc] >= 0) ? phid[(int64_t)2 * p * n + boff[tc]] : 0.0;
+      }
+    };
+    auto mma = [&](const double (&a)[MEAS_KC][TI], const double (&b)[MEAS_KC][T2]) {
+#pragma unroll
+      for (int u = 0; u < MEAS_KC; ++u)
+#pragma unroll
+        for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
           for (int tc = 0; tc < T2; ++tc)
-            acc[tc] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u][tc], acc[tc], 0, 0, 0);
-      }
+            acc[ti][tc] =
+                __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][ti], b[u][tc], acc[ti][tc], 0, 0, 0);
+    };
+    const int nch = (nkg + MEAS_KC - 1) / MEAS_KC;
+    load(a0, b0, 0);
+    for (int ch = 0; ch < nch; ch += 2) {
+      if (ch + 1 < nch) load(a1, b1, (ch + 1) * MEAS_KC);
+      mma(a0, b0);
+      if (ch + 2 < nch) load(a0, b0, (ch + 2) * MEAS_KC);
+      if (ch + 1 < nch) mma(a1, b1);
+    }
 #pragma unroll
-      for (int tc = 0; tc < T2; ++tc) {
-        if (boff[tc] < 0) continue;
-        const int c = tc * 16 + col;
-        const int cj = c < n ? c : c - n, ci = c < n ? 0 : 1;
+    for (int tc = 0; tc < T2; ++tc) {
+      if (boff[tc] < 0) continue;
+      const int c = tc * 16 + col;
+      const int cj = c < n ? c : c - n, ci = c < n ? 0 : 1;
+#pragma unroll
+      for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = ti * 16 + s + 4 * r;
-          if (row < n) reinterpret_cast<double*>(&M[row * n2 + cj])[ci] = acc[tc][r];
+          const int row = (tb + ti) * 16 + s + 4 * r;
+          if (row < n) reinterpret_cast<double*>(&M[row * n2 + cj])[ci] = acc[ti][tc][r];
         }
-      }
+    }
     }
     for (int e = lane; e < n * n; e += 64) {
       const int i = e / n, j = e - (e / n) * n;
@@ -89,26 +122,26 @@ __global__ __launch_bounds__(64) void measure_kernel(
   __syncthreads();
 
   // ---- Gauss-Jordan with partial pivoting: [O | I] -> [* | X], det from pivots.
+  // Lane c' owns column k + 1 + c' of the trailing matrix (2n - k - 1 <= 127 columns,
+  // at most two per lane), so the elimination needs no index division.
   cplx det = cplx{1.0, 0.0};
   for (int k = 0; k < n; ++k) {
+    int pr;
     {
+      // first maximum of |re| + |im| in column k (rows >= k): one 64-bit key per lane,
+      // float32 magnitude in the high word, 63 - row in the low word
       const int r = k + lane;
-      double v = (r < n) ? cabs1(M[r * n2 + k]) : -1.0;
-      int idx = r;
-      for (int off = 32; off > 0; off >>= 1) {
-        const double v2 = __shfl_xor(v, off, 64);
-        const int i2 = __shfl_xor(idx, off, 64);
-        if (v2 > v || (v2 == v && i2 < idx)) {
-          v = v2;
-          idx = i2;
-        }
+      unsigned long long key = 0ull;
+      if (r < n) {
+        const float v = (float)cabs1(M[r * n2 + k]);
+        key = (((unsigned long long)__float_as_uint(v)) << 32) | (unsigned)(63 - lane);
       }
-      if (lane == 0) piv_s = idx;
+      key = __ockl_wfred_max_u64(key);  // DPP wave reduction, result in every lane
+      pr = k + 63 - (int)__builtin_amdgcn_readfirstlane((unsigned)(key & 0xFFFFFFFFull));
     }
-    __syncthreads();
-    const int pr = piv_s;
     const cplx pivot = M[pr * n2 + k];
-    const cplx inv = cdiv(cplx{1.0, 0.0}, pivot);
+    const double d2 = 1.0 / (pivot.re * pivot.re + pivot.im * pivot.im);
+    const cplx inv = cplx{pivot.re * d2, -pivot.im * d2};
     if (lane == 0) {
       det = cmul(det, pivot);
       if (pr != k) det = cplx{-det.re, -det.im};
@@ -116,20 +149,38 @@ __global__ __launch_bounds__(64) void measure_kernel(
     // pivot column as seen after the row exchange; exchange + scale the pivot row
     for (int r = lane; r < n; r += 64) fcol[r] = M[(r == pr ? k : r) * n2 + k];
     for (int c = k + 1 + lane; c < n2; c += 64) {
-      const cplx a = M[k * n2 + c];
-      M[k * n2 + c] = cmul(M[pr * n2 + c], inv);
-      if (pr != k) M[pr * n2 + c] = a;
+      double* mk = reinterpret_cast<double*>(&M[k * n2 + c]);
+      double* mp = reinterpret_cast<double*>(&M[pr * n2 + c]);
+      const double ar = mk[0], ai = mk[1], pr_ = mp[0], pi_ = mp[1];
+      mk[0] = pr_ * inv.re - pi_ * inv.im;
+      mk[1] = pr_ * inv.im + pi_ * inv.re;
+      if (pr != k) {
+        mp[0] = ar;
+        mp[1] = ai;
+      }
     }
     __syncthreads();
-    const int width = n2 - k - 1;
-    for (int e = lane; e < n * width; e += 64) {
-      const int r = e / width, c = k + 1 + (e - (e / width) * width);
-      if (r != k) {
-        const cplx f = fcol[r], m = M[k * n2 + c];
-        cplx x = M[r * n2 + c];
-        x.re -= f.re * m.re - f.im * m.im;
-        x.im -= f.re * m.im + f.im * m.re;
-        M[r * n2 + c] = x;
+    for (int c = k + 1 + lane; c < n2; c += 64) {
+      const cplx m = M[k * n2 + c];
+      for (int r0 = 0; r0 < n; r0 += 8) {
+        // eight independent loads, then the updates, then the stores
+        cplx x[8], f[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int r = r0 + u < n ? r0 + u : n - 1;
+          x[u] = M[r * n2 + c];
+          f[u] = fcol[r];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int r = r0 + u;
+          if (r < n && r != k) {
+            cplx y = x[u];
+            y.re -= f[u].re * m.re - f[u].im * m.im;
+            y.im -= f[u].re * m.im + f[u].im * m.re;
+            M[r * n2 + c] = y;
+          }
+        }
       }
     }
     __syncthreads();
@@ -191,11 +242,24 @@ __global__ __launch_bounds__(64) void measure_kernel(
 
 template <int T2>
 int launch_measure_t(afqmc_ctx* c, int W, int n, const cplx* wk, int s, bool g64, bool gbf) {
-  const size_t shm = (size_t)(n * 2 * n + n) * sizeof(cplx);
+  // O row tiles per pass: all ceil(n / 16) of them unless the accumulators would spill
+  constexpr int TI = T2 > 4 ? 1 : (T2 + 1) / 2;
+  const int N = c->d.norb;
+  const size_t base = (size_t)(n * 2 * n + n) * sizeof(cplx);
+  const size_t staged = base + (size_t)N * n * sizeof(cplx);
+  if (staged <= 24 * 1024) {  // small walkers only: staging costs occupancy
+    hipLaunchKernelGGL((measure_kernel<T2, TI, true>), dim3(W), dim3(64), staged, c->stream, N,
+                       n, wk, c->trial[s], c->dets, s, g64 ? c->green[s] : nullptr,
+                       gbf ? c->green_bf[s][0] : nullptr, gbf ? c->green_bf[s][1] : nullptr,
+                       c->Np32);
+    LAUNCH_CHECK();
+    return AFQMC_OK;
+  }
+  const size_t shm = base;
   static int lds_limit = 0;
-  const int rc = ensure_dyn_lds((const void*)measure_kernel<T2>, shm, &lds_limit);
+  const int rc = ensure_dyn_lds((const void*)measure_kernel<T2, TI, false>, shm, &lds_limit);
   if (rc) return rc;
-  hipLaunchKernelGGL((measure_kernel<T2>), dim3(W), dim3(64), shm, c->stream, c->d.norb, n, wk,
+  hipLaunchKernelGGL((measure_kernel<T2, TI, false>), dim3(W), dim3(64), shm, c->stream, c->d.norb, n, wk,
                      c->trial[s], c->dets, s, g64 ? c->green[s] : nullptr,
                      gbf ? c->green_bf[s][0] : nullptr, gbf ? c->green_bf[s][1] : nullptr, c->Np32);
   LAUNCH_CHECK();

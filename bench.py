@@ -1,4 +1,5 @@
-"""Benchmark: walker-steps/s of the phaseless AFQMC step on one MI355X.
+"""Benchmark: walker-steps/s of the phaseless AFQMC step on one MI355X, mixed precision
+against the plain float64 GPU form at the same fields.
 
 Workload (default): synthetic closed-shell RHF, norb=100, nocc=20, nchol=400, 1024
 complex128 walkers, dt=0.005; a "step" is one phaseless propagation of all walkers
@@ -6,20 +7,26 @@ complex128 walkers, dt=0.005; a "step" is one phaseless propagation of all walke
 run in blocks of n_prop_steps=50 as sampler.propagate_phaseless does: auxiliary fields
 are drawn on the device per block, and each block ends with QR re-orthonormalisation,
 overlaps, local energies and block statistics; stochastic reconfiguration follows
-every 4 blocks (one 200-step SR block).  Block work is inside the timed region and
-amortised into the per-step figure.
+every 4 blocks.  Block work is inside the timed region and amortised into the per-step
+figure.
+
+Default mode runs the same trajectory twice from the same seed (same device fields,
+same SR draws): first in float64, then in the mixed mode (split-bf16 MFMA Cholesky
+contractions, float32 Taylor products, float64 overlaps / determinants / QR / weights /
+energies).  `value` is the mixed run's walker-steps/s; the line also carries the
+float64 rate, the speed-up and the largest block-energy difference |dE| between the
+two runs.  `--precision fp64|bf16x3` runs one mode only.
 
 Prints ONE JSON line.  `value` = walker-steps/s over the timed region (inputs already
-resident in HBM).  `roofline` describes the dominant kernel, timed with HIP events on
-the library's stream during the timed region.  `cpu_baseline` times the numpy
-restatement of the reference algorithm (the oracle) on a bounded sample on the host
-cores, one single-threaded process per core (the reference's own model: one MPI rank
-per core with one intra-op thread).
+resident in HBM).  `roofline` describes the dominant kernel of the reported run, timed
+with HIP events on the library's stream during the timed region.  `cpu_baseline` times
+the numpy restatement of the reference algorithm (the oracle) on a bounded sample on
+the host cores, one single-threaded process per core (the reference's own model: one
+MPI rank per core with one intra-op thread).
 """
 
 import argparse
 import json
-import math
 import multiprocessing as mp
 import os
 import sys
@@ -29,11 +36,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+# published MI355X dense peaks
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 HBM_PEAK_GBS = 8000.0
 
 N_PROP_STEPS = 50
 N_ENE_BLOCKS = 4
+
+# device kernel behind each timed phase (symbol prefix as rocprofv3 lists it)
+PHASE_KERNEL = {
+    "fp64": {"vhs_gemm": "gemm_nt_f64_kernel", "taylor": "taylor_reg_kernel",
+             "fb_gemm": "gemm_nt_f64_kernel", "overlap": "measure_kernel",
+             "energy": "energy_reduce_kernel"},
+    "bf16x3": {"vhs_gemm": "gemm_nt_bf16x3_kernel", "taylor": "taylor_reg_f32_kernel",
+               "fb_gemm": "gemm_nt_bf16x3_kernel", "overlap": "measure_kernel",
+               "energy": "energy_reduce_kernel"},
+}
 
 
 def flops_per_walker(norb, nelec, nchol, n_exp_terms, walker_type):
@@ -55,6 +73,36 @@ def flops_per_walker(norb, nelec, nchol, n_exp_terms, walker_type):
     return f
 
 
+def phase_peak(precision, phase, norb, nelec):
+    """Published peak of the arithmetic a phase runs on."""
+    if precision == "fp64":
+        return PEAK_TFLOPS["fp64"]
+    if phase in ("vhs_gemm", "fb_gemm"):
+        return PEAK_TFLOPS["bf16"]
+    # float32 Taylor kernel for N <= 128 and n <= 32 (the library's dispatch rule)
+    if phase == "taylor" and norb <= 128 and max(nelec) <= 32:
+        return PEAK_TFLOPS["fp32"]
+    return PEAK_TFLOPS["fp64"]
+
+
+def traffic_from_profiles(kernel, workload, precision):
+    """HBM bytes per launch of `kernel` from the PMC table in profiles/ (FETCH_SIZE and
+    WRITE_SIZE per dispatch, scaled by the copy-kernel factor), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 7 and cells[0] == kernel and cells[1] == workload \
+                    and cells[2] == precision:
+                try:
+                    return float(cells[6])
+                except ValueError:
+                    return None
+    return None
+
+
 # ------------------------------------------------------------------ CPU baseline
 
 
@@ -66,33 +114,36 @@ def _cpu_worker(args):
     from cs_afqmc_amd import problems
     from oracle import afqmc_oracle as o
 
-    hd, wd, nelec = problems.synthetic_rhf(norb, nocc, nchol, seed=seed)
-    oh = o.build_measurement_intermediates("rhf", hd, wd, norb)
-    oh = o.build_propagation_intermediates("rhf", oh, wd["rdm1"], norb, nelec, dt)
+    if walker_type == "rhf":
+        hd, wd, nelec = problems.synthetic_rhf(norb, nocc, nchol, seed=seed)
+    else:
+        hd, wd, nelec = problems.synthetic_uhf(norb, nocc, nocc, nchol, seed=seed)
+    oh = o.build_measurement_intermediates(walker_type, hd, wd, norb)
+    oh = o.build_propagation_intermediates(walker_type, oh, wd["rdm1"], norb, nelec, dt)
     rng = np.random.default_rng(seed + 17)
-    walkers = o.get_init_walkers(wd["rdm1"], nelec, n_walk, restricted=True)
+    walkers = o.get_init_walkers(wd["rdm1"], nelec, n_walk, restricted=walker_type == "rhf")
     prop = {"walkers": walkers, "weights": np.ones(n_walk), "e_estimate": 0.0,
             "pop_control_ene_shift": 0.0,
-            "overlaps": o.calc_overlap("rhf", walkers, wd)}
+            "overlaps": o.calc_overlap(walker_type, walkers, wd)}
     steps = 0
     t_prop = 0.0
     while True:
         t0 = time.perf_counter()
         fields = rng.standard_normal((n_walk, nchol))
-        prop = o.propagate("rhf", "rhf", oh, prop, fields, wd, dt, 6, norb,
+        prop = o.propagate(walker_type, walker_type, oh, prop, fields, wd, dt, 6, norb,
                            n_walkers=n_walk)
         t_prop += time.perf_counter() - t0
         steps += 1
         if steps % N_PROP_STEPS == 0:  # keep the walkers orthonormal as the blocks do
-            prop["walkers"] = o.orthonormalize_walkers("rhf", prop["walkers"])
-            prop["overlaps"] = o.calc_overlap("rhf", prop["walkers"], wd)
+            prop["walkers"] = o.orthonormalize_walkers(walker_type, prop["walkers"])
+            prop["overlaps"] = o.calc_overlap(walker_type, prop["walkers"], wd)
         if t_prop > budget_s:
             break
     t_step = t_prop / steps
     t1 = time.perf_counter()
-    prop["walkers"] = o.orthonormalize_walkers("rhf", prop["walkers"])
-    o.calc_overlap("rhf", prop["walkers"], wd)
-    o.calc_energy("rhf", prop["walkers"], oh, wd)
+    prop["walkers"] = o.orthonormalize_walkers(walker_type, prop["walkers"])
+    o.calc_overlap(walker_type, prop["walkers"], wd)
+    o.calc_energy(walker_type, prop["walkers"], oh, wd)
     t_block = time.perf_counter() - t1
     return n_walk, steps, t_step, t_block
 
@@ -124,7 +175,7 @@ def cpu_baseline(workload, budget_s):
         rate += n_w / (t_step + t_block / N_PROP_STEPS)
     steps_done = sum(r[1] for r in res)
     return {
-        "value": rate,
+        "value": round(rate, 1),
         "unit": "walker-steps/s",
         "cores": procs,
         "kind": "port",
@@ -137,7 +188,18 @@ def cpu_baseline(workload, budget_s):
 # ------------------------------------------------------------------ GPU run
 
 
-def run(args):
+def copy_probe(nbytes=1 << 30):
+    """A plain device copy of known size, for scaling PMC byte counters."""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
+def run(args, precision):
     import numpy as np
     import torch
 
@@ -150,19 +212,17 @@ def run(args):
     if walker_type == "rhf":
         hd, wd, nelec = problems.synthetic_rhf(norb, nocc, nchol, seed=0)
         trial = wavefunctions.rhf(norb, nelec)
-        prop = propagation.propagator_restricted(dt=dt, n_walkers=W, precision=args.precision)
+        prop = propagation.propagator_restricted(dt=dt, n_walkers=W, precision=precision)
     else:
         hd, wd, nelec = problems.synthetic_uhf(norb, nocc, nocc, nchol, seed=0)
         trial = wavefunctions.uhf(norb, nelec)
-        prop = propagation.propagator_unrestricted(dt=dt, n_walkers=W,
-                                                   precision=args.precision)
+        prop = propagation.propagator_unrestricted(dt=dt, n_walkers=W, precision=precision)
     ham = hamiltonian.hamiltonian(norb)
     ham_data = ham.build_measurement_intermediates(hd, trial, wd)
     ham_data = ham.build_propagation_intermediates(ham_data, prop, trial, wd)
     prop_data = prop.init_prop_data(trial, wd, ham_data)
     prop_data["key"] = rng.PRNGKey(0)
     dev = ham_data["_afqmc_device"]
-    smp = sampling.sampler(n_prop_steps=N_PROP_STEPS, n_ene_blocks=N_ENE_BLOCKS, n_sr_blocks=1)
 
     # device state as in sampler.propagate_phaseless
     ddev = dev.device
@@ -171,8 +231,9 @@ def run(args):
     prop_data["_scalars"] = scalars
     prop_data["_energies"] = torch.empty(W, dtype=torch.complex128, device=ddev)
     fields_buf = torch.empty((N_PROP_STEPS, W, nchol), dtype=torch.float64, device=ddev)
-    stats = torch.zeros((max(1, math.ceil(args.steps / N_PROP_STEPS)) + 64, 3),
-                        dtype=torch.float64, device=ddev)
+    total_steps = args.warmup + args.steps
+    stats = torch.zeros((total_steps // N_PROP_STEPS + 1, 3), dtype=torch.float64,
+                        device=ddev)
 
     state = {"block": 0, "in_block": 0, "blocks_since_sr": 0, "fields": None}
 
@@ -192,8 +253,7 @@ def run(args):
             dev.orthonormalize(prop_data["walkers"])
             dev.overlap(prop_data["walkers"], out=prop_data["overlaps"])
             energies = dev.energy(prop_data["walkers"], out=prop_data["_energies"])
-            slot = state["block"] % stats.shape[0]
-            dev.block_stats(energies, prop_data["weights"], scalars, stats[slot])
+            dev.block_stats(energies, prop_data["weights"], scalars, stats[state["block"]])
             state["block"] += 1
             state["blocks_since_sr"] += 1
             if state["blocks_since_sr"] == N_ENE_BLOCKS:
@@ -204,7 +264,9 @@ def run(args):
     for _ in range(args.warmup):
         one_step()
     torch.cuda.synchronize()
-    print(f"# warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+    print(f"# {precision}: warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+    if args.copy_probe:
+        copy_probe()
 
     dev.prof_collect()  # drop anything recorded before
     dev.prof_enable(True)
@@ -216,64 +278,62 @@ def run(args):
     elapsed = time.perf_counter() - t0
     dev.prof_enable(False)
     phases = dev.prof_collect()
-    print(f"# timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    print(f"# {precision}: timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr,
+          flush=True)
 
-    # energy sanity (mixed estimator of the timed blocks)
-    nb = min(state["block"], stats.shape[0])
+    nb = state["block"]
     st = stats[:nb].cpu().numpy() if nb else np.zeros((0, 3))
     e_mixed = float(np.sum(st[:, 0] * st[:, 1]) / np.sum(st[:, 1])) if nb else float("nan")
 
-    walker_steps = W * args.steps
-    value = walker_steps / elapsed
+    value = W * args.steps / elapsed
     fl = flops_per_walker(norb, nelec, nchol, prop.n_exp_terms, walker_type)
-    # dominant kernel = phase with the largest device time in the timed region
     per_phase = {p: {"ms_total": round(ms, 4), "launches": n,
                      "ms_avg": round(ms / n, 5) if n else None}
                  for p, (ms, n) in phases.items() if n}
+    # dominant kernel = phase with the largest device time in the timed region
     dom = max((p for p in per_phase if p in fl), key=lambda p: phases[p][0])
     avg_s = phases[dom][0] / phases[dom][1] / 1e3
     flop_launch = fl[dom] * W
     achieved = flop_launch / avg_s / 1e12
-    peak = PEAK_TFLOPS[args.precision]
+    peak = phase_peak(precision, dom, norb, nelec)
+    kernel = PHASE_KERNEL.get(precision, {}).get(dom, dom)
     roofline = {
-        "kernel": dom,
+        "phase": dom,
+        "kernel": kernel,
         "bound": "mfma",
         "achieved": round(achieved, 3),
         "peak": peak,
         "unit": "TFLOP/s",
         "frac": round(achieved / peak, 4),
-        "traffic": None,
+        "traffic": traffic_from_profiles(kernel, args.workload, precision),
         "flop_per_launch": flop_launch,
         "avg_launch_ms": round(avg_s * 1e3, 5),
     }
     step_flops = sum(v for k, v in fl.items() if k not in ("energy", "qr")) + (
         fl["energy"] + fl["qr"]) / N_PROP_STEPS
-    out = {
-        "metric": "walker-steps/s (phaseless AFQMC step, one MI355X)",
-        "value": round(value, 1),
-        "unit": "walker-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else args.precision,
-        "data": "synthetic (seeded Cholesky vectors, h1, Haar trial; device Philox fields)",
+    res = {
+        "value": value,
+        "elapsed": elapsed,
+        "block_energies": st[:, 0].tolist(),
+        "block_weights": st[:, 1].tolist(),
+        "mixed_energy": e_mixed,
+        "phases": per_phase,
+        "roofline": roofline,
+        "achieved_tflops_step": round(step_flops * value / 1e12, 3),
         "config": {
             "workload": args.workload,
             "walker_type": walker_type, "norb": norb, "nocc": nocc, "nchol": nchol,
             "n_walkers": W, "dt": dt, "n_exp_terms": prop.n_exp_terms,
             "n_prop_steps": N_PROP_STEPS, "n_ene_blocks": N_ENE_BLOCKS,
-            "precision": args.precision,
         },
-        "achieved_tflops_step": round(step_flops * value / 1e12, 3),
-        "mixed_energy": e_mixed,
-        "phases": per_phase,
-        "roofline": roofline,
     }
-    return out, workload
+    del dev, ham_data, prop_data, fields_buf
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return res, workload
+
+
+DTYPE = {"fp64": "f64", "bf16x3": "mixed: bf16x3 MFMA contractions, f32 Taylor, f64 rest"}
 
 
 def main():
@@ -282,20 +342,60 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="rhf_n100_o20_g400_w1024")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--precision", default="compare", choices=["compare", "fp64", "bf16x3"],
+                    help="compare: float64 then mixed at the same fields (default)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 1 GiB device copy before the timed region (PMC scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs the walker population on one GPU; --gpus {args.gpus} "
               "is not supported (multi-GPU sharding is a design note only).", file=sys.stderr)
         sys.exit(2)
-    out, workload = run(args)
+    modes = ["fp64", "bf16x3"] if args.precision == "compare" else [args.precision]
+    cpu = None
     if not args.no_cpu_baseline:
+        # before anything touches the GPU: the worker processes are started by spawn
+        from cs_afqmc_amd import problems
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(workload, args.cpu_seconds)
-    else:
-        out["cpu_baseline"] = None
+        cpu = cpu_baseline(problems.WORKLOADS[args.workload], args.cpu_seconds)
+    runs = {}
+    for m in modes:
+        runs[m], _ = run(args, m)
+    main_mode = modes[-1]
+    r = runs[main_mode]
+    out = {
+        "metric": "walker-steps/s mixed vs FP64 GPU at |dE|<0.1 mEh (phaseless AFQMC step)",
+        "value": round(r["value"], 1),
+        "unit": "walker-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r["elapsed"] / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE[main_mode],
+        "data": "synthetic (seeded Cholesky vectors, h1, Haar trial; device Philox fields)",
+        "config": dict(r["config"], precision=main_mode),
+    }
+    if "fp64" in runs and "bf16x3" in runs:
+        b64 = runs["fp64"]["block_energies"]
+        bmx = runs["bf16x3"]["block_energies"]
+        d = [abs(a - b) for a, b in zip(bmx, b64)]
+        out["fp64_value"] = round(runs["fp64"]["value"], 1)
+        out["speedup_vs_fp64"] = round(runs["bf16x3"]["value"] / runs["fp64"]["value"], 3)
+        out["delta_e_max_eh"] = max(d) if d else None
+        out["delta_e_mean_eh"] = abs(runs["bf16x3"]["mixed_energy"] - runs["fp64"]["mixed_energy"])
+        out["fp64_phases"] = runs["fp64"]["phases"]
+        out["fp64_roofline"] = runs["fp64"]["roofline"]
+    out["mixed_energy"] = r["mixed_energy"]
+    out["block_energies"] = r["block_energies"]
+    out["achieved_tflops_step"] = r["achieved_tflops_step"]
+    out["phases"] = r["phases"]
+    out["roofline"] = r["roofline"]
+    out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
 

@@ -12,6 +12,7 @@ from . import _build
 AFQMC_RHF = 1
 AFQMC_UHF = 2
 AFQMC_STEP_REUSE_GREEN = 1
+AFQMC_SR_GLOBAL_ORDER = 1
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 
 # every symbol declared in include/cs_afqmc_amd.h
@@ -91,7 +92,7 @@ def _declare(lib):
     lib.afqmc_force_bias.argtypes = [_vp, _i32, _vp, _vp, _vp]
     lib.afqmc_energy.argtypes = [_vp, _i32, _vp, _vp, _vp]
     lib.afqmc_orthonormalize.argtypes = [_vp, _i32, _vp, _vp]
-    lib.afqmc_sr.argtypes = [_vp, _i32, _vp, _vp, _vp, ctypes.c_double, _vp]
+    lib.afqmc_sr.argtypes = [_vp, _i32, _vp, _vp, _vp, ctypes.c_double, _vp, _i32]
     lib.afqmc_block_stats.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp]
     lib.afqmc_prof_enable.argtypes = [_vp, _i32]
     lib.afqmc_prof_collect.argtypes = [_vp, _vp, _vp]

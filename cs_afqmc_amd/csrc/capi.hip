@@ -442,7 +442,7 @@ extern "C" int afqmc_orthonormalize(afqmc_ctx* c, int32_t W, void* walkers_up,
 }
 
 extern "C" int afqmc_sr(afqmc_ctx* c, int32_t W, void* walkers_up, void* walkers_dn,
-                        double* weights, double zeta, int32_t* idx_out) {
+                        double* weights, double zeta, int32_t* idx_out, int32_t flags) {
   int rc = check_w(c, W);
   if (rc) return rc;
   if ((rc = check_walkers(c, walkers_up, walkers_dn))) return rc;
@@ -452,8 +452,9 @@ extern "C" int afqmc_sr(afqmc_ctx* c, int32_t W, void* walkers_up, void* walkers
   if ((rc = set_device(c))) return rc;
   c->green_src = nullptr;
   PhaseTimer t(c, AFQMC_PH_SR);
+  if (flags & ~AFQMC_SR_GLOBAL_ORDER) return afqmc_set_error(AFQMC_ERR_ARG, "unknown sr flags");
   return launch_sr(c, W, static_cast<cplx*>(walkers_up), static_cast<cplx*>(walkers_dn), weights,
-                   zeta, idx_out);
+                   zeta, idx_out, (flags & AFQMC_SR_GLOBAL_ORDER) != 0);
 }
 
 extern "C" int afqmc_fields_normal(afqmc_ctx* c, uint64_t seed, uint64_t counter,

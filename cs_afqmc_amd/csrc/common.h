@@ -218,6 +218,6 @@ int launch_energy(afqmc_ctx* c, int W, cplx* energies);
 int launch_qr(afqmc_ctx* c, int W, cplx* wup, cplx* wdn);
 // sr.hip
 int launch_sr(afqmc_ctx* c, int W, cplx* wup, cplx* wdn, double* weights, double zeta,
-              int32_t* idx_out);
+              int32_t* idx_out, bool global_order);
 int launch_fields_normal(afqmc_ctx* c, uint64_t seed, uint64_t counter, double* out,
                          int64_t count);

@@ -1,7 +1,8 @@
 // Stochastic reconfiguration (sr.py:24-48) and the auxiliary-field generator.
 //
 // SR: c = cumsum |w| (sequential, left to right, as numpy.cumsum), total = c[W-1],
-// z_i = total * (i + zeta) / W, idx_i = searchsorted(c, z_i, side='left') clamped to
+// z_i = total * (i + zeta) / W (or ((i + zeta) / W) * total, the order of the
+// gather/scatter variant), idx_i = searchsorted(c, z_i, side='left') clamped to
 // W - 1 (jax gathers clamp out-of-range indices), walkers <- walkers[idx],
 // weights <- total / W.  Indices are integers, so they are reproduced bit for bit.
 //
@@ -13,26 +14,39 @@
 namespace {
 
 constexpr int SR_NT = 1024;
+constexpr int SR_CHUNK = 4096;  // |w| staged through LDS in chunks of this many
 
-__global__ __launch_bounds__(SR_NT) void sr_index_kernel(int W, double zeta,
+// One workgroup.  The running sum stays sequential (left to right, one thread) so that
+// the cumulative weights -- and hence the integer indices -- are those of numpy.cumsum;
+// the other threads only stage |w| into LDS ahead of it and run the searches.
+// global_order selects z_i = ((i + zeta) / W) * total (the gather/scatter form of
+// sr.py:66-106 on one rank) instead of z_i = total * (i + zeta) / W (sr.py:24-34).
+__global__ __launch_bounds__(SR_NT) void sr_index_kernel(int W, double zeta, int global_order,
                                                          double* __restrict__ weights,
                                                          double* __restrict__ cum,
                                                          int32_t* __restrict__ idx,
                                                          int32_t* __restrict__ idx_out) {
+  __shared__ double absw[SR_CHUNK];
   __shared__ double total_s;
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < W; ++i) {
-      s += fabs(weights[i]);
-      cum[i] = s;
+  double run = 0.0;
+  for (int base = 0; base < W; base += SR_CHUNK) {
+    const int len = W - base < SR_CHUNK ? W - base : SR_CHUNK;
+    for (int i = threadIdx.x; i < len; i += SR_NT) absw[i] = fabs(weights[base + i]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int i = 0; i < len; ++i) {
+        run += absw[i];
+        cum[base + i] = run;
+      }
+      total_s = run;
     }
-    total_s = s;
+    __syncthreads();
   }
-  __syncthreads();
   const double total = total_s;
   const double avg = total / (double)W;
   for (int i = threadIdx.x; i < W; i += SR_NT) {
-    const double z = total * ((double)i + zeta) / (double)W;
+    const double z = global_order ? (((double)i + zeta) / (double)W) * total
+                                  : total * ((double)i + zeta) / (double)W;
     // first j with cum[j] >= z
     int lo = 0, hi = W;
     while (lo < hi) {
@@ -106,9 +120,9 @@ __global__ void normal_kernel(uint64_t seed, uint64_t counter, double* __restric
 }  // namespace
 
 int launch_sr(afqmc_ctx* c, int W, cplx* wup, cplx* wdn, double* weights, double zeta,
-              int32_t* idx_out) {
-  hipLaunchKernelGGL(sr_index_kernel, dim3(1), dim3(SR_NT), 0, c->stream, W, zeta, weights,
-                     c->sr_cum, c->sr_idx, idx_out);
+              int32_t* idx_out, bool global_order) {
+  hipLaunchKernelGGL(sr_index_kernel, dim3(1), dim3(SR_NT), 0, c->stream, W, zeta,
+                     global_order ? 1 : 0, weights, c->sr_cum, c->sr_idx, idx_out);
   LAUNCH_CHECK();
   const int N = c->d.norb;
   for (int s = 0; s < c->nspin; ++s) {

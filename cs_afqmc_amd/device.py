@@ -181,14 +181,17 @@ class DeviceStep:
         _lib.check(self._lib.afqmc_orthonormalize(self._h, W, _dptr(up), _dptr(dn)))
         return walkers
 
-    def sr(self, walkers, weights, zeta, idx_out=None):
+    def sr(self, walkers, weights, zeta, idx_out=None, global_order=False):
+        """Systematic resampling in place; global_order selects the comb-point rounding
+        of the gather/scatter variant (sr.py:66-106)."""
         up, dn, W = self._walkers(walkers)
         self._vec(weights, W, torch.float64, "weights")
         if idx_out is not None:
             self._vec(idx_out, W, torch.int32, "idx_out")
         self._prep()
+        flags = _lib.AFQMC_SR_GLOBAL_ORDER if global_order else 0
         _lib.check(self._lib.afqmc_sr(self._h, W, _dptr(up), _dptr(dn), _dptr(weights),
-                                      float(zeta), _dptr(idx_out)))
+                                      float(zeta), _dptr(idx_out), flags))
         return idx_out
 
     def fields_normal(self, seed, counter, shape, out=None):

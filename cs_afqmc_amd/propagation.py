@@ -153,6 +153,17 @@ class propagator_restricted:
         _device_of(prop_data).sr(prop_data["walkers"], prop_data["weights"], zeta)
         return prop_data
 
+    def stochastic_reconfiguration_global(self, prop_data, comm=None):
+        """propagation.py:243-249 -> sr.py:66-106 (and :109-156 for two spins): the
+        gather / comb / scatter form.  The whole population lives on this device, so
+        the gather and scatter are the identity and the comb runs on the device with
+        that variant's rounding of the comb points.  `comm` is ignored."""
+        prop_data["key"], subkey = rng.split(prop_data["key"])
+        zeta = rng.uniform(subkey)
+        _device_of(prop_data).sr(prop_data["walkers"], prop_data["weights"], zeta,
+                                 global_order=True)
+        return prop_data
+
     def __hash__(self):
         return hash(tuple(self.__dict__.values()))
 

@@ -138,9 +138,14 @@ AFQMC_API int afqmc_orthonormalize(afqmc_ctx* ctx, int32_t n_walkers, void* walk
 
 /* stochastic_reconfiguration (sr.py:24-34 / :37-48): systematic resampling with
  * the uniform draw zeta.  walkers gathered in place, weights <- sum|w| / W.
- * idx_out dev (W) int32 or NULL: the resampled indices. */
+ * idx_out dev (W) int32 or NULL: the resampled indices.
+ * flags: 0, or AFQMC_SR_GLOBAL_ORDER for the gather/scatter form that
+ * propagator.stochastic_reconfiguration_global uses (propagation.py:243-249 ->
+ * sr.py:66-106, here with the whole population on one device), which forms the
+ * comb points as ((i + zeta) / W) * total instead of total * (i + zeta) / W. */
+enum { AFQMC_SR_GLOBAL_ORDER = 1 };
 AFQMC_API int afqmc_sr(afqmc_ctx* ctx, int32_t n_walkers, void* walkers_up, void* walkers_dn,
-             double* weights, double zeta, int32_t* idx_out);
+             double* weights, double zeta, int32_t* idx_out, int32_t flags);
 
 /* Block statistics of sampler._block_scan (sampling.py:74-91), on the device:
  *   energies dev (W) complex128 local energies; the real part is clamped to

@@ -396,10 +396,28 @@ def sr_indices(weights, zeta):
     return np.minimum(idx, nwalkers - 1).astype(np.int32), total_weight
 
 
-def stochastic_reconfiguration(walker_type, walkers, weights, zeta):
-    """sr.py:24-34 (restricted) and sr.py:37-48 (unrestricted)."""
+def sr_indices_global(weights, zeta, size=1):
+    """sr.py:66-106 (stochastic_reconfiguration_mpi) index part on the gathered weights
+    of `size` equal ranks: z = (i + zeta) / nwalkers / size, new_i =
+    searchsorted(cumsum |w|, z * total), one walker at a time."""
+    nwalkers = weights.shape[0] // size
+    cumulative_weights = np.cumsum(np.abs(weights))
+    total_weight = cumulative_weights[-1]
+    idx = np.empty(nwalkers * size, dtype=np.int32)
+    for i in range(nwalkers * size):
+        z = (i + zeta) / nwalkers / size
+        idx[i] = np.searchsorted(cumulative_weights, z * total_weight)
+    return idx, total_weight
+
+
+def stochastic_reconfiguration(walker_type, walkers, weights, zeta, global_order=False):
+    """sr.py:24-34 (restricted) and sr.py:37-48 (unrestricted); global_order: the
+    gather/scatter variant sr.py:66-106 / :109-156 on one rank."""
     nwalkers = weights.shape[0]
-    idx, total_weight = sr_indices(weights, zeta)
+    if global_order:
+        idx, total_weight = sr_indices_global(weights, zeta)
+    else:
+        idx, total_weight = sr_indices(weights, zeta)
     new_weights = np.ones(nwalkers) * (total_weight / nwalkers)
     if walker_type == "rhf":
         return walkers[idx], new_weights, idx

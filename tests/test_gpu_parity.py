@@ -185,7 +185,8 @@ def test_orthonormalize(case):
         assert rel(got[0], ref[0]) < 1e-12 and rel(got[1], ref[1]) < 1e-12
 
 
-def test_sr_indices_bit_exact(case):
+@pytest.mark.parametrize("global_order", [False, True], ids=["local", "global"])
+def test_sr_indices_bit_exact(case, global_order):
     rng = np.random.default_rng(9)
     weights = rng.random(case.W) * (rng.random(case.W) > 0.3)
     weights[0] = 0.7
@@ -194,9 +195,10 @@ def test_sr_indices_bit_exact(case):
     walkers = case.walkers()
     w_t = to_dev(weights, case.dev)
     idx = torch.empty(case.W, dtype=torch.int32, device=case.dev)
-    dev_ctx.sr(walkers, w_t, zeta, idx_out=idx)
+    dev_ctx.sr(walkers, w_t, zeta, idx_out=idx, global_order=global_order)
     ref_walkers, ref_w, ref_idx = o.stochastic_reconfiguration(case.wt, case.walkers_np,
-                                                               weights, zeta)
+                                                               weights, zeta,
+                                                               global_order=global_order)
     assert np.array_equal(to_np(idx), ref_idx)
     got = to_np(walkers)
     if case.wt == "rhf":

@@ -237,14 +237,17 @@ def test_sr_systematic_resampling_counts():
         if w.sum() == 0:
             w[0] = 1.0
         zeta = rng.random()
-        idx, total = o.sr_indices(w, zeta)
-        assert np.all(np.diff(idx) >= 0)
-        counts = np.bincount(idx, minlength=W)
-        assert counts.sum() == W
-        expect = W * np.abs(w) / np.abs(w).sum()
-        assert np.all(counts >= np.floor(expect) - 1e-9)
-        assert np.all(counts <= np.ceil(expect) + 1e-9)
-        assert np.all(counts[w == 0] == 0)
+        for fn in (o.sr_indices, o.sr_indices_global):
+            idx, total = fn(w, zeta)
+            assert np.all(np.diff(idx) >= 0)
+            counts = np.bincount(idx, minlength=W)
+            assert counts.sum() == W
+            expect = W * np.abs(w) / np.abs(w).sum()
+            assert np.all(counts >= np.floor(expect) - 1e-9)
+            assert np.all(counts <= np.ceil(expect) + 1e-9)
+            assert np.all(counts[w == 0] == 0)
+        # the two comb-point roundings agree away from ties
+        assert np.array_equal(o.sr_indices(w, zeta)[0], o.sr_indices_global(w, zeta)[0])
 
 
 def test_propagation_intermediates_by_loops():

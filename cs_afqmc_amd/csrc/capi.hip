@@ -57,7 +57,7 @@ void free_ctx(afqmc_ctx* c) {
                   c->chol_t_f32, c->rot_chol_f32[0], c->rot_chol_f32[1], c->chol_t_bf[0],
                   c->chol_t_bf[1], c->rot_chol_bf[0][0], c->rot_chol_bf[0][1],
                   c->rot_chol_bf[1][0], c->rot_chol_bf[1][1], c->green[0], c->green[1], c->fb,
-                  c->xs, c->vhs, c->terms, c->dets, c->energy_f, c->energy_acc, c->walk_tmp,
+                  c->xs, c->vhs, c->terms, c->dets, c->energy_x, c->energy_acc, c->walk_tmp,
                   c->sr_cum, c->sr_idx, c->xs_bf[0], c->xs_bf[1], c->green_bf[0][0],
                   c->green_bf[0][1], c->green_bf[1][0], c->green_bf[1][1], c->vhs_f32,
                   c->exp_h1_f32[0], c->exp_h1_f32[1]};
@@ -92,14 +92,15 @@ int set_device(const afqmc_ctx* c) {
 
 // fb[w][c][g] = mult * sum_s green_s[w][c][:] . rot_chol_s[g][:], as c->fb_parts split-K
 // slices of (2W, G) that the consumers add in slice order
-int force_bias_gemm(afqmc_ctx* c, int W) {
+// bf16: split-bf16 planes of the mixed mode; otherwise the float64 Green's function
+int force_bias_gemm(afqmc_ctx* c, int W, bool bf16) {
   const int G = c->d.nchol;
   const int64_t slice = (int64_t)2 * W * G;
   int part = 0;
   for (int s = 0; s < c->nspin; ++s) {
     int rc;
     int sp;
-    if (c->mixed) {
+    if (bf16) {
       // split-bf16: rows (w, re/im) of the padded Green planes, K = n_s * Np32
       const int kd = c->nocc[s] * c->Np32;
       sp = gemm_f64_splits(2 * W, G, kd);
@@ -248,14 +249,10 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
     TRY(dalloc(c, &c->terms, (size_t)W * 2));
     TRY(dalloc(c, &c->dets, (size_t)W * 2));
     {
-      // energy f_g chunk: per walker G * n_s * 2 n_s doubles per spin; cap at 512 MiB
-      size_t per_w = 0;
-      for (int s = 0; s < c->nspin; ++s) per_w += (size_t)G * c->nocc[s] * 2 * c->nocc[s];
-      size_t wc = (size_t)(512u << 20) / (per_w * sizeof(double));
-      if (wc < 1) wc = 1;
-      if (wc > (size_t)W) wc = W;
-      c->energy_wchunk = (int)wc;
-      TRY(dalloc(c, &c->energy_f, per_w * wc));
+      // exchange partial sums of the local energy: one complex per (g-tile, walker)
+      size_t tiles = 0;
+      for (int s = 0; s < c->nspin; ++s) tiles += (size_t)energy_g_tiles(c, s);
+      TRY(dalloc(c, &c->energy_x, tiles * W));
     }
     TRY(dalloc(c, &c->walk_tmp, (size_t)W * N * (c->nocc[0] > c->nocc[1] ? c->nocc[0] : c->nocc[1])));
     TRY(dalloc(c, &c->sr_cum, (size_t)W));
@@ -310,7 +307,7 @@ extern "C" int afqmc_propagate(afqmc_ctx* c, int32_t W, void* walkers_up, void* 
   }
   {
     PhaseTimer t(c, AFQMC_PH_FB_GEMM);
-    if ((rc = force_bias_gemm(c, W))) return rc;
+    if ((rc = force_bias_gemm(c, W, c->mixed))) return rc;
   }
   {
     PhaseTimer t(c, AFQMC_PH_FIELDS);
@@ -410,7 +407,7 @@ extern "C" int afqmc_force_bias(afqmc_ctx* c, int32_t W, const void* walkers_up,
   if ((rc = launch_measure(c, W, static_cast<const cplx*>(walkers_up),
                            static_cast<const cplx*>(walkers_dn), !c->mixed, c->mixed)))
     return rc;
-  if ((rc = force_bias_gemm(c, W))) return rc;
+  if ((rc = force_bias_gemm(c, W, c->mixed))) return rc;
   return launch_fb_out(c, W, static_cast<cplx*>(force_bias));
 }
 
@@ -422,11 +419,14 @@ extern "C" int afqmc_energy(afqmc_ctx* c, int32_t W, const void* walkers_up,
   if (!energies) return afqmc_set_error(AFQMC_ERR_ARG, "null energies");
   if ((rc = set_device(c))) return rc;
   c->green_src = nullptr;
-  // local energies are contracted in float64 in every precision mode
+  // Green's function in float64 (one-body term, c_g), plus split-bf16 planes for the
+  // exchange contraction of the mixed mode
   if ((rc = launch_measure(c, W, static_cast<const cplx*>(walkers_up),
-                           static_cast<const cplx*>(walkers_dn), true, false)))
+                           static_cast<const cplx*>(walkers_dn), true, c->mixed)))
     return rc;
   PhaseTimer t(c, AFQMC_PH_ENERGY);
+  // c_g = force bias, always contracted in float64
+  if ((rc = force_bias_gemm(c, W, false))) return rc;
   return launch_energy(c, W, static_cast<cplx*>(energies));
 }
 

@@ -117,12 +117,11 @@ struct afqmc_ctx {
   double* vhs;           // (W, 2, N*N)
   cplx* terms;           // (W, 2): shift_term, fb_term
   cplx* dets;            // (W, 2): det(C_sigma^H w_sigma)
-  double* energy_f;      // (G*n, Wc*2*n) chunk of f_g matrices
+  cplx* energy_x;        // (sum_s g_tiles_s, W) exchange partial sums
   cplx* energy_acc;      // (W): partial energies
   cplx* walk_tmp;        // gather target for SR (W * N * n_max)
   double* sr_cum;        // (W) cumulative |w|
   int32_t* sr_idx;       // (W)
-  int energy_wchunk;
   int64_t device_bytes;
   // the Green's function in `green` belongs to these walkers (set by afqmc_propagate)
   const void* green_src;
@@ -214,6 +213,7 @@ int launch_block_stats(afqmc_ctx* c, int W, const cplx* energies, const double* 
                        double* scalars, double* out);
 // energy.hip
 int launch_energy(afqmc_ctx* c, int W, cplx* energies);
+int energy_g_tiles(const afqmc_ctx* c, int s);
 // qr.hip
 int launch_qr(afqmc_ctx* c, int W, cplx* wup, cplx* wdn);
 // sr.hip

@@ -5,109 +5,412 @@
 // with m_s = 2 for restricted walkers (one spin standing for both) and 1 otherwise;
 // for m = 2 this is 2 sum c^2 - exc + 2 sum G rot_h1 + h0, the restricted formula.
 //
-// The f_g for a chunk of walkers come from one FP64 MFMA GEMM per spin:
-//   F[(g,i)][(w,c,k)] = sum_p rot_chol[g][i][p] * green[w][c][k][p]
-// (rows of the split Green buffer are (w, re/im, k) with stride N), then a reduction
-// kernel forms c_g, x_g and the energy per walker.
+// c_g equals the force bias of the walker (fb_g = sum_s m_s sum_ip rot_chol_s[g,i,p]
+// G_s[i,p], wavefunctions.py:503-511 / :694-709), so it comes from the float64
+// force-bias GEMM.  The exchange sums x_g need every f_g: a fused kernel forms a tile
+// of F[(g,i)][(w,c,j)] = sum_p rot_chol[g][i][p] green[w][c][j][p] on MFMA (float64, or
+// split-bf16 with float32 accumulation in the mixed mode), holding whole (g, w) blocks
+// -- g_per_tile = BM / n rows of g and w_per_tile = BN / 2n walkers -- so the tile is
+// reduced to sum_g x_g per walker in LDS and F never reaches HBM.  Per-tile partial
+// sums are added in tile order by the finishing kernel, so results are deterministic.
+// The tile holds whole (g, w) blocks for n <= 64 occupied orbitals per spin, the
+// library's limit (AFQMC_MAX_NOCC).
 #include "common.h"
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
 namespace {
 
 constexpr int ENER_NT = 256;
 
-// one workgroup per walker of the chunk; wave v handles g = v, v+4, ...
-__global__ __launch_bounds__(ENER_NT) void energy_reduce_kernel(
-    int G, int N, int nspin, int mult, int n0, int n1, int wc, const double* __restrict__ F0,
-    const double* __restrict__ F1, const double* __restrict__ green0,
-    const double* __restrict__ green1, const double* __restrict__ rot_h1_0,
-    const double* __restrict__ rot_h1_1, double h0, int w_base, cplx* __restrict__ energies) {
-  __shared__ double red[ENER_NT / 64];
-  const int wl = blockIdx.x;  // walker within chunk
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int ns[2] = {n0, n1};
-  const double* Fs[2] = {F0, F1};
-  const double* gs[2] = {green0, green1};
-  const double* hs[2] = {rot_h1_0, rot_h1_1};
+// ---------------------------------------------------------------- fused exchange
 
-  // two-body part
-  double e2r = 0.0, e2i = 0.0;  // accumulated in lane 0 of each wave
-  for (int g = wave; g < G; g += ENER_NT / 64) {
-    double cr = 0.0, ci = 0.0, xr = 0.0, xi = 0.0;
-    for (int s = 0; s < nspin; ++s) {
-      const int n = ns[s];
-      const int64_t ldc = (int64_t)wc * 2 * n;
-      const double* base = Fs[s] + (int64_t)g * n * ldc + (int64_t)wl * 2 * n;
-      double tr = 0.0, ti = 0.0, ar = 0.0, ai = 0.0;
-      for (int e = lane; e < n * n; e += 64) {
-        const int i = e / n, j = e - (e / n) * n;
-        const double fr = base[i * ldc + j], fi = base[i * ldc + n + j];
-        const double tr2 = base[j * ldc + i], ti2 = base[j * ldc + n + i];
-        ar += fr * tr2 - fi * ti2;
-        ai += fr * ti2 + fi * tr2;
-        if (i == j) {
-          tr += fr;
-          ti += fi;
-        }
-      }
-      cr += mult * tr;
-      ci += mult * ti;
-      xr += mult * ar;
-      xi += mult * ai;
+// Exchange sums of the (g, w) blocks of an F tile held in LDS (row stride ldf):
+// F row gl*n + i, column wl*2n + c*n + j.  Wave v takes walkers wl = v, v+4, ...; its
+// lanes run over (gl, i, j); the sum over the tile's g is written to xpart[w].
+template <typename T>
+__device__ inline void exchange_tile(const T* __restrict__ Fs, int ldf, int n, int g_used,
+                                     int w_used, int w0, cplx* __restrict__ xpart) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nn = n * n;
+  const int tot = g_used * nn;
+  for (int wl = wave; wl < w_used; wl += ENER_NT / 64) {
+    const T* base = Fs + wl * 2 * n;
+    double xr = 0.0, xi = 0.0;
+    for (int e = lane; e < tot; e += 64) {
+      const int gl = e / nn;
+      const int rem = e - gl * nn;
+      const int i = rem / n, j = rem - (rem / n) * n;
+      const T* rg = base + gl * n * ldf;
+      const double ar = (double)rg[i * ldf + j], ai = (double)rg[i * ldf + n + j];
+      const double br = (double)rg[j * ldf + i], bi = (double)rg[j * ldf + n + i];
+      xr += ar * br - ai * bi;
+      xi += ar * bi + ai * br;
     }
     for (int off = 32; off > 0; off >>= 1) {
-      cr += __shfl_down(cr, off, 64);
-      ci += __shfl_down(ci, off, 64);
       xr += __shfl_down(xr, off, 64);
       xi += __shfl_down(xi, off, 64);
     }
-    if (lane == 0) {
-      e2r += (cr * cr - ci * ci - xr) / 2.0;
-      e2i += (2.0 * cr * ci - xi) / 2.0;
-    }
+    if (lane == 0) xpart[w0 + wl] = cplx{xr, xi};
   }
-  // one-body part: sum_s m_s sum_{k,p} G_s[k][p] rot_h1_s[k][p]
+}
+
+// float64 MFMA tile 64 x 128 (4 waves as 2 x 2, each 32 x 64), BK = 16, operands k-major
+// in LDS, double buffered.
+constexpr int EF_BM = 64, EF_BN = 128, EF_BK = 16, EF_PAD = 16;
+constexpr int EF_LDF = 129;  // F tile row stride (odd: transposed reads spread over banks)
+constexpr int EF_OPS_BYTES = 2 * EF_BK * (EF_BM + EF_PAD) * 8 + 2 * EF_BK * (EF_BN + EF_PAD) * 8;
+constexpr int EF_F_BYTES = EF_BM * EF_LDF * 8;
+constexpr int EF_LDS = EF_OPS_BYTES > EF_F_BYTES ? EF_OPS_BYTES : EF_F_BYTES;
+
+template <int VEC>
+__global__ __launch_bounds__(ENER_NT) void exchange_f64_kernel(
+    int G, int n, int N, int W, int gpt, int wpt, int g_tiles, const double* __restrict__ A,
+    const double* __restrict__ B, cplx* __restrict__ xpart) {
+  constexpr int BM = EF_BM, BN = EF_BN, BK = EF_BK;
+  constexpr int TM = BM / 2 / 16, TN = BN / 2 / 16;
+  constexpr int KT = BK / VEC, RSTEP = ENER_NT / KT, A_LD = BM / RSTEP, B_LD = BN / RSTEP;
+  __shared__ __attribute__((aligned(16))) unsigned char lds[EF_LDS];
+  typedef double AsT[BK][BM + EF_PAD];
+  typedef double BsT[BK][BN + EF_PAD];
+  AsT* As = reinterpret_cast<AsT*>(lds);
+  BsT* Bs = reinterpret_cast<BsT*>(lds + 2 * sizeof(AsT));
+  double* Fs = reinterpret_cast<double*>(lds);
+
+  const int gt = blockIdx.x % g_tiles, wt = blockIdx.x / g_tiles;
+  const int g0 = gt * gpt, w0 = wt * wpt;
+  const int M = G * n, NB = 2 * W * n;
+  const int m0 = g0 * n, n0 = w0 * 2 * n;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int lk = (t % KT) * VEC, lr = t / KT;
+
+  double ra[A_LD][VEC], rb[B_LD][VEC];
+  auto load_tile = [&](int k0) {
+    const int k = k0 + lk;
+    const bool kok = k < N;  // VEC == 2 needs N even (checked at launch)
+#pragma unroll
+    for (int i = 0; i < A_LD; ++i) {
+      const int r = m0 + lr + i * RSTEP;
+      if (VEC == 2) {
+        double2 v = make_double2(0.0, 0.0);
+        if (r < M && kok) v = *reinterpret_cast<const double2*>(A + (int64_t)r * N + k);
+        ra[i][0] = v.x;
+        ra[i][VEC - 1] = v.y;
+      } else {
+        ra[i][0] = (r < M && kok) ? A[(int64_t)r * N + k] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < B_LD; ++i) {
+      const int r = n0 + lr + i * RSTEP;
+      if (VEC == 2) {
+        double2 v = make_double2(0.0, 0.0);
+        if (r < NB && kok) v = *reinterpret_cast<const double2*>(B + (int64_t)r * N + k);
+        rb[i][0] = v.x;
+        rb[i][VEC - 1] = v.y;
+      } else {
+        rb[i][0] = (r < NB && kok) ? B[(int64_t)r * N + k] : 0.0;
+      }
+    }
+  };
+  auto store_tile = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < A_LD; ++i)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) As[buf][lk + v][lr + i * RSTEP] = ra[i][v];
+#pragma unroll
+    for (int i = 0; i < B_LD; ++i)
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) Bs[buf][lk + v][lr + i * RSTEP] = rb[i][v];
+  };
+
+  double4_t acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+
+  const int nk = (N + BK - 1) / BK;
+  load_tile(0);
+  store_tile(0);
+  __syncthreads();
+  const int am = wm * (BM / 2) + (lane & 15);
+  const int bn = wn * (BN / 2) + (lane & 15);
+  const int kq = lane >> 4;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load_tile((kt + 1) * BK);
+#pragma unroll
+    for (int ks = 0; ks < BK / 4; ++ks) {
+      double af[TM], bf[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) af[i] = As[cur][ks * 4 + kq][am + i * 16];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bf[j] = Bs[cur][ks * 4 + kq][bn + j * 16];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+    if (kt + 1 < nk) store_tile(cur ^ 1);
+    __syncthreads();
+  }
+  // F tile -> LDS (the operand buffers are dead after the last barrier)
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int col = wn * (BN / 2) + j * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = wm * (BM / 2) + i * 16 + (lane >> 4) + 4 * r;
+        Fs[row * EF_LDF + col] = acc[i][j][r];
+      }
+    }
+  __syncthreads();
+  const int g_used = (G - g0 < gpt) ? G - g0 : gpt;
+  const int w_used = (W - w0 < wpt) ? W - w0 : wpt;
+  exchange_tile<double>(Fs, EF_LDF, n, g_used, w_used, w0, xpart + (int64_t)gt * W);
+}
+
+// split-bf16 tile 128 x 128 on v_mfma_f32_16x16x32_bf16 (4 waves 2 x 2, each 64 x 64),
+// BK = 32; operand planes have K = Np32 (zero padded), so there is no k tail.
+constexpr int EB_BM = 128, EB_BN = 128, EB_BK = 32, EB_LDSW = EB_BK + 8;
+constexpr int EB_LDF = 129;
+constexpr int EB_OPS_BYTES = 2 * 2 * EB_BM * EB_LDSW * 2 + 2 * 2 * EB_BN * EB_LDSW * 2;
+constexpr int EB_F_BYTES = EB_BM * EB_LDF * 4;
+constexpr int EB_LDS = EB_OPS_BYTES > EB_F_BYTES ? EB_OPS_BYTES : EB_F_BYTES;
+
+__global__ __launch_bounds__(ENER_NT) void exchange_bf16x3_kernel(
+    int G, int n, int K, int W, int gpt, int wpt, int g_tiles, const uint16_t* __restrict__ Ah,
+    const uint16_t* __restrict__ Al, const uint16_t* __restrict__ Bh,
+    const uint16_t* __restrict__ Bl, cplx* __restrict__ xpart) {
+  constexpr int BM = EB_BM, BN = EB_BN, LDSW = EB_LDSW;
+  constexpr int TM = BM / 2 / 16, TN = BN / 2 / 16;
+  constexpr int A_LD = BM * 4 / ENER_NT, B_LD = BN * 4 / ENER_NT;
+  __shared__ __attribute__((aligned(16))) unsigned char lds[EB_LDS];
+  typedef uint16_t PlaneA[2][BM * LDSW];  // [hi/lo]
+  typedef uint16_t PlaneB[2][BN * LDSW];
+  PlaneA* sA = reinterpret_cast<PlaneA*>(lds);
+  PlaneB* sB = reinterpret_cast<PlaneB*>(lds + 2 * sizeof(PlaneA));
+  float* Fs = reinterpret_cast<float*>(lds);
+
+  const int gt = blockIdx.x % g_tiles, wt = blockIdx.x / g_tiles;
+  const int g0 = gt * gpt, w0 = wt * wpt;
+  const int M = G * n, NB = 2 * W * n;
+  const int m0 = g0 * n, n0 = w0 * 2 * n;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  u32x4_t ra[2][A_LD], rb[2][B_LD];
+  auto load_tile = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < A_LD; ++j) {
+      const int q = t + ENER_NT * j, row = q >> 2, ch = q & 3;
+      const int r = m0 + row;
+      u32x4_t vh = {0, 0, 0, 0}, vl = {0, 0, 0, 0};
+      if (r < M) {
+        vh = *reinterpret_cast<const u32x4_t*>(Ah + (int64_t)r * K + k0 + ch * 8);
+        vl = *reinterpret_cast<const u32x4_t*>(Al + (int64_t)r * K + k0 + ch * 8);
+      }
+      ra[0][j] = vh;
+      ra[1][j] = vl;
+    }
+#pragma unroll
+    for (int j = 0; j < B_LD; ++j) {
+      const int q = t + ENER_NT * j, row = q >> 2, ch = q & 3;
+      const int r = n0 + row;
+      u32x4_t vh = {0, 0, 0, 0}, vl = {0, 0, 0, 0};
+      if (r < NB) {
+        vh = *reinterpret_cast<const u32x4_t*>(Bh + (int64_t)r * K + k0 + ch * 8);
+        vl = *reinterpret_cast<const u32x4_t*>(Bl + (int64_t)r * K + k0 + ch * 8);
+      }
+      rb[0][j] = vh;
+      rb[1][j] = vl;
+    }
+  };
+  auto store_tile = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < A_LD; ++j) {
+      const int q = t + ENER_NT * j, row = q >> 2, ch = q & 3;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        *reinterpret_cast<u32x4_t*>(&sA[buf][h][row * LDSW + ch * 8]) = ra[h][j];
+    }
+#pragma unroll
+    for (int j = 0; j < B_LD; ++j) {
+      const int q = t + ENER_NT * j, row = q >> 2, ch = q & 3;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        *reinterpret_cast<u32x4_t*>(&sB[buf][h][row * LDSW + ch * 8]) = rb[h][j];
+    }
+  };
+
+  float4_t acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+
+  const int nk = K / EB_BK;
+  load_tile(0);
+  store_tile(0);
+  __syncthreads();
+  const int arow = wm * (BM / 2) + (lane & 15);
+  const int brow = wn * (BN / 2) + (lane & 15);
+  const int koff = (lane >> 4) * 8;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load_tile((kt + 1) * EB_BK);
+    bf16x8_t ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int o = (arow + i * 16) * LDSW + koff;
+      ah[i] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t*>(&sA[cur][0][o]));
+      al[i] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t*>(&sA[cur][1][o]));
+    }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int o = (brow + j * 16) * LDSW + koff;
+      bh[j] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t*>(&sB[cur][0][o]));
+      bl[j] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t*>(&sB[cur][1][o]));
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      }
+    if (kt + 1 < nk) store_tile(cur ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int col = wn * (BN / 2) + j * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = wm * (BM / 2) + i * 16 + 4 * (lane >> 4) + r;
+        Fs[row * EB_LDF + col] = acc[i][j][r];
+      }
+    }
+  __syncthreads();
+  const int g_used = (G - g0 < gpt) ? G - g0 : gpt;
+  const int w_used = (W - w0 < wpt) ? W - w0 : wpt;
+  exchange_tile<float>(Fs, EB_LDF, n, g_used, w_used, w0, xpart + (int64_t)gt * W);
+}
+
+// E_w = sum_g (c_g^2 - x_g) / 2 + sum_s m_s sum G_s rot_h1_s + h0 with c_g the summed
+// force-bias slices and x_g = sum over spins and g-tiles (in order) of m_s * xpart.
+__global__ __launch_bounds__(ENER_NT) void energy_finish_kernel(
+    int G, int N, int nspin, int mult, int n0, int n1, const double* __restrict__ fb, int nparts,
+    int64_t part_stride, const cplx* __restrict__ xpart, int gt0, int gt1, int W,
+    const double* __restrict__ green0, const double* __restrict__ green1,
+    const double* __restrict__ rot_h1_0, const double* __restrict__ rot_h1_1, double h0,
+    cplx* __restrict__ energies) {
+  __shared__ double red[ENER_NT / 64];
+  const int w = blockIdx.x;
+  double c2r = 0.0, c2i = 0.0;
+  for (int g = threadIdx.x; g < G; g += ENER_NT) {
+    const double* p = fb + (int64_t)w * 2 * G + g;
+    double re = p[0], im = p[G];
+    for (int s = 1; s < nparts; ++s) {
+      re += p[s * part_stride];
+      im += p[s * part_stride + G];
+    }
+    c2r += re * re - im * im;
+    c2i += 2.0 * re * im;
+  }
+  double xr = 0.0, xi = 0.0;
+  const int gts[2] = {gt0, gt1};
+  const cplx* xs = xpart;
+  for (int s = 0; s < nspin; ++s) {
+    for (int t = threadIdx.x; t < gts[s]; t += ENER_NT) {
+      const cplx v = xs[(int64_t)t * W + w];
+      xr += mult * v.re;
+      xi += mult * v.im;
+    }
+    xs += (int64_t)gts[s] * W;
+  }
+  const int ns[2] = {n0, n1};
+  const double* gs[2] = {green0, green1};
+  const double* hs[2] = {rot_h1_0, rot_h1_1};
   double e1r = 0.0, e1i = 0.0;
   for (int s = 0; s < nspin; ++s) {
     const int kd = ns[s] * N;
-    const double* gr = gs[s] + (int64_t)(w_base + wl) * 2 * kd;
+    const double* gr = gs[s] + (int64_t)w * 2 * kd;
     const double* gi = gr + kd;
     for (int e = threadIdx.x; e < kd; e += ENER_NT) {
       e1r += mult * gr[e] * hs[s][e];
       e1i += mult * gi[e] * hs[s][e];
     }
   }
-  const double E2r = block_sum<ENER_NT>(e2r, red);
-  const double E2i = block_sum<ENER_NT>(e2i, red);
+  const double C2r = block_sum<ENER_NT>(c2r, red);
+  const double C2i = block_sum<ENER_NT>(c2i, red);
+  const double Xr = block_sum<ENER_NT>(xr, red);
+  const double Xi = block_sum<ENER_NT>(xi, red);
   const double E1r = block_sum<ENER_NT>(e1r, red);
   const double E1i = block_sum<ENER_NT>(e1i, red);
-  if (threadIdx.x == 0) energies[w_base + wl] = cplx{E2r + E1r + h0, E2i + E1i};
+  if (threadIdx.x == 0)
+    energies[w] = cplx{(C2r - Xr) / 2.0 + E1r + h0, (C2i - Xi) / 2.0 + E1i};
 }
 
 }  // namespace
 
+int energy_g_tiles(const afqmc_ctx* c, int s) {
+  const int n = c->nocc[s];
+  if (n < 1) return 0;
+  const int gpt = (c->mixed ? EB_BM : EF_BM) / n;
+  return (c->d.nchol + gpt - 1) / gpt;
+}
+
+// Needs the float64 Green's function of every spin in c->green (and, in the mixed mode,
+// its split-bf16 planes) and the float64 force bias in c->fb (c->fb_parts slices).
 int launch_energy(afqmc_ctx* c, int W, cplx* energies) {
+  static_assert(AFQMC_MAX_NOCC <= EF_BM && 2 * AFQMC_MAX_NOCC <= EF_BN,
+                "a (g, w) block must fit one tile");
+  static_assert(AFQMC_MAX_NOCC <= EB_BM && 2 * AFQMC_MAX_NOCC <= EB_BN,
+                "a (g, w) block must fit one tile");
   const int N = c->d.norb, G = c->d.nchol;
-  const int wc_max = c->energy_wchunk;
-  double* Fs[2] = {c->energy_f, nullptr};
-  if (c->nspin == 2) {
-    const int64_t f0 = (int64_t)G * c->nocc[0] * wc_max * 2 * c->nocc[0];
-    Fs[1] = c->energy_f + f0;
-  }
-  for (int w0 = 0; w0 < W; w0 += wc_max) {
-    const int wc = (W - w0 < wc_max) ? (W - w0) : wc_max;
-    for (int s = 0; s < c->nspin; ++s) {
-      const int n = c->nocc[s];
-      // F[(g,i)][(w,c,k)], M = G*n, Ncols = wc*2*n, K = N
-      int rc = launch_gemm_nt_f64(c->stream, G * n, wc * 2 * n, N, 1.0, c->rot_chol[s], N,
-                                  c->green[s] + (int64_t)w0 * 2 * n * N, N, 0.0, Fs[s],
-                                  wc * 2 * n);
-      if (rc) return rc;
+  int gts[2] = {0, 0};
+  cplx* xp = c->energy_x;
+  for (int s = 0; s < c->nspin; ++s) {
+    const int n = c->nocc[s];
+    if (c->mixed) {
+      const int gpt = EB_BM / n, wpt = EB_BN / (2 * n);
+      const int g_tiles = (G + gpt - 1) / gpt, w_tiles = (W + wpt - 1) / wpt;
+      gts[s] = g_tiles;
+      hipLaunchKernelGGL(exchange_bf16x3_kernel, dim3(g_tiles * w_tiles), dim3(ENER_NT), 0,
+                         c->stream, G, n, c->Np32, W, gpt, wpt, g_tiles, c->rot_chol_bf[s][0],
+                         c->rot_chol_bf[s][1], c->green_bf[s][0], c->green_bf[s][1], xp);
+    } else {
+      const int gpt = EF_BM / n, wpt = EF_BN / (2 * n);
+      const int g_tiles = (G + gpt - 1) / gpt, w_tiles = (W + wpt - 1) / wpt;
+      gts[s] = g_tiles;
+      const bool vec2 = (N % 2 == 0);
+      if (vec2)
+        hipLaunchKernelGGL(exchange_f64_kernel<2>, dim3(g_tiles * w_tiles), dim3(ENER_NT), 0,
+                           c->stream, G, n, N, W, gpt, wpt, g_tiles, c->rot_chol[s], c->green[s],
+                           xp);
+      else
+        hipLaunchKernelGGL(exchange_f64_kernel<1>, dim3(g_tiles * w_tiles), dim3(ENER_NT), 0,
+                           c->stream, G, n, N, W, gpt, wpt, g_tiles, c->rot_chol[s], c->green[s],
+                           xp);
     }
-    hipLaunchKernelGGL(energy_reduce_kernel, dim3(wc), dim3(ENER_NT), 0, c->stream, G, N,
-                       c->nspin, c->mult, c->nocc[0], c->nocc[1], wc, Fs[0], Fs[1],
-                       c->green[0], c->green[1], c->rot_h1[0], c->rot_h1[1], c->h0, w0,
-                       energies);
     LAUNCH_CHECK();
+    xp += (int64_t)gts[s] * W;
   }
+  const int64_t slice = (int64_t)2 * W * G;
+  hipLaunchKernelGGL(energy_finish_kernel, dim3(W), dim3(ENER_NT), 0, c->stream, G, N, c->nspin,
+                     c->mult, c->nocc[0], c->nocc[1], c->fb, c->fb_parts, slice, c->energy_x,
+                     gts[0], gts[1], W, c->green[0], c->green[1], c->rot_h1[0], c->rot_h1[1],
+                     c->h0, energies);
+  LAUNCH_CHECK();
   return AFQMC_OK;
 }

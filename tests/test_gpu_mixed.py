@@ -94,6 +94,25 @@ def test_mixed_step_close_to_oracle(gpu, case):
     assert rel(to_np(fb_out), fb) > 1e-13
 
 
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-N{c[1]}" for c in CASES])
+def test_mixed_energy_close_to_oracle(gpu, case):
+    """Local energies with the split-bf16 exchange contraction (float64 Green's
+    function, c_g, one-body term and accumulation): 1e-5 relative."""
+    walker_type, norb, nelec, nchol, W = case
+    print(f"mixed energy case {case}", flush=True)
+    cs = 0.5 / np.sqrt(norb)
+    hd, wd, nelec, trial, prop, ham, hdat, wave = build(walker_type, norb, nelec, nchol, W,
+                                                        "bf16x3", chol_std=cs)
+    oh = o.build_measurement_intermediates(walker_type, dict(hd), wd, norb)
+    r = np.random.default_rng(4)
+    walkers = problems.random_walkers(walker_type, norb, nelec, W, r, near=wd["mo_coeff"],
+                                      noise=0.15)
+    got = to_np(trial.calc_energy(to_dev(walkers, gpu), hdat, wave))
+    ref = o.calc_energy(walker_type, walkers, oh, wd)
+    assert rel(got, ref) < 1e-5
+    assert rel(got, ref) > 1e-14  # the exchange really ran in split-bf16
+
+
 @pytest.mark.parametrize("walker_type", ["rhf", "uhf"])
 def test_mixed_energy_matches_fp64_sampler(gpu, walker_type):
     """Same device fields and SR draws: block energies of the mixed and the float64 runs

@@ -43,15 +43,21 @@ HBM_PEAK_GBS = 8000.0
 N_PROP_STEPS = 50
 N_ENE_BLOCKS = 4
 
-# device kernel behind each timed phase (symbol prefix as rocprofv3 lists it)
-PHASE_KERNEL = {
-    "fp64": {"vhs_gemm": "gemm_nt_f64_kernel", "taylor": "taylor_reg_kernel",
-             "fb_gemm": "gemm_nt_f64_kernel", "overlap": "measure_kernel",
-             "energy": "energy_reduce_kernel"},
-    "bf16x3": {"vhs_gemm": "gemm_nt_bf16x3_kernel", "taylor": "taylor_reg_f32_kernel",
-               "fb_gemm": "gemm_nt_bf16x3_kernel", "overlap": "measure_kernel",
-               "energy": "energy_reduce_kernel"},
-}
+def phase_kernel(precision, phase, norb, nelec):
+    """Device kernel behind a timed phase (symbol prefix as rocprofv3 lists it), by the
+    library's dispatch rules."""
+    reg = norb <= 128 and max(nelec) <= 32
+    if phase == "taylor":
+        if precision == "fp64":
+            return "taylor_reg_kernel" if reg else "taylor_kernel"
+        return "taylor_reg_f32_kernel" if reg else "taylor_stream_f32_kernel"
+    if phase in ("vhs_gemm", "fb_gemm"):
+        return "gemm_nt_f64_kernel" if precision == "fp64" else "gemm_nt_bf16x3_kernel"
+    if phase in ("overlap", "green"):
+        return "measure_kernel"
+    if phase == "energy":
+        return "exchange_f64_kernel" if precision == "fp64" else "exchange_bf16x3_kernel"
+    return phase
 
 
 def flops_per_walker(norb, nelec, nchol, n_exp_terms, walker_type):
@@ -79,8 +85,7 @@ def phase_peak(precision, phase, norb, nelec):
         return PEAK_TFLOPS["fp64"]
     if phase in ("vhs_gemm", "fb_gemm"):
         return PEAK_TFLOPS["bf16"]
-    # float32 Taylor kernel for N <= 128 and n <= 32 (the library's dispatch rule)
-    if phase == "taylor" and norb <= 128 and max(nelec) <= 32:
+    if phase == "taylor":  # float32 Taylor kernels in the mixed mode
         return PEAK_TFLOPS["fp32"]
     return PEAK_TFLOPS["fp64"]
 
@@ -296,7 +301,7 @@ def run(args, precision):
     flop_launch = fl[dom] * W
     achieved = flop_launch / avg_s / 1e12
     peak = phase_peak(precision, dom, norb, nelec)
-    kernel = PHASE_KERNEL.get(precision, {}).get(dom, dom)
+    kernel = phase_kernel(precision, dom, norb, nelec)
     roofline = {
         "phase": dom,
         "kernel": kernel,

@@ -203,7 +203,7 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
     c->mixed = d.precision == AFQMC_PREC_BF16X3;
     c->Gp = (G + 31) / 32 * 32;
     c->Np32 = (N + 31) / 32 * 32;
-    c->taylor_f32 = c->mixed && N <= 128 && c->nocc[0] <= 32 && c->nocc[1] <= 32;
+    c->taylor_f32 = c->mixed;  // float32 Taylor kernels (register or streaming form)
     if (c->mixed) {
       const size_t nn = (size_t)N * N;
       for (int h = 0; h < 2; ++h) TRY(dalloc(c, &c->chol_t_bf[h], nn * c->Gp));
@@ -316,14 +316,10 @@ extern "C" int afqmc_propagate(afqmc_ctx* c, int32_t W, void* walkers_up, void* 
   {
     // VHS[w] = sum_g (i sqrt(dt) x~_wg) L_g  : (2W, G) x (N*N, G)^T
     PhaseTimer t(c, AFQMC_PH_VHS_GEMM);
-    if (c->mixed && c->taylor_f32)
+    if (c->mixed)
       rc = launch_gemm_nt_bf16x3_f32(c->stream, 2 * W, N * N, c->Gp, 1.0f, c->xs_bf[0],
                                      c->xs_bf[1], c->Gp, c->chol_t_bf[0], c->chol_t_bf[1], c->Gp,
                                      c->vhs_f32, N * N, 1, 0);
-    else if (c->mixed)
-      rc = launch_gemm_nt_bf16x3_f64(c->stream, 2 * W, N * N, c->Gp, 1.0f, c->xs_bf[0],
-                                     c->xs_bf[1], c->Gp, c->chol_t_bf[0], c->chol_t_bf[1], c->Gp,
-                                     c->vhs, N * N, 1, 0);
     else
       rc = launch_gemm_nt_f64(c->stream, 2 * W, N * N, c->d.nchol, 1.0, c->xs, c->d.nchol,
                               c->chol_t, c->d.nchol, 0.0, c->vhs, N * N);

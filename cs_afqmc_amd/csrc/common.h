@@ -96,7 +96,7 @@ struct afqmc_ctx {
 
   // reduced-precision copies (mixed mode, AFQMC_PREC_BF16X3)
   int mixed;                 // 1 when the contractions run split-bf16
-  int taylor_f32;            // 1 when the Taylor kernel runs in float32 (N <= 128, n <= 32)
+  int taylor_f32;            // 1 when the Taylor kernels run in float32 (mixed mode)
   int Gp;                    // nchol rounded up to 32 (bf16 plane leading dimension)
   int Np32;                  // norb rounded up to 32
   float* chol_t_f32;         // unused

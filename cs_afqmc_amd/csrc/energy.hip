@@ -27,26 +27,30 @@ constexpr int ENER_NT = 256;
 // ---------------------------------------------------------------- fused exchange
 
 // Exchange sums of the (g, w) blocks of an F tile held in LDS (row stride ldf):
-// F row gl*n + i, column wl*2n + c*n + j.  Wave v takes walkers wl = v, v+4, ...; its
-// lanes run over (gl, i, j); the sum over the tile's g is written to xpart[w].
+// F row gl*n + i, column wl*2n + c*n + j.  Wave v takes walkers wl = v, v+4, ...; each
+// lane owns entries (i, j) of the n x n block and runs down the tile's g; the wave sum
+// is written to xpart[w].
 template <typename T>
 __device__ inline void exchange_tile(const T* __restrict__ Fs, int ldf, int n, int g_used,
                                      int w_used, int w0, cplx* __restrict__ xpart) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nn = n * n;
-  const int tot = g_used * nn;
+  const int gstride = n * ldf;
   for (int wl = wave; wl < w_used; wl += ENER_NT / 64) {
     const T* base = Fs + wl * 2 * n;
     double xr = 0.0, xi = 0.0;
-    for (int e = lane; e < tot; e += 64) {
-      const int gl = e / nn;
-      const int rem = e - gl * nn;
-      const int i = rem / n, j = rem - (rem / n) * n;
-      const T* rg = base + gl * n * ldf;
-      const double ar = (double)rg[i * ldf + j], ai = (double)rg[i * ldf + n + j];
-      const double br = (double)rg[j * ldf + i], bi = (double)rg[j * ldf + n + i];
-      xr += ar * br - ai * bi;
-      xi += ar * bi + ai * br;
+    for (int e = lane; e < nn; e += 64) {
+      const int i = e / n, j = e - i * n;
+      const T* pa = base + i * ldf + j;  // f(i, j): re pa[0], im pa[n]
+      const T* pb = base + j * ldf + i;  // f(j, i)
+      for (int gl = 0; gl < g_used; ++gl) {
+        const double ar = (double)pa[0], ai = (double)pa[n];
+        const double br = (double)pb[0], bi = (double)pb[n];
+        xr += ar * br - ai * bi;
+        xi += ar * bi + ai * br;
+        pa += gstride;
+        pb += gstride;
+      }
     }
     for (int off = 32; off > 0; off >>= 1) {
       xr += __shfl_down(xr, off, 64);

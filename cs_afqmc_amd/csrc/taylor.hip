@@ -440,6 +440,162 @@ __global__ __launch_bounds__(512) void taylor_reg_f32_kernel(
     }
 }
 
+// ------------------------------------------------------------------------------------
+// Float32 streaming form for the mixed mode beyond the register kernel's range (N > 128
+// or n > 32).  One workgroup per walker of one spin; wave r owns output rows
+// [16 r, 16 r + 16) and all columns.  The current vector y (N x n complex) lives in LDS
+// as float32 re / im planes; V and E rows stream from global memory (L2) as the A
+// operand, one k-group ahead.  The complex product is four real products,
+//   ur = Vr yr - Vi yi,  ui = Vr yi + Vi yr,
+// on v_mfma_f32_16x16x4f32 (D element r of lane l is row 4 (l / 16) + r).  The series is
+// evaluated in Horner form y <- t0 + V y / k with t0 = E w kept in registers.
+template <int CT>
+struct StreamTileF {
+  // plane row stride in floats: 4 rows of a k-group fall on distinct banks (== 16 or 48
+  // mod 64) except for CT = 4, which keeps 64 so that N = 256 fits the LDS
+  static constexpr int LD = (CT == 2) ? 48 : (CT == 4 ? 64 : 16 * CT);
+};
+
+template <int CT>
+__global__ __launch_bounds__(1024) void taylor_stream_f32_kernel(
+    int N, int n, int n_terms_k, cplx* __restrict__ walkers, const float* __restrict__ expE,
+    const float* __restrict__ vhs) {
+  constexpr int LD = StreamTileF<CT>::LD;
+  extern __shared__ float Y[];  // [2][Np][LD]: re plane, im plane
+  const int w = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rb = wave * 16;
+  const int Np = (N + 3) & ~3;
+  const int nkg = Np >> 2;
+  const int s = lane >> 4, col = lane & 15;
+  const int p = rb + col;  // A row of this lane
+  const bool prow = p < N;
+  cplx* phi = walkers + (int64_t)w * N * n;
+  const float* Vr = vhs + (int64_t)w * 2 * N * N;
+  const float* Vi = Vr + (int64_t)N * N;
+  float* Yr = Y;
+  float* Yi = Y + Np * LD;
+
+  for (int e = threadIdx.x; e < Np * LD; e += blockDim.x) {
+    const int q = e / LD, j = e - (e / LD) * LD;
+    float re = 0.f, im = 0.f;
+    if (q < N && j < n) {
+      const cplx z = phi[(int64_t)q * n + j];
+      re = (float)z.re;
+      im = (float)z.im;
+    }
+    Yr[e] = re;
+    Yi[e] = im;
+  }
+  __syncthreads();
+  const float* yr_base = Yr + s * LD + col;
+  const float* yi_base = Yi + s * LD + col;
+
+  auto e_product = [&](float4_t* ar, float4_t* ai) {
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+      ar[ct] = float4_t{0.f, 0.f, 0.f, 0.f};
+      ai[ct] = float4_t{0.f, 0.f, 0.f, 0.f};
+    }
+    float a = (prow && s < N) ? expE[(int64_t)p * N + s] : 0.f;
+    for (int kg = 0; kg < nkg; ++kg) {
+      const int qn = 4 * (kg + 1) + s;
+      const float an = (kg + 1 < nkg && prow && qn < N) ? expE[(int64_t)p * N + qn] : 0.f;
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) {
+        const float yr = yr_base[kg * 4 * LD + ct * 16];
+        const float yi = yi_base[kg * 4 * LD + ct * 16];
+        ar[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, yr, ar[ct], 0, 0, 0);
+        ai[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, yi, ai[ct], 0, 0, 0);
+      }
+      a = an;
+    }
+  };
+  auto v_product = [&](float4_t* ur, float4_t* ui) {
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+      ur[ct] = float4_t{0.f, 0.f, 0.f, 0.f};
+      ui[ct] = float4_t{0.f, 0.f, 0.f, 0.f};
+    }
+    const bool ok0 = prow && s < N;
+    float vr = ok0 ? Vr[(int64_t)p * N + s] : 0.f;
+    float vi = ok0 ? Vi[(int64_t)p * N + s] : 0.f;
+    for (int kg = 0; kg < nkg; ++kg) {
+      const int qn = 4 * (kg + 1) + s;
+      const bool okn = kg + 1 < nkg && prow && qn < N;
+      const float vrn = okn ? Vr[(int64_t)p * N + qn] : 0.f;
+      const float vin = okn ? Vi[(int64_t)p * N + qn] : 0.f;
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) {
+        const float yr = yr_base[kg * 4 * LD + ct * 16];
+        const float yi = yi_base[kg * 4 * LD + ct * 16];
+        ur[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(vr, yr, ur[ct], 0, 0, 0);
+        ur[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(-vi, yi, ur[ct], 0, 0, 0);
+        ui[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(vr, yi, ui[ct], 0, 0, 0);
+        ui[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(vi, yr, ui[ct], 0, 0, 0);
+      }
+      vr = vrn;
+      vi = vin;
+    }
+  };
+  auto put = [&](const float4_t* vr, const float4_t* vi) {
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rb + 4 * s + r;
+        if (row < Np) {
+          Yr[row * LD + ct * 16 + col] = vr[ct][r];
+          Yi[row * LD + ct * 16 + col] = vi[ct][r];
+        }
+      }
+  };
+
+  float4_t t0r[CT], t0i[CT], ur[CT], ui[CT];
+  e_product(t0r, t0i);
+  __syncthreads();
+  put(t0r, t0i);
+  __syncthreads();
+  for (int k = n_terms_k; k >= 1; --k) {
+    v_product(ur, ui);
+    __syncthreads();
+    const float rk = 1.0f / (float)k;
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        ur[ct][r] = t0r[ct][r] + ur[ct][r] * rk;
+        ui[ct][r] = t0i[ct][r] + ui[ct][r] * rk;
+      }
+    put(ur, ui);
+    __syncthreads();
+  }
+  e_product(ur, ui);
+#pragma unroll
+  for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = rb + 4 * s + r;
+      const int j = ct * 16 + col;
+      if (row < N && j < n) phi[(int64_t)row * n + j] = cplx{(double)ur[ct][r], (double)ui[ct][r]};
+    }
+}
+
+template <int CT>
+int launch_stream_f32(afqmc_ctx* c, int W, int n, cplx* wk, const float* E) {
+  const int N = c->d.norb;
+  const int nwaves = (N + 15) / 16;
+  const int Np = (N + 3) & ~3;
+  const size_t shm = (size_t)2 * Np * StreamTileF<CT>::LD * sizeof(float);
+  static int lds_limit = 0;
+  const int rc = ensure_dyn_lds((const void*)taylor_stream_f32_kernel<CT>, shm, &lds_limit);
+  if (rc) return rc;
+  hipLaunchKernelGGL((taylor_stream_f32_kernel<CT>), dim3(W), dim3(nwaves * 64), shm, c->stream,
+                     N, n, c->d.n_exp_terms - 1, wk, E, c->vhs_f32);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
 template <int NKGV, int CTP>
 int launch_reg_f32(afqmc_ctx* c, int W, int n, cplx* wk, const float* E) {
   const int N = c->d.norb;
@@ -484,11 +640,22 @@ int launch_taylor(afqmc_ctx* c, int W, cplx* wup, cplx* wdn) {
     for (int s = 0; s < c->nspin; ++s) {
       const int n = c->nocc[s];
       cplx* wk = (s == 0) ? wup : wdn;
-      const int nkgv = 2 * ((N + 3) / 4);
-      const int rc = nkgv <= 32   ? launch_reg_f32_ct<32>(c, W, n, wk, c->exp_h1_f32[s])
-                     : nkgv <= 48 ? launch_reg_f32_ct<48>(c, W, n, wk, c->exp_h1_f32[s])
-                     : nkgv <= 56 ? launch_reg_f32_ct<56>(c, W, n, wk, c->exp_h1_f32[s])
-                                  : launch_reg_f32_ct<64>(c, W, n, wk, c->exp_h1_f32[s]);
+      const float* E = c->exp_h1_f32[s];
+      int rc;
+      if (N <= 128 && n <= 32) {
+        const int nkgv = 2 * ((N + 3) / 4);
+        rc = nkgv <= 32   ? launch_reg_f32_ct<32>(c, W, n, wk, E)
+             : nkgv <= 48 ? launch_reg_f32_ct<48>(c, W, n, wk, E)
+             : nkgv <= 56 ? launch_reg_f32_ct<56>(c, W, n, wk, E)
+                          : launch_reg_f32_ct<64>(c, W, n, wk, E);
+      } else {
+        switch ((n + 15) / 16) {
+          case 1: rc = launch_stream_f32<1>(c, W, n, wk, E); break;
+          case 2: rc = launch_stream_f32<2>(c, W, n, wk, E); break;
+          case 3: rc = launch_stream_f32<3>(c, W, n, wk, E); break;
+          default: rc = launch_stream_f32<4>(c, W, n, wk, E); break;
+        }
+      }
       if (rc) return rc;
     }
     return AFQMC_OK;

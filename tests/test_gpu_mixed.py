@@ -1,5 +1,6 @@
-"""Mixed-precision step (split-bf16 Cholesky contractions, float32 Taylor products,
-float64 overlaps / determinants / QR / weights / energies) against the float64 oracle.
+"""Mixed-precision step (split-bf16 Cholesky contractions, float32 Taylor products in the
+register-resident or the streaming kernel, float64 overlaps / determinants / QR / weights /
+energies) against the float64 oracle.
 
 Tolerances: one step agrees to 1e-4 relative (split-bf16 carries 16 mantissa bits);
 block energies of the device sampler driven by identical fields agree with the plain
@@ -18,7 +19,9 @@ CASES = [
     ("rhf", 40, (8, 8), 100, 32),
     ("rhf", 100, (20, 20), 400, 16),
     ("uhf", 36, (7, 5), 90, 24),
-    ("uhf", 136, (34, 30), 160, 8),  # float64 Taylor fallback (N > 128)
+    ("uhf", 136, (34, 30), 160, 8),  # float32 streaming Taylor (N > 128)
+    ("rhf", 96, (40, 40), 200, 8),  # float32 streaming Taylor (n > 32)
+    ("uhf", 200, (40, 40), 300, 4),
 ]
 
 

@@ -32,6 +32,14 @@ EXPORTS = (
     "afqmc_prof_enable",
     "afqmc_prof_collect",
     "afqmc_fields_normal",
+    "afqmc_group_create",
+    "afqmc_group_destroy",
+    "afqmc_group_set_stream",
+    "afqmc_group_clear",
+    "afqmc_group_add",
+    "afqmc_group_size",
+    "afqmc_group_propagate",
+    "afqmc_group_tune",
 )
 
 # phase indices of afqmc_prof_collect (AFQMC_PH_* in the header)
@@ -98,6 +106,14 @@ def _declare(lib):
     lib.afqmc_prof_collect.argtypes = [_vp, _vp, _vp]
     lib.afqmc_fields_normal.argtypes = [
         _vp, ctypes.c_uint64, ctypes.c_uint64, _vp, ctypes.c_int64]
+    lib.afqmc_group_create.argtypes = [ctypes.POINTER(_vp), _i32, _vp]
+    lib.afqmc_group_destroy.argtypes = [_vp]
+    lib.afqmc_group_set_stream.argtypes = [_vp, _vp]
+    lib.afqmc_group_clear.argtypes = [_vp]
+    lib.afqmc_group_add.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp]
+    lib.afqmc_group_size.argtypes = [_vp]
+    lib.afqmc_group_propagate.argtypes = [_vp, _i32, _i32]
+    lib.afqmc_group_tune.argtypes = [_vp, _vp]
     for name in EXPORTS:
         fn = getattr(lib, name)
         if fn.restype is ctypes.c_int:  # default

@@ -221,3 +221,88 @@ int launch_sr(afqmc_ctx* c, int W, cplx* wup, cplx* wdn, double* weights, double
               int32_t* idx_out, bool global_order);
 int launch_fields_normal(afqmc_ctx* c, uint64_t seed, uint64_t counter, double* out,
                          int64_t count);
+
+// ---------------------------------------------------------------- grouped launches
+// Several independent problems (fragments x spins) share one grid: problem p owns the
+// blocks [start[p], start[p + 1]) and reads its kernel arguments from args[p]; both
+// tables live in device memory.  The kernel bodies are the single-problem ones.
+__device__ inline int group_of(const int* __restrict__ start, int nprob, int b) {
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+struct MeasureArgs {
+  int N, n, spin, ld16;
+  const cplx* walkers;
+  const double* trial;
+  cplx* dets;
+  double* green;
+  uint16_t* ghi;
+  uint16_t* glo;
+};
+struct FieldsArgs {
+  int G, nparts, Gp, W;
+  double sqdt;
+  int64_t part_stride;
+  const double* fb;
+  const cplx* mf;
+  const double* fields;  // step s reads fields + s * W * G
+  double* xs;
+  uint16_t* xs_hi;
+  uint16_t* xs_lo;
+  cplx* terms;
+};
+struct WeightsArgs {
+  int W, nspin;
+  double dt, sqdt, h0_prop;
+  const cplx* dets;
+  const cplx* terms;
+  double* weights;
+  cplx* overlaps;
+  double* scalars;
+};
+struct TaylorArgs {
+  int N, n, K, rows_pad;
+  cplx* walkers;
+  const void* E;    // float64 or float32 by kernel
+  const void* vhs;  // float64 or float32 by kernel
+};
+struct GemmArgs {
+  int M, N, K, tiles_m, tiles_n;
+  float alpha;
+  int64_t lda, ldb, ldc;
+  const void* A;   // float64, or the bf16 hi plane
+  const void* Al;  // bf16 lo plane (split-bf16 only)
+  const void* B;
+  const void* Bl;
+  void* C;
+};
+
+enum TaylorKind { TAYLOR_REG_F64 = 0, TAYLOR_REG_F32 = 1, TAYLOR_STREAM_F32 = 2, TAYLOR_STREAM_F64 = 3 };
+
+struct TaylorPlan {
+  int kind, nkgv, ct, nthreads, blocks_per_walker;
+  size_t shm;
+};
+void taylor_plan(const afqmc_ctx* c, int s, TaylorPlan* pl);
+void measure_plan(const afqmc_ctx* c, int s, int* t2, bool* stage, size_t* shm);
+
+int launch_measure_grouped(hipStream_t s, int t2, bool stage, const MeasureArgs* args,
+                           const int* start, int nprob, int nblocks, size_t shm);
+int launch_fields_grouped(hipStream_t s, const FieldsArgs* args, const int* start, int nprob,
+                          int nblocks, int step);
+int launch_weights_grouped(hipStream_t s, const WeightsArgs* args, int nprob);
+int launch_taylor_grouped(hipStream_t s, int kind, int nkgv, int ct, const TaylorArgs* args,
+                          const int* start, int nprob, int nblocks, int nthreads, size_t shm);
+// bm: 64 or 128 square tiles; out: 0 float64 C, 1 float32 C (split-bf16 only)
+int launch_gemm_bf16x3_grouped(hipStream_t s, int bm, int out, const GemmArgs* args,
+                               const int* start, int nprob, int nblocks);
+int launch_gemm_f64_grouped(hipStream_t s, int bm, bool vec2, const GemmArgs* args,
+                            const int* start, int nprob, int nblocks);

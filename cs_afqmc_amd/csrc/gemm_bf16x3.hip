@@ -49,24 +49,24 @@ __global__ void split_bf16_kernel(int rows, int cols, const double* __restrict__
 }
 
 template <int BM, int BN, typename OutT>
-__global__ __launch_bounds__(256) void gemm_nt_bf16x3_kernel(
+__device__ __forceinline__ void gemm_nt_bf16x3_body(
     int M, int N, int K, float alpha, const uint16_t* __restrict__ Ah,
     const uint16_t* __restrict__ Al, int64_t lda, const uint16_t* __restrict__ Bh,
     const uint16_t* __restrict__ Bl, int64_t ldb, OutT* __restrict__ C, int64_t ldc, int tiles_m,
-    int tiles_n, int k_per_split, int64_t c_split_stride) {
+    int tiles_n, int k_per_split, int64_t c_split_stride, const int bx, const int by) {
   constexpr int TM = BM / 2 / 16, TN = BN / 2 / 16;  // 2 x 2 waves
   constexpr int A_LD = BM * 4 / 256, B_LD = BN * 4 / 256;  // 16-byte chunks per thread
   __shared__ __attribute__((aligned(16))) uint16_t sA[2][2][BM * LDSW];  // [buf][hi/lo]
   __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][BN * LDSW];
 
   const int nblk = tiles_m * tiles_n;
-  int b = blockIdx.x;
+  int b = bx;
   if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);  // XCD-contiguous tile runs
   const int tile_m = b % tiles_m, tile_n = b / tiles_m;
   const int m0 = tile_m * BM, n0 = tile_n * BN;
-  const int kbeg = blockIdx.y * k_per_split;
+  const int kbeg = by * k_per_split;
   const int kend = (kbeg + k_per_split < K) ? kbeg + k_per_split : K;
-  C += (int64_t)blockIdx.y * c_split_stride;
+  C += (int64_t)by * c_split_stride;
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave & 1, wn = wave >> 1;
@@ -172,6 +172,30 @@ __global__ __launch_bounds__(256) void gemm_nt_bf16x3_kernel(
 }
 
 template <int BM, int BN, typename OutT>
+__global__ __launch_bounds__(256) void gemm_nt_bf16x3_kernel(
+    int M, int N, int K, float alpha, const uint16_t* __restrict__ Ah,
+    const uint16_t* __restrict__ Al, int64_t lda, const uint16_t* __restrict__ Bh,
+    const uint16_t* __restrict__ Bl, int64_t ldb, OutT* __restrict__ C, int64_t ldc, int tiles_m,
+    int tiles_n, int k_per_split, int64_t c_split_stride) {
+  gemm_nt_bf16x3_body<BM, BN, OutT>(M, N, K, alpha, Ah, Al, lda, Bh, Bl, ldb, C, ldc, tiles_m,
+                                    tiles_n, k_per_split, c_split_stride, blockIdx.x, blockIdx.y);
+}
+
+// grouped: problem p owns tiles [start[p], start[p + 1]), no split-K
+template <int BM, int BN, typename OutT>
+__global__ __launch_bounds__(256) void gemm_bf16x3_grouped_kernel(const GemmArgs* __restrict__ args,
+                                                                  const int* __restrict__ start,
+                                                                  int nprob) {
+  const int p = group_of(start, nprob, blockIdx.x);
+  const GemmArgs a = args[p];
+  gemm_nt_bf16x3_body<BM, BN, OutT>(
+      a.M, a.N, a.K, a.alpha, static_cast<const uint16_t*>(a.A),
+      static_cast<const uint16_t*>(a.Al), a.lda, static_cast<const uint16_t*>(a.B),
+      static_cast<const uint16_t*>(a.Bl), a.ldb, static_cast<OutT*>(a.C), a.ldc, a.tiles_m,
+      a.tiles_n, a.K, 0, blockIdx.x - start[p], 0);
+}
+
+template <int BM, int BN, typename OutT>
 int launch_bf(hipStream_t s, int M, int N, int K, float alpha, const uint16_t* Ah,
               const uint16_t* Al, int64_t lda, const uint16_t* Bh, const uint16_t* Bl, int64_t ldb,
               OutT* C, int64_t ldc, int splits, int kps, int64_t c_split_stride) {
@@ -228,4 +252,25 @@ int launch_gemm_nt_bf16x3_f64(hipStream_t s, int M, int N, int K, float alpha,
                               int64_t ldc, int splits, int64_t c_split_stride) {
   return gemm_bf16x3<double>(s, M, N, K, alpha, Ah, Al, lda, Bh, Bl, ldb, C, ldc, splits,
                              c_split_stride);
+}
+
+int launch_gemm_bf16x3_grouped(hipStream_t s, int bm, int out, const GemmArgs* args,
+                               const int* start, int nprob, int nblocks) {
+  if (bm == 128) {
+    if (out)
+      hipLaunchKernelGGL((gemm_bf16x3_grouped_kernel<128, 128, float>), dim3(nblocks), dim3(256),
+                         0, s, args, start, nprob);
+    else
+      hipLaunchKernelGGL((gemm_bf16x3_grouped_kernel<128, 128, double>), dim3(nblocks),
+                         dim3(256), 0, s, args, start, nprob);
+  } else {
+    if (out)
+      hipLaunchKernelGGL((gemm_bf16x3_grouped_kernel<64, 64, float>), dim3(nblocks), dim3(256), 0,
+                         s, args, start, nprob);
+    else
+      hipLaunchKernelGGL((gemm_bf16x3_grouped_kernel<64, 64, double>), dim3(nblocks), dim3(256),
+                         0, s, args, start, nprob);
+  }
+  LAUNCH_CHECK();
+  return AFQMC_OK;
 }

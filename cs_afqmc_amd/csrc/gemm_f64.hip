@@ -20,10 +20,11 @@ constexpr int GEMM_BK = 16;
 constexpr int GEMM_PAD = 16;  // row stride == 16 (mod 32) doubles: k and k+1 rows hit disjoint bank halves
 
 template <int BM, int BN, int WM, int WN, int VEC>
-__global__ __launch_bounds__(256) void gemm_nt_f64_kernel(
+__device__ __forceinline__ void gemm_nt_f64_body(
     int M, int N, int K, double alpha, const double* __restrict__ A, int lda,
     const double* __restrict__ B, int ldb, double beta, double* __restrict__ C, int ldc,
-    int tiles_m, int tiles_n, int k_per_split, int64_t c_split_stride) {
+    int tiles_m, int tiles_n, int k_per_split, int64_t c_split_stride, const int bx,
+    const int by) {
   constexpr int NT = 256;
   constexpr int BK = GEMM_BK;
   constexpr int TM = BM / WM / 16;
@@ -41,13 +42,13 @@ __global__ __launch_bounds__(256) void gemm_nt_f64_kernel(
   // XCD-aware block -> tile map: blocks are dealt round-robin to the 8 XCDs, so give
   // each XCD a contiguous run of tiles (m fastest: the run shares B column panels).
   const int nblk = tiles_m * tiles_n;
-  int b = blockIdx.x;
+  int b = bx;
   if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
   const int tile_m = b % tiles_m, tile_n = b / tiles_m;
   const int m0 = tile_m * BM, n0 = tile_n * BN;
-  const int kbeg = blockIdx.y * k_per_split;
+  const int kbeg = by * k_per_split;
   const int kend = (kbeg + k_per_split < K) ? kbeg + k_per_split : K;
-  C += (int64_t)blockIdx.y * c_split_stride;
+  C += (int64_t)by * c_split_stride;
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave % WM, wn = wave / WM;
@@ -146,6 +147,31 @@ __global__ __launch_bounds__(256) void gemm_nt_f64_kernel(
     }
 }
 
+template <int BM, int BN, int WM, int WN, int VEC>
+__global__ __launch_bounds__(256) void gemm_nt_f64_kernel(
+    int M, int N, int K, double alpha, const double* __restrict__ A, int lda,
+    const double* __restrict__ B, int ldb, double beta, double* __restrict__ C, int ldc,
+    int tiles_m, int tiles_n, int k_per_split, int64_t c_split_stride) {
+  gemm_nt_f64_body<BM, BN, WM, WN, VEC>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tiles_m,
+                                        tiles_n, k_per_split, c_split_stride, blockIdx.x,
+                                        blockIdx.y);
+}
+
+// grouped: problem p owns tiles [start[p], start[p + 1]), no split-K, beta = 0
+template <int BM, int BN, int VEC>
+__global__ __launch_bounds__(256) void gemm_f64_grouped_kernel(const GemmArgs* __restrict__ args,
+                                                               const int* __restrict__ start,
+                                                               int nprob) {
+  const int p = group_of(start, nprob, blockIdx.x);
+  const GemmArgs a = args[p];
+  gemm_nt_f64_body<BM, BN, 2, 2, VEC>(a.M, a.N, a.K, (double)a.alpha,
+                                      static_cast<const double*>(a.A), (int)a.lda,
+                                      static_cast<const double*>(a.B), (int)a.ldb, 0.0,
+                                      static_cast<double*>(a.C), (int)a.ldc, a.tiles_m, a.tiles_n,
+                                      (a.K + GEMM_BK - 1) / GEMM_BK * GEMM_BK, 0,
+                                      blockIdx.x - start[p], 0);
+}
+
 template <int BM, int BN>
 int launch_tiles(hipStream_t s, int M, int N, int K, double alpha, const double* A, int lda,
                  const double* B, int ldb, double beta, double* C, int ldc, int splits,
@@ -206,4 +232,25 @@ int launch_gemm_nt_f64(hipStream_t s, int M, int N, int K, double alpha, const d
     return launch_tiles<128, 128>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, 1, K, 0,
                                   vec2);
   return launch_tiles<64, 64>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, 1, K, 0, vec2);
+}
+
+int launch_gemm_f64_grouped(hipStream_t s, int bm, bool vec2, const GemmArgs* args,
+                            const int* start, int nprob, int nblocks) {
+  if (bm == 128) {
+    if (vec2)
+      hipLaunchKernelGGL((gemm_f64_grouped_kernel<128, 128, 2>), dim3(nblocks), dim3(256), 0, s,
+                         args, start, nprob);
+    else
+      hipLaunchKernelGGL((gemm_f64_grouped_kernel<128, 128, 1>), dim3(nblocks), dim3(256), 0, s,
+                         args, start, nprob);
+  } else {
+    if (vec2)
+      hipLaunchKernelGGL((gemm_f64_grouped_kernel<64, 64, 2>), dim3(nblocks), dim3(256), 0, s,
+                         args, start, nprob);
+    else
+      hipLaunchKernelGGL((gemm_f64_grouped_kernel<64, 64, 1>), dim3(nblocks), dim3(256), 0, s,
+                         args, start, nprob);
+  }
+  LAUNCH_CHECK();
+  return AFQMC_OK;
 }

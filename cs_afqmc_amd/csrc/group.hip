@@ -1,0 +1,512 @@
+// Grouped step over many fragments: independent AFQMC problems of uneven size (norb,
+// nocc, nchol, walkers), one afqmc_ctx each, advanced together.  Each phase of the step
+// is one launch over all fragments (or a few, one per kernel template key): problem p
+// owns a range of blocks and reads its arguments from a device table, so small
+// fragments fill the GPU together instead of one after another.
+//
+// Per step:  [Green of all walkers]  force-bias GEMMs  fields  VHS GEMMs  Taylor
+//            new overlaps + Green  weights
+// Tables are rebuilt when the member set changes and uploaded once; a step only passes
+// the step index (which selects each fragment's fields in its bound fields block).
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "common.h"
+
+namespace {
+
+struct Member {
+  afqmc_ctx* c;
+  cplx* wup;
+  cplx* wdn;
+  double* weights;
+  cplx* overlaps;
+  const double* fields;
+  int n_steps;
+  double* scalars;
+};
+
+struct Launch {
+  int k0, k1, k2, k3;  // kernel key (meaning per table)
+  size_t shm;
+  int nprob, nblocks;
+  size_t args_off, start_off;  // byte offsets into the device table
+};
+
+}  // namespace
+
+struct afqmc_group {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int max_frag = 0;
+  std::vector<Member> mem;
+  bool dirty = true;
+  bool mixed = false;
+  int fb_bm = 0, vhs_bm = 0;  // GEMM tile edge (0: heuristic)
+  bool f64_vec2 = true;
+  // built tables
+  std::vector<Launch> meas, taylor;
+  Launch fb{}, vhs{}, fields{}, weights{};
+  unsigned char* dev_tab = nullptr;
+  size_t dev_cap = 0;
+  std::vector<unsigned char> host_tab;
+};
+
+namespace {
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+template <typename T>
+void put_table(std::vector<unsigned char>& tab, Launch& L, const std::vector<T>& args,
+               const std::vector<int>& start) {
+  L.args_off = align256(tab.size());
+  tab.resize(L.args_off + args.size() * sizeof(T));
+  std::memcpy(tab.data() + L.args_off, args.data(), args.size() * sizeof(T));
+  L.start_off = align256(tab.size());
+  tab.resize(L.start_off + start.size() * sizeof(int));
+  std::memcpy(tab.data() + L.start_off, start.data(), start.size() * sizeof(int));
+}
+
+int heuristic_bm(int64_t tiles128) { return tiles128 >= 512 ? 128 : 64; }
+
+int build(afqmc_group* g) {
+  g->host_tab.clear();
+  g->f64_vec2 = true;
+  g->meas.clear();
+  g->taylor.clear();
+  std::vector<unsigned char>& tab = g->host_tab;
+
+  // ---- measurement (Green / overlaps): key (t2, stage)
+  {
+    std::map<std::pair<int, int>, std::vector<std::pair<MeasureArgs, int>>> by_key;
+    std::map<std::pair<int, int>, size_t> shm;
+    for (const Member& m : g->mem) {
+      afqmc_ctx* c = m.c;
+      for (int s = 0; s < c->nspin; ++s) {
+        int t2;
+        bool stage;
+        size_t sh;
+        measure_plan(c, s, &t2, &stage, &sh);
+        MeasureArgs a{};
+        a.N = c->d.norb;
+        a.n = c->nocc[s];
+        a.spin = s;
+        a.ld16 = c->Np32;
+        a.walkers = s == 0 ? m.wup : m.wdn;
+        a.trial = c->trial[s];
+        a.dets = c->dets;
+        a.green = g->mixed ? nullptr : c->green[s];
+        a.ghi = g->mixed ? c->green_bf[s][0] : nullptr;
+        a.glo = g->mixed ? c->green_bf[s][1] : nullptr;
+        const auto key = std::make_pair(t2, stage ? 1 : 0);
+        by_key[key].push_back({a, c->d.n_walkers});
+        shm[key] = std::max(shm[key], sh);
+      }
+    }
+    for (auto& kv : by_key) {
+      Launch L{};
+      L.k0 = kv.first.first;
+      L.k1 = kv.first.second;
+      L.shm = shm[kv.first];
+      std::vector<MeasureArgs> args;
+      std::vector<int> start;
+      int b = 0;
+      for (auto& pa : kv.second) {
+        args.push_back(pa.first);
+        start.push_back(b);
+        b += pa.second;
+      }
+      L.nprob = (int)args.size();
+      L.nblocks = b;
+      put_table(tab, L, args, start);
+      g->meas.push_back(L);
+    }
+  }
+
+  // ---- force-bias GEMMs: one problem per (fragment, spin); part s of fb gets spin s
+  {
+    std::vector<GemmArgs> args;
+    std::vector<int> start;
+    int64_t t128 = 0;
+    for (const Member& m : g->mem)
+      for (int s = 0; s < m.c->nspin; ++s)
+        t128 += (int64_t)((2 * m.c->d.n_walkers + 127) / 128) * ((m.c->d.nchol + 127) / 128);
+    const int bm = g->fb_bm ? g->fb_bm : heuristic_bm(t128);
+    int b = 0;
+    for (const Member& m : g->mem) {
+      afqmc_ctx* c = m.c;
+      const int W = c->d.n_walkers, G = c->d.nchol;
+      for (int s = 0; s < c->nspin; ++s) {
+        GemmArgs a{};
+        a.M = 2 * W;
+        a.N = G;
+        a.alpha = (float)c->mult;
+        a.ldc = G;
+        a.C = c->fb + (int64_t)s * 2 * W * G;
+        if (g->mixed) {
+          a.K = c->nocc[s] * c->Np32;
+          a.A = c->green_bf[s][0];
+          a.Al = c->green_bf[s][1];
+          a.B = c->rot_chol_bf[s][0];
+          a.Bl = c->rot_chol_bf[s][1];
+        } else {
+          a.K = c->kdim[s];
+          a.A = c->green[s];
+          a.B = c->rot_chol[s];
+          if (a.K % 2) g->f64_vec2 = false;
+        }
+        a.lda = a.ldb = a.K;
+        a.tiles_m = (a.M + bm - 1) / bm;
+        a.tiles_n = (a.N + bm - 1) / bm;
+        args.push_back(a);
+        start.push_back(b);
+        b += a.tiles_m * a.tiles_n;
+      }
+    }
+    g->fb = Launch{};
+    g->fb.k0 = bm;
+    g->fb.nprob = (int)args.size();
+    g->fb.nblocks = b;
+    put_table(tab, g->fb, args, start);
+  }
+
+  // ---- fields: one block per walker
+  {
+    std::vector<FieldsArgs> args;
+    std::vector<int> start;
+    int b = 0;
+    for (const Member& m : g->mem) {
+      afqmc_ctx* c = m.c;
+      FieldsArgs a{};
+      a.G = c->d.nchol;
+      a.nparts = c->nspin;
+      a.Gp = c->Gp;
+      a.W = c->d.n_walkers;
+      a.sqdt = sqrt(c->d.dt);
+      a.part_stride = (int64_t)2 * a.W * a.G;
+      a.fb = c->fb;
+      a.mf = c->mf_shifts;
+      a.fields = m.fields;
+      a.xs = c->xs;
+      a.xs_hi = g->mixed ? c->xs_bf[0] : nullptr;
+      a.xs_lo = g->mixed ? c->xs_bf[1] : nullptr;
+      a.terms = c->terms;
+      args.push_back(a);
+      start.push_back(b);
+      b += a.W;
+    }
+    g->fields = Launch{};
+    g->fields.nprob = (int)args.size();
+    g->fields.nblocks = b;
+    put_table(tab, g->fields, args, start);
+  }
+
+  // ---- VHS GEMMs: one problem per fragment
+  {
+    std::vector<GemmArgs> args;
+    std::vector<int> start;
+    int64_t t128 = 0;
+    for (const Member& m : g->mem)
+      t128 += (int64_t)((2 * m.c->d.n_walkers + 127) / 128) *
+              (((int64_t)m.c->d.norb * m.c->d.norb + 127) / 128);
+    const int bm = g->vhs_bm ? g->vhs_bm : heuristic_bm(t128);
+    int b = 0;
+    for (const Member& m : g->mem) {
+      afqmc_ctx* c = m.c;
+      const int W = c->d.n_walkers, N = c->d.norb;
+      GemmArgs a{};
+      a.M = 2 * W;
+      a.N = N * N;
+      a.alpha = 1.0f;
+      a.ldc = (int64_t)N * N;
+      if (g->mixed) {
+        a.K = c->Gp;
+        a.A = c->xs_bf[0];
+        a.Al = c->xs_bf[1];
+        a.B = c->chol_t_bf[0];
+        a.Bl = c->chol_t_bf[1];
+        a.C = c->vhs_f32;
+      } else {
+        a.K = c->d.nchol;
+        a.A = c->xs;
+        a.B = c->chol_t;
+        a.C = c->vhs;
+        if (a.K % 2) g->f64_vec2 = false;
+      }
+      a.lda = a.ldb = a.K;
+      a.tiles_m = (a.M + bm - 1) / bm;
+      a.tiles_n = (a.N + bm - 1) / bm;
+      args.push_back(a);
+      start.push_back(b);
+      b += a.tiles_m * a.tiles_n;
+    }
+    g->vhs = Launch{};
+    g->vhs.k0 = bm;
+    g->vhs.nprob = (int)args.size();
+    g->vhs.nblocks = b;
+    put_table(tab, g->vhs, args, start);
+  }
+
+  // ---- Taylor: key (kind, nkgv, ct, block size)
+  {
+    std::map<std::tuple<int, int, int, int>, std::vector<std::pair<TaylorArgs, int>>> by_key;
+    std::map<std::tuple<int, int, int, int>, size_t> shm;
+    for (const Member& m : g->mem) {
+      afqmc_ctx* c = m.c;
+      for (int s = 0; s < c->nspin; ++s) {
+        TaylorPlan pl;
+        taylor_plan(c, s, &pl);
+        TaylorArgs a{};
+        a.N = c->d.norb;
+        a.n = c->nocc[s];
+        a.K = c->d.n_exp_terms - 1;
+        a.rows_pad = ((a.N + 15) / 16) * 16;
+        a.walkers = s == 0 ? m.wup : m.wdn;
+        a.E = g->mixed ? (const void*)c->exp_h1_f32[s] : (const void*)c->exp_h1[s];
+        a.vhs = g->mixed ? (const void*)c->vhs_f32 : (const void*)c->vhs;
+        const auto key = std::make_tuple(pl.kind, pl.nkgv, pl.ct, pl.nthreads);
+        by_key[key].push_back({a, c->d.n_walkers * pl.blocks_per_walker});
+        shm[key] = std::max(shm[key], pl.shm);
+      }
+    }
+    for (auto& kv : by_key) {
+      Launch L{};
+      L.k0 = std::get<0>(kv.first);
+      L.k1 = std::get<1>(kv.first);
+      L.k2 = std::get<2>(kv.first);
+      L.k3 = std::get<3>(kv.first);
+      L.shm = shm[kv.first];
+      std::vector<TaylorArgs> args;
+      std::vector<int> start;
+      int b = 0;
+      for (auto& pa : kv.second) {
+        args.push_back(pa.first);
+        start.push_back(b);
+        b += pa.second;
+      }
+      L.nprob = (int)args.size();
+      L.nblocks = b;
+      put_table(tab, L, args, start);
+      g->taylor.push_back(L);
+    }
+  }
+
+  // ---- weights: one block per fragment
+  {
+    std::vector<WeightsArgs> args;
+    std::vector<int> start;
+    for (const Member& m : g->mem) {
+      afqmc_ctx* c = m.c;
+      WeightsArgs a{};
+      a.W = c->d.n_walkers;
+      a.nspin = c->nspin;
+      a.dt = c->d.dt;
+      a.sqdt = sqrt(c->d.dt);
+      a.h0_prop = c->h0_prop;
+      a.dets = c->dets;
+      a.terms = c->terms;
+      a.weights = m.weights;
+      a.overlaps = m.overlaps;
+      a.scalars = m.scalars;
+      start.push_back((int)args.size());
+      args.push_back(a);
+    }
+    g->weights = Launch{};
+    g->weights.nprob = (int)args.size();
+    g->weights.nblocks = (int)args.size();
+    put_table(tab, g->weights, args, start);
+  }
+
+  // upload (the previous step's launches are stream-ordered before this copy)
+  if (tab.size() > g->dev_cap) {
+    if (g->dev_tab) HIP_TRY(hipFree(g->dev_tab));
+    g->dev_tab = nullptr;
+    g->dev_cap = 0;
+    HIP_TRY(hipMalloc(&g->dev_tab, tab.size()));
+    g->dev_cap = tab.size();
+  }
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  HIP_TRY(hipMemcpy(g->dev_tab, tab.data(), tab.size(), hipMemcpyHostToDevice));
+  g->dirty = false;
+  return AFQMC_OK;
+}
+
+template <typename T>
+const T* targs(const afqmc_group* g, const Launch& L) {
+  return reinterpret_cast<const T*>(g->dev_tab + L.args_off);
+}
+const int* tstart(const afqmc_group* g, const Launch& L) {
+  return reinterpret_cast<const int*>(g->dev_tab + L.start_off);
+}
+
+int run_measure(afqmc_group* g) {
+  for (const Launch& L : g->meas) {
+    const int rc = launch_measure_grouped(g->stream, L.k0, L.k1 != 0, targs<MeasureArgs>(g, L),
+                                          tstart(g, L), L.nprob, L.nblocks, L.shm);
+    if (rc) return rc;
+  }
+  return AFQMC_OK;
+}
+
+int run_gemm(afqmc_group* g, const Launch& L, int out_f32) {
+  if (g->mixed)
+    return launch_gemm_bf16x3_grouped(g->stream, L.k0, out_f32, targs<GemmArgs>(g, L),
+                                      tstart(g, L), L.nprob, L.nblocks);
+  return launch_gemm_f64_grouped(g->stream, L.k0, g->f64_vec2, targs<GemmArgs>(g, L),
+                                 tstart(g, L), L.nprob, L.nblocks);
+}
+
+int set_group_device(const afqmc_group* g) {
+  HIP_TRY(hipSetDevice(g->device));
+  return AFQMC_OK;
+}
+
+}  // namespace
+
+extern "C" int afqmc_group_create(afqmc_group** out, int32_t max_fragments, void* stream) {
+  if (!out) return afqmc_set_error(AFQMC_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (max_fragments < 1) return afqmc_set_error(AFQMC_ERR_ARG, "max_fragments=%d", max_fragments);
+  afqmc_group* g = new afqmc_group();
+  g->max_frag = max_fragments;
+  g->stream = static_cast<hipStream_t>(stream);
+  if (hipGetDevice(&g->device) != hipSuccess) {
+    delete g;
+    return afqmc_set_error(AFQMC_ERR_HIP, "hipGetDevice failed");
+  }
+  *out = g;
+  return AFQMC_OK;
+}
+
+extern "C" int afqmc_group_destroy(afqmc_group* g) {
+  if (!g) return AFQMC_OK;
+  if (g->dev_tab) {
+    (void)hipSetDevice(g->device);
+    (void)hipStreamSynchronize(g->stream);
+    (void)hipFree(g->dev_tab);
+  }
+  delete g;
+  return AFQMC_OK;
+}
+
+extern "C" int afqmc_group_set_stream(afqmc_group* g, void* stream) {
+  if (!g) return afqmc_set_error(AFQMC_ERR_ARG, "null group");
+  g->stream = static_cast<hipStream_t>(stream);
+  return AFQMC_OK;
+}
+
+extern "C" int afqmc_group_clear(afqmc_group* g) {
+  if (!g) return afqmc_set_error(AFQMC_ERR_ARG, "null group");
+  g->mem.clear();
+  g->dirty = true;
+  return AFQMC_OK;
+}
+
+extern "C" int afqmc_group_add(afqmc_group* g, afqmc_ctx* c, void* walkers_up, void* walkers_dn,
+                               double* weights, void* overlaps, const double* fields_block,
+                               int32_t n_steps, double* scalars) {
+  if (!g || !c) return afqmc_set_error(AFQMC_ERR_ARG, "null group or context");
+  if ((int)g->mem.size() >= g->max_frag)
+    return afqmc_set_error(AFQMC_ERR_SIZE, "group holds at most %d fragments", g->max_frag);
+  if (!walkers_up || (c->nspin == 2 && !walkers_dn) || !weights || !overlaps || !fields_block ||
+      !scalars || n_steps < 1)
+    return afqmc_set_error(AFQMC_ERR_ARG, "null buffer or n_steps < 1");
+  const bool mixed = c->mixed != 0;
+  if (!g->mem.empty() && mixed != g->mixed)
+    return afqmc_set_error(AFQMC_ERR_ARG, "all fragments of a group share one precision mode");
+  if (c->device != g->device)
+    return afqmc_set_error(AFQMC_ERR_ARG, "context on device %d, group on device %d", c->device,
+                           g->device);
+  for (const Member& m : g->mem)
+    if (m.c == c) return afqmc_set_error(AFQMC_ERR_ARG, "context already in the group");
+  if (g->mem.empty()) g->mixed = mixed;
+  Member m{c, static_cast<cplx*>(walkers_up), static_cast<cplx*>(walkers_dn), weights,
+           static_cast<cplx*>(overlaps), fields_block, n_steps, scalars};
+  g->mem.push_back(m);
+  g->dirty = true;
+  return AFQMC_OK;
+}
+
+extern "C" int afqmc_group_size(const afqmc_group* g) { return g ? (int)g->mem.size() : 0; }
+
+extern "C" int afqmc_group_propagate(afqmc_group* g, int32_t step, int32_t flags) {
+  if (!g) return afqmc_set_error(AFQMC_ERR_ARG, "null group");
+  if (g->mem.empty()) return AFQMC_OK;
+  int rc = set_group_device(g);
+  if (rc) return rc;
+  for (const Member& m : g->mem)
+    if (step < 0 || step >= m.n_steps)
+      return afqmc_set_error(AFQMC_ERR_ARG, "step %d outside the bound fields block [0, %d)",
+                             step, m.n_steps);
+  if (g->dirty && (rc = build(g))) return rc;
+  bool reuse = (flags & AFQMC_STEP_REUSE_GREEN) != 0;
+  for (const Member& m : g->mem)
+    if (m.c->green_src != m.wup || m.c->green_w != m.c->d.n_walkers) reuse = false;
+  for (const Member& m : g->mem) m.c->green_src = nullptr;
+  if (!reuse && (rc = run_measure(g))) return rc;
+  if ((rc = run_gemm(g, g->fb, 0))) return rc;
+  for (const Member& m : g->mem) m.c->fb_parts = m.c->nspin;
+  if ((rc = launch_fields_grouped(g->stream, targs<FieldsArgs>(g, g->fields),
+                                  tstart(g, g->fields), g->fields.nprob, g->fields.nblocks,
+                                  step)))
+    return rc;
+  if ((rc = run_gemm(g, g->vhs, 1))) return rc;
+  for (const Launch& L : g->taylor) {
+    rc = launch_taylor_grouped(g->stream, L.k0, L.k1, L.k2, targs<TaylorArgs>(g, L),
+                               tstart(g, L), L.nprob, L.nblocks, L.k3, L.shm);
+    if (rc) return rc;
+  }
+  if ((rc = run_measure(g))) return rc;
+  for (const Member& m : g->mem) {
+    m.c->green_src = m.wup;
+    m.c->green_w = m.c->d.n_walkers;
+  }
+  return launch_weights_grouped(g->stream, targs<WeightsArgs>(g, g->weights), g->weights.nprob);
+}
+
+// Times both GEMM tile edges (64, 128) for the force-bias and VHS launches of the current
+// member set and keeps the faster one.  Runs the GEMMs on the bound buffers (their
+// outputs are workspace that every step rewrites).  Synchronises.
+extern "C" int afqmc_group_tune(afqmc_group* g, double* ms_out) {
+  if (!g) return afqmc_set_error(AFQMC_ERR_ARG, "null group");
+  if (g->mem.empty()) return AFQMC_OK;
+  int rc = set_group_device(g);
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  float best[2][2];  // [fb / vhs][64 / 128]
+  for (int bi = 0; bi < 2; ++bi) {
+    g->fb_bm = g->vhs_bm = bi ? 128 : 64;
+    if ((rc = build(g))) break;
+    for (int which = 0; which < 2 && !rc; ++which) {
+      const Launch& L = which ? g->vhs : g->fb;
+      rc = run_gemm(g, L, which);  // warm-up
+      if (rc) break;
+      (void)hipEventRecord(e0, g->stream);
+      for (int r = 0; r < 3 && !rc; ++r) rc = run_gemm(g, L, which);
+      (void)hipEventRecord(e1, g->stream);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      best[which][bi] = ms / 3.0f;
+    }
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  g->fb_bm = best[0][1] < best[0][0] ? 128 : 64;
+  g->vhs_bm = best[1][1] < best[1][0] ? 128 : 64;
+  if (ms_out) {
+    ms_out[0] = best[0][0];
+    ms_out[1] = best[0][1];
+    ms_out[2] = best[1][0];
+    ms_out[3] = best[1][1];
+  }
+  g->dirty = true;
+  return AFQMC_OK;
+}

@@ -23,16 +23,18 @@ constexpr int MEAS_KC = 4;  // k-groups whose operands are loaded ahead of their
 
 // T2 = ceil(2n / 16): column tiles of [Or | Oi] and row tiles of [Gr ; Gi]
 template <int T2, int TI, bool STAGE>
-__global__ __launch_bounds__(64) void measure_kernel(
-    int N, int n, const cplx* __restrict__ walkers, const double* __restrict__ trial,
-    cplx* __restrict__ dets, int spin, double* __restrict__ green, uint16_t* __restrict__ ghi,
-    uint16_t* __restrict__ glo, int ld16) {
+__device__ __forceinline__ void measure_body(int N, int n, const cplx* __restrict__ walkers,
+                                             const double* __restrict__ trial,
+                                             cplx* __restrict__ dets, int spin,
+                                             double* __restrict__ green,
+                                             uint16_t* __restrict__ ghi,
+                                             uint16_t* __restrict__ glo, int ld16, const int w) {
   extern __shared__ double smem[];
   const int n2 = 2 * n;
   cplx* M = reinterpret_cast<cplx*>(smem);  // n x 2n augmented matrix
   cplx* fcol = M + n * n2;                  // pivot column copy (n)
 
-  const int w = blockIdx.x, lane = threadIdx.x;
+  const int lane = threadIdx.x;
   const int s = lane >> 4, col = lane & 15;
   const cplx* phi = walkers + (int64_t)w * N * n;
   // walker operand: staged once into LDS with 16-byte loads (all in flight together)
@@ -240,6 +242,44 @@ __global__ __launch_bounds__(64) void measure_kernel(
   }
 }
 
+template <int T2, int TI, bool STAGE>
+__global__ __launch_bounds__(64) void measure_kernel(
+    int N, int n, const cplx* __restrict__ walkers, const double* __restrict__ trial,
+    cplx* __restrict__ dets, int spin, double* __restrict__ green, uint16_t* __restrict__ ghi,
+    uint16_t* __restrict__ glo, int ld16) {
+  measure_body<T2, TI, STAGE>(N, n, walkers, trial, dets, spin, green, ghi, glo, ld16,
+                              blockIdx.x);
+}
+
+template <int T2, int TI, bool STAGE>
+__global__ __launch_bounds__(64) void measure_grouped_kernel(const MeasureArgs* __restrict__ args,
+                                                             const int* __restrict__ start,
+                                                             int nprob) {
+  const int p = group_of(start, nprob, blockIdx.x);
+  const MeasureArgs a = args[p];
+  measure_body<T2, TI, STAGE>(a.N, a.n, a.walkers, a.trial, a.dets, a.spin, a.green, a.ghi,
+                              a.glo, a.ld16, blockIdx.x - start[p]);
+}
+
+template <int T2>
+int launch_measure_grouped_t(hipStream_t st, bool stage, const MeasureArgs* args,
+                             const int* start, int nprob, int nblocks, size_t shm) {
+  constexpr int TI = T2 > 4 ? 1 : (T2 + 1) / 2;
+  if (stage) {
+    hipLaunchKernelGGL((measure_grouped_kernel<T2, TI, true>), dim3(nblocks), dim3(64), shm, st,
+                       args, start, nprob);
+  } else {
+    static int lds_limit = 0;
+    const int rc =
+        ensure_dyn_lds((const void*)measure_grouped_kernel<T2, TI, false>, shm, &lds_limit);
+    if (rc) return rc;
+    hipLaunchKernelGGL((measure_grouped_kernel<T2, TI, false>), dim3(nblocks), dim3(64), shm, st,
+                       args, start, nprob);
+  }
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
 template <int T2>
 int launch_measure_t(afqmc_ctx* c, int W, int n, const cplx* wk, int s, bool g64, bool gbf) {
   // O row tiles per pass: all ceil(n / 16) of them unless the accumulators would spill
@@ -286,4 +326,31 @@ int launch_measure(afqmc_ctx* c, int W, const cplx* wup, const cplx* wdn, bool g
     if (rc) return rc;
   }
   return AFQMC_OK;
+}
+
+// Grouped form: one launch for every (fragment, spin) problem with the same column-tile
+// count t2 and staging choice; shm is the largest LDS need among them.
+int launch_measure_grouped(hipStream_t st, int t2, bool stage, const MeasureArgs* args,
+                           const int* start, int nprob, int nblocks, size_t shm) {
+  switch (t2) {
+    case 1: return launch_measure_grouped_t<1>(st, stage, args, start, nprob, nblocks, shm);
+    case 2: return launch_measure_grouped_t<2>(st, stage, args, start, nprob, nblocks, shm);
+    case 3: return launch_measure_grouped_t<3>(st, stage, args, start, nprob, nblocks, shm);
+    case 4: return launch_measure_grouped_t<4>(st, stage, args, start, nprob, nblocks, shm);
+    case 5: return launch_measure_grouped_t<5>(st, stage, args, start, nprob, nblocks, shm);
+    case 6: return launch_measure_grouped_t<6>(st, stage, args, start, nprob, nblocks, shm);
+    case 7: return launch_measure_grouped_t<7>(st, stage, args, start, nprob, nblocks, shm);
+    default: return launch_measure_grouped_t<8>(st, stage, args, start, nprob, nblocks, shm);
+  }
+}
+
+// The kernel launch_measure picks for spin s, as a plan for grouped launches.
+void measure_plan(const afqmc_ctx* c, int s, int* t2, bool* stage, size_t* shm) {
+  const int n = c->nocc[s], N = c->d.norb;
+  const int t = (2 * n + 15) / 16;
+  *t2 = t > 8 ? 8 : t;
+  const size_t base = (size_t)(n * 2 * n + n) * sizeof(cplx);
+  const size_t staged = base + (size_t)N * n * sizeof(cplx);
+  *stage = staged <= 24 * 1024;
+  *shm = *stage ? staged : base;
 }

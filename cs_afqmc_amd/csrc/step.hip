@@ -35,13 +35,12 @@ __device__ inline cplx fb_sum(const double* __restrict__ fb, int nparts, int64_t
   return cplx{re, im};
 }
 
-__global__ __launch_bounds__(FIELDS_NT) void fields_kernel(
+__device__ __forceinline__ void fields_body(
     int G, double sqdt, const double* __restrict__ fb, int nparts, int64_t part_stride,
     const cplx* __restrict__ mf, const double* __restrict__ fields, double* __restrict__ xs,
     uint16_t* __restrict__ xs_hi, uint16_t* __restrict__ xs_lo, int Gp,
-    cplx* __restrict__ terms, cplx* __restrict__ fb_out) {
+    cplx* __restrict__ terms, cplx* __restrict__ fb_out, const int w) {
   __shared__ double red[FIELDS_NT / 64];
-  const int w = blockIdx.x;
   const double* x = fields + (int64_t)w * G;
   double* xr = xs + (int64_t)w * 2 * G;
   double* xi = xr + G;
@@ -80,8 +79,27 @@ __global__ __launch_bounds__(FIELDS_NT) void fields_kernel(
   }
 }
 
+__global__ __launch_bounds__(FIELDS_NT) void fields_kernel(
+    int G, double sqdt, const double* __restrict__ fb, int nparts, int64_t part_stride,
+    const cplx* __restrict__ mf, const double* __restrict__ fields, double* __restrict__ xs,
+    uint16_t* __restrict__ xs_hi, uint16_t* __restrict__ xs_lo, int Gp,
+    cplx* __restrict__ terms, cplx* __restrict__ fb_out) {
+  fields_body(G, sqdt, fb, nparts, part_stride, mf, fields, xs, xs_hi, xs_lo, Gp, terms, fb_out,
+              blockIdx.x);
+}
+
+// one block per walker of every fragment; step selects the fields of this step
+__global__ __launch_bounds__(FIELDS_NT) void fields_grouped_kernel(
+    const FieldsArgs* __restrict__ args, const int* __restrict__ start, int nprob, int step) {
+  const int p = group_of(start, nprob, blockIdx.x);
+  const FieldsArgs a = args[p];
+  fields_body(a.G, a.sqdt, a.fb, a.nparts, a.part_stride, a.mf,
+              a.fields + (int64_t)step * a.W * a.G, a.xs, a.xs_hi, a.xs_lo, a.Gp, a.terms,
+              nullptr, blockIdx.x - start[p]);
+}
+
 // single workgroup over all walkers (the weight sum needs every walker)
-__global__ __launch_bounds__(WEIGHTS_NT) void weights_kernel(
+__device__ __forceinline__ void weights_body(
     int W, int nspin, double dt, double sqdt, double h0_prop, const cplx* __restrict__ dets,
     const cplx* __restrict__ terms, double* __restrict__ weights, cplx* __restrict__ overlaps,
     double* __restrict__ scalars) {
@@ -109,6 +127,21 @@ __global__ __launch_bounds__(WEIGHTS_NT) void weights_kernel(
   }
   const double total = block_sum<WEIGHTS_NT>(part, red);
   if (threadIdx.x == 0) scalars[1] = e_estimate - 0.1 * (log(total / (double)W) / dt);
+}
+
+__global__ __launch_bounds__(WEIGHTS_NT) void weights_kernel(
+    int W, int nspin, double dt, double sqdt, double h0_prop, const cplx* __restrict__ dets,
+    const cplx* __restrict__ terms, double* __restrict__ weights, cplx* __restrict__ overlaps,
+    double* __restrict__ scalars) {
+  weights_body(W, nspin, dt, sqdt, h0_prop, dets, terms, weights, overlaps, scalars);
+}
+
+// one block per fragment
+__global__ __launch_bounds__(WEIGHTS_NT) void weights_grouped_kernel(
+    const WeightsArgs* __restrict__ args) {
+  const WeightsArgs a = args[blockIdx.x];
+  weights_body(a.W, a.nspin, a.dt, a.sqdt, a.h0_prop, a.dets, a.terms, a.weights, a.overlaps,
+               a.scalars);
 }
 
 // sampling.py:79-91: clamp, weighted block energy, population-control mix
@@ -193,6 +226,20 @@ int launch_fb_out(afqmc_ctx* c, int W, cplx* fb_out) {
   hipLaunchKernelGGL(fb_out_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
                      c->stream, W, c->d.nchol, c->fb, c->fb_parts, (int64_t)2 * W * c->d.nchol,
                      fb_out);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+int launch_fields_grouped(hipStream_t st, const FieldsArgs* args, const int* start, int nprob,
+                          int nblocks, int step) {
+  hipLaunchKernelGGL(fields_grouped_kernel, dim3(nblocks), dim3(FIELDS_NT), 0, st, args, start,
+                     nprob, step);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+int launch_weights_grouped(hipStream_t st, const WeightsArgs* args, int nprob) {
+  hipLaunchKernelGGL(weights_grouped_kernel, dim3(nprob), dim3(WEIGHTS_NT), 0, st, args);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }

@@ -67,11 +67,12 @@ __device__ inline void put_tile(double* tr, double* ti, int rb, int lane, const 
   }
 }
 
-__global__ __launch_bounds__(1024) void taylor_kernel(
-    int N, int n, int n_terms_k, cplx* __restrict__ walkers, const double* __restrict__ expE,
-    const double* __restrict__ vhs, int rows_pad) {
+__device__ __forceinline__ void taylor_body(int N, int n, int n_terms_k,
+                                            cplx* __restrict__ walkers,
+                                            const double* __restrict__ expE,
+                                            const double* __restrict__ vhs, int rows_pad,
+                                            const int w, const int chunk) {
   extern __shared__ double T[];
-  const int w = blockIdx.x, chunk = blockIdx.y;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rb = wave * 16;
   const int c0 = chunk * 16;
@@ -133,6 +134,23 @@ __global__ __launch_bounds__(1024) void taylor_kernel(
   }
 }
 
+__global__ __launch_bounds__(1024) void taylor_kernel(
+    int N, int n, int n_terms_k, cplx* __restrict__ walkers, const double* __restrict__ expE,
+    const double* __restrict__ vhs, int rows_pad) {
+  taylor_body(N, n, n_terms_k, walkers, expE, vhs, rows_pad, blockIdx.x, blockIdx.y);
+}
+
+// grouped: local block b = walker * chunks + chunk, chunks = ceil(n / 16)
+__global__ __launch_bounds__(1024) void taylor_grouped_kernel(const TaylorArgs* __restrict__ args,
+                                                              const int* __restrict__ start,
+                                                              int nprob) {
+  const int p = group_of(start, nprob, blockIdx.x);
+  const TaylorArgs a = args[p];
+  const int b = blockIdx.x - start[p], chunks = (a.n + 15) / 16;
+  taylor_body(a.N, a.n, a.K, a.walkers, static_cast<const double*>(a.E),
+              static_cast<const double*>(a.vhs), a.rows_pad, b / chunks, b % chunks);
+}
+
 // ------------------------------------------------------------------------------------
 // Register-resident form (N <= 128, n <= 32).  One workgroup per (walker, spin); wave r
 // keeps rows [16 r, 16 r + 16) of A = [Vr | Vi] (N x 2Np, Np = N rounded up to 4) in
@@ -148,12 +166,12 @@ struct RegTile {
 };
 
 template <int NKGV, int CTP>
-__global__ __launch_bounds__(512) void taylor_reg_kernel(
-    int N, int n, int n_terms_k, cplx* __restrict__ walkers, const double* __restrict__ expE,
-    const double* __restrict__ vhs) {
+__device__ __forceinline__ void taylor_reg_body(int N, int n, int n_terms_k,
+                                                cplx* __restrict__ walkers,
+                                                const double* __restrict__ expE,
+                                                const double* __restrict__ vhs, const int w) {
   constexpr int RS = RegTile<CTP>::RS;
   extern __shared__ double TT[];  // [2 Np][RS]
-  const int w = blockIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rb = wave * 16;
   const int Np = (N + 3) & ~3;
@@ -284,6 +302,22 @@ __global__ __launch_bounds__(512) void taylor_reg_kernel(
 }
 
 template <int NKGV, int CTP>
+__global__ __launch_bounds__(512) void taylor_reg_kernel(
+    int N, int n, int n_terms_k, cplx* __restrict__ walkers, const double* __restrict__ expE,
+    const double* __restrict__ vhs) {
+  taylor_reg_body<NKGV, CTP>(N, n, n_terms_k, walkers, expE, vhs, blockIdx.x);
+}
+
+template <int NKGV, int CTP>
+__global__ __launch_bounds__(512) void taylor_reg_grouped_kernel(
+    const TaylorArgs* __restrict__ args, const int* __restrict__ start, int nprob) {
+  const int p = group_of(start, nprob, blockIdx.x);
+  const TaylorArgs a = args[p];
+  taylor_reg_body<NKGV, CTP>(a.N, a.n, a.K, a.walkers, static_cast<const double*>(a.E),
+                             static_cast<const double*>(a.vhs), blockIdx.x - start[p]);
+}
+
+template <int NKGV, int CTP>
 int launch_reg(afqmc_ctx* c, int W, int n, cplx* wk, const double* E) {
   const int N = c->d.norb;
   const int nwaves = (N + 15) / 16;
@@ -309,12 +343,12 @@ struct RegTileF {
 };
 
 template <int NKGV, int CTP>
-__global__ __launch_bounds__(512) void taylor_reg_f32_kernel(
-    int N, int n, int n_terms_k, cplx* __restrict__ walkers, const float* __restrict__ expE,
-    const float* __restrict__ vhs) {
+__device__ __forceinline__ void taylor_reg_f32_body(int N, int n, int n_terms_k,
+                                                    cplx* __restrict__ walkers,
+                                                    const float* __restrict__ expE,
+                                                    const float* __restrict__ vhs, const int w) {
   constexpr int RS = RegTileF<CTP>::RS;
   extern __shared__ float TF[];  // [2 Np][RS]
-  const int w = blockIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rb = wave * 16;
   const int Np = (N + 3) & ~3;
@@ -440,6 +474,22 @@ __global__ __launch_bounds__(512) void taylor_reg_f32_kernel(
     }
 }
 
+template <int NKGV, int CTP>
+__global__ __launch_bounds__(512) void taylor_reg_f32_kernel(
+    int N, int n, int n_terms_k, cplx* __restrict__ walkers, const float* __restrict__ expE,
+    const float* __restrict__ vhs) {
+  taylor_reg_f32_body<NKGV, CTP>(N, n, n_terms_k, walkers, expE, vhs, blockIdx.x);
+}
+
+template <int NKGV, int CTP>
+__global__ __launch_bounds__(512) void taylor_reg_f32_grouped_kernel(
+    const TaylorArgs* __restrict__ args, const int* __restrict__ start, int nprob) {
+  const int p = group_of(start, nprob, blockIdx.x);
+  const TaylorArgs a = args[p];
+  taylor_reg_f32_body<NKGV, CTP>(a.N, a.n, a.K, a.walkers, static_cast<const float*>(a.E),
+                                 static_cast<const float*>(a.vhs), blockIdx.x - start[p]);
+}
+
 // ------------------------------------------------------------------------------------
 // Float32 streaming form for the mixed mode beyond the register kernel's range (N > 128
 // or n > 32).  One workgroup per walker of one spin; wave r owns output rows
@@ -457,12 +507,13 @@ struct StreamTileF {
 };
 
 template <int CT>
-__global__ __launch_bounds__(1024) void taylor_stream_f32_kernel(
-    int N, int n, int n_terms_k, cplx* __restrict__ walkers, const float* __restrict__ expE,
-    const float* __restrict__ vhs) {
+__device__ __forceinline__ void taylor_stream_f32_body(int N, int n, int n_terms_k,
+                                                       cplx* __restrict__ walkers,
+                                                       const float* __restrict__ expE,
+                                                       const float* __restrict__ vhs,
+                                                       const int w) {
   constexpr int LD = StreamTileF<CT>::LD;
   extern __shared__ float Y[];  // [2][Np][LD]: re plane, im plane
-  const int w = blockIdx.x;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rb = wave * 16;
   const int Np = (N + 3) & ~3;
@@ -582,6 +633,22 @@ __global__ __launch_bounds__(1024) void taylor_stream_f32_kernel(
 }
 
 template <int CT>
+__global__ __launch_bounds__(1024) void taylor_stream_f32_kernel(
+    int N, int n, int n_terms_k, cplx* __restrict__ walkers, const float* __restrict__ expE,
+    const float* __restrict__ vhs) {
+  taylor_stream_f32_body<CT>(N, n, n_terms_k, walkers, expE, vhs, blockIdx.x);
+}
+
+template <int CT>
+__global__ __launch_bounds__(1024) void taylor_stream_f32_grouped_kernel(
+    const TaylorArgs* __restrict__ args, const int* __restrict__ start, int nprob) {
+  const int p = group_of(start, nprob, blockIdx.x);
+  const TaylorArgs a = args[p];
+  taylor_stream_f32_body<CT>(a.N, a.n, a.K, a.walkers, static_cast<const float*>(a.E),
+                             static_cast<const float*>(a.vhs), blockIdx.x - start[p]);
+}
+
+template <int CT>
 int launch_stream_f32(afqmc_ctx* c, int W, int n, cplx* wk, const float* E) {
   const int N = c->d.norb;
   const int nwaves = (N + 15) / 16;
@@ -633,6 +700,46 @@ int launch_reg_ct(afqmc_ctx* c, int W, int n, cplx* wk, const double* E) {
 }
 
 }  // namespace
+
+// The kernel launch_taylor picks for spin s, as a plan for grouped launches.
+void taylor_plan(const afqmc_ctx* c, int s, TaylorPlan* pl) {
+  const int N = c->d.norb, n = c->nocc[s];
+  const int Np = (N + 3) & ~3;
+  const int nkgv = 2 * ((N + 3) / 4);
+  const int bucket = nkgv <= 32 ? 32 : nkgv <= 48 ? 48 : nkgv <= 56 ? 56 : 64;
+  const int ctp = (2 * n + 15) / 16 > 4 ? 4 : (2 * n + 15) / 16;
+  pl->nthreads = ((N + 15) / 16) * 64;
+  pl->blocks_per_walker = 1;
+  pl->nkgv = 0;
+  pl->ct = 0;
+  if (c->taylor_f32) {
+    if (N <= 128 && n <= 32) {
+      pl->kind = TAYLOR_REG_F32;
+      pl->nkgv = bucket;
+      pl->ct = ctp;
+      int rs = 16 * ctp;
+      if (ctp == 2) rs = 48;
+      if (ctp == 4) rs = 80;
+      pl->shm = (size_t)2 * Np * rs * sizeof(float);
+    } else {
+      const int ct = (n + 15) / 16 > 4 ? 4 : (n + 15) / 16;
+      pl->kind = TAYLOR_STREAM_F32;
+      pl->ct = ct;
+      const int ld = (ct == 2) ? 48 : (ct == 4 ? 64 : 16 * ct);
+      pl->shm = (size_t)2 * Np * ld * sizeof(float);
+    }
+  } else if (N <= 128 && c->nocc[0] <= 32 && c->nocc[1] <= 32) {
+    pl->kind = TAYLOR_REG_F64;
+    pl->nkgv = bucket;
+    pl->ct = ctp;
+    const int rs = 16 * ctp + ((ctp % 2 == 0) ? 16 : 0);
+    pl->shm = (size_t)2 * Np * rs * sizeof(double);
+  } else {
+    pl->kind = TAYLOR_STREAM_F64;
+    pl->blocks_per_walker = (n + 15) / 16;
+    pl->shm = (size_t)((N + 15) / 16) * 16 * 16 * 4 * sizeof(double);
+  }
+}
 
 int launch_taylor(afqmc_ctx* c, int W, cplx* wup, cplx* wdn) {
   const int N = c->d.norb;
@@ -689,4 +796,67 @@ int launch_taylor(afqmc_ctx* c, int W, cplx* wup, cplx* wdn) {
     LAUNCH_CHECK();
   }
   return AFQMC_OK;
+}
+
+namespace {
+
+template <typename KernelFn>
+int launch_grouped_fn(KernelFn fn, hipStream_t st, const TaylorArgs* args, const int* start,
+                      int nprob, int nblocks, int nthreads, size_t shm, int* lds_limit) {
+  const int rc = ensure_dyn_lds((const void*)fn, shm, lds_limit);
+  if (rc) return rc;
+  hipLaunchKernelGGL(fn, dim3(nblocks), dim3(nthreads), shm, st, args, start, nprob);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+template <int NKGV>
+int reg_grouped(bool f32, int ct, hipStream_t st, const TaylorArgs* args, const int* start,
+                int nprob, int nblocks, int nthreads, size_t shm) {
+  static int lim[2][4] = {};
+  const int ci = ct < 1 ? 0 : (ct > 4 ? 3 : ct - 1);
+#define REG_CASE(C)                                                                          \
+  case C:                                                                                    \
+    return f32 ? launch_grouped_fn(taylor_reg_f32_grouped_kernel<NKGV, C>, st, args, start,  \
+                                   nprob, nblocks, nthreads, shm, &lim[1][ci])               \
+               : launch_grouped_fn(taylor_reg_grouped_kernel<NKGV, C>, st, args, start,      \
+                                   nprob, nblocks, nthreads, shm, &lim[0][ci]);
+  switch (ct) {
+    REG_CASE(1)
+    REG_CASE(2)
+    REG_CASE(3)
+    REG_CASE(4)
+  }
+#undef REG_CASE
+  return afqmc_set_error(AFQMC_ERR_ARG, "grouped taylor: column tiles %d", ct);
+}
+
+}  // namespace
+
+// Grouped Taylor launch for problems of one kernel kind and template key (nkgv bucket,
+// column tiles ct) and one block size nthreads = 64 * ceil(N / 16).
+int launch_taylor_grouped(hipStream_t st, int kind, int nkgv, int ct, const TaylorArgs* args,
+                          const int* start, int nprob, int nblocks, int nthreads, size_t shm) {
+  if (kind == TAYLOR_REG_F64 || kind == TAYLOR_REG_F32) {
+    const bool f32 = kind == TAYLOR_REG_F32;
+    switch (nkgv) {
+      case 32: return reg_grouped<32>(f32, ct, st, args, start, nprob, nblocks, nthreads, shm);
+      case 48: return reg_grouped<48>(f32, ct, st, args, start, nprob, nblocks, nthreads, shm);
+      case 56: return reg_grouped<56>(f32, ct, st, args, start, nprob, nblocks, nthreads, shm);
+      case 64: return reg_grouped<64>(f32, ct, st, args, start, nprob, nblocks, nthreads, shm);
+      default: return afqmc_set_error(AFQMC_ERR_ARG, "grouped taylor: nkgv %d", nkgv);
+    }
+  }
+  if (kind == TAYLOR_STREAM_F32) {
+    static int lim[4] = {};
+    switch (ct) {
+      case 1: return launch_grouped_fn(taylor_stream_f32_grouped_kernel<1>, st, args, start, nprob, nblocks, nthreads, shm, &lim[0]);
+      case 2: return launch_grouped_fn(taylor_stream_f32_grouped_kernel<2>, st, args, start, nprob, nblocks, nthreads, shm, &lim[1]);
+      case 3: return launch_grouped_fn(taylor_stream_f32_grouped_kernel<3>, st, args, start, nprob, nblocks, nthreads, shm, &lim[2]);
+      default: return launch_grouped_fn(taylor_stream_f32_grouped_kernel<4>, st, args, start, nprob, nblocks, nthreads, shm, &lim[3]);
+    }
+  }
+  static int lim64 = 0;
+  return launch_grouped_fn(taylor_grouped_kernel, st, args, start, nprob, nblocks, nthreads, shm,
+                           &lim64);
 }

@@ -223,3 +223,63 @@ class DeviceStep:
         _lib.check(self._lib.afqmc_prof_collect(
             self._h, ms.ctypes.data_as(ctypes.c_void_p), cnt.ctypes.data_as(ctypes.c_void_p)))
         return {p: (float(ms[i]), int(cnt[i])) for i, p in enumerate(_lib.PHASES)}
+
+
+class DeviceGroup:
+    """A fragment group of the HIP library (afqmc_group_*): many DeviceStep contexts
+    advanced by one grouped step.  Members bind their walker state and a fields block
+    (n_steps, W, nchol); ``propagate(step)`` advances every member by one step."""
+
+    def __init__(self, max_fragments=256):
+        if not torch.cuda.is_available():
+            raise _lib.AfqmcError("cs_afqmc_amd needs a ROCm GPU (torch.cuda is unavailable)")
+        L = _lib.lib()
+        handle = ctypes.c_void_p()
+        _lib.check(L.afqmc_group_create(ctypes.byref(handle), int(max_fragments), _stream()))
+        self._h = handle
+        self._lib = L
+        self._members = []  # keeps the bound tensors alive
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.afqmc_group_destroy(h)
+            self._h = None
+
+    def _prep(self):
+        _lib.check(self._lib.afqmc_group_set_stream(self._h, _stream()))
+
+    def clear(self):
+        _lib.check(self._lib.afqmc_group_clear(self._h))
+        self._members = []
+
+    def add(self, dev, walkers, weights, overlaps, fields_block, scalars):
+        up, dn, W = dev._walkers(walkers)
+        if W != dev.n_walkers:
+            raise ValueError("group members run at their full walker capacity")
+        dev._vec(weights, W, torch.float64, "weights")
+        dev._vec(overlaps, W, torch.complex128, "overlaps")
+        if (fields_block.dtype != torch.float64 or not fields_block.is_contiguous()
+                or fields_block.dim() != 3 or tuple(fields_block.shape[1:]) != (W, dev.nchol)):
+            raise ValueError("fields_block must be contiguous float64 (n_steps, W, nchol)")
+        if scalars.dtype != torch.float64 or scalars.numel() != 2:
+            raise ValueError("scalars must be float64 [e_estimate, pop_control_ene_shift]")
+        _lib.check(self._lib.afqmc_group_add(
+            self._h, dev._h, _dptr(up), _dptr(dn), _dptr(weights), _dptr(overlaps),
+            _dptr(fields_block), int(fields_block.shape[0]), _dptr(scalars)))
+        self._members.append((dev, walkers, weights, overlaps, fields_block, scalars))
+
+    def __len__(self):
+        return int(self._lib.afqmc_group_size(self._h))
+
+    def propagate(self, step, reuse_green=False):
+        self._prep()
+        flags = _lib.AFQMC_STEP_REUSE_GREEN if reuse_green else 0
+        _lib.check(self._lib.afqmc_group_propagate(self._h, int(step), flags))
+
+    def tune(self):
+        """Pick GEMM tiles by timing; returns {fb: (ms64, ms128), vhs: (ms64, ms128)}."""
+        self._prep()
+        ms = np.zeros(4, dtype=np.float64)
+        _lib.check(self._lib.afqmc_group_tune(self._h, ms.ctypes.data_as(ctypes.c_void_p)))
+        return {"fb": (ms[0], ms[1]), "vhs": (ms[2], ms[3])}

@@ -80,3 +80,16 @@ def random_walkers(walker_type, norb, nelec, n_walkers, rng, near=None, noise=0.
         return one(nelec[0], near)
     base = near if near is not None else [None, None]
     return [one(nelec[0], base[0]), one(nelec[1], base[1])]
+
+
+def fragment_specs(count, seed=0, norb_range=(20, 120), walkers=(256,)):
+    """Synthetic LNO-style fragment set: norb uniform in norb_range, nocc = norb // 5,
+    nchol = 4 norb, closed shell, walkers drawn from `walkers`.  Returns a list of
+    (norb, nocc, nchol, n_walkers, seed) tuples."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        norb = int(rng.integers(norb_range[0], norb_range[1] + 1))
+        nocc = max(1, norb // 5)
+        out.append((norb, nocc, 4 * norb, int(rng.choice(walkers)), int(seed * 100003 + i)))
+    return out

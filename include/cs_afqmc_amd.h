@@ -185,6 +185,36 @@ AFQMC_API int afqmc_prof_collect(afqmc_ctx* ctx, double* ms, int64_t* counts);
 AFQMC_API int afqmc_fields_normal(afqmc_ctx* ctx, uint64_t seed, uint64_t counter,
                         double* fields, int64_t count);
 
+/* ---- Fragment groups: many independent problems advanced by one grouped step.
+ *
+ * No single counterpart in the reference: it replaces the loop a fragment driver runs
+ * over propagator.propagate (propagation.py:100-165) of each LNO fragment
+ * (lno_afqmc/lno_afqmc.py:442-605 runs one AFQMC per fragment), with every fragment's
+ * step in the same launches.  Each fragment keeps its own context (integrals,
+ * workspace) and caller-owned state; afqmc_group_add binds that state:
+ *   walkers_up / walkers_dn, weights, overlaps, scalars: as for afqmc_propagate, with
+ *     n_walkers = the context's capacity;
+ *   fields_block dev (n_steps, W, nchol) float64: step s of afqmc_group_propagate
+ *     reads fields_block + s * W * nchol.
+ * All fragments of a group share one precision mode and device.  A fragment's result
+ * does not depend on the other members (no split-K, fixed k order in every GEMM).
+ * Calls on a group must not overlap in time with calls on its member contexts. */
+typedef struct afqmc_group afqmc_group;
+AFQMC_API int afqmc_group_create(afqmc_group** out, int32_t max_fragments, void* stream);
+AFQMC_API int afqmc_group_destroy(afqmc_group* g);
+AFQMC_API int afqmc_group_set_stream(afqmc_group* g, void* stream);
+AFQMC_API int afqmc_group_clear(afqmc_group* g);
+AFQMC_API int afqmc_group_add(afqmc_group* g, afqmc_ctx* ctx, void* walkers_up,
+                              void* walkers_dn, double* weights, void* overlaps,
+                              const double* fields_block, int32_t n_steps, double* scalars);
+AFQMC_API int afqmc_group_size(const afqmc_group* g);
+/* One phaseless step of every member (flags: 0 or AFQMC_STEP_REUSE_GREEN). */
+AFQMC_API int afqmc_group_propagate(afqmc_group* g, int32_t step, int32_t flags);
+/* Picks the GEMM tile edge (64 or 128) of the force-bias and VHS launches by timing
+ * both on the current members; ms_out (4 floats64, may be NULL) receives the times
+ * [fb 64, fb 128, vhs 64, vhs 128].  Synchronises. */
+AFQMC_API int afqmc_group_tune(afqmc_group* g, double* ms_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,3 +41,17 @@ def test_error_path_without_gpu_work():
     rc = L.afqmc_ctx_create(ctypes.byref(h), ctypes.byref(dims), ctypes.byref(ham), None)
     assert rc == 3 and b"norb" in L.afqmc_last_error()
     assert L.afqmc_propagate(None, 4, None, None, None, None, None, None, None, 0) == 1
+
+
+def test_group_error_paths_without_gpu_work():
+    L = _lib.lib()
+    assert L.afqmc_group_create(None, 4, None) == 1
+    g = ctypes.c_void_p()
+    assert L.afqmc_group_create(ctypes.byref(g), 0, None) == 1
+    assert b"max_fragments" in L.afqmc_last_error()
+    assert L.afqmc_group_add(None, None, None, None, None, None, None, 1, None) == 1
+    assert L.afqmc_group_propagate(None, 0, 0) == 1
+    assert L.afqmc_group_tune(None, None) == 1
+    assert L.afqmc_group_clear(None) == 1
+    assert L.afqmc_group_size(None) == 0
+    assert L.afqmc_group_destroy(None) == 0

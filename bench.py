@@ -338,6 +338,183 @@ def run(args, precision):
     return res, workload
 
 
+# ------------------------------------------------------------------ fragment batch
+
+
+FRAG_WORKLOAD = "fragments64_n20-120_w256"
+
+
+def _make_fragments(specs, precision, n_prop_steps):
+    from cs_afqmc_amd import batch, problems
+    frags = []
+    for norb, nocc, nchol, W, seed in specs:
+        hd, wd, ne = problems.synthetic_rhf(norb, nocc, nchol, seed=seed)
+        frags.append(batch.Fragment(hd, wd, "rhf", ne, W, dt=0.005, seed=seed,
+                                    precision=precision, n_prop_steps=n_prop_steps,
+                                    tag=(norb, nocc, nchol, W)))
+    return frags
+
+
+def _single_block(f):
+    """The plain (ungrouped) form: one fragment's block through afqmc_propagate."""
+    pd = f.prop_data
+    f.begin_block()
+    for s in range(f.n_prop_steps):
+        f.dev.propagate(pd["walkers"], pd["weights"], pd["overlaps"], f.fields[s], f.scalars,
+                        reuse_green=s > 0)
+    f.end_block()
+
+
+def run_fragments(args):
+    """Survey config 4: 64 closed-shell fragments, norb uniform in [20, 120], nocc =
+    norb / 5, nchol = 4 norb, 256 walkers each.  value = walker-steps/s of the grouped
+    mixed-precision step; fp64_value = the plain float64 form (each fragment through
+    afqmc_propagate in turn); also the grouped float64 rate."""
+    import numpy as np
+    import torch
+
+    from cs_afqmc_amd import batch, problems
+
+    torch.cuda.set_device(0)
+    n_prop = N_PROP_STEPS
+    blocks = max(1, args.steps // n_prop)
+    wblocks = max(1, args.warmup // n_prop)
+    specs = problems.fragment_specs(64, seed=0)
+    total_w = sum(s[3] for s in specs)
+    res = {}
+    energies = {}
+    for mode in ("bf16x3", "fp64", "fp64_plain"):
+        prec = "fp64" if mode.startswith("fp64") else mode
+        frags = _make_fragments(specs, prec, n_prop)
+        bt = None if mode == "fp64_plain" else batch.FragmentBatch(frags, tune=True)
+        for _ in range(wblocks):
+            if bt is None:
+                for f in frags:
+                    _single_block(f)
+            else:
+                bt.run_block()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(blocks):
+            if bt is None:
+                for f in frags:
+                    _single_block(f)
+            else:
+                bt.run_block()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        res[mode] = total_w * n_prop * blocks / el
+        energies[mode] = [f.block_energies()[0] for f in frags]
+        print(f"# {mode}: {blocks} blocks in {el:.3f} s", file=sys.stderr, flush=True)
+        if bt is not None and bt.tuned:
+            res[mode + "_tuned"] = bt.tuned
+        del frags, bt
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+    d = [float(np.max(np.abs(a - b))) for a, b in zip(energies["bf16x3"], energies["fp64_plain"])
+         if a.size]
+    dg = [float(np.max(np.abs(a - b))) for a, b in zip(energies["fp64"], energies["fp64_plain"])
+          if a.size]
+    finite = [bool(np.all(np.isfinite(a))) for a in energies["bf16x3"]]
+    return {
+        "metric": "walker-steps/s mixed vs FP64 GPU at |dE|<0.1 mEh (batched fragments)",
+        "value": round(res["bf16x3"], 1),
+        "unit": "walker-steps/s",
+        "n_gpus": 1,
+        "steps": blocks * n_prop,
+        "warmup": wblocks * n_prop,
+        "ms_per_step": round(total_w / res["bf16x3"] * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE["bf16x3"],
+        "data": "synthetic fragments (seeded Cholesky vectors, h1, Haar trial; device Philox "
+                "fields)",
+        "config": {"workload": FRAG_WORKLOAD, "n_fragments": len(specs),
+                   "norb": [s[0] for s in specs], "n_walkers_total": total_w,
+                   "n_prop_steps": n_prop, "dt": 0.005, "precision": "bf16x3"},
+        "fp64_value": round(res["fp64_plain"], 1),
+        "fp64_grouped_value": round(res["fp64"], 1),
+        "speedup_vs_fp64": round(res["bf16x3"] / res["fp64_plain"], 3),
+        "delta_e_max_eh": max(d) if d else None,
+        "delta_e_fp64_grouped_vs_plain_eh": max(dg) if dg else None,
+        "fragments_finite": sum(finite),
+        "gemm_tuning_ms": {k: v for k, v in res.items() if k.endswith("_tuned")},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
+# ------------------------------------------------------------------ serving
+
+
+SERVE_WORKLOAD = "serve_poisson200_n20-250"
+
+
+def run_serving(args):
+    """Survey config 5: Poisson arrivals (mean 200 requests/s) of fragment solves with
+    norb uniform in [20, 250], nocc = norb / 5, nchol = 4 norb, walkers in {128, 256,
+    512, 1024}, 100 steps each; a bounded sample of --requests requests.  Integrals are
+    the requests' payload (made before the clock starts); intermediates, context
+    creation and the solve are the server's work.  value = walker-steps/s of the whole
+    run (completed walker-steps / makespan); p50 / p99 latency from arrival to GPU
+    completion."""
+    import numpy as np
+    import torch
+
+    from cs_afqmc_amd import problems, serving
+
+    torch.cuda.set_device(0)
+    rng = np.random.default_rng(2024)
+    t = 0.0
+    reqs = []
+    for i in range(args.requests):
+        t += rng.exponential(1.0 / 200.0)
+        norb = int(rng.integers(20, 251))
+        nocc = max(1, norb // 5)
+        W = int(rng.choice([128, 256, 512, 1024]))
+        hd, wd, ne = problems.synthetic_rhf(norb, nocc, 4 * norb, seed=5000 + i)
+        reqs.append(serving.Request(arrival=t, rid=i, ham_data=hd, wave_data=wd, nelec=ne,
+                                    n_walkers=W, n_steps=100, seed=17 + i))
+    print(f"# serving: {len(reqs)} requests generated", file=sys.stderr, flush=True)
+    srv = serving.FragmentServer(precision="bf16x3", n_prop_steps=N_PROP_STEPS,
+                                 hbm_budget=200 << 30, max_active=256)
+    t0 = time.perf_counter()
+    res = srv.serve(reqs)
+    wall = time.perf_counter() - t0
+    ws = sum(r.walker_steps for r in res)
+    makespan = max(r.done for r in res)
+    st = serving.latency_stats(res)
+    return {
+        "metric": "p50/p99 fragment latency under load, with walker-steps/s (serving)",
+        "value": round(ws / makespan, 1),
+        "unit": "walker-steps/s",
+        "n_gpus": 1,
+        "steps": 100,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE["bf16x3"],
+        "data": "synthetic fragment requests (seeded integrals, Haar trial), Poisson arrivals",
+        "config": {"workload": SERVE_WORKLOAD, "n_requests": len(reqs),
+                   "arrival_rate_per_s": 200.0, "n_prop_steps": N_PROP_STEPS,
+                   "steps_per_request": 100, "hbm_budget_gib": 200, "precision": "bf16x3"},
+        "latency_p50_s": st["p50_s"],
+        "latency_p99_s": st["p99_s"],
+        "latency_mean_s": st["mean_s"],
+        "latency_max_s": st.get("max_s"),
+        "makespan_s": makespan,
+        "wall_s": wall,
+        "peak_active_fragments": srv.peak_active,
+        "peak_reserved_fragments": srv.peak_reserved,
+        "completed": len(res),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 DTYPE = {"fp64": "f64", "bf16x3": "mixed: bf16x3 MFMA contractions, f32 Taylor, f64 rest"}
 
 
@@ -351,6 +528,8 @@ def main():
                     help="compare: float64 then mixed at the same fields (default)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--requests", type=int, default=32,
+                    help="number of requests of the serving workload")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy before the timed region (PMC scaling)")
     args = ap.parse_args()
@@ -358,6 +537,12 @@ def main():
         print(f"bench.py: this build runs the walker population on one GPU; --gpus {args.gpus} "
               "is not supported (multi-GPU sharding is a design note only).", file=sys.stderr)
         sys.exit(2)
+    if args.workload == FRAG_WORKLOAD:
+        print(json.dumps(run_fragments(args)), flush=True)
+        return
+    if args.workload == SERVE_WORKLOAD:
+        print(json.dumps(run_serving(args)), flush=True)
+        return
     modes = ["fp64", "bf16x3"] if args.precision == "compare" else [args.precision]
     cpu = None
     if not args.no_cpu_baseline:

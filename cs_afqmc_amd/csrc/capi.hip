@@ -32,11 +32,13 @@ int dalloc(afqmc_ctx* c, T** p, size_t count) {
   return AFQMC_OK;
 }
 
+// stream-ordered on the context's stream; ctx creation synchronises that stream (only)
+// before it returns, so the host arrays may be released afterwards
 template <typename T>
 int upload(afqmc_ctx* c, T** p, const T* host, size_t count) {
   int rc = dalloc(c, p, count);
   if (rc) return rc;
-  HIP_TRY(hipMemcpy(*p, host, count * sizeof(T), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpyAsync(*p, host, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
   return AFQMC_OK;
 }
 
@@ -192,11 +194,20 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
       off_h1 += (size_t)n * N;
       off_rc += (size_t)G * n * N;
     }
-    // transposed Cholesky vectors (N*N, G): K-contiguous B operand of the VHS GEMM
-    std::vector<double> ct((size_t)N * N * G);
-    for (int g = 0; g < G; ++g)
-      for (size_t pq = 0; pq < (size_t)N * N; ++pq) ct[pq * G + g] = ham->chol[(size_t)g * N * N + pq];
-    TRY(upload(c, &c->chol_t, ct.data(), ct.size()));
+    // transposed Cholesky vectors (N*N, G): K-contiguous B operand of the VHS GEMM,
+    // transposed on the device from the (G, N*N) upload
+    {
+      double* raw = nullptr;
+      rc = upload(c, &raw, ham->chol, (size_t)G * N * N);
+      if (!rc) rc = dalloc(c, &c->chol_t, (size_t)G * N * N);
+      if (!rc) rc = launch_transpose_f64(c->stream, G, N * N, raw, c->chol_t);
+      (void)hipStreamSynchronize(c->stream);
+      if (raw) {
+        (void)hipFree(raw);
+        c->device_bytes -= (int64_t)G * N * N * (int64_t)sizeof(double);
+      }
+      if (rc) goto fail;
+    }
     TRY(upload(c, &c->mf_shifts, reinterpret_cast<const cplx*>(ham->mf_shifts), (size_t)G));
 
     // mixed mode: split-bf16 planes of the contraction operands (zero padded to 32)
@@ -257,11 +268,13 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
     TRY(dalloc(c, &c->walk_tmp, (size_t)W * N * (c->nocc[0] > c->nocc[1] ? c->nocc[0] : c->nocc[1])));
     TRY(dalloc(c, &c->sr_cum, (size_t)W));
     TRY(dalloc(c, &c->sr_idx, (size_t)W));
+    TRY(sync_stream(c->stream));
   }
 #undef TRY
   *out = c;
   return AFQMC_OK;
 fail:
+  (void)hipStreamSynchronize(c->stream);
   free_ctx(c);
   return rc;
 }

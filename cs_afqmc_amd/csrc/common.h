@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../include/cs_afqmc_amd.h"
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
@@ -171,9 +173,16 @@ int afqmc_set_error(int code, const char* fmt, ...);
   } while (0)
 
 // Raise a kernel's dynamic-LDS limit to `bytes` when it exceeds the 64 KiB default and
-// the limit already set (`current`, one static per kernel).
+// the limit already set (`current`, one static per kernel).  The limit only grows, under
+// a lock, so threads launching the same kernel with different sizes never lower it.
+inline std::mutex& dyn_lds_mutex() {
+  static std::mutex m;
+  return m;
+}
 inline int ensure_dyn_lds(const void* fn, size_t bytes, int* current) {
-  if (bytes <= 64 * 1024 || (int)bytes <= *current) return AFQMC_OK;
+  if (bytes <= 64 * 1024) return AFQMC_OK;
+  std::lock_guard<std::mutex> lock(dyn_lds_mutex());
+  if ((int)bytes <= *current) return AFQMC_OK;
   HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
   *current = (int)bytes;
   return AFQMC_OK;
@@ -188,6 +197,8 @@ int launch_gemm_nt_f64_split(hipStream_t s, int M, int N, int K, double alpha, c
                              int lda, const double* B, int ldb, double* C, int ldc, int splits,
                              int64_t c_split_stride);
 int gemm_f64_splits(int M, int N, int K);
+// dst (cols x rows) = src (rows x cols)^T, float64
+int launch_transpose_f64(hipStream_t s, int rows, int cols, const double* src, double* dst);
 // gemm_bf16x3.hip: split-bf16 NT GEMM (K, lda, ldb multiples of 32 / 8), float / double out
 int launch_gemm_nt_bf16x3_f32(hipStream_t s, int M, int N, int K, float alpha,
                               const uint16_t* Ah, const uint16_t* Al, int64_t lda,

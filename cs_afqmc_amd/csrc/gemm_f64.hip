@@ -254,3 +254,34 @@ int launch_gemm_f64_grouped(hipStream_t s, int bm, bool vec2, const GemmArgs* ar
   LAUNCH_CHECK();
   return AFQMC_OK;
 }
+
+namespace {
+
+// dst (cols x rows) = src (rows x cols)^T through 32 x 32 LDS tiles
+__global__ __launch_bounds__(256) void transpose_f64_kernel(int rows, int cols,
+                                                           const double* __restrict__ src,
+                                                           double* __restrict__ dst) {
+  __shared__ double tile[32][33];
+  const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int j = ty; j < 32; j += 8) {
+    const int r = r0 + j, c = c0 + tx;
+    if (r < rows && c < cols) tile[j][tx] = src[(int64_t)r * cols + c];
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int c = c0 + j, r = r0 + tx;
+    if (r < rows && c < cols) dst[(int64_t)c * rows + r] = tile[tx][j];
+  }
+}
+
+}  // namespace
+
+int launch_transpose_f64(hipStream_t s, int rows, int cols, const double* src, double* dst) {
+  if (rows <= 0 || cols <= 0) return AFQMC_OK;
+  dim3 grid((cols + 31) / 32, (rows + 31) / 32);
+  if (grid.y > 65535) return afqmc_set_error(AFQMC_ERR_SIZE, "transpose: %d rows", rows);
+  hipLaunchKernelGGL(transpose_f64_kernel, grid, dim3(256), 0, s, rows, cols, src, dst);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}

@@ -53,6 +53,9 @@ struct afqmc_group {
   unsigned char* dev_tab = nullptr;
   size_t dev_cap = 0;
   std::vector<unsigned char> host_tab;
+  unsigned char* pinned = nullptr;  // staging of the table upload
+  size_t pinned_cap = 0;
+  hipEvent_t upload_done = nullptr;  // the last upload has read `pinned`
 };
 
 namespace {
@@ -320,16 +323,30 @@ int build(afqmc_group* g) {
     put_table(tab, g->weights, args, start);
   }
 
-  // upload (the previous step's launches are stream-ordered before this copy)
+  // Upload without draining the stream: launches already queued read the old table
+  // before this stream-ordered copy overwrites it; the pinned staging buffer is reused
+  // once the previous copy has read it.
+  if (!g->upload_done) HIP_TRY(hipEventCreateWithFlags(&g->upload_done, hipEventDisableTiming));
   if (tab.size() > g->dev_cap) {
+    HIP_TRY(hipStreamSynchronize(g->stream));
     if (g->dev_tab) HIP_TRY(hipFree(g->dev_tab));
     g->dev_tab = nullptr;
     g->dev_cap = 0;
-    HIP_TRY(hipMalloc(&g->dev_tab, tab.size()));
-    g->dev_cap = tab.size();
+    HIP_TRY(hipMalloc(&g->dev_tab, 2 * tab.size()));
+    g->dev_cap = 2 * tab.size();
   }
-  HIP_TRY(hipStreamSynchronize(g->stream));
-  HIP_TRY(hipMemcpy(g->dev_tab, tab.data(), tab.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipEventSynchronize(g->upload_done));
+  if (tab.size() > g->pinned_cap) {
+    if (g->pinned) HIP_TRY(hipHostFree(g->pinned));
+    g->pinned = nullptr;
+    g->pinned_cap = 0;
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g->pinned), 2 * tab.size(),
+                          hipHostMallocDefault));
+    g->pinned_cap = 2 * tab.size();
+  }
+  std::memcpy(g->pinned, tab.data(), tab.size());
+  HIP_TRY(hipMemcpyAsync(g->dev_tab, g->pinned, tab.size(), hipMemcpyHostToDevice, g->stream));
+  HIP_TRY(hipEventRecord(g->upload_done, g->stream));
   g->dirty = false;
   return AFQMC_OK;
 }
@@ -383,18 +400,25 @@ extern "C" int afqmc_group_create(afqmc_group** out, int32_t max_fragments, void
 
 extern "C" int afqmc_group_destroy(afqmc_group* g) {
   if (!g) return AFQMC_OK;
-  if (g->dev_tab) {
-    (void)hipSetDevice(g->device);
-    (void)hipStreamSynchronize(g->stream);
-    (void)hipFree(g->dev_tab);
-  }
+  (void)hipSetDevice(g->device);
+  if (g->dev_tab || g->pinned) (void)hipStreamSynchronize(g->stream);
+  if (g->dev_tab) (void)hipFree(g->dev_tab);
+  if (g->pinned) (void)hipHostFree(g->pinned);
+  if (g->upload_done) (void)hipEventDestroy(g->upload_done);
   delete g;
   return AFQMC_OK;
 }
 
 extern "C" int afqmc_group_set_stream(afqmc_group* g, void* stream) {
   if (!g) return afqmc_set_error(AFQMC_ERR_ARG, "null group");
-  g->stream = static_cast<hipStream_t>(stream);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (st != g->stream && g->dev_tab) {
+    // launches queued on the old stream may still read the table
+    int rc = set_group_device(g);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(g->stream));
+  }
+  g->stream = st;
   return AFQMC_OK;
 }
 

@@ -73,8 +73,10 @@ class propagator_restricted:
         ham_data["mf_shifts"] = mf_shifts
         h0 = float(np.real(_as_np(ham_data["h0"])))
         ham_data["h0_prop"] = -h0 - np.sum(mf_shifts**2) / 2.0
-        v0 = 0.5 * np.einsum("gik,gjk->ij", chol3, chol3, optimize=True)
-        v1 = np.real(1.0j * np.einsum("g,gik->ik", mf_shifts, chol3, optimize=True))
+        # v0[i, j] = 1/2 sum_{g,k} L_g[i, k] L_g[j, k]: one (N x G N) product
+        lt = np.ascontiguousarray(chol3.transpose(1, 0, 2)).reshape(norb, -1)
+        v0 = 0.5 * (lt @ lt.T)
+        v1 = np.real(1.0j * (mf_shifts @ chol)).reshape(norb, norb)
         h1 = np.asarray(_as_np(ham_data["h1"]), dtype=np.float64)
         if self.walker_type == "rhf":
             h1_mod = (h1[0] + h1[1]) / 2.0 - v0 - v1

@@ -116,7 +116,7 @@ class rhf:
         mo = np.asarray(_as_np(wave_data["mo_coeff"]), dtype=np.float64)
         chol3 = np.asarray(_as_np(ham_data["chol"]), dtype=np.float64).reshape(-1, norb, norb)
         ham_data["rot_h1"] = mo.T @ ((h1[0] + h1[1]) / 2.0)
-        ham_data["rot_chol"] = np.einsum("pi,gij->gpj", mo.T, chol3, optimize=True)
+        ham_data["rot_chol"] = np.matmul(mo.T, chol3)
         _invalidate(ham_data)
         return ham_data
 
@@ -166,10 +166,7 @@ class uhf:
         mo = [np.asarray(_as_np(m), dtype=np.float64) for m in wave_data["mo_coeff"]]
         chol3 = np.asarray(_as_np(ham_data["chol"]), dtype=np.float64).reshape(-1, norb, norb)
         ham_data["rot_h1"] = [mo[0].T @ h1[0], mo[1].T @ h1[1]]
-        ham_data["rot_chol"] = [
-            np.einsum("pi,gij->gpj", mo[0].T, chol3, optimize=True),
-            np.einsum("pi,gij->gpj", mo[1].T, chol3, optimize=True),
-        ]
+        ham_data["rot_chol"] = [np.matmul(mo[0].T, chol3), np.matmul(mo[1].T, chol3)]
         _invalidate(ham_data)
         return ham_data
 

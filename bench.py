@@ -386,7 +386,8 @@ def run_fragments(args):
     for mode in ("bf16x3", "fp64", "fp64_plain"):
         prec = "fp64" if mode.startswith("fp64") else mode
         frags = _make_fragments(specs, prec, n_prop)
-        bt = None if mode == "fp64_plain" else batch.FragmentBatch(frags, tune=True)
+        bt = None if mode == "fp64_plain" else batch.FragmentBatch(frags, tune=True,
+                                                                   use_graph=True)
         for _ in range(wblocks):
             if bt is None:
                 for f in frags:

@@ -105,24 +105,41 @@ class Fragment:
 
 
 class FragmentBatch:
-    """A set of fragments advanced by grouped steps (one group, one stream)."""
+    """A set of fragments advanced by grouped steps (one group, one stream).
 
-    def __init__(self, fragments=(), max_fragments=1024, tune=False):
-        self.group = DeviceGroup(max_fragments)
+    With use_graph, the n_prop_steps grouped steps of a block are captured once into a
+    HIP graph (torch.cuda.CUDAGraph over the batch's stream) and replayed every block
+    while the member set is unchanged; the first block after a change runs eagerly
+    (it rebuilds the group's tables), the next one captures."""
+
+    def __init__(self, fragments=(), max_fragments=1024, tune=False, use_graph=False,
+                 stream=None):
+        self.stream = stream if stream is not None else torch.cuda.current_stream()
+        with torch.cuda.stream(self.stream):
+            self.group = DeviceGroup(max_fragments)
         self.fragments = []
+        self.use_graph = use_graph
+        self._version = 0        # bumped on every membership change
+        self._warm_version = -1  # version whose tables an eager block has built
+        self._graph = None
+        self._graph_version = -1
         for f in fragments:
             self.add(f)
         self.tuned = None
         if tune and self.fragments:
-            self.tuned = self.group.tune()
+            with torch.cuda.stream(self.stream):
+                self.tuned = self.group.tune()
 
     def add(self, frag):
         if self.fragments and frag.n_prop_steps != self.fragments[0].n_prop_steps:
             raise ValueError("fragments of a batch share n_prop_steps")
         pd = frag.prop_data
+        # work that set the fragment up on the caller's stream comes first
+        self.stream.wait_stream(torch.cuda.current_stream())
         self.group.add(frag.dev, pd["walkers"], pd["weights"], pd["overlaps"], frag.fields,
                        frag.scalars)
         self.fragments.append(frag)
+        self._version += 1
 
     def remove(self, frags):
         """Drop finished fragments (the group's tables are rebuilt on the next step)."""
@@ -133,13 +150,26 @@ class FragmentBatch:
         for f in keep:
             self.add(f)
 
+    def _steps(self):
+        for s in range(self.fragments[0].n_prop_steps):
+            self.group.propagate(s, reuse_green=s > 0)
+
     def run_block(self):
         """One block of every fragment: fields, n_prop_steps grouped steps, block end."""
         if not self.fragments:
             return
-        for f in self.fragments:
-            f.begin_block()
-        for s in range(self.fragments[0].n_prop_steps):
-            self.group.propagate(s, reuse_green=s > 0)
-        for f in self.fragments:
-            f.end_block()
+        with torch.cuda.stream(self.stream):
+            for f in self.fragments:
+                f.begin_block()
+            if not self.use_graph or self._warm_version != self._version:
+                self._steps()
+                self._warm_version = self._version
+            else:
+                if self._graph_version != self._version:
+                    self._graph = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(self._graph, stream=self.stream):
+                        self._steps()
+                    self._graph_version = self._version
+                self._graph.replay()
+            for f in self.fragments:
+                f.end_block()

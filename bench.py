@@ -50,7 +50,7 @@ def phase_kernel(precision, phase, norb, nelec):
     if phase == "taylor":
         if precision == "fp64":
             return "taylor_reg_kernel" if reg else "taylor_kernel"
-        return "taylor_reg_f32_kernel" if reg else "taylor_stream_f32_kernel"
+        return "taylor_bf_kernel" if reg else "taylor_stream_f32_kernel"
     if phase in ("vhs_gemm", "fb_gemm"):
         return "gemm_nt_f64_kernel" if precision == "fp64" else "gemm_nt_bf16x3_kernel"
     if phase in ("overlap", "green"):
@@ -85,8 +85,9 @@ def phase_peak(precision, phase, norb, nelec):
         return PEAK_TFLOPS["fp64"]
     if phase in ("vhs_gemm", "fb_gemm"):
         return PEAK_TFLOPS["bf16"]
-    if phase == "taylor":  # float32 Taylor kernels in the mixed mode
-        return PEAK_TFLOPS["fp32"]
+    if phase == "taylor":  # split-bf16 register kernel, or the float32 streaming kernel
+        reg = norb <= 128 and max(nelec) <= 32
+        return PEAK_TFLOPS["bf16"] if reg else PEAK_TFLOPS["fp32"]
     return PEAK_TFLOPS["fp64"]
 
 
@@ -516,7 +517,9 @@ def run_serving(args):
     }
 
 
-DTYPE = {"fp64": "f64", "bf16x3": "mixed: bf16x3 MFMA contractions, f32 Taylor, f64 rest"}
+DTYPE = {"fp64": "f64",
+         "bf16x3": "mixed: bf16x3 MFMA contractions and Taylor products (f32 accumulation), "
+                   "f64 overlaps/QR/weights/energy accumulation"}
 
 
 def main():

@@ -56,7 +56,7 @@ void free_ctx(afqmc_ctx* c) {
   if (!c) return;
   void* ptrs[] = {c->trial[0], c->trial[1], c->rot_h1[0], c->rot_h1[1], c->rot_chol[0],
                   c->rot_chol[1], c->chol_t, c->exp_h1[0], c->exp_h1[1], c->mf_shifts,
-                  c->chol_t_f32, c->rot_chol_f32[0], c->rot_chol_f32[1], c->chol_t_bf[0],
+                  c->emi_bf[0][0], c->emi_bf[0][1], c->emi_bf[1][0], c->emi_bf[1][1], c->chol_t_bf[0],
                   c->chol_t_bf[1], c->rot_chol_bf[0][0], c->rot_chol_bf[0][1],
                   c->rot_chol_bf[1][0], c->rot_chol_bf[1][1], c->green[0], c->green[1], c->fb,
                   c->xs, c->vhs, c->terms, c->dets, c->energy_x, c->energy_acc, c->walk_tmp,
@@ -241,6 +241,19 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
         for (int s = 0; s < c->nspin; ++s) {
           for (size_t e = 0; e < nn; ++e) ef[e] = (float)ham->exp_h1[(size_t)s * nn + e];
           TRY(upload(c, &c->exp_h1_f32[s], ef.data(), nn));
+          // exp_h1 - I as split-bf16 planes, rows padded to Np32 (bf16 Taylor kernel)
+          std::vector<double> emi(nn);
+          for (size_t e = 0; e < nn; ++e)
+            emi[e] = ham->exp_h1[(size_t)s * nn + e] - ((e / N == e % N) ? 1.0 : 0.0);
+          double* emi_dev = nullptr;
+          TRY(upload(c, &emi_dev, emi.data(), nn));
+          for (int h = 0; h < 2; ++h) TRY(dalloc(c, &c->emi_bf[s][h], (size_t)N * c->Np32));
+          rc = launch_split_bf16(c->stream, N, N, emi_dev, N, c->emi_bf[s][0], c->emi_bf[s][1],
+                                 c->Np32);
+          if (!rc) rc = sync_stream(c->stream);
+          (void)hipFree(emi_dev);
+          c->device_bytes -= (int64_t)nn * (int64_t)sizeof(double);
+          if (rc) goto fail;
         }
       }
       TRY(sync_stream(c->stream));

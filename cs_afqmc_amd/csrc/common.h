@@ -101,14 +101,13 @@ struct afqmc_ctx {
   int taylor_f32;            // 1 when the Taylor kernels run in float32 (mixed mode)
   int Gp;                    // nchol rounded up to 32 (bf16 plane leading dimension)
   int Np32;                  // norb rounded up to 32
-  float* chol_t_f32;         // unused
-  float* rot_chol_f32[2];    // unused
   uint16_t* chol_t_bf[2];    // hi / lo planes of chol_t, (N*N, Gp)
   uint16_t* rot_chol_bf[2][2];  // [spin][hi/lo], (G, n_sigma, Np32)
   uint16_t* xs_bf[2];        // [hi/lo] (2W, Gp): i sqrt(dt) x~
   uint16_t* green_bf[2][2];  // [spin][hi/lo] (W, 2, n_sigma, Np32)
   float* vhs_f32;            // (W, 2, N*N)
   float* exp_h1_f32[2];      // (N, N)
+  uint16_t* emi_bf[2][2];    // [spin][hi/lo] (N, Np32): split-bf16 exp_h1 - I
 
   // workspace (device)
   double* green[2];      // (W, 2, n_sigma*N): split re / im planes per walker
@@ -282,7 +281,8 @@ struct WeightsArgs {
 struct TaylorArgs {
   int N, n, K, rows_pad;
   cplx* walkers;
-  const void* E;    // float64 or float32 by kernel
+  const void* E;    // float64 / float32 exp_h1, or the hi plane of exp_h1 - I (bf16 kernel)
+  const void* E2;   // lo plane of exp_h1 - I (bf16 kernel)
   const void* vhs;  // float64 or float32 by kernel
 };
 struct GemmArgs {
@@ -296,7 +296,13 @@ struct GemmArgs {
   void* C;
 };
 
-enum TaylorKind { TAYLOR_REG_F64 = 0, TAYLOR_REG_F32 = 1, TAYLOR_STREAM_F32 = 2, TAYLOR_STREAM_F64 = 3 };
+enum TaylorKind {
+  TAYLOR_REG_F64 = 0,
+  TAYLOR_REG_F32 = 1,
+  TAYLOR_STREAM_F32 = 2,
+  TAYLOR_STREAM_F64 = 3,
+  TAYLOR_REG_BF16X3 = 4
+};
 
 struct TaylorPlan {
   int kind, nkgv, ct, nthreads, blocks_per_walker;

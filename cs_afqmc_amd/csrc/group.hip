@@ -269,6 +269,10 @@ int build(afqmc_group* g) {
         a.rows_pad = ((a.N + 15) / 16) * 16;
         a.walkers = s == 0 ? m.wup : m.wdn;
         a.E = g->mixed ? (const void*)c->exp_h1_f32[s] : (const void*)c->exp_h1[s];
+        if (pl.kind == TAYLOR_REG_BF16X3) {
+          a.E = c->emi_bf[s][0];
+          a.E2 = c->emi_bf[s][1];
+        }
         a.vhs = g->mixed ? (const void*)c->vhs_f32 : (const void*)c->vhs;
         const auto key = std::make_tuple(pl.kind, pl.nkgv, pl.ct, pl.nthreads);
         by_key[key].push_back({a, c->d.n_walkers * pl.blocks_per_walker});

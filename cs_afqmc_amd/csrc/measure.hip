@@ -21,6 +21,83 @@ namespace {
 
 constexpr int MEAS_KC = 4;  // k-groups whose operands are loaded ahead of their MFMAs
 
+__device__ inline double readlane_f64(double x, int l) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Gauss-Jordan on [O | I] (n x 2n, n <= 8 T2 <= 24) held in registers: lane c keeps
+// column c.  Pivot k: lane k finds the first maximum of |re| + |im| in its column
+// (rows >= k), the row exchange and the pivot-row scaling are per-lane register
+// operations, and each other row r is updated with the multiplier M[r][k] read from
+// lane k (v_readlane).  No LDS traffic and no barriers inside the elimination.
+// Returns det(O); on exit M (LDS) holds [* | O^-1].
+template <int T2>
+__device__ __forceinline__ cplx gauss_jordan_registers(cplx* M, int n, int lane) {
+  constexpr int NR = 8 * T2;
+  const int n2 = 2 * n;
+  double vr[NR], vi[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    cplx x = cplx{0.0, 0.0};
+    if (r < n && lane < n2) x = M[r * n2 + lane];
+    vr[r] = x.re;
+    vi[r] = x.im;
+  }
+  cplx det = cplx{1.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    if (k < n) {
+      int best = k;
+      double bv = -1.0;
+#pragma unroll
+      for (int r = k; r < NR; ++r) {
+        const double a = fabs(vr[r]) + fabs(vi[r]);
+        if (r < n && a > bv) {
+          bv = a;
+          best = r;
+        }
+      }
+      const int pr = __builtin_amdgcn_readlane(best, k);
+      if (pr != k) {
+#pragma unroll
+        for (int r = k + 1; r < NR; ++r) {
+          if (r == pr) {
+            const double tr = vr[r], ti = vi[r];
+            vr[r] = vr[k];
+            vi[r] = vi[k];
+            vr[k] = tr;
+            vi[k] = ti;
+          }
+        }
+      }
+      const cplx piv = cplx{readlane_f64(vr[k], k), readlane_f64(vi[k], k)};
+      det = cmul(det, piv);
+      if (pr != k) det = cplx{-det.re, -det.im};
+      const double d2 = 1.0 / (piv.re * piv.re + piv.im * piv.im);
+      const cplx inv = cplx{piv.re * d2, -piv.im * d2};
+      const double mr = vr[k] * inv.re - vi[k] * inv.im;
+      const double mi = vr[k] * inv.im + vi[k] * inv.re;
+      vr[k] = mr;
+      vi[k] = mi;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        if (r != k && r < n) {
+          const double fr = readlane_f64(vr[r], k), fi = readlane_f64(vi[r], k);
+          vr[r] -= fr * mr - fi * mi;
+          vi[r] -= fr * mi + fi * mr;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+    if (r < n && lane < n2) M[r * n2 + lane] = cplx{vr[r], vi[r]};
+  return det;
+}
+
 // T2 = ceil(2n / 16): column tiles of [Or | Oi] and row tiles of [Gr ; Gi]
 template <int T2, int TI, bool STAGE>
 __device__ __forceinline__ void measure_body(int N, int n, const cplx* __restrict__ walkers,
@@ -124,9 +201,13 @@ __device__ __forceinline__ void measure_body(int N, int n, const cplx* __restric
   __syncthreads();
 
   // ---- Gauss-Jordan with partial pivoting: [O | I] -> [* | X], det from pivots.
-  // Lane c' owns column k + 1 + c' of the trailing matrix (2n - k - 1 <= 127 columns,
-  // at most two per lane), so the elimination needs no index division.
   cplx det = cplx{1.0, 0.0};
+  if constexpr (T2 <= 3) {  // n <= 24: the column fits the registers without spilling
+    det = gauss_jordan_registers<T2>(M, n, lane);
+    __syncthreads();
+  } else {
+  // Wider systems in LDS: lane c' owns column k + 1 + c' of the trailing matrix
+  // (2n - k - 1 <= 127 columns, at most two per lane).
   for (int k = 0; k < n; ++k) {
     int pr;
     {
@@ -186,6 +267,7 @@ __device__ __forceinline__ void measure_body(int N, int n, const cplx* __restric
       }
     }
     __syncthreads();
+  }
   }
   if (lane == 0) dets[(int64_t)w * 2 + spin] = det;
   if (!green && !ghi) return;

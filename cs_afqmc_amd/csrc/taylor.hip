@@ -491,6 +491,226 @@ __global__ __launch_bounds__(512) void taylor_reg_f32_grouped_kernel(
 }
 
 // ------------------------------------------------------------------------------------
+// Split-bf16 register-resident form for the mixed mode (N <= 128, n <= 32).  Same real
+// block product as above, [ur | ui] = [Vr | Vi] [[yr, yi], [-yi, yr]], on
+// v_mfma_f32_16x16x32_bf16 with each operand x = hi + lo (bf16 planes) and the three
+// leading products al bh + ah bl + ah bh accumulated in float32:
+//   A (V rows, registers): lane l holds A[l % 16][8 (l / 16) + e] of each 32-wide k block
+//   B (y, LDS [column][k], k contiguous): lane l holds B[8 (l / 16) + e][l % 16]
+//   D (float4): lane l, element r holds D[4 (l / 16) + r][l % 16]
+// The one-body factor is applied as E y = y + (E - I) y with E - I in split bf16: its
+// entries are O(dt), so the product's relative error stays far below float32 rounding of
+// E y itself.  The Horner recursion y <- t0 + V y / k and all additions run in float32;
+// the walker is read and written in float64.
+template <int KB>  // 32-wide k blocks of [Vr | Vi]: 2 Np32 / 32
+struct BfTile {
+  static constexpr int LDK = 32 * KB + 8;  // LDS row (one column of y) in bf16 elements
+};
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ inline uint16_t bf16_bits(float f) { return f32_to_bf16_rne(f); }
+__device__ inline float bf16_val(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+template <int KB, int CTP>
+__device__ __forceinline__ void taylor_bf_body(int N, int n, int n_terms_k,
+                                               cplx* __restrict__ walkers,
+                                               const uint16_t* __restrict__ emi_hi,
+                                               const uint16_t* __restrict__ emi_lo,
+                                               const float* __restrict__ vhs, const int w) {
+  constexpr int LDK = BfTile<KB>::LDK;
+  constexpr int KH = KB / 2;  // k blocks per half (Np32 / 32)
+  extern __shared__ uint16_t YB[];  // [2 planes][CTP * 16 columns][LDK]
+  uint16_t* Yh = YB;
+  uint16_t* Yl = YB + CTP * 16 * LDK;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rb = wave * 16;
+  const int np32 = 32 * KH;
+  const int q4 = lane >> 4, col = lane & 15;
+  const int p = rb + col;  // A row of this lane
+  const bool prow = p < N;
+  const int n2 = 2 * n;
+  cplx* phi = walkers + (int64_t)w * N * n;
+  const float* Vr = vhs + (int64_t)w * 2 * N * N;
+  const float* Vi = Vr + (int64_t)N * N;
+
+  // V rows in registers, split once
+  bf16x8_t vh[KB], vl[KB];
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb) {
+    const float* src = kb < KH ? Vr : Vi;
+    const int k0 = (kb < KH ? kb : kb - KH) * 32 + 8 * q4;
+    uint16_t hb[8], lb[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = k0 + e;
+      const float v = (prow && k < N) ? src[(int64_t)p * N + k] : 0.f;
+      hb[e] = bf16_bits(v);
+      lb[e] = bf16_bits(v - bf16_val(hb[e]));
+    }
+    u32x4_t ph, pl;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      ph[j] = (uint32_t)hb[2 * j] | ((uint32_t)hb[2 * j + 1] << 16);
+      pl[j] = (uint32_t)lb[2 * j] | ((uint32_t)lb[2 * j + 1] << 16);
+    }
+    vh[kb] = __builtin_bit_cast(bf16x8_t, ph);
+    vl[kb] = __builtin_bit_cast(bf16x8_t, pl);
+  }
+
+  for (int e = threadIdx.x; e < 2 * CTP * 16 * LDK; e += blockDim.x) YB[e] = 0;
+  __syncthreads();
+
+  // y (D layout, float32) -> LDS planes, top half [yr | yi] and bottom half [-yi | yr]
+  auto put = [&](const float4_t* y) {
+#pragma unroll
+    for (int ct = 0; ct < CTP; ++ct)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rb + 4 * q4 + r;
+        const int c = ct * 16 + col;
+        if (row < np32 && c < n2) {
+          const float v = y[ct][r];
+          const uint16_t h = bf16_bits(v);
+          const uint16_t l = bf16_bits(v - bf16_val(h));
+          Yh[c * LDK + row] = h;
+          Yl[c * LDK + row] = l;
+          const int cb = c < n ? c + n : c - n;
+          const uint16_t sgn = c < n ? 0 : 0x8000;  // bottom half: +yr, -yi
+          Yh[cb * LDK + np32 + row] = h ^ sgn;
+          Yl[cb * LDK + np32 + row] = l ^ sgn;
+        }
+      }
+  };
+  auto bload = [&](const uint16_t* plane, int ct, int kb) {
+    return __builtin_bit_cast(
+        bf16x8_t, *reinterpret_cast<const u32x4_t*>(plane + (ct * 16 + col) * LDK + kb * 32 + 8 * q4));
+  };
+  // acc = (E - I) y over the top half (k < Np32); A streams from L2
+  auto e_product = [&](float4_t* acc) {
+#pragma unroll
+    for (int ct = 0; ct < CTP; ++ct) acc[ct] = float4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < KH; ++kb) {
+      u32x4_t ah = {0, 0, 0, 0}, al = {0, 0, 0, 0};
+      if (prow) {
+        ah = *reinterpret_cast<const u32x4_t*>(emi_hi + (int64_t)p * np32 + kb * 32 + 8 * q4);
+        al = *reinterpret_cast<const u32x4_t*>(emi_lo + (int64_t)p * np32 + kb * 32 + 8 * q4);
+      }
+      const bf16x8_t a_h = __builtin_bit_cast(bf16x8_t, ah), a_l = __builtin_bit_cast(bf16x8_t, al);
+#pragma unroll
+      for (int ct = 0; ct < CTP; ++ct) {
+        const bf16x8_t bh = bload(Yh, ct, kb), bl = bload(Yl, ct, kb);
+        acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l, bh, acc[ct], 0, 0, 0);
+        acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, bl, acc[ct], 0, 0, 0);
+        acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, bh, acc[ct], 0, 0, 0);
+      }
+    }
+  };
+
+  // walker -> registers (D layout) and LDS
+  float4_t y[CTP], t0[CTP];
+#pragma unroll
+  for (int ct = 0; ct < CTP; ++ct)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = rb + 4 * q4 + r;
+      const int c = ct * 16 + col;
+      float v = 0.f;
+      if (row < N && c < n2) {
+        const double* z = reinterpret_cast<const double*>(phi + (int64_t)row * n + (c < n ? c : c - n));
+        v = (float)z[c < n ? 0 : 1];
+      }
+      y[ct][r] = v;
+    }
+  put(y);
+  __syncthreads();
+  e_product(t0);
+#pragma unroll
+  for (int ct = 0; ct < CTP; ++ct) t0[ct] += y[ct];
+  __syncthreads();
+  put(t0);
+  __syncthreads();
+  for (int k = n_terms_k; k >= 1; --k) {
+#pragma unroll
+    for (int ct = 0; ct < CTP; ++ct) y[ct] = float4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+#pragma unroll
+      for (int ct = 0; ct < CTP; ++ct) {
+        const bf16x8_t bh = bload(Yh, ct, kb), bl = bload(Yl, ct, kb);
+        y[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vl[kb], bh, y[ct], 0, 0, 0);
+        y[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh[kb], bl, y[ct], 0, 0, 0);
+        y[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh[kb], bh, y[ct], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+    const float rk = 1.0f / (float)k;
+#pragma unroll
+    for (int ct = 0; ct < CTP; ++ct) y[ct] = t0[ct] + y[ct] * rk;
+    put(y);
+    __syncthreads();
+  }
+  float4_t acc[CTP];
+  e_product(acc);
+#pragma unroll
+  for (int ct = 0; ct < CTP; ++ct)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = rb + 4 * q4 + r;
+      const int c = ct * 16 + col;
+      if (row < N && c < n2) {
+        double* dst = reinterpret_cast<double*>(phi + (int64_t)row * n + (c < n ? c : c - n));
+        dst[c < n ? 0 : 1] = (double)(y[ct][r] + acc[ct][r]);
+      }
+    }
+}
+
+template <int KB, int CTP>
+__global__ __launch_bounds__(512) void taylor_bf_kernel(int N, int n, int n_terms_k,
+                                                        cplx* __restrict__ walkers,
+                                                        const uint16_t* __restrict__ emi_hi,
+                                                        const uint16_t* __restrict__ emi_lo,
+                                                        const float* __restrict__ vhs) {
+  taylor_bf_body<KB, CTP>(N, n, n_terms_k, walkers, emi_hi, emi_lo, vhs, blockIdx.x);
+}
+
+template <int KB, int CTP>
+__global__ __launch_bounds__(512) void taylor_bf_grouped_kernel(
+    const TaylorArgs* __restrict__ args, const int* __restrict__ start, int nprob) {
+  const int pidx = group_of(start, nprob, blockIdx.x);
+  const TaylorArgs a = args[pidx];
+  taylor_bf_body<KB, CTP>(a.N, a.n, a.K, a.walkers, static_cast<const uint16_t*>(a.E),
+                          static_cast<const uint16_t*>(a.E2), static_cast<const float*>(a.vhs),
+                          blockIdx.x - start[pidx]);
+}
+
+template <int KB, int CTP>
+int launch_bf(afqmc_ctx* c, int W, int n, cplx* wk, int s) {
+  const int N = c->d.norb;
+  const int nwaves = (N + 15) / 16;
+  const size_t shm = (size_t)2 * CTP * 16 * BfTile<KB>::LDK * sizeof(uint16_t);
+  static int lds_limit = 0;
+  const int rc = ensure_dyn_lds((const void*)taylor_bf_kernel<KB, CTP>, shm, &lds_limit);
+  if (rc) return rc;
+  hipLaunchKernelGGL((taylor_bf_kernel<KB, CTP>), dim3(W), dim3(nwaves * 64), shm, c->stream, N,
+                     n, c->d.n_exp_terms - 1, wk, c->emi_bf[s][0], c->emi_bf[s][1], c->vhs_f32);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+template <int KB>
+int launch_bf_ct(afqmc_ctx* c, int W, int n, cplx* wk, int s) {
+  switch ((2 * n + 15) / 16) {
+    case 1: return launch_bf<KB, 1>(c, W, n, wk, s);
+    case 2: return launch_bf<KB, 2>(c, W, n, wk, s);
+    case 3: return launch_bf<KB, 3>(c, W, n, wk, s);
+    default: return launch_bf<KB, 4>(c, W, n, wk, s);
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // Float32 streaming form for the mixed mode beyond the register kernel's range (N > 128
 // or n > 32).  One workgroup per walker of one spin; wave r owns output rows
 // [16 r, 16 r + 16) and all columns.  The current vector y (N x n complex) lives in LDS
@@ -714,13 +934,11 @@ void taylor_plan(const afqmc_ctx* c, int s, TaylorPlan* pl) {
   pl->ct = 0;
   if (c->taylor_f32) {
     if (N <= 128 && n <= 32) {
-      pl->kind = TAYLOR_REG_F32;
-      pl->nkgv = bucket;
+      const int kb = c->Np32 / 16;
+      pl->kind = TAYLOR_REG_BF16X3;
+      pl->nkgv = kb;
       pl->ct = ctp;
-      int rs = 16 * ctp;
-      if (ctp == 2) rs = 48;
-      if (ctp == 4) rs = 80;
-      pl->shm = (size_t)2 * Np * rs * sizeof(float);
+      pl->shm = (size_t)2 * ctp * 16 * (32 * kb + 8) * sizeof(uint16_t);
     } else {
       const int ct = (n + 15) / 16 > 4 ? 4 : (n + 15) / 16;
       pl->kind = TAYLOR_STREAM_F32;
@@ -750,11 +968,13 @@ int launch_taylor(afqmc_ctx* c, int W, cplx* wup, cplx* wdn) {
       const float* E = c->exp_h1_f32[s];
       int rc;
       if (N <= 128 && n <= 32) {
-        const int nkgv = 2 * ((N + 3) / 4);
-        rc = nkgv <= 32   ? launch_reg_f32_ct<32>(c, W, n, wk, E)
-             : nkgv <= 48 ? launch_reg_f32_ct<48>(c, W, n, wk, E)
-             : nkgv <= 56 ? launch_reg_f32_ct<56>(c, W, n, wk, E)
-                          : launch_reg_f32_ct<64>(c, W, n, wk, E);
+        switch (c->Np32 / 16) {
+          case 2: rc = launch_bf_ct<2>(c, W, n, wk, s); break;
+          case 4: rc = launch_bf_ct<4>(c, W, n, wk, s); break;
+          case 6: rc = launch_bf_ct<6>(c, W, n, wk, s); break;
+          default: rc = launch_bf_ct<8>(c, W, n, wk, s); break;
+        }
+        (void)E;
       } else {
         switch ((n + 15) / 16) {
           case 1: rc = launch_stream_f32<1>(c, W, n, wk, E); break;
@@ -846,6 +1066,20 @@ int launch_taylor_grouped(hipStream_t st, int kind, int nkgv, int ct, const Tayl
       case 64: return reg_grouped<64>(f32, ct, st, args, start, nprob, nblocks, nthreads, shm);
       default: return afqmc_set_error(AFQMC_ERR_ARG, "grouped taylor: nkgv %d", nkgv);
     }
+  }
+  if (kind == TAYLOR_REG_BF16X3) {
+    static int lim[4][4] = {};
+    const int ci = ct < 1 ? 0 : (ct > 4 ? 3 : ct - 1);
+#define BF_CASE(KB, C)                                                                      \
+  if (nkgv == KB && ct == C)                                                                \
+    return launch_grouped_fn(taylor_bf_grouped_kernel<KB, C>, st, args, start, nprob, nblocks, \
+                             nthreads, shm, &lim[KB / 2 - 1][ci]);
+    BF_CASE(2, 1) BF_CASE(2, 2) BF_CASE(2, 3) BF_CASE(2, 4)
+    BF_CASE(4, 1) BF_CASE(4, 2) BF_CASE(4, 3) BF_CASE(4, 4)
+    BF_CASE(6, 1) BF_CASE(6, 2) BF_CASE(6, 3) BF_CASE(6, 4)
+    BF_CASE(8, 1) BF_CASE(8, 2) BF_CASE(8, 3) BF_CASE(8, 4)
+#undef BF_CASE
+    return afqmc_set_error(AFQMC_ERR_ARG, "grouped taylor: k blocks %d, column tiles %d", nkgv, ct);
   }
   if (kind == TAYLOR_STREAM_F32) {
     static int lim[4] = {};

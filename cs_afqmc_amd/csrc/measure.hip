@@ -273,7 +273,62 @@ __device__ __forceinline__ void measure_body(int N, int n, const cplx* __restric
   if (!green && !ghi) return;
 
   // ---- [Gr ; Gi] = [[Xr^T, -Xi^T], [Xi^T, Xr^T]] [wr^T ; wi^T], K = 2n, columns p
-  {
+  if constexpr (T2 <= 3) {
+    // All k groups of a tile's walker operand are loaded at once (one round of global
+    // loads per tile); A comes from X in LDS.  Prefetching the next tile as well costs
+    // more registers than it saves once several waves share a SIMD.
+    constexpr int NKG = 4 * T2;  // k groups of 2n <= 16 T2
+    const int tp_n = (N + 15) >> 4;
+    auto a_of = [&](int kg, int tr) {
+      const int k = 4 * kg + s;
+      const int r = tr * 16 + col;
+      double v = 0.0;
+      if (k < n2 && r < n2) {
+        const int kj = k < n ? k : k - n;
+        const cplx x = M[kj * n2 + n + (r < n ? r : r - n)];  // X[kj][ri]
+        v = (r < n) == (k < n) ? x.re : ((r < n) ? -x.im : x.im);
+      }
+      return v;
+    };
+    auto load_b = [&](double (&b)[NKG], int tp) {
+      const int p = tp * 16 + col;
+#pragma unroll
+      for (int kg = 0; kg < NKG; ++kg) {
+        const int k = 4 * kg + s;
+        const int kj = k < n ? k : k - n;
+        b[kg] = (k < n2 && p < N) ? phid[2 * ((int64_t)p * n + kj) + (k < n ? 0 : 1)] : 0.0;
+      }
+    };
+    double* gw = green ? green + (int64_t)w * n2 * N : nullptr;
+    uint16_t* gh = ghi ? ghi + (int64_t)w * n2 * ld16 : nullptr;
+    uint16_t* gl = ghi ? glo + (int64_t)w * n2 * ld16 : nullptr;
+    auto tile = [&](const double (&b)[NKG], int tp) {
+      double4_t acc[T2];
+#pragma unroll
+      for (int tr = 0; tr < T2; ++tr) acc[tr] = double4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kg = 0; kg < NKG; ++kg)
+#pragma unroll
+        for (int tr = 0; tr < T2; ++tr)
+          acc[tr] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_of(kg, tr), b[kg], acc[tr], 0, 0, 0);
+      const int p = tp * 16 + col;
+#pragma unroll
+      for (int tr = 0; tr < T2; ++tr)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int row = tr * 16 + s + 4 * q;
+          if (row < n2 && p < N) {
+            if (gw) gw[(int64_t)row * N + p] = acc[tr][q];
+            if (gh) split_bf16(acc[tr][q], &gh[(int64_t)row * ld16 + p], &gl[(int64_t)row * ld16 + p]);
+          }
+        }
+    };
+    for (int tp = 0; tp < tp_n; ++tp) {
+      double b[NKG];
+      load_b(b, tp);
+      tile(b, tp);
+    }
+  } else {
     const int tp_n = (N + 15) >> 4;
     const int nkg = (n2 + 3) >> 2;
     double* gw = green ? green + (int64_t)w * n2 * N : nullptr;

@@ -32,6 +32,7 @@ EXPORTS = (
     "afqmc_prof_enable",
     "afqmc_prof_collect",
     "afqmc_fields_normal",
+    "afqmc_cs_outliers",
     "afqmc_group_create",
     "afqmc_group_destroy",
     "afqmc_group_set_stream",
@@ -106,6 +107,7 @@ def _declare(lib):
     lib.afqmc_prof_collect.argtypes = [_vp, _vp, _vp]
     lib.afqmc_fields_normal.argtypes = [
         _vp, ctypes.c_uint64, ctypes.c_uint64, _vp, ctypes.c_int64]
+    lib.afqmc_cs_outliers.argtypes = [_vp, _i32, _vp, _vp, ctypes.c_double]
     lib.afqmc_group_create.argtypes = [ctypes.POINTER(_vp), _i32, _vp]
     lib.afqmc_group_destroy.argtypes = [_vp]
     lib.afqmc_group_set_stream.argtypes = [_vp, _vp]

@@ -377,6 +377,15 @@ extern "C" int afqmc_block_stats(afqmc_ctx* c, int32_t W, const void* energies,
   return launch_block_stats(c, W, static_cast<const cplx*>(energies), weights, scalars, out);
 }
 
+extern "C" int afqmc_cs_outliers(afqmc_ctx* c, int32_t W, double* weights1, double* weights2,
+                                 double z_thresh) {
+  int rc = check_w(c, W);
+  if (rc) return rc;
+  if (!weights1 || !weights2) return afqmc_set_error(AFQMC_ERR_ARG, "null weights");
+  if ((rc = set_device(c))) return rc;
+  return launch_cs_outliers(c, W, weights1, weights2, z_thresh);
+}
+
 extern "C" int afqmc_prof_enable(afqmc_ctx* c, int32_t on) {
   if (!c) return afqmc_set_error(AFQMC_ERR_ARG, "null context");
   if (on && !c->prof_ev) {

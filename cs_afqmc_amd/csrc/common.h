@@ -221,6 +221,7 @@ int launch_overlap_combine(afqmc_ctx* c, int W, cplx* overlaps);
 int launch_fb_out(afqmc_ctx* c, int W, cplx* fb_out);
 int launch_block_stats(afqmc_ctx* c, int W, const cplx* energies, const double* weights,
                        double* scalars, double* out);
+int launch_cs_outliers(afqmc_ctx* c, int W, double* w1, double* w2, double z_thresh);
 // energy.hip
 int launch_energy(afqmc_ctx* c, int W, cplx* energies);
 int energy_g_tiles(const afqmc_ctx* c, int s);

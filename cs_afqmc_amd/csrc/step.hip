@@ -172,6 +172,38 @@ __global__ __launch_bounds__(WEIGHTS_NT) void block_stats_kernel(
   }
 }
 
+// corr_sample.py:551-585 (corr_otler_rm): z-scores of the weights of two correlated
+// populations, z = |w - mean| / std (population std); a walker whose larger z-score
+// exceeds z_thresh gets weight 0 in both populations.  Two-pass mean / variance with
+// the deterministic block reduction.
+__global__ __launch_bounds__(WEIGHTS_NT) void cs_outlier_kernel(int W, double z_thresh,
+                                                                double* __restrict__ w1,
+                                                                double* __restrict__ w2) {
+  __shared__ double red[WEIGHTS_NT / 64];
+  double s1 = 0.0, s2 = 0.0;
+  for (int i = threadIdx.x; i < W; i += WEIGHTS_NT) {
+    s1 += w1[i];
+    s2 += w2[i];
+  }
+  const double m1 = block_sum<WEIGHTS_NT>(s1, red) / (double)W;
+  const double m2 = block_sum<WEIGHTS_NT>(s2, red) / (double)W;
+  double v1 = 0.0, v2 = 0.0;
+  for (int i = threadIdx.x; i < W; i += WEIGHTS_NT) {
+    v1 += (w1[i] - m1) * (w1[i] - m1);
+    v2 += (w2[i] - m2) * (w2[i] - m2);
+  }
+  const double sd1 = sqrt(block_sum<WEIGHTS_NT>(v1, red) / (double)W);
+  const double sd2 = sqrt(block_sum<WEIGHTS_NT>(v2, red) / (double)W);
+  for (int i = threadIdx.x; i < W; i += WEIGHTS_NT) {
+    const double z1 = fabs(w1[i] - m1) / sd1, z2 = fabs(w2[i] - m2) / sd2;
+    const double z = z1 > z2 ? z1 : z2;  // a NaN z1 picks z2, as jnp.where does
+    if (z > z_thresh) {
+      w1[i] = 0.0;
+      w2[i] = 0.0;
+    }
+  }
+}
+
 __global__ void overlap_combine_kernel(int W, int nspin, const cplx* __restrict__ dets,
                                        cplx* __restrict__ overlaps) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
@@ -240,6 +272,13 @@ int launch_fields_grouped(hipStream_t st, const FieldsArgs* args, const int* sta
 
 int launch_weights_grouped(hipStream_t st, const WeightsArgs* args, int nprob) {
   hipLaunchKernelGGL(weights_grouped_kernel, dim3(nprob), dim3(WEIGHTS_NT), 0, st, args);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+int launch_cs_outliers(afqmc_ctx* c, int W, double* w1, double* w2, double z_thresh) {
+  hipLaunchKernelGGL(cs_outlier_kernel, dim3(1), dim3(WEIGHTS_NT), 0, c->stream, W, z_thresh, w1,
+                     w2);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }

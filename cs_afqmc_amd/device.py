@@ -212,6 +212,15 @@ class DeviceStep:
                                                _dptr(scalars), _dptr(out)))
         return out
 
+    def cs_outliers(self, weights1, weights2, z_thresh=5.0):
+        """corr_sample.py:551-585: zero the weights of joint z-score outliers."""
+        W = weights1.shape[0]
+        self._vec(weights1, W, torch.float64, "weights1")
+        self._vec(weights2, W, torch.float64, "weights2")
+        self._prep()
+        _lib.check(self._lib.afqmc_cs_outliers(self._h, W, _dptr(weights1), _dptr(weights2),
+                                               float(z_thresh)))
+
     # ------------------------------------------------------------------ phase timing
     def prof_enable(self, on=True):
         _lib.check(self._lib.afqmc_prof_enable(self._h, 1 if on else 0))

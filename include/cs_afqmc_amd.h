@@ -179,6 +179,14 @@ enum {
 AFQMC_API int afqmc_prof_enable(afqmc_ctx* ctx, int32_t on);
 AFQMC_API int afqmc_prof_collect(afqmc_ctx* ctx, double* ms, int64_t* counts);
 
+/* Correlated sampling outlier rule, corr_sample.corr_otler_rm (corr_sample.py:551-585):
+ * for two populations of n_walkers propagated with the same fields, z_i =
+ * |w_i - mean(w)| / std(w) (population std) per population; walkers whose larger
+ * z-score exceeds z_thresh get weight 0 in both.  weights1 / weights2 dev (W) float64,
+ * updated in place on ctx's stream. */
+AFQMC_API int afqmc_cs_outliers(afqmc_ctx* ctx, int32_t n_walkers, double* weights1,
+                                double* weights2, double z_thresh);
+
 /* Auxiliary fields for one step: count float64 standard normals written to fields
  * (dev), from a counter-based generator (Philox4x32-10 + Box-Muller) keyed by
  * (seed, counter).  Stands in for jax.random.normal in sampling.py:61-69. */

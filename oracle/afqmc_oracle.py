@@ -458,6 +458,38 @@ def block_scan(walker_type, trial, ham_data, prop_data, wave_data, fields_block,
     return prop_data, block_energy, block_weight
 
 
+def corr_outlier_rm(w1, w2, z_thresh=5.0):
+    """corr_sample.py:551-585 (corr_otler_rm): joint z-score rule on two weight arrays."""
+    w1 = np.asarray(w1, dtype=np.float64)
+    w2 = np.asarray(w2, dtype=np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        z1 = np.abs(w1 - np.mean(w1)) / np.std(w1)
+        z2 = np.abs(w2 - np.mean(w2)) / np.std(w2)
+    z = np.where(z1 > z2, z1, z2)
+    return np.where(z > z_thresh, 0.0, w1), np.where(z > z_thresh, 0.0, w2)
+
+
+def cs_block(systems, fields_block, dt, n_exp_terms, z_thresh=5.0):
+    """corr_sample.py:588-618 (cs_block_scan) with the fields passed in, followed by the
+    local sums of cs_steps_scan (:667-690).  systems: two dicts with keys walker_type,
+    ham_data, prop_data, wave_data, norb; both use fields_block (n_prop_steps, W, G).
+    Returns (updated prop_data pair, (loc_en1, loc_w1, loc_en2, loc_w2))."""
+    pds = []
+    for sy in systems:
+        pd, _, _ = block_scan(sy["walker_type"], sy["walker_type"], sy["ham_data"],
+                              sy["prop_data"], sy["wave_data"], fields_block, dt, n_exp_terms,
+                              sy["norb"])
+        pds.append(pd)
+    w1, w2 = corr_outlier_rm(pds[0]["weights"], pds[1]["weights"], z_thresh)
+    pds[0]["weights"], pds[1]["weights"] = w1, w2
+    sums = []
+    for sy, pd in zip(systems, pds):
+        e = np.real(calc_energy(sy["walker_type"], pd["walkers"], sy["ham_data"], sy["wave_data"]))
+        e = np.where(np.abs(e - pd["e_estimate"]) > np.sqrt(2.0 / dt), pd["e_estimate"], e)
+        sums += [np.sum(e * pd["weights"]), np.sum(pd["weights"])]
+    return pds, tuple(sums)
+
+
 def propagate_phaseless(walker_type, trial, ham_data, prop_data, wave_data, fields,
                         zetas, dt, n_exp_terms, norb, n_ene_blocks, n_sr_blocks):
     """sampling.py:319-341 (propagate_phaseless) -> :126-144 (_sr_block_scan) ->

@@ -279,3 +279,19 @@ def test_propagation_intermediates_by_loops():
         term = term @ A / k
         ser = ser + term
     assert np.allclose(out["exp_h1"], ser, rtol=1e-13, atol=1e-15)
+
+
+def test_corr_outlier_rule_closed_form():
+    """corr_otler_rm: one weight far out in system 1 zeroes that walker in both systems;
+    equal weights (std 0, z = 0/0) are kept, as jnp.where keeps them."""
+    w1 = np.ones(30)
+    w1[7] = 100.0
+    w2 = np.linspace(0.5, 1.5, 30)
+    m, sd = w1.mean(), w1.std()
+    assert abs(100.0 - m) / sd > 5.0 and abs(1.0 - m) / sd < 5.0  # the case is what it claims
+    a, b = o.corr_outlier_rm(w1, w2)
+    assert a[7] == 0.0 and b[7] == 0.0
+    keep = np.arange(30) != 7
+    assert np.array_equal(a[keep], w1[keep]) and np.array_equal(b[keep], w2[keep])
+    c, d = o.corr_outlier_rm(np.ones(8), np.ones(8))
+    assert np.array_equal(c, np.ones(8)) and np.array_equal(d, np.ones(8))

@@ -45,11 +45,19 @@ __host__ __device__ inline double cabs1(cplx a) { return fabs(a.re) + fabs(a.im)
 
 // ---------------------------------------------------------------- bf16 split
 // x ~= hi + lo with hi = bf16(x), lo = bf16(x - hi), both rounded to nearest even
+// (gfx950's v_cvt_pk_bf16_f32 rounds to nearest even)
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 __device__ inline uint16_t f32_to_bf16_rne(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+  return __builtin_bit_cast(uint16_t, (__bf16)f);
 }
+// two floats -> one dword of bf16 (a in the low half)
+__device__ inline uint32_t pack_bf16x2(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2_t{a, b}, bf16x2_t));
+}
+// the floats a packed dword stands for
+__device__ inline float bf16_lo_val(uint32_t u) { return __uint_as_float(u << 16); }
+__device__ inline float bf16_hi_val(uint32_t u) { return __uint_as_float(u & 0xFFFF0000u); }
 __device__ inline void split_bf16(double x, uint16_t* hi, uint16_t* lo) {
   const uint16_t h = f32_to_bf16_rne((float)x);
   *hi = h;

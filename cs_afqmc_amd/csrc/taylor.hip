@@ -491,27 +491,41 @@ __global__ __launch_bounds__(512) void taylor_reg_f32_grouped_kernel(
 }
 
 // ------------------------------------------------------------------------------------
-// Split-bf16 register-resident form for the mixed mode (N <= 128, n <= 32).  Same real
-// block product as above, [ur | ui] = [Vr | Vi] [[yr, yi], [-yi, yr]], on
-// v_mfma_f32_16x16x32_bf16 with each operand x = hi + lo (bf16 planes) and the three
+// Split-bf16 register-resident form for the mixed mode (N <= 128, n <= 32).  The complex
+// product is one real block product [ur | ui] = [Vr | Vi] [[yr, yi], [-yi, yr]] on
+// v_mfma_f32_16x16x32_bf16, with each operand x = hi + lo (bf16 planes) and the three
 // leading products al bh + ah bl + ah bh accumulated in float32:
 //   A (V rows, registers): lane l holds A[l % 16][8 (l / 16) + e] of each 32-wide k block
 //   B (y, LDS [column][k], k contiguous): lane l holds B[8 (l / 16) + e][l % 16]
 //   D (float4): lane l, element r holds D[4 (l / 16) + r][l % 16]
-// The one-body factor is applied as E y = y + (E - I) y with E - I in split bf16: its
-// entries are O(dt), so the product's relative error stays far below float32 rounding of
-// E y itself.  The Horner recursion y <- t0 + V y / k and all additions run in float32;
-// the walker is read and written in float64.
-template <int KB>  // 32-wide k blocks of [Vr | Vi]: 2 Np32 / 32
+// The k axis is packed tightly: with Nh = N rounded up to 4, k in [0, Nh) is the real
+// part of V (top half of the y column), k in [Nh, 2 Nh) the imaginary part (bottom half),
+// and KB = ceil(2 Nh / 32) k blocks cover it.  A lane's four D rows 4 (l / 16) + r are
+// consecutive k of one LDS column, so y goes to LDS as 8-byte stores.
+// The one-body factor is applied as E y = y + (E - I) y with E - I in split bf16 (rows of
+// Np32 = 32 ceil(N / 32) entries, zero beyond N): its entries are O(dt), so the product's
+// relative error stays far below float32 rounding of E y itself.  The Horner recursion
+// y <- t0 + V y / k and all additions run in float32; the walker is read and written in
+// float64.
+template <int KB>
 struct BfTile {
   static constexpr int LDK = 32 * KB + 8;  // LDS row (one column of y) in bf16 elements
+  static constexpr int KH = (KB + 1) / 2;  // 32-wide k blocks of E - I: ceil(N / 32)
 };
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
 
-__device__ inline uint16_t bf16_bits(float f) { return f32_to_bf16_rne(f); }
-__device__ inline float bf16_val(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// 8 floats -> hi and lo bf16 planes (4 dwords each)
+__device__ inline void split8(const float (&v)[8], u32x4_t& ph, u32x4_t& pl) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t h = pack_bf16x2(v[2 * j], v[2 * j + 1]);
+    ph[j] = h;
+    pl[j] = pack_bf16x2(v[2 * j] - bf16_lo_val(h), v[2 * j + 1] - bf16_hi_val(h));
+  }
+}
 
 template <int KB, int CTP>
 __device__ __forceinline__ void taylor_bf_body(int N, int n, int n_terms_k,
@@ -520,13 +534,14 @@ __device__ __forceinline__ void taylor_bf_body(int N, int n, int n_terms_k,
                                                const uint16_t* __restrict__ emi_lo,
                                                const float* __restrict__ vhs, const int w) {
   constexpr int LDK = BfTile<KB>::LDK;
-  constexpr int KH = KB / 2;  // k blocks per half (Np32 / 32)
+  constexpr int KH = BfTile<KB>::KH;
   extern __shared__ uint16_t YB[];  // [2 planes][CTP * 16 columns][LDK]
   uint16_t* Yh = YB;
   uint16_t* Yl = YB + CTP * 16 * LDK;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rb = wave * 16;
   const int np32 = 32 * KH;
+  const int Nh = (N + 3) & ~3;
   const int q4 = lane >> 4, col = lane & 15;
   const int p = rb + col;  // A row of this lane
   const bool prow = p < N;
@@ -535,59 +550,63 @@ __device__ __forceinline__ void taylor_bf_body(int N, int n, int n_terms_k,
   const float* Vr = vhs + (int64_t)w * 2 * N * N;
   const float* Vi = Vr + (int64_t)N * N;
 
-  // V rows in registers, split once
+  // V rows in registers, split once; loads are clamped into the row and masked after,
+  // so they issue back to back without branches
   bf16x8_t vh[KB], vl[KB];
+  const float* vrow_r = Vr + (int64_t)(prow ? p : 0) * N;
+  const float* vrow_i = Vi + (int64_t)(prow ? p : 0) * N;
 #pragma unroll
   for (int kb = 0; kb < KB; ++kb) {
-    const float* src = kb < KH ? Vr : Vi;
-    const int k0 = (kb < KH ? kb : kb - KH) * 32 + 8 * q4;
-    uint16_t hb[8], lb[8];
+    float v[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const int k = k0 + e;
-      const float v = (prow && k < N) ? src[(int64_t)p * N + k] : 0.f;
-      hb[e] = bf16_bits(v);
-      lb[e] = bf16_bits(v - bf16_val(hb[e]));
+      const int k = kb * 32 + 8 * q4 + e;
+      const bool im = k >= Nh;
+      const int kk = im ? k - Nh : k;
+      const bool ok = prow && kk < N;
+      const float t = (im ? vrow_i : vrow_r)[ok ? kk : 0];
+      v[e] = ok ? t : 0.f;
     }
     u32x4_t ph, pl;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      ph[j] = (uint32_t)hb[2 * j] | ((uint32_t)hb[2 * j + 1] << 16);
-      pl[j] = (uint32_t)lb[2 * j] | ((uint32_t)lb[2 * j + 1] << 16);
-    }
+    split8(v, ph, pl);
     vh[kb] = __builtin_bit_cast(bf16x8_t, ph);
     vl[kb] = __builtin_bit_cast(bf16x8_t, pl);
   }
 
-  for (int e = threadIdx.x; e < 2 * CTP * 16 * LDK; e += blockDim.x) YB[e] = 0;
+  {
+    u32x4_t* z = reinterpret_cast<u32x4_t*>(YB);
+    const int nz = 2 * CTP * 16 * LDK / 8;
+    for (int e = threadIdx.x; e < nz; e += blockDim.x) z[e] = u32x4_t{0u, 0u, 0u, 0u};
+  }
   __syncthreads();
 
-  // y (D layout, float32) -> LDS planes, top half [yr | yi] and bottom half [-yi | yr]
+  // y (D layout, float32) -> LDS planes: column c < n holds [yr_c ; -yi_c], column n + c
+  // holds [yi_c ; yr_c] (top half at k = row, bottom half at k = Nh + row)
   auto put = [&](const float4_t* y) {
+    const int row0 = rb + 4 * q4;
 #pragma unroll
-    for (int ct = 0; ct < CTP; ++ct)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = rb + 4 * q4 + r;
-        const int c = ct * 16 + col;
-        if (row < np32 && c < n2) {
-          const float v = y[ct][r];
-          const uint16_t h = bf16_bits(v);
-          const uint16_t l = bf16_bits(v - bf16_val(h));
-          Yh[c * LDK + row] = h;
-          Yl[c * LDK + row] = l;
-          const int cb = c < n ? c + n : c - n;
-          const uint16_t sgn = c < n ? 0 : 0x8000;  // bottom half: +yr, -yi
-          Yh[cb * LDK + np32 + row] = h ^ sgn;
-          Yl[cb * LDK + np32 + row] = l ^ sgn;
-        }
+    for (int ct = 0; ct < CTP; ++ct) {
+      const int c = ct * 16 + col;
+      if (row0 < N && c < n2) {
+        const uint32_t h01 = pack_bf16x2(y[ct][0], y[ct][1]);
+        const uint32_t h23 = pack_bf16x2(y[ct][2], y[ct][3]);
+        const uint32_t l01 = pack_bf16x2(y[ct][0] - bf16_lo_val(h01), y[ct][1] - bf16_hi_val(h01));
+        const uint32_t l23 = pack_bf16x2(y[ct][2] - bf16_lo_val(h23), y[ct][3] - bf16_hi_val(h23));
+        *reinterpret_cast<u32x2_t*>(Yh + c * LDK + row0) = u32x2_t{h01, h23};
+        *reinterpret_cast<u32x2_t*>(Yl + c * LDK + row0) = u32x2_t{l01, l23};
+        const int cb = c < n ? c + n : c - n;
+        const uint32_t sg = c < n ? 0u : 0x80008000u;  // bottom half: +yr, -yi
+        *reinterpret_cast<u32x2_t*>(Yh + cb * LDK + Nh + row0) = u32x2_t{h01 ^ sg, h23 ^ sg};
+        *reinterpret_cast<u32x2_t*>(Yl + cb * LDK + Nh + row0) = u32x2_t{l01 ^ sg, l23 ^ sg};
       }
+    }
   };
   auto bload = [&](const uint16_t* plane, int ct, int kb) {
     return __builtin_bit_cast(
         bf16x8_t, *reinterpret_cast<const u32x4_t*>(plane + (ct * 16 + col) * LDK + kb * 32 + 8 * q4));
   };
-  // acc = (E - I) y over the top half (k < Np32); A streams from L2
+  // acc = (E - I) y: A (rows of E - I) streams from L2; k < Np32 reads the top half of
+  // each column and, past Nh, bottom-half entries that meet zero columns of E - I
   auto e_product = [&](float4_t* acc) {
 #pragma unroll
     for (int ct = 0; ct < CTP; ++ct) acc[ct] = float4_t{0.f, 0.f, 0.f, 0.f};
@@ -667,17 +686,21 @@ __device__ __forceinline__ void taylor_bf_body(int N, int n, int n_terms_k,
     }
 }
 
+// Two 7- or 8-wave workgroups per CU need at most 128 VGPRs (4 waves per SIMD); the
+// widest tiles (n > 24 with N > 96) would spill inside the series at that budget.
+constexpr int bf_waves_per_simd(int KB, int CTP) { return (CTP == 4 && KB >= 7) ? 3 : 4; }
+
 template <int KB, int CTP>
-__global__ __launch_bounds__(512) void taylor_bf_kernel(int N, int n, int n_terms_k,
-                                                        cplx* __restrict__ walkers,
-                                                        const uint16_t* __restrict__ emi_hi,
-                                                        const uint16_t* __restrict__ emi_lo,
-                                                        const float* __restrict__ vhs) {
+__global__ __launch_bounds__(512, bf_waves_per_simd(KB, CTP)) void taylor_bf_kernel(int N, int n, int n_terms_k,
+                                                           cplx* __restrict__ walkers,
+                                                           const uint16_t* __restrict__ emi_hi,
+                                                           const uint16_t* __restrict__ emi_lo,
+                                                           const float* __restrict__ vhs) {
   taylor_bf_body<KB, CTP>(N, n, n_terms_k, walkers, emi_hi, emi_lo, vhs, blockIdx.x);
 }
 
 template <int KB, int CTP>
-__global__ __launch_bounds__(512) void taylor_bf_grouped_kernel(
+__global__ __launch_bounds__(512, bf_waves_per_simd(KB, CTP)) void taylor_bf_grouped_kernel(
     const TaylorArgs* __restrict__ args, const int* __restrict__ start, int nprob) {
   const int pidx = group_of(start, nprob, blockIdx.x);
   const TaylorArgs a = args[pidx];
@@ -686,9 +709,14 @@ __global__ __launch_bounds__(512) void taylor_bf_grouped_kernel(
                           blockIdx.x - start[pidx]);
 }
 
+// k blocks of the packed [Vr | Vi] axis
+inline int bf_kblocks(int N) { return (2 * ((N + 3) & ~3) + 31) / 32; }
+
 template <int KB, int CTP>
 int launch_bf(afqmc_ctx* c, int W, int n, cplx* wk, int s) {
   const int N = c->d.norb;
+  if (BfTile<KB>::KH * 32 != c->Np32 || bf_kblocks(N) != KB)
+    return afqmc_set_error(AFQMC_ERR_ARG, "taylor bf16x3: k blocks %d do not fit N %d", KB, N);
   const int nwaves = (N + 15) / 16;
   const size_t shm = (size_t)2 * CTP * 16 * BfTile<KB>::LDK * sizeof(uint16_t);
   static int lds_limit = 0;
@@ -934,7 +962,7 @@ void taylor_plan(const afqmc_ctx* c, int s, TaylorPlan* pl) {
   pl->ct = 0;
   if (c->taylor_f32) {
     if (N <= 128 && n <= 32) {
-      const int kb = c->Np32 / 16;
+      const int kb = bf_kblocks(N);
       pl->kind = TAYLOR_REG_BF16X3;
       pl->nkgv = kb;
       pl->ct = ctp;
@@ -968,10 +996,14 @@ int launch_taylor(afqmc_ctx* c, int W, cplx* wup, cplx* wdn) {
       const float* E = c->exp_h1_f32[s];
       int rc;
       if (N <= 128 && n <= 32) {
-        switch (c->Np32 / 16) {
+        switch (bf_kblocks(N)) {
+          case 1: rc = launch_bf_ct<1>(c, W, n, wk, s); break;
           case 2: rc = launch_bf_ct<2>(c, W, n, wk, s); break;
+          case 3: rc = launch_bf_ct<3>(c, W, n, wk, s); break;
           case 4: rc = launch_bf_ct<4>(c, W, n, wk, s); break;
+          case 5: rc = launch_bf_ct<5>(c, W, n, wk, s); break;
           case 6: rc = launch_bf_ct<6>(c, W, n, wk, s); break;
+          case 7: rc = launch_bf_ct<7>(c, W, n, wk, s); break;
           default: rc = launch_bf_ct<8>(c, W, n, wk, s); break;
         }
         (void)E;
@@ -1068,16 +1100,16 @@ int launch_taylor_grouped(hipStream_t st, int kind, int nkgv, int ct, const Tayl
     }
   }
   if (kind == TAYLOR_REG_BF16X3) {
-    static int lim[4][4] = {};
+    static int lim[8][4] = {};
     const int ci = ct < 1 ? 0 : (ct > 4 ? 3 : ct - 1);
 #define BF_CASE(KB, C)                                                                      \
   if (nkgv == KB && ct == C)                                                                \
     return launch_grouped_fn(taylor_bf_grouped_kernel<KB, C>, st, args, start, nprob, nblocks, \
-                             nthreads, shm, &lim[KB / 2 - 1][ci]);
-    BF_CASE(2, 1) BF_CASE(2, 2) BF_CASE(2, 3) BF_CASE(2, 4)
-    BF_CASE(4, 1) BF_CASE(4, 2) BF_CASE(4, 3) BF_CASE(4, 4)
-    BF_CASE(6, 1) BF_CASE(6, 2) BF_CASE(6, 3) BF_CASE(6, 4)
-    BF_CASE(8, 1) BF_CASE(8, 2) BF_CASE(8, 3) BF_CASE(8, 4)
+                             nthreads, shm, &lim[KB - 1][ci]);
+#define BF_CASES(KB) BF_CASE(KB, 1) BF_CASE(KB, 2) BF_CASE(KB, 3) BF_CASE(KB, 4)
+    BF_CASES(1) BF_CASES(2) BF_CASES(3) BF_CASES(4)
+    BF_CASES(5) BF_CASES(6) BF_CASES(7) BF_CASES(8)
+#undef BF_CASES
 #undef BF_CASE
     return afqmc_set_error(AFQMC_ERR_ARG, "grouped taylor: k blocks %d, column tiles %d", nkgv, ct);
   }

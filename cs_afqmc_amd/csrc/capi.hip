@@ -356,15 +356,17 @@ extern "C" int afqmc_propagate(afqmc_ctx* c, int32_t W, void* walkers_up, void* 
     if ((rc = launch_taylor(c, W, wu, wd))) return rc;
   }
   {
-    // new overlaps and the weight update
-    // new overlaps; the Green's function of the new walkers is kept for the next step
+    // new overlaps and each walker's weight update; the Green's function of the new
+    // walkers is kept for the next step
     PhaseTimer t(c, AFQMC_PH_OVERLAP);
-    if ((rc = launch_measure(c, W, wu, wd, !c->mixed, c->mixed))) return rc;
+    const WeightsArgs tail = weights_args(c, W, weights, static_cast<cplx*>(overlaps), scalars);
+    if ((rc = launch_measure(c, W, wu, wd, !c->mixed, c->mixed, &tail))) return rc;
     c->green_src = walkers_up;
     c->green_w = W;
   }
+  // population-control shift from the summed weights
   PhaseTimer t(c, AFQMC_PH_WEIGHTS);
-  return launch_weights(c, W, weights, static_cast<cplx*>(overlaps), scalars);
+  return launch_pcs(c, W, weights, scalars);
 }
 
 extern "C" int afqmc_block_stats(afqmc_ctx* c, int32_t W, const void* energies,

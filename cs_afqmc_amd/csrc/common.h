@@ -64,6 +64,46 @@ __device__ inline void split_bf16(double x, uint16_t* hi, uint16_t* lo) {
   *lo = f32_to_bf16_rne((float)(x - (double)__uint_as_float((uint32_t)h << 16)));
 }
 
+// ---------------------------------------------------------------- walker weights
+struct WeightsArgs {
+  int W, nspin;
+  double dt, sqdt, h0_prop;
+  const cplx* dets;
+  const cplx* terms;
+  double* weights;
+  cplx* overlaps;
+  double* scalars;
+};
+
+__device__ inline cplx cexp_(cplx a) {
+  const double e = exp(a.re);
+  double s, c;
+  sincos(a.im, &s, &c);
+  return cplx{e * c, e * s};
+}
+
+// Importance function, phaseless projection, clipping and the weight cap of walker w
+// (propagation.py:133-160) given its new overlap and the population-control shift pcs;
+// writes weights[w] and overlaps[w] and returns the new weight.
+__device__ inline double walker_weight_update(const WeightsArgs& a, double pcs, int w,
+                                              cplx onew) {
+  const cplx oold = a.overlaps[w];
+  const cplx st = a.terms[2 * w], ft = a.terms[2 * w + 1];
+  const cplx e = cplx{-a.sqdt * st.re + ft.re + a.dt * (pcs + a.h0_prop), -a.sqdt * st.im + ft.im};
+  const cplx imp = cdiv(cmul(cexp_(e), onew), oold);
+  const cplx rb = cdiv(cmul(cexp_(cplx{-a.sqdt * st.re, -a.sqdt * st.im}), onew), oold);
+  const double theta = atan2(rb.im, rb.re);
+  double ipl = hypot(imp.re, imp.im) * cos(theta);
+  if (isnan(ipl)) ipl = 0.0;
+  if (ipl < 1.0e-3) ipl = 0.0;
+  if (ipl > 100.0) ipl = 0.0;
+  double wt = ipl * a.weights[w];
+  if (wt > 100.0) wt = 0.0;
+  a.weights[w] = wt;
+  a.overlaps[w] = onew;
+  return wt;
+}
+
 // ---------------------------------------------------------------- reductions
 template <int NT>
 __device__ inline double block_sum(double v, double* red) {
@@ -218,13 +258,17 @@ int launch_gemm_nt_bf16x3_f64(hipStream_t s, int M, int N, int K, float alpha,
 int launch_split_bf16(hipStream_t s, int rows, int cols, const double* x, int64_t ld_src,
                       uint16_t* hi, uint16_t* lo, int64_t ld_dst);
 // measure.hip: green_f64 -> c->green (float64 split layout); green_bf -> c->green_bf
+// With tail, the last spin's launch also applies the per-walker weight update (thread 0 of
+// each walker's block); the population-control shift then comes from launch_pcs.
 int launch_measure(afqmc_ctx* c, int W, const cplx* wup, const cplx* wdn, bool green_f64,
-                   bool green_bf = false);
+                   bool green_bf = false, const WeightsArgs* tail = nullptr);
 // taylor.hip
 int launch_taylor(afqmc_ctx* c, int W, cplx* wup, cplx* wdn);
 // step.hip
 int launch_fields(afqmc_ctx* c, int W, const double* fields, cplx* fb_out);
-int launch_weights(afqmc_ctx* c, int W, double* weights, cplx* overlaps, double* scalars);
+WeightsArgs weights_args(const afqmc_ctx* c, int W, double* weights, cplx* overlaps,
+                         double* scalars);
+int launch_pcs(afqmc_ctx* c, int W, const double* weights, double* scalars);
 int launch_overlap_combine(afqmc_ctx* c, int W, cplx* overlaps);
 int launch_fb_out(afqmc_ctx* c, int W, cplx* fb_out);
 int launch_block_stats(afqmc_ctx* c, int W, const cplx* energies, const double* weights,
@@ -277,15 +321,6 @@ struct FieldsArgs {
   uint16_t* xs_hi;
   uint16_t* xs_lo;
   cplx* terms;
-};
-struct WeightsArgs {
-  int W, nspin;
-  double dt, sqdt, h0_prop;
-  const cplx* dets;
-  const cplx* terms;
-  double* weights;
-  cplx* overlaps;
-  double* scalars;
 };
 struct TaylorArgs {
   int N, n, K, rows_pad;

@@ -5,8 +5,12 @@
 // Follows rhf._calc_overlap_restricted / _calc_green (wavefunctions.py:478-501) and
 // uhf._calc_overlap / _calc_green (wavefunctions.py:658-692).  The trial C is real.
 //
-// One wave (a 64-lane workgroup) per (walker, spin), so many walkers are in flight on
-// each CU and the elimination needs no cross-wave barriers.  Both products run on
+// One workgroup per (walker, spin).  For n <= 24 it has two waves: they split the k
+// chunks of the overlap product (partials summed in a fixed order through LDS) and the
+// column tiles of the Green's function, and wave 0 runs the register Gauss-Jordan while
+// both waves' first Green operands are in flight; with one walker per SIMD (W = 4 x CUs)
+// this halves the chains of dependent global loads each walker waits on.  Wider systems
+// use one wave and the LDS elimination.  Both products run on
 // v_mfma_f64_16x16x4f64 in real block form:
 //   [Or | Oi]  = C^T [wr | wi]                         (n x 2n,  K = N)
 //   [Gr ; Gi]  = [[Xr^T, -Xi^T], [Xi^T, Xr^T]] [wr^T ; wi^T]   (2n x N, K = 2n)
@@ -98,6 +102,14 @@ __device__ __forceinline__ cplx gauss_jordan_registers(cplx* M, int n, int lane)
   return det;
 }
 
+// waves per walker and the LDS the cross-wave overlap reduction needs; the two-wave
+// kernels keep two waves per SIMD (at most 256 registers per lane, AGPRs included)
+constexpr int meas_waves(int t2) { return t2 <= 3 ? 2 : 1; }
+constexpr int meas_ti(int t2) { return t2 > 4 ? 1 : (t2 + 1) / 2; }
+constexpr size_t meas_red_bytes(int t2) {
+  return meas_waves(t2) > 1 ? (size_t)meas_ti(t2) * t2 * 4 * 64 * sizeof(double) : 0;
+}
+
 // T2 = ceil(2n / 16): column tiles of [Or | Oi] and row tiles of [Gr ; Gi]
 template <int T2, int TI, bool STAGE>
 __device__ __forceinline__ void measure_body(int N, int n, const cplx* __restrict__ walkers,
@@ -105,13 +117,17 @@ __device__ __forceinline__ void measure_body(int N, int n, const cplx* __restric
                                              cplx* __restrict__ dets, int spin,
                                              double* __restrict__ green,
                                              uint16_t* __restrict__ ghi,
-                                             uint16_t* __restrict__ glo, int ld16, const int w) {
+                                             uint16_t* __restrict__ glo, int ld16, const int w,
+                                             const WeightsArgs& tail) {
+  constexpr int NW = meas_waves(T2);
   extern __shared__ double smem[];
   const int n2 = 2 * n;
   cplx* M = reinterpret_cast<cplx*>(smem);  // n x 2n augmented matrix
   cplx* fcol = M + n * n2;                  // pivot column copy (n)
+  // overlap partials of wave 1 ([TI][T2][4][64] doubles), after the staged walker
+  double* red = reinterpret_cast<double*>(fcol + n + (STAGE ? N * n : 0));
 
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int s = lane >> 4, col = lane & 15;
   const cplx* phi = walkers + (int64_t)w * N * n;
   // walker operand: staged once into LDS with 16-byte loads (all in flight together)
@@ -119,7 +135,7 @@ __device__ __forceinline__ void measure_body(int N, int n, const cplx* __restric
   const double* phid = reinterpret_cast<const double*>(phi);
   if (STAGE) {
     cplx* ph = fcol + n;
-    for (int e = lane; e < N * n; e += 64) ph[e] = phi[e];
+    for (int e = threadIdx.x; e < N * n; e += 64 * NW) ph[e] = phi[e];
     phid = reinterpret_cast<const double*>(ph);
   }
   __syncthreads();
@@ -171,14 +187,36 @@ __device__ __forceinline__ void measure_body(int N, int n, const cplx* __restric
             acc[ti][tc] =
                 __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][ti], b[u][tc], acc[ti][tc], 0, 0, 0);
     };
+    // wave wv takes chunks wv, wv + NW, wv + 2 NW, ...
     const int nch = (nkg + MEAS_KC - 1) / MEAS_KC;
-    load(a0, b0, 0);
-    for (int ch = 0; ch < nch; ch += 2) {
-      if (ch + 1 < nch) load(a1, b1, (ch + 1) * MEAS_KC);
+    if (wv < nch) load(a0, b0, wv * MEAS_KC);
+    for (int ch = wv; ch < nch; ch += 2 * NW) {
+      if (ch + NW < nch) load(a1, b1, (ch + NW) * MEAS_KC);
       mma(a0, b0);
-      if (ch + 2 < nch) load(a0, b0, (ch + 2) * MEAS_KC);
-      if (ch + 1 < nch) mma(a1, b1);
+      if (ch + 2 * NW < nch) load(a0, b0, (ch + 2 * NW) * MEAS_KC);
+      if (ch + NW < nch) mma(a1, b1);
     }
+    if constexpr (NW > 1) {
+      // wave 0's partial + wave 1's partial, in that order
+      if (wv == 1) {
+#pragma unroll
+        for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+          for (int tc = 0; tc < T2; ++tc)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) red[((ti * T2 + tc) * 4 + r) * 64 + lane] = acc[ti][tc][r];
+      }
+      __syncthreads();
+      if (wv == 0) {
+#pragma unroll
+        for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+          for (int tc = 0; tc < T2; ++tc)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[ti][tc][r] += red[((ti * T2 + tc) * 4 + r) * 64 + lane];
+      }
+    }
+    if (wv == 0) {
 #pragma unroll
     for (int tc = 0; tc < T2; ++tc) {
       if (boff[tc] < 0) continue;
@@ -193,19 +231,89 @@ __device__ __forceinline__ void measure_body(int N, int n, const cplx* __restric
         }
     }
     }
-    for (int e = lane; e < n * n; e += 64) {
+    if constexpr (NW > 1) __syncthreads();  // red is reused by the next row-tile pass
+    }
+    for (int e = threadIdx.x; e < n * n; e += 64 * NW) {
       const int i = e / n, j = e - (e / n) * n;
       M[i * n2 + n + j] = cplx{i == j ? 1.0 : 0.0, 0.0};
     }
   }
   __syncthreads();
 
-  // ---- Gauss-Jordan with partial pivoting: [O | I] -> [* | X], det from pivots.
-  cplx det = cplx{1.0, 0.0};
-  if constexpr (T2 <= 3) {  // n <= 24: the column fits the registers without spilling
-    det = gauss_jordan_registers<T2>(M, n, lane);
+  // ---- Gauss-Jordan with partial pivoting: [O | I] -> [* | X], det from pivots;
+  // then [Gr ; Gi] = [[Xr^T, -Xi^T], [Xi^T, Xr^T]] [wr^T ; wi^T], K = 2n, columns p.
+  const bool want_green = green || ghi;
+  double* gw = green ? green + (int64_t)w * n2 * N : nullptr;
+  uint16_t* gh = ghi ? ghi + (int64_t)w * n2 * ld16 : nullptr;
+  uint16_t* gl = ghi ? glo + (int64_t)w * n2 * ld16 : nullptr;
+  const int tp_n = (N + 15) >> 4;
+  // thread 0: det of this spin and, on the last spin's launch, the walker's weight
+  auto finish_walker = [&](cplx det) {
+    dets[(int64_t)w * 2 + spin] = det;
+    if (tail.weights) {
+      const cplx onew = tail.nspin == 1 ? cmul(det, det) : cmul(dets[(int64_t)w * 2], det);
+      walker_weight_update(tail, tail.scalars[1], w, onew);
+    }
+  };
+  if constexpr (T2 <= 3) {
+    // n <= 24: the column fits the registers without spilling.  All k groups of a
+    // tile's walker operand are loaded at once (one round of global loads per tile) and
+    // each wave's first tile is in flight during the elimination; A comes from X in LDS.
+    constexpr int NKG = 4 * T2;  // k groups of 2n <= 16 T2
+    auto a_of = [&](int kg, int tr) {
+      const int k = 4 * kg + s;
+      const int r = tr * 16 + col;
+      double v = 0.0;
+      if (k < n2 && r < n2) {
+        const int kj = k < n ? k : k - n;
+        const cplx x = M[kj * n2 + n + (r < n ? r : r - n)];  // X[kj][ri]
+        v = (r < n) == (k < n) ? x.re : ((r < n) ? -x.im : x.im);
+      }
+      return v;
+    };
+    auto load_b = [&](double (&b)[NKG], int tp) {
+      const int p = tp * 16 + col;
+#pragma unroll
+      for (int kg = 0; kg < NKG; ++kg) {
+        const int k = 4 * kg + s;
+        const int kj = k < n ? k : k - n;
+        b[kg] = (k < n2 && p < N) ? phid[2 * ((int64_t)p * n + kj) + (k < n ? 0 : 1)] : 0.0;
+      }
+    };
+    auto tile = [&](const double (&b)[NKG], int tp) {
+      double4_t acc[T2];
+#pragma unroll
+      for (int tr = 0; tr < T2; ++tr) acc[tr] = double4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kg = 0; kg < NKG; ++kg)
+#pragma unroll
+        for (int tr = 0; tr < T2; ++tr)
+          acc[tr] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_of(kg, tr), b[kg], acc[tr], 0, 0, 0);
+      const int p = tp * 16 + col;
+#pragma unroll
+      for (int tr = 0; tr < T2; ++tr)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int row = tr * 16 + s + 4 * q;
+          if (row < n2 && p < N) {
+            if (gw) gw[(int64_t)row * N + p] = acc[tr][q];
+            if (gh) split_bf16(acc[tr][q], &gh[(int64_t)row * ld16 + p], &gl[(int64_t)row * ld16 + p]);
+          }
+        }
+    };
+    double b[NKG];
+    if (want_green && wv < tp_n) load_b(b, wv);
+    cplx det = cplx{1.0, 0.0};
+    if (wv == 0) det = gauss_jordan_registers<T2>(M, n, lane);
     __syncthreads();
+    if (threadIdx.x == 0) finish_walker(det);
+    if (!want_green) return;
+    for (int tp = wv; tp < tp_n; tp += NW) {
+      if (tp != wv) load_b(b, tp);
+      tile(b, tp);
+    }
   } else {
+  cplx det = cplx{1.0, 0.0};
   // Wider systems in LDS: lane c' owns column k + 1 + c' of the trailing matrix
   // (2n - k - 1 <= 127 columns, at most two per lane).
   for (int k = 0; k < n; ++k) {
@@ -268,72 +376,10 @@ __device__ __forceinline__ void measure_body(int N, int n, const cplx* __restric
     }
     __syncthreads();
   }
-  }
-  if (lane == 0) dets[(int64_t)w * 2 + spin] = det;
-  if (!green && !ghi) return;
-
-  // ---- [Gr ; Gi] = [[Xr^T, -Xi^T], [Xi^T, Xr^T]] [wr^T ; wi^T], K = 2n, columns p
-  if constexpr (T2 <= 3) {
-    // All k groups of a tile's walker operand are loaded at once (one round of global
-    // loads per tile); A comes from X in LDS.  Prefetching the next tile as well costs
-    // more registers than it saves once several waves share a SIMD.
-    constexpr int NKG = 4 * T2;  // k groups of 2n <= 16 T2
-    const int tp_n = (N + 15) >> 4;
-    auto a_of = [&](int kg, int tr) {
-      const int k = 4 * kg + s;
-      const int r = tr * 16 + col;
-      double v = 0.0;
-      if (k < n2 && r < n2) {
-        const int kj = k < n ? k : k - n;
-        const cplx x = M[kj * n2 + n + (r < n ? r : r - n)];  // X[kj][ri]
-        v = (r < n) == (k < n) ? x.re : ((r < n) ? -x.im : x.im);
-      }
-      return v;
-    };
-    auto load_b = [&](double (&b)[NKG], int tp) {
-      const int p = tp * 16 + col;
-#pragma unroll
-      for (int kg = 0; kg < NKG; ++kg) {
-        const int k = 4 * kg + s;
-        const int kj = k < n ? k : k - n;
-        b[kg] = (k < n2 && p < N) ? phid[2 * ((int64_t)p * n + kj) + (k < n ? 0 : 1)] : 0.0;
-      }
-    };
-    double* gw = green ? green + (int64_t)w * n2 * N : nullptr;
-    uint16_t* gh = ghi ? ghi + (int64_t)w * n2 * ld16 : nullptr;
-    uint16_t* gl = ghi ? glo + (int64_t)w * n2 * ld16 : nullptr;
-    auto tile = [&](const double (&b)[NKG], int tp) {
-      double4_t acc[T2];
-#pragma unroll
-      for (int tr = 0; tr < T2; ++tr) acc[tr] = double4_t{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int kg = 0; kg < NKG; ++kg)
-#pragma unroll
-        for (int tr = 0; tr < T2; ++tr)
-          acc[tr] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_of(kg, tr), b[kg], acc[tr], 0, 0, 0);
-      const int p = tp * 16 + col;
-#pragma unroll
-      for (int tr = 0; tr < T2; ++tr)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int row = tr * 16 + s + 4 * q;
-          if (row < n2 && p < N) {
-            if (gw) gw[(int64_t)row * N + p] = acc[tr][q];
-            if (gh) split_bf16(acc[tr][q], &gh[(int64_t)row * ld16 + p], &gl[(int64_t)row * ld16 + p]);
-          }
-        }
-    };
-    for (int tp = 0; tp < tp_n; ++tp) {
-      double b[NKG];
-      load_b(b, tp);
-      tile(b, tp);
-    }
-  } else {
-    const int tp_n = (N + 15) >> 4;
+  if (lane == 0) finish_walker(det);
+  if (!want_green) return;
+  {
     const int nkg = (n2 + 3) >> 2;
-    double* gw = green ? green + (int64_t)w * n2 * N : nullptr;
-    uint16_t* gh = ghi ? ghi + (int64_t)w * n2 * ld16 : nullptr;
-    uint16_t* gl = ghi ? glo + (int64_t)w * n2 * ld16 : nullptr;
     for (int tp = 0; tp < tp_n; ++tp) {
       const int p = tp * 16 + col;  // B column = orbital p
       double4_t acc[T2];
@@ -377,40 +423,42 @@ __device__ __forceinline__ void measure_body(int N, int n, const cplx* __restric
         }
     }
   }
+  }
 }
 
 template <int T2, int TI, bool STAGE>
-__global__ __launch_bounds__(64) void measure_kernel(
+__global__ __launch_bounds__(64 * meas_waves(T2), meas_waves(T2)) void measure_kernel(
     int N, int n, const cplx* __restrict__ walkers, const double* __restrict__ trial,
     cplx* __restrict__ dets, int spin, double* __restrict__ green, uint16_t* __restrict__ ghi,
-    uint16_t* __restrict__ glo, int ld16) {
+    uint16_t* __restrict__ glo, int ld16, WeightsArgs tail) {
   measure_body<T2, TI, STAGE>(N, n, walkers, trial, dets, spin, green, ghi, glo, ld16,
-                              blockIdx.x);
+                              blockIdx.x, tail);
 }
 
 template <int T2, int TI, bool STAGE>
-__global__ __launch_bounds__(64) void measure_grouped_kernel(const MeasureArgs* __restrict__ args,
+__global__ __launch_bounds__(64 * meas_waves(T2), meas_waves(T2)) void measure_grouped_kernel(const MeasureArgs* __restrict__ args,
                                                              const int* __restrict__ start,
                                                              int nprob) {
   const int p = group_of(start, nprob, blockIdx.x);
   const MeasureArgs a = args[p];
   measure_body<T2, TI, STAGE>(a.N, a.n, a.walkers, a.trial, a.dets, a.spin, a.green, a.ghi,
-                              a.glo, a.ld16, blockIdx.x - start[p]);
+                              a.glo, a.ld16, blockIdx.x - start[p], WeightsArgs{});
 }
 
 template <int T2>
 int launch_measure_grouped_t(hipStream_t st, bool stage, const MeasureArgs* args,
                              const int* start, int nprob, int nblocks, size_t shm) {
-  constexpr int TI = T2 > 4 ? 1 : (T2 + 1) / 2;
+  constexpr int TI = meas_ti(T2);
+  const dim3 block(64 * meas_waves(T2));
   if (stage) {
-    hipLaunchKernelGGL((measure_grouped_kernel<T2, TI, true>), dim3(nblocks), dim3(64), shm, st,
+    hipLaunchKernelGGL((measure_grouped_kernel<T2, TI, true>), dim3(nblocks), block, shm, st,
                        args, start, nprob);
   } else {
     static int lds_limit = 0;
     const int rc =
         ensure_dyn_lds((const void*)measure_grouped_kernel<T2, TI, false>, shm, &lds_limit);
     if (rc) return rc;
-    hipLaunchKernelGGL((measure_grouped_kernel<T2, TI, false>), dim3(nblocks), dim3(64), shm, st,
+    hipLaunchKernelGGL((measure_grouped_kernel<T2, TI, false>), dim3(nblocks), block, shm, st,
                        args, start, nprob);
   }
   LAUNCH_CHECK();
@@ -418,47 +466,53 @@ int launch_measure_grouped_t(hipStream_t st, bool stage, const MeasureArgs* args
 }
 
 template <int T2>
-int launch_measure_t(afqmc_ctx* c, int W, int n, const cplx* wk, int s, bool g64, bool gbf) {
+int launch_measure_t(afqmc_ctx* c, int W, int n, const cplx* wk, int s, bool g64, bool gbf,
+                     const WeightsArgs& tail) {
   // O row tiles per pass: all ceil(n / 16) of them unless the accumulators would spill
-  constexpr int TI = T2 > 4 ? 1 : (T2 + 1) / 2;
+  constexpr int TI = meas_ti(T2);
+  const dim3 block(64 * meas_waves(T2));
   const int N = c->d.norb;
-  const size_t base = (size_t)(n * 2 * n + n) * sizeof(cplx);
-  const size_t staged = base + (size_t)N * n * sizeof(cplx);
-  if (staged <= 24 * 1024) {  // small walkers only: staging costs occupancy
-    hipLaunchKernelGGL((measure_kernel<T2, TI, true>), dim3(W), dim3(64), staged, c->stream, N,
+  int t2;
+  bool stage;
+  size_t shm;
+  measure_plan(c, s, &t2, &stage, &shm);
+  if (stage) {  // small walkers only: staging costs occupancy
+    hipLaunchKernelGGL((measure_kernel<T2, TI, true>), dim3(W), block, shm, c->stream, N,
                        n, wk, c->trial[s], c->dets, s, g64 ? c->green[s] : nullptr,
                        gbf ? c->green_bf[s][0] : nullptr, gbf ? c->green_bf[s][1] : nullptr,
-                       c->Np32);
+                       c->Np32, tail);
     LAUNCH_CHECK();
     return AFQMC_OK;
   }
-  const size_t shm = base;
   static int lds_limit = 0;
   const int rc = ensure_dyn_lds((const void*)measure_kernel<T2, TI, false>, shm, &lds_limit);
   if (rc) return rc;
-  hipLaunchKernelGGL((measure_kernel<T2, TI, false>), dim3(W), dim3(64), shm, c->stream, c->d.norb, n, wk,
+  hipLaunchKernelGGL((measure_kernel<T2, TI, false>), dim3(W), block, shm, c->stream, c->d.norb, n, wk,
                      c->trial[s], c->dets, s, g64 ? c->green[s] : nullptr,
-                     gbf ? c->green_bf[s][0] : nullptr, gbf ? c->green_bf[s][1] : nullptr, c->Np32);
+                     gbf ? c->green_bf[s][0] : nullptr, gbf ? c->green_bf[s][1] : nullptr, c->Np32,
+                     tail);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }
 
 }  // namespace
 
-int launch_measure(afqmc_ctx* c, int W, const cplx* wup, const cplx* wdn, bool g64, bool gbf) {
+int launch_measure(afqmc_ctx* c, int W, const cplx* wup, const cplx* wdn, bool g64, bool gbf,
+                   const WeightsArgs* tail) {
   for (int sp = 0; sp < c->nspin; ++sp) {
     const int n = c->nocc[sp];
     const cplx* wk = (sp == 0) ? wup : wdn;
+    const WeightsArgs t = (tail && sp == c->nspin - 1) ? *tail : WeightsArgs{};
     int rc;
     switch ((2 * n + 15) / 16) {
-      case 1: rc = launch_measure_t<1>(c, W, n, wk, sp, g64, gbf); break;
-      case 2: rc = launch_measure_t<2>(c, W, n, wk, sp, g64, gbf); break;
-      case 3: rc = launch_measure_t<3>(c, W, n, wk, sp, g64, gbf); break;
-      case 4: rc = launch_measure_t<4>(c, W, n, wk, sp, g64, gbf); break;
-      case 5: rc = launch_measure_t<5>(c, W, n, wk, sp, g64, gbf); break;
-      case 6: rc = launch_measure_t<6>(c, W, n, wk, sp, g64, gbf); break;
-      case 7: rc = launch_measure_t<7>(c, W, n, wk, sp, g64, gbf); break;
-      default: rc = launch_measure_t<8>(c, W, n, wk, sp, g64, gbf); break;
+      case 1: rc = launch_measure_t<1>(c, W, n, wk, sp, g64, gbf, t); break;
+      case 2: rc = launch_measure_t<2>(c, W, n, wk, sp, g64, gbf, t); break;
+      case 3: rc = launch_measure_t<3>(c, W, n, wk, sp, g64, gbf, t); break;
+      case 4: rc = launch_measure_t<4>(c, W, n, wk, sp, g64, gbf, t); break;
+      case 5: rc = launch_measure_t<5>(c, W, n, wk, sp, g64, gbf, t); break;
+      case 6: rc = launch_measure_t<6>(c, W, n, wk, sp, g64, gbf, t); break;
+      case 7: rc = launch_measure_t<7>(c, W, n, wk, sp, g64, gbf, t); break;
+      default: rc = launch_measure_t<8>(c, W, n, wk, sp, g64, gbf, t); break;
     }
     if (rc) return rc;
   }
@@ -489,5 +543,5 @@ void measure_plan(const afqmc_ctx* c, int s, int* t2, bool* stage, size_t* shm) 
   const size_t base = (size_t)(n * 2 * n + n) * sizeof(cplx);
   const size_t staged = base + (size_t)N * n * sizeof(cplx);
   *stage = staged <= 24 * 1024;
-  *shm = *stage ? staged : base;
+  *shm = (*stage ? staged : base) + meas_red_bytes(*t2);
 }

@@ -1,8 +1,10 @@
 // Element-wise parts of the phaseless step (propagator.propagate, propagation.py:100-165):
 //   fields_kernel  : field shifts, shifted fields, shift / force-bias terms
 //                    (propagation.py:121-127) and the VHS GEMM operand i sqrt(dt) x~
-//   weights_kernel : importance function, phaseless projection, clipping, weight cap and
-//                    the population-control shift (propagation.py:133-164)
+//   weights_grouped_kernel : importance function, phaseless projection, clipping, weight
+//                    cap and the population-control shift (propagation.py:133-164), one
+//                    block per fragment; the single-context step applies the per-walker
+//                    part in the measurement kernel and the shift in pcs_kernel
 #include "common.h"
 
 namespace {
@@ -13,13 +15,6 @@ constexpr int WEIGHTS_NT = 1024;
 __device__ inline cplx overlap_of(const cplx* dets, int w, int nspin) {
   const cplx d0 = dets[2 * w];
   return nspin == 1 ? cmul(d0, d0) : cmul(d0, dets[2 * w + 1]);
-}
-
-__device__ inline cplx cexp_(cplx a) {
-  const double e = exp(a.re);
-  double s, c;
-  sincos(a.im, &s, &c);
-  return cplx{e * c, e * s};
 }
 
 // one workgroup per walker
@@ -106,34 +101,24 @@ __device__ __forceinline__ void weights_body(
   __shared__ double red[WEIGHTS_NT / 64];
   const double e_estimate = scalars[0];
   const double pcs = scalars[1];
+  const WeightsArgs a{W, nspin, dt, sqdt, h0_prop, dets, terms, weights, overlaps, scalars};
   double part = 0.0;
-  for (int w = threadIdx.x; w < W; w += WEIGHTS_NT) {
-    const cplx onew = overlap_of(dets, w, nspin);
-    const cplx oold = overlaps[w];
-    const cplx st = terms[2 * w], ft = terms[2 * w + 1];
-    const cplx a = cplx{-sqdt * st.re + ft.re + dt * (pcs + h0_prop), -sqdt * st.im + ft.im};
-    const cplx imp = cdiv(cmul(cexp_(a), onew), oold);
-    const cplx rb = cdiv(cmul(cexp_(cplx{-sqdt * st.re, -sqdt * st.im}), onew), oold);
-    const double theta = atan2(rb.im, rb.re);
-    double ipl = hypot(imp.re, imp.im) * cos(theta);
-    if (isnan(ipl)) ipl = 0.0;
-    if (ipl < 1.0e-3) ipl = 0.0;
-    if (ipl > 100.0) ipl = 0.0;
-    double wt = ipl * weights[w];
-    if (wt > 100.0) wt = 0.0;
-    weights[w] = wt;
-    overlaps[w] = onew;
-    part += wt;
-  }
+  for (int w = threadIdx.x; w < W; w += WEIGHTS_NT)
+    part += walker_weight_update(a, pcs, w, overlap_of(dets, w, nspin));
   const double total = block_sum<WEIGHTS_NT>(part, red);
   if (threadIdx.x == 0) scalars[1] = e_estimate - 0.1 * (log(total / (double)W) / dt);
 }
 
-__global__ __launch_bounds__(WEIGHTS_NT) void weights_kernel(
-    int W, int nspin, double dt, double sqdt, double h0_prop, const cplx* __restrict__ dets,
-    const cplx* __restrict__ terms, double* __restrict__ weights, cplx* __restrict__ overlaps,
-    double* __restrict__ scalars) {
-  weights_body(W, nspin, dt, sqdt, h0_prop, dets, terms, weights, overlaps, scalars);
+// population-control shift from weights the measurement kernel already updated
+// (propagation.py:161-164): same strided sum and tree as weights_body
+__global__ __launch_bounds__(WEIGHTS_NT) void pcs_kernel(int W, double dt,
+                                                         const double* __restrict__ weights,
+                                                         double* __restrict__ scalars) {
+  __shared__ double red[WEIGHTS_NT / 64];
+  double part = 0.0;
+  for (int w = threadIdx.x; w < W; w += WEIGHTS_NT) part += weights[w];
+  const double total = block_sum<WEIGHTS_NT>(part, red);
+  if (threadIdx.x == 0) scalars[1] = scalars[0] - 0.1 * (log(total / (double)W) / dt);
 }
 
 // one block per fragment
@@ -230,9 +215,14 @@ int launch_fields(afqmc_ctx* c, int W, const double* fields, cplx* fb_out) {
   return AFQMC_OK;
 }
 
-int launch_weights(afqmc_ctx* c, int W, double* weights, cplx* overlaps, double* scalars) {
-  hipLaunchKernelGGL(weights_kernel, dim3(1), dim3(WEIGHTS_NT), 0, c->stream, W, c->nspin,
-                     c->d.dt, sqrt(c->d.dt), c->h0_prop, c->dets, c->terms, weights, overlaps,
+WeightsArgs weights_args(const afqmc_ctx* c, int W, double* weights, cplx* overlaps,
+                         double* scalars) {
+  return WeightsArgs{W, c->nspin, c->d.dt, sqrt(c->d.dt), c->h0_prop, c->dets, c->terms,
+                     weights, overlaps, scalars};
+}
+
+int launch_pcs(afqmc_ctx* c, int W, const double* weights, double* scalars) {
+  hipLaunchKernelGGL(pcs_kernel, dim3(1), dim3(WEIGHTS_NT), 0, c->stream, W, c->d.dt, weights,
                      scalars);
   LAUNCH_CHECK();
   return AFQMC_OK;

@@ -18,8 +18,10 @@ float64 rate, the speed-up and the largest block-energy difference |dE| between 
 two runs.  `--precision fp64|bf16x3` runs one mode only.
 
 Prints ONE JSON line.  `value` = walker-steps/s over the timed region (inputs already
-resident in HBM).  `roofline` describes the dominant kernel of the reported run, timed
-with HIP events on the library's stream during the timed region.  `cpu_baseline` times
+resident in HBM).  `phases` / `rooflines` come from a profiled pass of up to 50 steps
+just before the timed region, with a HIP event pair around every phase (each pair costs
+device time); the timed region records events only around the dominant phase, and
+`roofline` describes that kernel from those events.  `cpu_baseline` times
 the numpy restatement of the reference algorithm (the oracle) on a bounded sample on
 the host cores, one single-threaded process per core (the reference's own model: one
 MPI rank per core with one intra-op thread).
@@ -42,6 +44,7 @@ HBM_PEAK_GBS = 8000.0
 
 N_PROP_STEPS = 50
 N_ENE_BLOCKS = 4
+PROFILE_STEPS = 50  # steps of the all-phase profiled pass before the timed region
 
 def phase_kernel(precision, phase, norb, nelec):
     """Device kernel behind a timed phase (symbol prefix as rocprofv3 lists it), by the
@@ -57,33 +60,100 @@ def phase_kernel(precision, phase, norb, nelec):
         return "measure_kernel"
     if phase == "energy":
         return "exchange_f64_kernel" if precision == "fp64" else "exchange_bf16x3_kernel"
-    return phase
+    return {"fields": "fields_kernel", "weights": "pcs_kernel", "qr": "qr_kernel",
+            "sr": "sr_index_kernel", "rng": "normal_kernel"}.get(phase, phase)
 
 
 def flops_per_walker(norb, nelec, nchol, n_exp_terms, walker_type):
     """Algorithmic real flops per walker per phase (real x complex MAC = 4 flop,
-    complex x complex MAC = 8 flop)."""
+    complex x complex MAC = 8 flop).  The measurement kernel ("green" at a block's start,
+    "overlap" after each step) forms C^T w (4 N n^2), the inverse by elimination (8 n^3)
+    and the Green's function w O^-1 (8 N n^2)."""
     K = n_exp_terms - 1
     N, G = norb, nchol
     ns = [nelec[0]] if walker_type == "rhf" else [nelec[0], nelec[1]]
     f = {
         "vhs_gemm": 4.0 * G * N * N,
         "fb_gemm": sum(4.0 * G * N * n for n in ns),
-        "green": sum(16.0 * N * n * n + 8.0 * n**3 for n in ns),
+        "green": sum(12.0 * N * n * n + 8.0 * n**3 for n in ns),
         "taylor": sum((8.0 + 8.0 * K) * N * N * n for n in ns),
-        "overlap": sum(4.0 * N * n * n + (8.0 / 3.0) * n**3 for n in ns),
+        "overlap": sum(12.0 * N * n * n + 8.0 * n**3 for n in ns),
         "energy": sum(4.0 * G * N * n * n + 8.0 * G * n * n + 12.0 * N * n * n + 8.0 * n**3
                       for n in ns),
         "qr": sum(16.0 * N * n * n for n in ns),
+        "fields": 40.0 * G,  # shifts, shifted fields and the two field sums, complex
+        "weights": 1.0,      # the population-control sum
     }
     return f
+
+
+def bytes_per_walker(norb, nelec, nchol, walker_type, precision, W):
+    """Algorithmic HBM bytes per walker per phase: each kernel's inputs read once and
+    outputs written once in the layouts the library keeps (complex128 walkers; float64
+    or split-bf16 Green's function / shifted fields / Cholesky planes; float64 or float32
+    VHS), with operands shared by all walkers (Cholesky vectors, trial, one-body factor)
+    counted once per launch and divided by W."""
+    N, G = norb, nchol
+    ns = [nelec[0]] if walker_type == "rhf" else [nelec[0], nelec[1]]
+    nsum, nspin = sum(ns), len(ns)
+    mixed = precision != "fp64"
+    Np32, Gp = -(-N // 32) * 32, -(-G // 32) * 32
+    walker = 16.0 * N * nsum
+    green = 8.0 * nsum * Np32 if mixed else 16.0 * nsum * N     # hi/lo bf16 or float64
+    vhs = 8.0 * N * N if mixed else 16.0 * N * N                 # float32 or float64 re/im
+    xs = 8.0 * Gp if mixed else 16.0 * G
+    chol = (4.0 * N * N * Gp if mixed else 8.0 * N * N * G) / W
+    rot = (4.0 * G * nsum * Np32 if mixed else 8.0 * G * nsum * N) / W
+    one_body = nspin * (4.0 * N * Np32 if mixed else 8.0 * N * N) / W
+    trial = 8.0 * N * nsum / W
+    tail = 32.0 + 32.0 + 16.0  # terms read, overlap and weight read + written
+    return {
+        "vhs_gemm": xs + chol + vhs,
+        "fb_gemm": green + rot + 16.0 * G,
+        "fields": 16.0 * G + 8.0 * G + xs + 32.0,
+        "taylor": 2.0 * walker + vhs + one_body,
+        "green": walker + trial + green + 16.0 * nspin,
+        "overlap": walker + trial + green + 16.0 * nspin + tail,
+        "weights": 8.0,
+        "qr": 2.0 * walker,
+        "energy": walker + trial + green + rot,
+    }
+
+
+def roofline_of(phase, precision, norb, nelec, nchol, n_exp_terms, walker_type, W, avg_s):
+    """Roofline record of one phase's kernel: the bound is whichever of MFMA time and
+    HBM time at published peaks is longer for the algorithmic flops and bytes of one
+    launch; achieved = those flops (or bytes) / the measured average launch time."""
+    fl = flops_per_walker(norb, nelec, nchol, n_exp_terms, walker_type).get(phase)
+    by = bytes_per_walker(norb, nelec, nchol, walker_type, precision, W).get(phase)
+    if fl is None or by is None:
+        return None
+    flop, byt = fl * W, by * W
+    peak_tf = phase_peak(precision, phase, norb, nelec)
+    mfma_bound = flop / (peak_tf * 1e12) >= byt / (HBM_PEAK_GBS * 1e9)
+    if mfma_bound:
+        achieved, peak, unit = flop / avg_s / 1e12, peak_tf, "TFLOP/s"
+    else:
+        achieved, peak, unit = byt / avg_s / 1e9, HBM_PEAK_GBS, "GB/s"
+    return {
+        "phase": phase,
+        "kernel": phase_kernel(precision, phase, norb, nelec),
+        "bound": "mfma" if mfma_bound else "hbm",
+        "achieved": round(achieved, 3),
+        "peak": peak,
+        "unit": unit,
+        "frac": round(achieved / peak, 4),
+        "flop_per_launch": flop,
+        "bytes_per_launch": byt,
+        "avg_launch_ms": round(avg_s * 1e3, 5),
+    }
 
 
 def phase_peak(precision, phase, norb, nelec):
     """Published peak of the arithmetic a phase runs on."""
     if precision == "fp64":
         return PEAK_TFLOPS["fp64"]
-    if phase in ("vhs_gemm", "fb_gemm"):
+    if phase in ("vhs_gemm", "fb_gemm", "energy"):  # split-bf16 contractions
         return PEAK_TFLOPS["bf16"]
     if phase == "taylor":  # split-bf16 register kernel, or the float32 streaming kernel
         reg = norb <= 128 and max(nelec) <= 32
@@ -274,8 +344,19 @@ def run(args, precision):
     if args.copy_probe:
         copy_probe()
 
+    # phase breakdown: a short pass with an event pair around every phase (the pairs
+    # cost device time, so the timed region below records only the dominant phase)
+    fl = flops_per_walker(norb, nelec, nchol, prop.n_exp_terms, walker_type)
     dev.prof_collect()  # drop anything recorded before
     dev.prof_enable(True)
+    for _ in range(min(args.steps, PROFILE_STEPS)):
+        one_step()
+    torch.cuda.synchronize()
+    dev.prof_enable(False)
+    phases = dev.prof_collect()
+    dom = max((p for p, (ms, n) in phases.items() if n and p in fl), key=lambda p: phases[p][0])
+
+    dev.prof_enable(True, phases=[dom])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -283,7 +364,8 @@ def run(args, precision):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     dev.prof_enable(False)
-    phases = dev.prof_collect()
+    timed = dev.prof_collect()
+    dev.prof_enable(False, phases=None)
     print(f"# {precision}: timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr,
           flush=True)
 
@@ -292,29 +374,23 @@ def run(args, precision):
     e_mixed = float(np.sum(st[:, 0] * st[:, 1]) / np.sum(st[:, 1])) if nb else float("nan")
 
     value = W * args.steps / elapsed
-    fl = flops_per_walker(norb, nelec, nchol, prop.n_exp_terms, walker_type)
     per_phase = {p: {"ms_total": round(ms, 4), "launches": n,
                      "ms_avg": round(ms / n, 5) if n else None}
                  for p, (ms, n) in phases.items() if n}
-    # dominant kernel = phase with the largest device time in the timed region
-    dom = max((p for p in per_phase if p in fl), key=lambda p: phases[p][0])
-    avg_s = phases[dom][0] / phases[dom][1] / 1e3
-    flop_launch = fl[dom] * W
-    achieved = flop_launch / avg_s / 1e12
-    peak = phase_peak(precision, dom, norb, nelec)
-    kernel = phase_kernel(precision, dom, norb, nelec)
-    roofline = {
-        "phase": dom,
-        "kernel": kernel,
-        "bound": "mfma",
-        "achieved": round(achieved, 3),
-        "peak": peak,
-        "unit": "TFLOP/s",
-        "frac": round(achieved / peak, 4),
-        "traffic": traffic_from_profiles(kernel, args.workload, precision),
-        "flop_per_launch": flop_launch,
-        "avg_launch_ms": round(avg_s * 1e3, 5),
-    }
+    # every phase's roofline from the profiled pass; the dominant kernel's from its
+    # events over the timed region
+    rooflines = {}
+    for p, (ms, n) in phases.items():
+        if n:
+            r = roofline_of(p, precision, norb, nelec, nchol, prop.n_exp_terms, walker_type, W,
+                            ms / n / 1e3)
+            if r is not None:
+                rooflines[p] = r
+    ms_dom, n_dom = timed[dom]
+    roofline = roofline_of(dom, precision, norb, nelec, nchol, prop.n_exp_terms, walker_type,
+                           W, ms_dom / n_dom / 1e3)
+    roofline["launches_timed"] = n_dom
+    roofline["traffic"] = traffic_from_profiles(roofline["kernel"], args.workload, precision)
     step_flops = sum(v for k, v in fl.items() if k not in ("energy", "qr")) + (
         fl["energy"] + fl["qr"]) / N_PROP_STEPS
     res = {
@@ -325,6 +401,8 @@ def run(args, precision):
         "mixed_energy": e_mixed,
         "phases": per_phase,
         "roofline": roofline,
+        "rooflines": {p: {k: r[k] for k in ("kernel", "bound", "achieved", "unit", "frac")}
+                      for p, r in rooflines.items()},
         "achieved_tflops_step": round(step_flops * value / 1e12, 3),
         "config": {
             "workload": args.workload,
@@ -589,6 +667,7 @@ def main():
     out["achieved_tflops_step"] = r["achieved_tflops_step"]
     out["phases"] = r["phases"]
     out["roofline"] = r["roofline"]
+    out["rooflines"] = r["rooflines"]
     out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 

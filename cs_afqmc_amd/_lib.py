@@ -31,6 +31,7 @@ EXPORTS = (
     "afqmc_block_stats",
     "afqmc_prof_enable",
     "afqmc_prof_collect",
+    "afqmc_prof_phases",
     "afqmc_fields_normal",
     "afqmc_cs_outliers",
     "afqmc_group_create",
@@ -105,6 +106,7 @@ def _declare(lib):
     lib.afqmc_block_stats.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp]
     lib.afqmc_prof_enable.argtypes = [_vp, _i32]
     lib.afqmc_prof_collect.argtypes = [_vp, _vp, _vp]
+    lib.afqmc_prof_phases.argtypes = [_vp, ctypes.c_uint32]
     lib.afqmc_fields_normal.argtypes = [
         _vp, ctypes.c_uint64, ctypes.c_uint64, _vp, ctypes.c_int64]
     lib.afqmc_cs_outliers.argtypes = [_vp, _i32, _vp, _vp, ctypes.c_double]

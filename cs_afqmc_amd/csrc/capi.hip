@@ -402,6 +402,12 @@ extern "C" int afqmc_prof_enable(afqmc_ctx* c, int32_t on) {
   return AFQMC_OK;
 }
 
+extern "C" int afqmc_prof_phases(afqmc_ctx* c, uint32_t mask) {
+  if (!c) return afqmc_set_error(AFQMC_ERR_ARG, "null context");
+  c->prof_mask = mask;
+  return AFQMC_OK;
+}
+
 extern "C" int afqmc_prof_collect(afqmc_ctx* c, double* ms, int64_t* counts) {
   if (!c || !ms || !counts) return afqmc_set_error(AFQMC_ERR_ARG, "null argument");
   for (int i = 0; i < c->prof_n; ++i) {

@@ -178,6 +178,7 @@ struct afqmc_ctx {
 
   // phase timing (afqmc_prof_*)
   int prof_on;
+  uint32_t prof_mask;         // phases that record (0: all)
   int prof_n;                 // recorded event pairs
   int prof_cap;
   hipEvent_t* prof_ev;        // 2 * prof_cap events
@@ -189,7 +190,8 @@ struct PhaseTimer {
   afqmc_ctx* c;
   int slot;
   PhaseTimer(afqmc_ctx* ctx, int phase) : c(ctx), slot(-1) {
-    if (c->prof_on && c->prof_n < c->prof_cap) {
+    if (c->prof_on && c->prof_n < c->prof_cap &&
+        (c->prof_mask == 0 || ((c->prof_mask >> phase) & 1u))) {
       slot = c->prof_n++;
       c->prof_phase[slot] = phase;
       (void)hipEventRecord(c->prof_ev[2 * slot], c->stream);

@@ -17,6 +17,8 @@
 // The second product's rows are exactly the rows of the split Green layout
 // green[w][c][i][p] (c = re / im), so results go straight to global memory.
 // [O | I] lives in LDS (n x 2n complex).
+#include <algorithm>
+
 #include "common.h"
 
 extern "C" __device__ unsigned long long __ockl_wfred_max_u64(unsigned long long);
@@ -109,25 +111,246 @@ constexpr int meas_ti(int t2) { return t2 > 4 ? 1 : (t2 + 1) / 2; }
 constexpr size_t meas_red_bytes(int t2) {
   return meas_waves(t2) > 1 ? (size_t)meas_ti(t2) * t2 * 4 * 64 * sizeof(double) : 0;
 }
+// walkers up to this size are staged in LDS by the two-wave kernel (four blocks per CU)
+constexpr size_t MEAS_STAGE_MAX = 40 * 1024;
 
-// T2 = ceil(2n / 16): column tiles of [Or | Oi] and row tiles of [Gr ; Gi]
+// [Or | Oi] = C^T [wr | wi] for the row tiles [tb, tb + TI) of O, into M (rows i, columns
+// c < 2n, K = N).  All TI x T2 accumulators are live, so each B value feeds TI MFMAs; the
+// operands of k-group chunk j + NW are loaded while chunk j runs.  With NW = 2 the waves
+// take alternate chunks, wave 1's partial sums reach wave 0 through red (summed in the
+// order wave 0 + wave 1) and wave 0 writes M; `aliased` adds the barrier that keeps red's
+// writes after both waves' last read of a staged walker sharing that LDS.
+template <int T2, int TI, int NW>
+__device__ __forceinline__ void overlap_tiles(int N, int n, int tb, const double* __restrict__ trial,
+                                              const double* __restrict__ wsrc, cplx* M,
+                                              double* red, bool aliased, int lane, int wv) {
+  const int n2 = 2 * n;
+  const int s = lane >> 4, col = lane & 15;
+  const int nkg = (N + 3) >> 2;
+  int boff[T2];
+#pragma unroll
+  for (int tc = 0; tc < T2; ++tc) {
+    const int c = tc * 16 + col;
+    boff[tc] = c < n2 ? 2 * (c < n ? c : c - n) + (c < n ? 0 : 1) : -1;
+  }
+  double4_t acc[TI][T2];
+#pragma unroll
+  for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+    for (int tc = 0; tc < T2; ++tc) acc[ti][tc] = double4_t{0.0, 0.0, 0.0, 0.0};
+  // two operand buffers with compile-time names (a runtime buffer index would put them
+  // in scratch memory)
+  double a0[MEAS_KC][TI], b0[MEAS_KC][T2], a1[MEAS_KC][TI], b1[MEAS_KC][T2];
+  auto load = [&](double (&a)[MEAS_KC][TI], double (&b)[MEAS_KC][T2], int kg0) {
+#pragma unroll
+    for (int u = 0; u < MEAS_KC; ++u) {
+      const int p = 4 * (kg0 + u) + s;
+      const bool pok = p < N;
+#pragma unroll
+      for (int ti = 0; ti < TI; ++ti) {
+        const int i = (tb + ti) * 16 + col;
+        a[u][ti] = (pok && i < n) ? trial[p * n + i] : 0.0;
+      }
+#pragma unroll
+      for (int tc = 0; tc < T2; ++tc)
+        b[u][tc] = (pok && boff[tc] >= 0) ? wsrc[(int64_t)2 * p * n + boff[tc]] : 0.0;
+    }
+  };
+  auto mma = [&](const double (&a)[MEAS_KC][TI], const double (&b)[MEAS_KC][T2]) {
+#pragma unroll
+    for (int u = 0; u < MEAS_KC; ++u)
+#pragma unroll
+      for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+        for (int tc = 0; tc < T2; ++tc)
+          acc[ti][tc] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][ti], b[u][tc], acc[ti][tc], 0, 0, 0);
+  };
+  const int nch = (nkg + MEAS_KC - 1) / MEAS_KC;
+  if (wv < nch) load(a0, b0, wv * MEAS_KC);
+  for (int ch = wv; ch < nch; ch += 2 * NW) {
+    if (ch + NW < nch) load(a1, b1, (ch + NW) * MEAS_KC);
+    mma(a0, b0);
+    if (ch + 2 * NW < nch) load(a0, b0, (ch + 2 * NW) * MEAS_KC);
+    if (ch + NW < nch) mma(a1, b1);
+  }
+  if constexpr (NW > 1) {
+    if (aliased) __syncthreads();
+    if (wv == 1) {
+#pragma unroll
+      for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+        for (int tc = 0; tc < T2; ++tc)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) red[((ti * T2 + tc) * 4 + r) * 64 + lane] = acc[ti][tc][r];
+    }
+    __syncthreads();
+    if (wv != 0) return;
+#pragma unroll
+    for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+      for (int tc = 0; tc < T2; ++tc)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[ti][tc][r] += red[((ti * T2 + tc) * 4 + r) * 64 + lane];
+  }
+#pragma unroll
+  for (int tc = 0; tc < T2; ++tc) {
+    if (boff[tc] < 0) continue;
+    const int c = tc * 16 + col;
+    const int cj = c < n ? c : c - n, ci = c < n ? 0 : 1;
+#pragma unroll
+    for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = (tb + ti) * 16 + s + 4 * r;
+        if (row < n) reinterpret_cast<double*>(&M[row * n2 + cj])[ci] = acc[ti][tc][r];
+      }
+  }
+}
+
+// thread 0: det of this spin and, on the last spin's launch, the walker's weight
+__device__ inline void finish_walker(cplx det, cplx* dets, int w, int spin,
+                                     const WeightsArgs& tail) {
+  dets[(int64_t)w * 2 + spin] = det;
+  if (tail.weights) {
+    const cplx onew = tail.nspin == 1 ? cmul(det, det) : cmul(dets[(int64_t)w * 2], det);
+    walker_weight_update(tail, tail.scalars[1], w, onew);
+  }
+}
+
+// n <= 24 (T2 <= 3): two waves per walker.  A walker of at most MEAS_STAGE_MAX bytes is
+// staged in LDS first (one round of loads), and the region is then reused for [O | I]
+// and the overlap partials.  Wave 0 runs the register Gauss-Jordan while each wave's
+// first Green tile of the walker operand loads; the Green product's A operand (from X) is
+// then read into registers once and the walker operand runs two tiles ahead.
+template <int T2, bool STAGE>
+__device__ __forceinline__ void measure_narrow_body(
+    int N, int n, const cplx* __restrict__ walkers, const double* __restrict__ trial,
+    cplx* __restrict__ dets, int spin, double* __restrict__ green, uint16_t* __restrict__ ghi,
+    uint16_t* __restrict__ glo, int ld16, const int w, const WeightsArgs& tail) {
+  constexpr int NW = 2, TI = meas_ti(T2);
+  constexpr int NKG = 4 * T2;  // k groups of 2n <= 16 T2
+  extern __shared__ double smem[];
+  const int n2 = 2 * n;
+  cplx* M = reinterpret_cast<cplx*>(smem);              // n x 2n augmented matrix
+  double* red = reinterpret_cast<double*>(M + n * n2);  // wave 1's overlap partials
+  // wave index as a scalar: the waves' paths below are real branches, so registers that
+  // live on one wave's path only can share storage with the other's
+  const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int s = lane >> 4, col = lane & 15;
+  const cplx* phi = walkers + (int64_t)w * N * n;
+  const double* phid = reinterpret_cast<const double*>(phi);
+
+  if constexpr (STAGE) {
+    cplx* ph = reinterpret_cast<cplx*>(smem);
+    for (int e = threadIdx.x; e < N * n; e += 64 * NW) ph[e] = phi[e];
+    __syncthreads();
+    overlap_tiles<T2, TI, NW>(N, n, 0, trial, reinterpret_cast<const double*>(ph), M, red,
+                              true, lane, wv);
+  } else {
+    overlap_tiles<T2, TI, NW>(N, n, 0, trial, phid, M, red, false, lane, wv);
+  }
+  for (int e = threadIdx.x; e < n * n; e += 64 * NW) {
+    const int i = e / n, j = e - (e / n) * n;
+    M[i * n2 + n + j] = cplx{i == j ? 1.0 : 0.0, 0.0};
+  }
+  __syncthreads();
+
+  // ---- Gauss-Jordan with partial pivoting on wave 0: [O | I] -> [* | X], det from
+  // pivots; then [Gr ; Gi] = [[Xr^T, -Xi^T], [Xi^T, Xr^T]] [wr^T ; wi^T], K = 2n,
+  // columns p in tiles of 16, tiles split between the waves.
+  const bool want_green = green || ghi;
+  const int tp_n = (N + 15) >> 4;
+  // B[k][p] = re / im of w[p][k mod n]: 32-bit offsets within a walker row (tile
+  // invariant), one row base per tile, clamped loads masked after
+  int boff[NKG];
+#pragma unroll
+  for (int kg = 0; kg < NKG; ++kg) {
+    const int k = 4 * kg + s;
+    boff[kg] = k < n2 ? 2 * (k < n ? k : k - n) + (k < n ? 0 : 1) : -1;
+  }
+  auto load_b = [&](double (&b)[NKG], int tp) {
+    const int p = tp * 16 + col;
+    const double* rowp = phid + (int64_t)2 * (p < N ? p : 0) * n;
+#pragma unroll
+    for (int kg = 0; kg < NKG; ++kg) {
+      const double v = rowp[boff[kg] < 0 ? 0 : boff[kg]];
+      b[kg] = (boff[kg] >= 0 && p < N) ? v : 0.0;
+    }
+  };
+  // wave 1 loads its first tile while wave 0 eliminates
+  double b[NKG];
+  cplx det = cplx{1.0, 0.0};
+  if (wv == 0)
+    det = gauss_jordan_registers<T2>(M, n, lane);
+  else if (want_green && wv < tp_n)
+    load_b(b, wv);
+  __syncthreads();
+  if (threadIdx.x == 0) finish_walker(det, dets, w, spin, tail);
+  if (!want_green) return;
+  if (wv == 0) load_b(b, 0);
+
+  // A operand of every tile, from X in LDS, read once
+  double a[NKG][T2];
+#pragma unroll
+  for (int kg = 0; kg < NKG; ++kg)
+#pragma unroll
+    for (int tr = 0; tr < T2; ++tr) {
+      const int k = 4 * kg + s;
+      const int r = tr * 16 + col;
+      double v = 0.0;
+      if (k < n2 && r < n2) {
+        const int kj = k < n ? k : k - n;
+        const double* x = reinterpret_cast<const double*>(&M[kj * n2 + n + (r < n ? r : r - n)]);
+        // X[kj][ri]: (r < n, k < n): Xr; (r < n, k >= n): -Xi; (r >= n, k < n): Xi; else Xr
+        v = (r < n) == (k < n) ? x[0] : ((r < n) ? -x[1] : x[1]);
+      }
+      a[kg][tr] = v;
+    }
+  double* gw = green ? green + (int64_t)w * n2 * N : nullptr;
+  uint16_t* gh = ghi ? ghi + (int64_t)w * n2 * ld16 : nullptr;
+  uint16_t* gl = ghi ? glo + (int64_t)w * n2 * ld16 : nullptr;
+  // tile tp; the walker operand of tile tp + NW loads into b once the MFMAs have read it
+  auto tile = [&](int tp) {
+    double4_t acc[T2];
+#pragma unroll
+    for (int tr = 0; tr < T2; ++tr) acc[tr] = double4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kg = 0; kg < NKG; ++kg)
+#pragma unroll
+      for (int tr = 0; tr < T2; ++tr)
+        acc[tr] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kg][tr], b[kg], acc[tr], 0, 0, 0);
+    if (tp + NW < tp_n) load_b(b, tp + NW);
+    // row tr * 16 + 4 q + s of column p: one base pointer per lane, wave-uniform offsets
+    const int p = tp * 16 + col;
+    if (p >= N) return;
+    double* gwp = gw ? gw + (int64_t)s * N + p : nullptr;
+    uint16_t* ghp = gh ? gh + (int64_t)s * ld16 + p : nullptr;
+    uint16_t* glp = gh ? gl + (int64_t)s * ld16 + p : nullptr;
+#pragma unroll
+    for (int tr = 0; tr < T2; ++tr)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r0 = tr * 16 + 4 * q;
+        if (r0 + s < n2) {
+          if (gwp) gwp[(int64_t)r0 * N] = acc[tr][q];
+          if (ghp) split_bf16(acc[tr][q], &ghp[(int64_t)r0 * ld16], &glp[(int64_t)r0 * ld16]);
+        }
+      }
+  };
+  for (int tp = wv; tp < tp_n; tp += NW) tile(tp);
+}
+
+// n > 24: one wave per walker; the elimination runs in LDS.
 template <int T2, int TI, bool STAGE>
-__device__ __forceinline__ void measure_body(int N, int n, const cplx* __restrict__ walkers,
-                                             const double* __restrict__ trial,
-                                             cplx* __restrict__ dets, int spin,
-                                             double* __restrict__ green,
-                                             uint16_t* __restrict__ ghi,
-                                             uint16_t* __restrict__ glo, int ld16, const int w,
-                                             const WeightsArgs& tail) {
-  constexpr int NW = meas_waves(T2);
+__device__ __forceinline__ void measure_wide_body(
+    int N, int n, const cplx* __restrict__ walkers, const double* __restrict__ trial,
+    cplx* __restrict__ dets, int spin, double* __restrict__ green, uint16_t* __restrict__ ghi,
+    uint16_t* __restrict__ glo, int ld16, const int w, const WeightsArgs& tail) {
   extern __shared__ double smem[];
   const int n2 = 2 * n;
   cplx* M = reinterpret_cast<cplx*>(smem);  // n x 2n augmented matrix
   cplx* fcol = M + n * n2;                  // pivot column copy (n)
-  // overlap partials of wave 1 ([TI][T2][4][64] doubles), after the staged walker
-  double* red = reinterpret_cast<double*>(fcol + n + (STAGE ? N * n : 0));
-
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x;
   const int s = lane >> 4, col = lane & 15;
   const cplx* phi = walkers + (int64_t)w * N * n;
   // walker operand: staged once into LDS with 16-byte loads (all in flight together)
@@ -135,184 +358,21 @@ __device__ __forceinline__ void measure_body(int N, int n, const cplx* __restric
   const double* phid = reinterpret_cast<const double*>(phi);
   if (STAGE) {
     cplx* ph = fcol + n;
-    for (int e = threadIdx.x; e < N * n; e += 64 * NW) ph[e] = phi[e];
+    for (int e = lane; e < N * n; e += 64) ph[e] = phi[e];
     phid = reinterpret_cast<const double*>(ph);
   }
   __syncthreads();
-
-  // ---- [Or | Oi] = C^T [wr | wi]: rows i (TI tiles of 16), columns c < 2n, K = N.
-  // All TI x T2 accumulators are live, so each B value feeds TI MFMAs; the operands of
-  // k-group chunk j + 1 are loaded while chunk j runs.
-  {
-    const int nkg = (N + 3) >> 2;
-    int boff[T2];
-#pragma unroll
-    for (int tc = 0; tc < T2; ++tc) {
-      const int c = tc * 16 + col;
-      boff[tc] = c < n2 ? 2 * (c < n ? c : c - n) + (c < n ? 0 : 1) : -1;
-    }
-    // row tiles in passes of TI (all of them when the accumulators fit in registers)
-    const int ti_total = (n + 15) >> 4;
-    for (int tb = 0; tb < ti_total; tb += TI) {
-    double4_t acc[TI][T2];
-#pragma unroll
-    for (int ti = 0; ti < TI; ++ti)
-#pragma unroll
-      for (int tc = 0; tc < T2; ++tc) acc[ti][tc] = double4_t{0.0, 0.0, 0.0, 0.0};
-    // two operand buffers with compile-time names (a runtime buffer index would put
-    // them in scratch memory)
-    double a0[MEAS_KC][TI], b0[MEAS_KC][T2], a1[MEAS_KC][TI], b1[MEAS_KC][T2];
-    auto load = [&](double (&a)[MEAS_KC][TI], double (&b)[MEAS_KC][T2], int kg0) {
-#pragma unroll
-      for (int u = 0; u < MEAS_KC; ++u) {
-        const int p = 4 * (kg0 + u) + s;
-        const bool pok = p < N;
-#pragma unroll
-        for (int ti = 0; ti < TI; ++ti) {
-          const int i = (tb + ti) * 16 + col;
-          a[u][ti] = (pok && i < n) ? trial[p * n + i] : 0.0;
-        }
-#pragma unroll
-        for (int tc = 0; tc < T2; ++tc)
-          b[u][tc] = (pok && boff[tc] >= 0) ? phid[(int64_t)2 * p * n + boff[tc]] : 0.0;
-      }
-    };
-    auto mma = [&](const double (&a)[MEAS_KC][TI], const double (&b)[MEAS_KC][T2]) {
-#pragma unroll
-      for (int u = 0; u < MEAS_KC; ++u)
-#pragma unroll
-        for (int ti = 0; ti < TI; ++ti)
-#pragma unroll
-          for (int tc = 0; tc < T2; ++tc)
-            acc[ti][tc] =
-                __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][ti], b[u][tc], acc[ti][tc], 0, 0, 0);
-    };
-    // wave wv takes chunks wv, wv + NW, wv + 2 NW, ...
-    const int nch = (nkg + MEAS_KC - 1) / MEAS_KC;
-    if (wv < nch) load(a0, b0, wv * MEAS_KC);
-    for (int ch = wv; ch < nch; ch += 2 * NW) {
-      if (ch + NW < nch) load(a1, b1, (ch + NW) * MEAS_KC);
-      mma(a0, b0);
-      if (ch + 2 * NW < nch) load(a0, b0, (ch + 2 * NW) * MEAS_KC);
-      if (ch + NW < nch) mma(a1, b1);
-    }
-    if constexpr (NW > 1) {
-      // wave 0's partial + wave 1's partial, in that order
-      if (wv == 1) {
-#pragma unroll
-        for (int ti = 0; ti < TI; ++ti)
-#pragma unroll
-          for (int tc = 0; tc < T2; ++tc)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) red[((ti * T2 + tc) * 4 + r) * 64 + lane] = acc[ti][tc][r];
-      }
-      __syncthreads();
-      if (wv == 0) {
-#pragma unroll
-        for (int ti = 0; ti < TI; ++ti)
-#pragma unroll
-          for (int tc = 0; tc < T2; ++tc)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) acc[ti][tc][r] += red[((ti * T2 + tc) * 4 + r) * 64 + lane];
-      }
-    }
-    if (wv == 0) {
-#pragma unroll
-    for (int tc = 0; tc < T2; ++tc) {
-      if (boff[tc] < 0) continue;
-      const int c = tc * 16 + col;
-      const int cj = c < n ? c : c - n, ci = c < n ? 0 : 1;
-#pragma unroll
-      for (int ti = 0; ti < TI; ++ti)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = (tb + ti) * 16 + s + 4 * r;
-          if (row < n) reinterpret_cast<double*>(&M[row * n2 + cj])[ci] = acc[ti][tc][r];
-        }
-    }
-    }
-    if constexpr (NW > 1) __syncthreads();  // red is reused by the next row-tile pass
-    }
-    for (int e = threadIdx.x; e < n * n; e += 64 * NW) {
-      const int i = e / n, j = e - (e / n) * n;
-      M[i * n2 + n + j] = cplx{i == j ? 1.0 : 0.0, 0.0};
-    }
+  // row tiles of O in passes of TI
+  const int ti_total = (n + 15) >> 4;
+  for (int tb = 0; tb < ti_total; tb += TI)
+    overlap_tiles<T2, TI, 1>(N, n, tb, trial, phid, M, nullptr, false, lane, 0);
+  for (int e = lane; e < n * n; e += 64) {
+    const int i = e / n, j = e - (e / n) * n;
+    M[i * n2 + n + j] = cplx{i == j ? 1.0 : 0.0, 0.0};
   }
   __syncthreads();
 
-  // ---- Gauss-Jordan with partial pivoting: [O | I] -> [* | X], det from pivots;
-  // then [Gr ; Gi] = [[Xr^T, -Xi^T], [Xi^T, Xr^T]] [wr^T ; wi^T], K = 2n, columns p.
-  const bool want_green = green || ghi;
-  double* gw = green ? green + (int64_t)w * n2 * N : nullptr;
-  uint16_t* gh = ghi ? ghi + (int64_t)w * n2 * ld16 : nullptr;
-  uint16_t* gl = ghi ? glo + (int64_t)w * n2 * ld16 : nullptr;
-  const int tp_n = (N + 15) >> 4;
-  // thread 0: det of this spin and, on the last spin's launch, the walker's weight
-  auto finish_walker = [&](cplx det) {
-    dets[(int64_t)w * 2 + spin] = det;
-    if (tail.weights) {
-      const cplx onew = tail.nspin == 1 ? cmul(det, det) : cmul(dets[(int64_t)w * 2], det);
-      walker_weight_update(tail, tail.scalars[1], w, onew);
-    }
-  };
-  if constexpr (T2 <= 3) {
-    // n <= 24: the column fits the registers without spilling.  All k groups of a
-    // tile's walker operand are loaded at once (one round of global loads per tile) and
-    // each wave's first tile is in flight during the elimination; A comes from X in LDS.
-    constexpr int NKG = 4 * T2;  // k groups of 2n <= 16 T2
-    auto a_of = [&](int kg, int tr) {
-      const int k = 4 * kg + s;
-      const int r = tr * 16 + col;
-      double v = 0.0;
-      if (k < n2 && r < n2) {
-        const int kj = k < n ? k : k - n;
-        const cplx x = M[kj * n2 + n + (r < n ? r : r - n)];  // X[kj][ri]
-        v = (r < n) == (k < n) ? x.re : ((r < n) ? -x.im : x.im);
-      }
-      return v;
-    };
-    auto load_b = [&](double (&b)[NKG], int tp) {
-      const int p = tp * 16 + col;
-#pragma unroll
-      for (int kg = 0; kg < NKG; ++kg) {
-        const int k = 4 * kg + s;
-        const int kj = k < n ? k : k - n;
-        b[kg] = (k < n2 && p < N) ? phid[2 * ((int64_t)p * n + kj) + (k < n ? 0 : 1)] : 0.0;
-      }
-    };
-    auto tile = [&](const double (&b)[NKG], int tp) {
-      double4_t acc[T2];
-#pragma unroll
-      for (int tr = 0; tr < T2; ++tr) acc[tr] = double4_t{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int kg = 0; kg < NKG; ++kg)
-#pragma unroll
-        for (int tr = 0; tr < T2; ++tr)
-          acc[tr] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_of(kg, tr), b[kg], acc[tr], 0, 0, 0);
-      const int p = tp * 16 + col;
-#pragma unroll
-      for (int tr = 0; tr < T2; ++tr)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int row = tr * 16 + s + 4 * q;
-          if (row < n2 && p < N) {
-            if (gw) gw[(int64_t)row * N + p] = acc[tr][q];
-            if (gh) split_bf16(acc[tr][q], &gh[(int64_t)row * ld16 + p], &gl[(int64_t)row * ld16 + p]);
-          }
-        }
-    };
-    double b[NKG];
-    if (want_green && wv < tp_n) load_b(b, wv);
-    cplx det = cplx{1.0, 0.0};
-    if (wv == 0) det = gauss_jordan_registers<T2>(M, n, lane);
-    __syncthreads();
-    if (threadIdx.x == 0) finish_walker(det);
-    if (!want_green) return;
-    for (int tp = wv; tp < tp_n; tp += NW) {
-      if (tp != wv) load_b(b, tp);
-      tile(b, tp);
-    }
-  } else {
+  // ---- Gauss-Jordan with partial pivoting: [O | I] -> [* | X], det from pivots.
   cplx det = cplx{1.0, 0.0};
   // Wider systems in LDS: lane c' owns column k + 1 + c' of the trailing matrix
   // (2n - k - 1 <= 127 columns, at most two per lane).
@@ -376,9 +436,14 @@ __device__ __forceinline__ void measure_body(int N, int n, const cplx* __restric
     }
     __syncthreads();
   }
-  if (lane == 0) finish_walker(det);
-  if (!want_green) return;
-  {
+  if (lane == 0) finish_walker(det, dets, w, spin, tail);
+  if (!(green || ghi)) return;
+
+  // ---- [Gr ; Gi] = [[Xr^T, -Xi^T], [Xi^T, Xr^T]] [wr^T ; wi^T], K = 2n, columns p
+  double* gw = green ? green + (int64_t)w * n2 * N : nullptr;
+  uint16_t* gh = ghi ? ghi + (int64_t)w * n2 * ld16 : nullptr;
+  uint16_t* gl = ghi ? glo + (int64_t)w * n2 * ld16 : nullptr;
+  const int tp_n = (N + 15) >> 4;
     const int nkg = (n2 + 3) >> 2;
     for (int tp = 0; tp < tp_n; ++tp) {
       const int p = tp * 16 + col;  // B column = orbital p
@@ -422,8 +487,6 @@ __device__ __forceinline__ void measure_body(int N, int n, const cplx* __restric
           }
         }
     }
-  }
-  }
 }
 
 template <int T2, int TI, bool STAGE>
@@ -431,8 +494,12 @@ __global__ __launch_bounds__(64 * meas_waves(T2), meas_waves(T2)) void measure_k
     int N, int n, const cplx* __restrict__ walkers, const double* __restrict__ trial,
     cplx* __restrict__ dets, int spin, double* __restrict__ green, uint16_t* __restrict__ ghi,
     uint16_t* __restrict__ glo, int ld16, WeightsArgs tail) {
-  measure_body<T2, TI, STAGE>(N, n, walkers, trial, dets, spin, green, ghi, glo, ld16,
-                              blockIdx.x, tail);
+  if constexpr (T2 <= 3)
+    measure_narrow_body<T2, STAGE>(N, n, walkers, trial, dets, spin, green, ghi, glo, ld16,
+                                   blockIdx.x, tail);
+  else
+    measure_wide_body<T2, TI, STAGE>(N, n, walkers, trial, dets, spin, green, ghi, glo, ld16,
+                                     blockIdx.x, tail);
 }
 
 template <int T2, int TI, bool STAGE>
@@ -441,8 +508,13 @@ __global__ __launch_bounds__(64 * meas_waves(T2), meas_waves(T2)) void measure_g
                                                              int nprob) {
   const int p = group_of(start, nprob, blockIdx.x);
   const MeasureArgs a = args[p];
-  measure_body<T2, TI, STAGE>(a.N, a.n, a.walkers, a.trial, a.dets, a.spin, a.green, a.ghi,
-                              a.glo, a.ld16, blockIdx.x - start[p], WeightsArgs{});
+  const WeightsArgs none{};
+  if constexpr (T2 <= 3)
+    measure_narrow_body<T2, STAGE>(a.N, a.n, a.walkers, a.trial, a.dets, a.spin, a.green,
+                                   a.ghi, a.glo, a.ld16, blockIdx.x - start[p], none);
+  else
+    measure_wide_body<T2, TI, STAGE>(a.N, a.n, a.walkers, a.trial, a.dets, a.spin, a.green,
+                                     a.ghi, a.glo, a.ld16, blockIdx.x - start[p], none);
 }
 
 template <int T2>
@@ -540,8 +612,14 @@ void measure_plan(const afqmc_ctx* c, int s, int* t2, bool* stage, size_t* shm) 
   const int n = c->nocc[s], N = c->d.norb;
   const int t = (2 * n + 15) / 16;
   *t2 = t > 8 ? 8 : t;
+  const size_t walker = (size_t)N * n * sizeof(cplx);
+  if (meas_waves(*t2) > 1) {  // two-wave kernel: [O | I] and partials alias the staged walker
+    const size_t m_red = (size_t)n * 2 * n * sizeof(cplx) + meas_red_bytes(*t2);
+    *stage = walker <= MEAS_STAGE_MAX;
+    *shm = *stage ? std::max(walker, m_red) : m_red;
+    return;
+  }
   const size_t base = (size_t)(n * 2 * n + n) * sizeof(cplx);
-  const size_t staged = base + (size_t)N * n * sizeof(cplx);
-  *stage = staged <= 24 * 1024;
-  *shm = (*stage ? staged : base) + meas_red_bytes(*t2);
+  *stage = base + walker <= 24 * 1024;  // small walkers only: staging costs occupancy
+  *shm = *stage ? base + walker : base;
 }

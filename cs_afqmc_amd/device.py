@@ -222,7 +222,12 @@ class DeviceStep:
                                                float(z_thresh)))
 
     # ------------------------------------------------------------------ phase timing
-    def prof_enable(self, on=True):
+    def prof_enable(self, on=True, phases=None):
+        """Record event pairs around phases; `phases` (names) restricts them, None = all."""
+        mask = 0
+        for p in phases or ():
+            mask |= 1 << _lib.PHASES.index(p)
+        _lib.check(self._lib.afqmc_prof_phases(self._h, mask))
         _lib.check(self._lib.afqmc_prof_enable(self._h, 1 if on else 0))
 
     def prof_collect(self):

@@ -178,6 +178,10 @@ enum {
 };
 AFQMC_API int afqmc_prof_enable(afqmc_ctx* ctx, int32_t on);
 AFQMC_API int afqmc_prof_collect(afqmc_ctx* ctx, double* ms, int64_t* counts);
+/* Restricts the event pairs to the phases whose bit (1 << AFQMC_PH_*) is set in mask;
+ * 0 (the default) records every phase.  Each pair costs a few microseconds of device
+ * time, so a throughput measurement records only the phase it reports on. */
+AFQMC_API int afqmc_prof_phases(afqmc_ctx* ctx, uint32_t mask);
 
 /* Correlated sampling outlier rule, corr_sample.corr_otler_rm (corr_sample.py:551-585):
  * for two populations of n_walkers propagated with the same fields, z_i =

@@ -91,9 +91,10 @@ __device__ __forceinline__ cplx gauss_jordan_registers(cplx* M, int n, int lane)
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
         if (r != k && r < n) {
+          // x -= f m as four fused multiply-adds
           const double fr = readlane_f64(vr[r], k), fi = readlane_f64(vi[r], k);
-          vr[r] -= fr * mr - fi * mi;
-          vi[r] -= fr * mi + fi * mr;
+          vr[r] = fma(-fr, mr, fma(fi, mi, vr[r]));
+          vi[r] = fma(-fr, mi, fma(-fi, mr, vi[r]));
         }
       }
     }

@@ -104,6 +104,26 @@ __device__ inline double walker_weight_update(const WeightsArgs& a, double pcs, 
   return wt;
 }
 
+// ---------------------------------------------------------------- GEMM tile order
+// Block -> (tile_m, tile_n) for a tiles_m x tiles_n grid.  Blocks are dealt round-robin
+// to the 8 XCDs (block b runs on XCD b % 8), each with its own L2, so every XCD gets a
+// contiguous run of tiles.  Within a run the tile index walks the dimension whose
+// operand panels are shared: rows fastest when B (the N side) is the larger operand, so
+// an XCD reads each B panel once; columns fastest when A is larger (m_major = true).
+__device__ inline void xcd_tile(int bx, int tiles_m, int tiles_n, bool m_major, int* tm,
+                                int* tn) {
+  const int nblk = tiles_m * tiles_n;
+  const int q = nblk >> 3, r = nblk & 7, x = bx & 7, i = bx >> 3;
+  const int b = x * q + (x < r ? x : r) + i;
+  if (m_major) {
+    *tm = b / tiles_n;
+    *tn = b - (b / tiles_n) * tiles_n;
+  } else {
+    *tm = b % tiles_m;
+    *tn = b / tiles_m;
+  }
+}
+
 // ---------------------------------------------------------------- reductions
 template <int NT>
 __device__ inline double block_sum(double v, double* red) {

@@ -59,10 +59,8 @@ __device__ __forceinline__ void gemm_nt_bf16x3_body(
   __shared__ __attribute__((aligned(16))) uint16_t sA[2][2][BM * LDSW];  // [buf][hi/lo]
   __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][BN * LDSW];
 
-  const int nblk = tiles_m * tiles_n;
-  int b = bx;
-  if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);  // XCD-contiguous tile runs
-  const int tile_m = b % tiles_m, tile_n = b / tiles_m;
+  int tile_m, tile_n;
+  xcd_tile(bx, tiles_m, tiles_n, M > N, &tile_m, &tile_n);
   const int m0 = tile_m * BM, n0 = tile_n * BN;
   const int kbeg = by * k_per_split;
   const int kend = (kbeg + k_per_split < K) ? kbeg + k_per_split : K;

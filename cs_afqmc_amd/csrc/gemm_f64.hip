@@ -39,12 +39,10 @@ __device__ __forceinline__ void gemm_nt_f64_body(
   __shared__ double As[2][BK][BM + GEMM_PAD];
   __shared__ double Bs[2][BK][BN + GEMM_PAD];
 
-  // XCD-aware block -> tile map: blocks are dealt round-robin to the 8 XCDs, so give
-  // each XCD a contiguous run of tiles (m fastest: the run shares B column panels).
-  const int nblk = tiles_m * tiles_n;
-  int b = bx;
-  if ((nblk & 7) == 0) b = (b & 7) * (nblk >> 3) + (b >> 3);
-  const int tile_m = b % tiles_m, tile_n = b / tiles_m;
+  // XCD-aware block -> tile map (contiguous tile runs per XCD; the larger operand's
+  // panels are the shared ones)
+  int tile_m, tile_n;
+  xcd_tile(bx, tiles_m, tiles_n, M > N, &tile_m, &tile_n);
   const int m0 = tile_m * BM, n0 = tile_n * BN;
   const int kbeg = by * k_per_split;
   const int kend = (kbeg + k_per_split < K) ? kbeg + k_per_split : K;

@@ -491,11 +491,24 @@ def run_fragments(args):
         del frags, bt
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
+    # |dE| of the first block (before the first resampling: the precision's own effect)
+    # and of every block (later blocks can also differ by a discrete event, a walker's
+    # weight clipped or resampled in one precision and not the other)
+    d1 = [float(abs(a[0] - b[0])) for a, b in zip(energies["bf16x3"], energies["fp64_plain"])
+          if a.size]
     d = [float(np.max(np.abs(a - b))) for a, b in zip(energies["bf16x3"], energies["fp64_plain"])
          if a.size]
     dg = [float(np.max(np.abs(a - b))) for a, b in zip(energies["fp64"], energies["fp64_plain"])
           if a.size]
     finite = [bool(np.all(np.isfinite(a))) for a in energies["bf16x3"]]
+    # per fragment: block-by-block |dE| and the energy scale, largest first
+    per = sorted(((float(np.max(np.abs(a - b))), i) for i, (a, b) in
+                  enumerate(zip(energies["bf16x3"], energies["fp64_plain"])) if a.size),
+                 reverse=True)
+    worst = [{"fragment": i, "norb": specs[i][0], "n_walkers": specs[i][3],
+              "delta_e_blocks_eh": np.abs(energies["bf16x3"][i] - energies["fp64_plain"][i]).tolist(),
+              "energy_scale_eh": float(np.max(np.abs(energies["fp64_plain"][i])))}
+             for _, i in per[:4]]
     return {
         "metric": "walker-steps/s mixed vs FP64 GPU at |dE|<0.1 mEh (batched fragments)",
         "value": round(res["bf16x3"], 1),
@@ -516,8 +529,10 @@ def run_fragments(args):
         "fp64_value": round(res["fp64_plain"], 1),
         "fp64_grouped_value": round(res["fp64"], 1),
         "speedup_vs_fp64": round(res["bf16x3"] / res["fp64_plain"], 3),
-        "delta_e_max_eh": max(d) if d else None,
+        "delta_e_max_eh": max(d1) if d1 else None,
+        "delta_e_max_all_blocks_eh": max(d) if d else None,
         "delta_e_fp64_grouped_vs_plain_eh": max(dg) if dg else None,
+        "delta_e_worst_fragments": worst,
         "fragments_finite": sum(finite),
         "gemm_tuning_ms": {k: v for k, v in res.items() if k.endswith("_tuned")},
         "roofline": None,

@@ -108,13 +108,17 @@ class FragmentBatch:
     """A set of fragments advanced by grouped steps (one group, one stream).
 
     With use_graph, the n_prop_steps grouped steps of a block are captured once into a
-    HIP graph (torch.cuda.CUDAGraph over the batch's stream) and replayed every block
-    while the member set is unchanged; the first block after a change runs eagerly
-    (it rebuilds the group's tables), the next one captures."""
+    HIP graph (torch.cuda.CUDAGraph over the batch's stream, which is then a stream of its
+    own unless one is given) and replayed every block while the member set is unchanged;
+    the first block after a change runs eagerly (it rebuilds the group's tables), the
+    next one captures.  The caller's current stream waits for each block."""
 
     def __init__(self, fragments=(), max_fragments=1024, tune=False, use_graph=False,
                  stream=None):
-        self.stream = stream if stream is not None else torch.cuda.current_stream()
+        # graph capture needs a stream other than the default one (default priority)
+        if stream is None:
+            stream = torch.cuda.Stream() if use_graph else torch.cuda.current_stream()
+        self.stream = stream
         with torch.cuda.stream(self.stream):
             self.group = DeviceGroup(max_fragments)
         self.fragments = []
@@ -173,3 +177,5 @@ class FragmentBatch:
                 self._graph.replay()
             for f in self.fragments:
                 f.end_block()
+        # later work on the caller's stream (reading results) follows the block
+        torch.cuda.current_stream().wait_stream(self.stream)

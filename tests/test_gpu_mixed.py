@@ -18,6 +18,13 @@ pytestmark = pytest.mark.gpu
 CASES = [
     ("rhf", 40, (8, 8), 100, 32),
     ("rhf", 100, (20, 20), 400, 16),
+    # fragment sizes (n = N / 5, nchol = 4 N): every k-block count of the split-bf16
+    # Taylor kernel and every measurement-kernel width
+    ("rhf", 20, (4, 4), 80, 16),
+    ("rhf", 50, (10, 10), 200, 16),
+    ("rhf", 70, (14, 14), 280, 8),
+    ("rhf", 85, (17, 17), 340, 8),
+    ("rhf", 120, (24, 24), 480, 4),
     ("uhf", 36, (7, 5), 90, 24),
     ("uhf", 136, (34, 30), 160, 8),  # float32 streaming Taylor (N > 128)
     ("rhf", 96, (40, 40), 200, 8),  # float32 streaming Taylor (n > 32)

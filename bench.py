@@ -602,6 +602,10 @@ def run_serving(args):
         "latency_max_s": st.get("max_s"),
         "makespan_s": makespan,
         "wall_s": wall,
+        "build_s_mean": float(np.mean([r.build_s for r in res])),
+        "build_s_max": float(np.max([r.build_s for r in res])),
+        "admit_to_done_s_mean": float(np.mean([r.done - r.admitted for r in res])),
+        "arrival_to_admit_s_mean": float(np.mean([r.admitted - r.arrival for r in res])),
         "peak_active_fragments": srv.peak_active,
         "peak_reserved_fragments": srv.peak_reserved,
         "completed": len(res),

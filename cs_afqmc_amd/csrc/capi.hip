@@ -32,13 +32,29 @@ int dalloc(afqmc_ctx* c, T** p, size_t count) {
   return AFQMC_OK;
 }
 
+// true for memory of a device allocation on `device`; host memory, registered or not,
+// and other devices' memory are false
+bool is_device_ptr(const void* p, int device) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();  // unregistered host memory reports an error: clear it
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice && a.device == device;
+}
+
 // stream-ordered on the context's stream; ctx creation synchronises that stream (only)
 // before it returns, so the host arrays may be released afterwards
 template <typename T>
 int upload(afqmc_ctx* c, T** p, const T* host, size_t count) {
   int rc = dalloc(c, p, count);
   if (rc) return rc;
-  HIP_TRY(hipMemcpyAsync(*p, host, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(*p, host, count * sizeof(T), hipMemcpyDefault, c->stream));
+  return AFQMC_OK;
+}
+
+int copy_async(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, s));
   return AFQMC_OK;
 }
 
@@ -195,12 +211,17 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
       off_rc += (size_t)G * n * N;
     }
     // transposed Cholesky vectors (N*N, G): K-contiguous B operand of the VHS GEMM,
-    // transposed on the device from the (G, N*N) upload
+    // transposed on the device from the (G, N*N) array (read in place when the caller
+    // passes device memory, else from a temporary upload)
     {
       double* raw = nullptr;
-      rc = upload(c, &raw, ham->chol, (size_t)G * N * N);
+      const double* src = ham->chol;
+      if (!is_device_ptr(src, c->device)) {
+        rc = upload(c, &raw, ham->chol, (size_t)G * N * N);
+        src = raw;
+      }
       if (!rc) rc = dalloc(c, &c->chol_t, (size_t)G * N * N);
-      if (!rc) rc = launch_transpose_f64(c->stream, G, N * N, raw, c->chol_t);
+      if (!rc) rc = launch_transpose_f64(c->stream, G, N * N, src, c->chol_t);
       (void)hipStreamSynchronize(c->stream);
       if (raw) {
         (void)hipFree(raw);
@@ -241,19 +262,18 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
         for (int s = 0; s < c->nspin; ++s) {
           for (size_t e = 0; e < nn; ++e) ef[e] = (float)ham->exp_h1[(size_t)s * nn + e];
           TRY(upload(c, &c->exp_h1_f32[s], ef.data(), nn));
-          // exp_h1 - I as split-bf16 planes, rows padded to Np32 (bf16 Taylor kernel)
+          // exp_h1 - I as split-bf16 planes, rows padded to Np32 (bf16 Taylor kernel).  The
+          // float64 matrix is staged in the VHS workspace (8 W N^2 bytes, unused until the
+          // first step): a temporary allocation's hipFree would wait for the whole device.
           std::vector<double> emi(nn);
           for (size_t e = 0; e < nn; ++e)
             emi[e] = ham->exp_h1[(size_t)s * nn + e] - ((e / N == e % N) ? 1.0 : 0.0);
-          double* emi_dev = nullptr;
-          TRY(upload(c, &emi_dev, emi.data(), nn));
+          double* emi_dev = reinterpret_cast<double*>(c->vhs_f32);
+          TRY(copy_async(emi_dev, emi.data(), nn * sizeof(double), c->stream));
           for (int h = 0; h < 2; ++h) TRY(dalloc(c, &c->emi_bf[s][h], (size_t)N * c->Np32));
-          rc = launch_split_bf16(c->stream, N, N, emi_dev, N, c->emi_bf[s][0], c->emi_bf[s][1],
-                                 c->Np32);
-          if (!rc) rc = sync_stream(c->stream);
-          (void)hipFree(emi_dev);
-          c->device_bytes -= (int64_t)nn * (int64_t)sizeof(double);
-          if (rc) goto fail;
+          TRY(launch_split_bf16(c->stream, N, N, emi_dev, N, c->emi_bf[s][0], c->emi_bf[s][1],
+                                c->Np32));
+          TRY(sync_stream(c->stream));  // the host copy `emi` is released next iteration
         }
       }
       TRY(sync_stream(c->stream));

@@ -19,12 +19,69 @@ def _cptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+def _aptr(a):
+    """Pointer to a host numpy array or a device tensor (float64)."""
+    if isinstance(a, torch.Tensor):
+        return ctypes.cast(ctypes.c_void_p(a.data_ptr()), ctypes.POINTER(ctypes.c_double))
+    return _cptr(a)
+
+
 def _dptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
 def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+_CACHE = "_afqmc_dev_cache"
+
+
+def cached_tensor(ham_data, key):
+    """Device copy of ham_data[key] made by `cache_tensor`, or None when ham_data[key]
+    has been replaced since (the cache is keyed by the host object itself)."""
+    hit = ham_data.get(_CACHE, {}).get(key)
+    return hit[1] if hit is not None and hit[0] is ham_data.get(key) else None
+
+
+def cache_tensor(ham_data, key, tensor):
+    """Remember `tensor` as the device copy of the current ham_data[key]."""
+    ham_data.setdefault(_CACHE, {})[key] = (ham_data.get(key), tensor)
+
+
+def drop_cached(ham_data):
+    ham_data.pop(_CACHE, None)
+
+
+def device_chol(ham_data, norb):
+    """ham_data["chol"] as a contiguous float64 (G, N*N) tensor on the current device,
+    uploaded once per host array (the set-up contractions and the context read it)."""
+    if not torch.cuda.is_available():
+        raise _lib.AfqmcError("cs_afqmc_amd needs a ROCm GPU (torch.cuda is unavailable)")
+    t = cached_tensor(ham_data, "chol")
+    if t is None or t.device.index != torch.cuda.current_device():
+        src = ham_data["chol"]
+        if isinstance(src, torch.Tensor):
+            t = src.detach().to(device="cuda", dtype=torch.float64)
+        else:
+            t = torch.from_numpy(np.ascontiguousarray(np.asarray(src, dtype=_f64)))
+            t = t.to(device="cuda")
+        t = t.reshape(-1, norb * norb).contiguous()
+        cache_tensor(ham_data, "chol", t)
+    return t
+
+
+def _dev_f64(t, count, name):
+    """A float64 device tensor passed to the context in place of a host array."""
+    if t.device.type != "cuda" or t.device.index != torch.cuda.current_device():
+        raise ValueError(f"{name}: device tensor on {t.device}, current device "
+                         f"{torch.cuda.current_device()}")
+    if t.is_complex():
+        raise NotImplementedError(f"{name}: complex arrays are not supported")
+    t = t.detach().to(torch.float64).contiguous()
+    if t.numel() != count:
+        raise ValueError(f"{name}: {t.numel()} elements, expected {count}")
+    return t
 
 
 class DeviceStep:
@@ -51,12 +108,16 @@ class DeviceStep:
         self.dt = float(dt)
         self.n_exp_terms = int(n_exp_terms)
         self.precision = precision
-        chol = np.ascontiguousarray(np.asarray(chol, dtype=_f64).reshape(-1, norb * norb))
-        self.nchol = chol.shape[0]
+        if isinstance(chol, torch.Tensor):
+            self.nchol = chol.numel() // (norb * norb)
+            chol = _dev_f64(chol, self.nchol * norb * norb, "chol")
+        else:
+            chol = np.ascontiguousarray(np.asarray(chol, dtype=_f64).reshape(-1, norb * norb))
+            self.nchol = chol.shape[0]
         if walker_type == "rhf":
             mo = np.ascontiguousarray(np.asarray(mo_coeff, dtype=_f64))
             rh1 = np.ascontiguousarray(np.asarray(rot_h1, dtype=_f64))
-            rch = np.ascontiguousarray(np.asarray(rot_chol, dtype=_f64))
+            rot_chol = [rot_chol]
             eh1 = np.ascontiguousarray(np.asarray(exp_h1, dtype=_f64).reshape(norb, norb))
             wt = _lib.AFQMC_RHF
         elif walker_type == "uhf":
@@ -64,14 +125,20 @@ class DeviceStep:
                 [np.asarray(m, dtype=_f64).ravel() for m in mo_coeff]))
             rh1 = np.ascontiguousarray(np.concatenate(
                 [np.asarray(m, dtype=_f64).ravel() for m in rot_h1]))
-            rch = np.ascontiguousarray(np.concatenate(
-                [np.asarray(m, dtype=_f64).ravel() for m in rot_chol]))
             eh1 = np.ascontiguousarray(np.asarray(exp_h1, dtype=_f64).reshape(2, norb, norb))
             wt = _lib.AFQMC_UHF
         else:
             raise ValueError(f"walker_type {walker_type!r}")
+        if any(isinstance(m, torch.Tensor) for m in rot_chol):
+            n_rc = sum(self.nchol * n * norb
+                       for n in self.nelec[: 1 if walker_type == "rhf" else 2])
+            rch = _dev_f64(torch.cat([torch.as_tensor(m, device="cuda").reshape(-1)
+                                      for m in rot_chol]), n_rc, "rot_chol")
+        else:
+            rch = np.ascontiguousarray(np.concatenate(
+                [np.asarray(m, dtype=_f64).ravel() for m in rot_chol]))
         for a in (mo, rh1, rch):
-            if np.iscomplexobj(a):
+            if (a.is_complex() if isinstance(a, torch.Tensor) else np.iscomplexobj(a)):
                 raise NotImplementedError("complex trial coefficients are not supported")
         mf = np.ascontiguousarray(np.asarray(mf_shifts, dtype=np.complex128).reshape(-1))
         if mf.shape[0] != self.nchol:
@@ -80,8 +147,8 @@ class DeviceStep:
         self._keep = (chol, mo, rh1, rch, eh1, mf_ri)
         dims = _lib.Dims(wt, self.norb, self.nelec[0], self.nelec[1], self.nchol,
                          self.n_walkers, self.n_exp_terms, _lib.PRECISIONS[precision], self.dt)
-        ham = _lib.HamHost(float(np.real(h0)), float(np.real(h0_prop)), _cptr(chol), _cptr(mo),
-                           _cptr(rh1), _cptr(rch), _cptr(mf_ri), _cptr(eh1))
+        ham = _lib.HamHost(float(np.real(h0)), float(np.real(h0_prop)), _aptr(chol), _cptr(mo),
+                           _cptr(rh1), _aptr(rch), _cptr(mf_ri), _cptr(eh1))
         self.device = torch.device("cuda", torch.cuda.current_device())
         handle = ctypes.c_void_p()
         L = _lib.lib()

@@ -21,7 +21,7 @@ import scipy.linalg
 import torch
 
 from . import rng
-from .device import DeviceStep
+from .device import DeviceStep, device_chol, drop_cached
 from .wavefunctions import _as_np
 
 
@@ -62,21 +62,26 @@ class propagator_restricted:
         """propagation.py:292-325 (restricted) / :464-509 (unrestricted):
         mf_shifts = i sum(L_g * rdm1), h0_prop = -h0 - sum(mf^2)/2,
         v0 = 1/2 sum_g L_g L_g^T, h1_mod = h1 - v0 - Re(i sum_g mf_g L_g),
-        exp_h1 = expm(-dt h1_mod / 2).  Float64 on the host, once per run; then the
-        device context (intermediates + workspace) is created."""
+        exp_h1 = expm(-dt h1_mod / 2).  Float64, once per run: the Cholesky contractions
+        on the device (over the vectors uploaded for the context), the rest on the host;
+        then the device context (intermediates + workspace) is created."""
         norb = trial.norb
         rdm1 = np.asarray(trial.get_rdm1(wave_data), dtype=np.float64)
         rdm1t = rdm1[0] + rdm1[1]
-        chol = np.asarray(_as_np(ham_data["chol"]), dtype=np.float64).reshape(-1, norb * norb)
-        chol3 = chol.reshape(-1, norb, norb)
-        mf_shifts = 1.0j * (chol @ rdm1t.reshape(-1))
+        chol = device_chol(ham_data, norb)  # (G, N*N)
+        G = chol.shape[0]
+        mf_re = chol @ torch.from_numpy(rdm1t.reshape(-1)).to(chol.device)
+        # v0 = 1/2 sum_g L_g L_g^T: G independent (N x N x N) products summed (one
+        # (N x GN x N) product would leave most of the GPU idle on its few output tiles)
+        chol3 = chol.view(G, norb, norb)
+        v0 = (0.5 * torch.bmm(chol3, chol3.transpose(1, 2).contiguous()).sum(0)).cpu().numpy()
+        # Re(i sum_g mf_g L_g) = -sum_g mf_re_g L_g: a real product
+        v1 = (-(mf_re @ chol)).view(norb, norb).cpu().numpy()
+        mf_re = mf_re.cpu().numpy()
+        mf_shifts = 1.0j * mf_re  # purely imaginary
         ham_data["mf_shifts"] = mf_shifts
         h0 = float(np.real(_as_np(ham_data["h0"])))
         ham_data["h0_prop"] = -h0 - np.sum(mf_shifts**2) / 2.0
-        # v0[i, j] = 1/2 sum_{g,k} L_g[i, k] L_g[j, k]: one (N x G N) product
-        lt = np.ascontiguousarray(chol3.transpose(1, 0, 2)).reshape(norb, -1)
-        v0 = 0.5 * (lt @ lt.T)
-        v1 = np.real(1.0j * (mf_shifts @ chol)).reshape(norb, norb)
         h1 = np.asarray(_as_np(ham_data["h1"]), dtype=np.float64)
         if self.walker_type == "rhf":
             h1_mod = (h1[0] + h1[1]) / 2.0 - v0 - v1
@@ -90,6 +95,7 @@ class propagator_restricted:
             self.walker_type, norb, trial.nelec, self.n_walkers, self.dt, self.n_exp_terms,
             h0, ham_data["h0_prop"], chol, wave_data["mo_coeff"], ham_data["rot_h1"],
             ham_data["rot_chol"], mf_shifts, ham_data["exp_h1"], precision=self.precision)
+        drop_cached(ham_data)  # the context holds its own copies
         ham_data["_afqmc_device"] = dev
         wave_data["_afqmc_device"] = dev
         return ham_data

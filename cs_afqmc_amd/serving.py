@@ -53,6 +53,7 @@ class Result:
     admitted: float
     done: float
     walker_steps: int
+    build_s: float = 0.0  # host + device set-up time of the request (prep thread)
 
     @property
     def latency(self):
@@ -60,9 +61,10 @@ class Result:
 
 
 def estimate_bytes(norb, nocc, nchol, n_walkers, n_prop_steps, precision):
-    """Device bytes of one fragment (context, workspace, walker state, fields block)."""
+    """Device bytes of one fragment (context, workspace, walker state, fields block, and
+    the rotated Cholesky vectors ham_data keeps on the device)."""
     N, n, G, W = norb, nocc, nchol, n_walkers
-    b = 8 * N * N * G + 8 * G * n * N + 16 * G  # chol_t, rot_chol, mf
+    b = 8 * N * N * G + 2 * 8 * G * n * N + 16 * G  # chol_t, rot_chol (ctx, ham_data), mf
     if precision != "fp64":
         gp, np32 = -(-G // 32) * 32, -(-N // 32) * 32
         b += 4 * N * N * gp + 4 * G * n * np32 + 4 * W * 2 * n * np32 + 4 * W * 2 * gp
@@ -77,8 +79,9 @@ def estimate_bytes(norb, nocc, nchol, n_walkers, n_prop_steps, precision):
 
 class FragmentServer:
     def __init__(self, precision="bf16x3", n_prop_steps=50, hbm_budget=64 << 30,
-                 max_active=64, dt=0.005, tune=False, prep_workers=4):
+                 max_active=64, dt=0.005, tune=False, prep_workers=4, max_inflight=2):
         self.prep_workers = max(1, int(prep_workers))
+        self.max_inflight = max(1, int(max_inflight))
         self.precision = precision
         self.n_prop_steps = int(n_prop_steps)
         self.hbm_budget = int(hbm_budget)
@@ -89,6 +92,7 @@ class FragmentServer:
     def _build(self, req, side):
         """Host intermediates + device context of one request, on the calling prep
         thread's stream; returns (fragment, event after its set-up work)."""
+        t = time.perf_counter()
         with torch.cuda.stream(side):
             frag = batch.Fragment(req.ham_data, req.wave_data, req.walker_type, req.nelec,
                                   req.n_walkers, dt=self.dt, seed=req.seed,
@@ -96,6 +100,7 @@ class FragmentServer:
                                   tag=req.rid)
             ev = torch.cuda.Event()
             ev.record(side)
+        frag.build_s = time.perf_counter() - t
         return frag, ev
 
     def serve(self, requests, poll_s=0.0005):
@@ -126,6 +131,7 @@ class FragmentServer:
         ev0 = torch.cuda.Event(enable_timing=True)
         ev0.record(tick)
         nxt = 0
+        inflight = []  # events after the queued ticks
         with torch.cuda.stream(tick):
             bt = batch.FragmentBatch(max_fragments=max(self.max_active, 1))
             while nxt < len(pending) or queue or building or active or retiring:
@@ -141,7 +147,7 @@ class FragmentServer:
                         done = ev0.elapsed_time(ev) / 1e3
                         energy = float(np.sum(be * bw) / np.sum(bw))
                         results.append(Result(req.rid, energy, be, bw, req.arrival, adm, done,
-                                              frag.walker_steps))
+                                              frag.walker_steps, frag.build_s))
                         graveyard.append(frag)
                     else:
                         still.append((frag, req, adm, ev, host))
@@ -184,8 +190,18 @@ class FragmentServer:
                         if wait > 0:
                             time.sleep(wait)
                     continue
-                # one tick: a block of every active fragment
+                # one tick: a block of every active fragment.  At most `max_inflight`
+                # ticks are queued on the GPU: fragments that join meanwhile enter the
+                # next tick instead of waiting behind a deep queue, and the loop sleeps
+                # (releasing the interpreter to the prep threads) rather than spinning.
+                inflight = [e for e in inflight if not e.query()]
+                if len(inflight) >= self.max_inflight:
+                    time.sleep(poll_s)
+                    continue
                 bt.run_block()
+                tev = torch.cuda.Event()
+                tev.record(tick)
+                inflight.append(tev)
                 done = [f for f, *_ in active.values() if f.blocks_done >= f.n_blocks_total]
                 if done:
                     for f in done:

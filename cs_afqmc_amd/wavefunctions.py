@@ -6,7 +6,7 @@ Mirrors ad_afqmc/wavefunctions.py: ``rhf`` (:453-635) and ``uhf`` (:638-844) wit
 (:602-619 / :817-841).  Walkers are cuda complex128 tensors: (W, N, n) for
 restricted walkers, [up, dn] for unrestricted ones.  Measurements run in the HIP
 library (afqmc_overlap / afqmc_force_bias / afqmc_energy); intermediates are built
-once on the host in float64, as the reference builds them once per run.
+once per run in float64, the Cholesky contractions on the device.
 
 Supported pairs (mpi_jax.py:314-352): restricted walkers with an rhf trial, and
 unrestricted walkers with a uhf trial.  The trial coefficients must be real.
@@ -18,7 +18,7 @@ from typing import Tuple
 import numpy as np
 import torch
 
-from .device import DeviceStep
+from .device import DeviceStep, device_chol
 
 
 def _as_np(x):
@@ -42,11 +42,11 @@ def _device_for(trial, ham_data, wave_data, walker_type, n_walkers):
     norb = trial.norb
     eye = np.eye(norb) if walker_type == "rhf" else np.array([np.eye(norb)] * 2)
     if ham_data is not None:
-        nchol = np.asarray(_as_np(ham_data["chol"])).reshape(-1, norb * norb).shape[0]
+        chol = device_chol(ham_data, norb)
         dev = DeviceStep(
             walker_type, norb, trial.nelec, n_walkers, 0.01, 2, _as_np(ham_data["h0"]), 0.0,
-            _as_np(ham_data["chol"]), wave_data["mo_coeff"], ham_data["rot_h1"],
-            ham_data["rot_chol"], np.zeros(nchol, dtype=np.complex128), eye)
+            chol, wave_data["mo_coeff"], ham_data["rot_h1"], ham_data["rot_chol"],
+            np.zeros(chol.shape[0], dtype=np.complex128), eye)
         ham_data["_afqmc_measure_device"] = dev
         return dev
     # overlap only: the Hamiltonian arrays are placeholders the overlap kernel never reads
@@ -114,10 +114,9 @@ class rhf:
         h1 = np.array([(h1[0] + h1[0].T) / 2.0, (h1[1] + h1[1].T) / 2.0])
         ham_data["h1"] = h1
         mo = np.asarray(_as_np(wave_data["mo_coeff"]), dtype=np.float64)
-        chol3 = np.asarray(_as_np(ham_data["chol"]), dtype=np.float64).reshape(-1, norb, norb)
         ham_data["rot_h1"] = mo.T @ ((h1[0] + h1[1]) / 2.0)
-        ham_data["rot_chol"] = np.matmul(mo.T, chol3)
         _invalidate(ham_data)
+        _rotate_chol(ham_data, norb, [mo], restricted=True)
         return ham_data
 
 
@@ -164,11 +163,21 @@ class uhf:
         h1 = np.array([(h1[0] + h1[0].T) / 2.0, (h1[1] + h1[1].T) / 2.0])
         ham_data["h1"] = h1
         mo = [np.asarray(_as_np(m), dtype=np.float64) for m in wave_data["mo_coeff"]]
-        chol3 = np.asarray(_as_np(ham_data["chol"]), dtype=np.float64).reshape(-1, norb, norb)
         ham_data["rot_h1"] = [mo[0].T @ h1[0], mo[1].T @ h1[1]]
-        ham_data["rot_chol"] = [np.matmul(mo[0].T, chol3), np.matmul(mo[1].T, chol3)]
         _invalidate(ham_data)
+        _rotate_chol(ham_data, norb, mo, restricted=False)
         return ham_data
+
+
+def _rotate_chol(ham_data, norb, mos, restricted):
+    """ham_data["rot_chol"]: C^T L_g for every Cholesky vector and each trial C in `mos`,
+    one batched float64 product on the device over the uploaded vectors.  The result
+    stays in device memory (a float64 cuda tensor (G, n, N); a list of two for uhf), as
+    the reference's jax arrays do on a GPU."""
+    chol = device_chol(ham_data, norb).view(-1, norb, norb)
+    rot = [torch.matmul(torch.from_numpy(np.ascontiguousarray(m.T)).to(chol.device), chol)
+           for m in mos]
+    ham_data["rot_chol"] = rot[0] if restricted else rot
 
 
 def _invalidate(ham_data):
@@ -176,12 +185,28 @@ def _invalidate(ham_data):
     ham_data.pop("_afqmc_measure_device", None)
 
 
+def _eigh_vectors(a):
+    """Eigenvectors of a symmetric matrix (numpy's LAPACK driver, so the basis chosen in
+    degenerate subspaces is the host reference's).  Run on one BLAS thread: next to the
+    HIP runtime's threads a threaded decomposition of an orbital-sized matrix was
+    measured 50x slower (110 ms against 2 ms at N = 250)."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:
+        return np.linalg.eigh(a)[1]
+    with threadpool_limits(limits=1, user_api="blas"):
+        return np.linalg.eigh(a)[1]
+
+
 def _init_walkers(trial, wave_data, n_walkers, restricted):
     """wavefunctions.py:298-355: natural orbitals (descending occupation) of each spin
     rdm1, copied n_walkers times, as cuda complex128 tensors."""
     rdm1 = trial.get_rdm1(wave_data)
-    natorbs_up = np.linalg.eigh(rdm1[0])[1][:, ::-1][:, : trial.nelec[0]]
-    natorbs_dn = np.linalg.eigh(rdm1[1])[1][:, ::-1][:, : trial.nelec[1]]
+    vec_up = _eigh_vectors(rdm1[0])
+    # identical spin densities (closed shells) share one decomposition
+    vec_dn = vec_up if np.array_equal(rdm1[0], rdm1[1]) else _eigh_vectors(rdm1[1])
+    natorbs_up = vec_up[:, ::-1][:, : trial.nelec[0]]
+    natorbs_dn = vec_dn[:, ::-1][:, : trial.nelec[1]]
     dev = torch.device("cuda", torch.cuda.current_device())
 
     def rep(m):

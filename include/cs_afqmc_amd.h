@@ -66,9 +66,11 @@ typedef struct afqmc_dims {
   double dt;            /* time step */
 } afqmc_dims;
 
-/* Host float64 intermediates, laid out as the reference's ham_data / wave_data
- * after build_measurement_intermediates (wavefunctions.py:602-619, :817-841) and
- * build_propagation_intermediates (propagation.py:292-325, :464-509). */
+/* Float64 intermediates, laid out as the reference's ham_data / wave_data after
+ * build_measurement_intermediates (wavefunctions.py:602-619, :817-841) and
+ * build_propagation_intermediates (propagation.py:292-325, :464-509).  Each pointer
+ * may address host memory or device memory of the context's device (copied with
+ * hipMemcpyDefault on the creation stream; device arrays must be ready on it). */
 typedef struct afqmc_ham_host {
   double h0;               /* ham_data["h0"]                                        */
   double h0_prop;          /* Re ham_data["h0_prop"]                                */

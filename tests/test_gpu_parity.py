@@ -99,8 +99,22 @@ def case(request, gpu):
 def test_intermediates_match_oracle(case):
     assert rel(case.ham["mf_shifts"], case.oham["mf_shifts"]) < 1e-13
     assert rel(case.ham["exp_h1"], case.oham["exp_h1"]) < 1e-13
-    assert rel(case.ham["rot_chol"], case.oham["rot_chol"]) < 1e-13
+    assert rel(to_np(case.ham["rot_chol"]), case.oham["rot_chol"]) < 1e-13
     assert abs(case.ham["h0_prop"] - case.oham["h0_prop"]) < 1e-12 * (1 + abs(case.oham["h0_prop"]))
+
+
+def test_context_from_host_arrays(case):
+    """afqmc_ctx_create copies the intermediates from host memory (numpy arrays) as it
+    does from device memory (the Cholesky and rotated Cholesky tensors the set-up leaves
+    on the device): same energies and force biases to the bit."""
+    from cs_afqmc_amd.device import DeviceStep
+    hd, dev = case.ham, case.ham["_afqmc_device"]
+    host = DeviceStep(case.wt, case.norb, case.nelec, case.W, case.dt, 6, hd["h0"],
+                      hd["h0_prop"], np.asarray(hd["chol"]), case.wave["mo_coeff"],
+                      hd["rot_h1"], to_np(hd["rot_chol"]), hd["mf_shifts"], hd["exp_h1"])
+    w = case.walkers()
+    assert torch.equal(host.energy(w), dev.energy(w))
+    assert torch.equal(host.force_bias(w), dev.force_bias(w))
 
 
 def test_overlap(case):

@@ -257,7 +257,9 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
         TRY(zero_async(c->xs_bf[h], (size_t)W * 2 * c->Gp * sizeof(uint16_t), c->stream));
       }
       if (c->taylor_f32) {
-        TRY(dalloc(c, &c->vhs_f32, (size_t)W * 2 * nn));
+        // 8 floats of tail padding: the streaming split-bf16 Taylor kernel reads V rows in
+        // 8-float chunks, and the chunk that straddles the end of the last row is masked
+        TRY(dalloc(c, &c->vhs_f32, (size_t)W * 2 * nn + 8));
         std::vector<float> ef(nn);
         for (int s = 0; s < c->nspin; ++s) {
           for (size_t e = 0; e < nn; ++e) ef[e] = (float)ham->exp_h1[(size_t)s * nn + e];

@@ -367,7 +367,8 @@ enum TaylorKind {
   TAYLOR_REG_F32 = 1,
   TAYLOR_STREAM_F32 = 2,
   TAYLOR_STREAM_F64 = 3,
-  TAYLOR_REG_BF16X3 = 4
+  TAYLOR_REG_BF16X3 = 4,
+  TAYLOR_STREAM_BF16X3 = 5
 };
 
 struct TaylorPlan {

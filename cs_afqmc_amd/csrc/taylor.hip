@@ -896,6 +896,316 @@ __global__ __launch_bounds__(1024) void taylor_stream_f32_grouped_kernel(
                              static_cast<const float*>(a.vhs), blockIdx.x - start[p]);
 }
 
+// ------------------------------------------------------------------------------------
+// Split-bf16 streaming form for the mixed mode beyond the register kernel's range
+// (N <= 256, n <= 64).  One workgroup per walker of one spin; wave r owns RT 16-row
+// output tiles starting at row 16 RT r, and every column.  y = [yr | yi] (N x 2 n, each
+// half padded to 16 CT columns) lives in LDS as hi / lo bf16 planes stored [column][k]
+// with k contiguous, so a lane's B operand (8 consecutive k of one column) is one 16-byte
+// read; k is padded with zeros to 32 ceil(N / 32).  V streams from global memory as
+// float32 rows, 32 k at a time with the next block in flight while the current one
+// multiplies, and is split into bf16 planes in registers.  The complex product is four
+// real products,
+//   ur = Vr yr - Vi yi,  ui = Vr yi + Vi yr,
+// each as three bf16 products (al bh + ah bl + ah bh) on v_mfma_f32_16x16x32_bf16 with
+// float32 accumulation; the planes of -Vi are those of Vi with the sign bits flipped
+// (round-to-nearest-even is symmetric).  E y = y + (E - I) y uses the split planes of
+// E - I (rows of 32 ceil(N / 32) entries) as the register form does, and the Horner
+// recursion y <- t0 + V y / k runs in float32 registers.  Two row tiles per wave (RT = 2,
+// at most 8 waves, 256 registers per lane) hold the wider column counts without spills
+// and share each B operand between both tiles.
+typedef float f4u_t __attribute__((ext_vector_type(4), aligned(4)));  // dword-aligned rows
+
+inline int bfs_ldk(int N) { return 32 * ((N + 31) / 32) + 8; }
+inline size_t bfs_shm(int N, int ct) {
+  return (size_t)2 * 32 * ct * bfs_ldk(N) * sizeof(uint16_t);
+}
+constexpr int bfs_rt(int ct) { return ct >= 3 ? 2 : 1; }
+inline int bfs_threads(int N, int ct) {
+  const int rows = 16 * bfs_rt(ct);
+  return 64 * ((N + rows - 1) / rows);
+}
+
+template <int CT, int RT>
+__device__ __forceinline__ void taylor_bfs_body(int N, int n, int n_terms_k,
+                                                cplx* __restrict__ walkers,
+                                                const uint16_t* __restrict__ emi_hi,
+                                                const uint16_t* __restrict__ emi_lo,
+                                                const float* __restrict__ vhs, const int w) {
+  constexpr int NC = 32 * CT;  // LDS columns: CT tiles of yr, then CT tiles of yi
+  const int KBk = (N + 31) >> 5;
+  const int LDK = 32 * KBk + 8;
+  const int np32 = 32 * KBk;
+  extern __shared__ uint16_t YB[];  // [2 planes][NC][LDK]
+  uint16_t* Yh = YB;
+  uint16_t* Yl = YB + NC * LDK;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rb = wave * 16 * RT;
+  const int q4 = lane >> 4, col = lane & 15;
+  // uniform bases with 32-bit lane offsets (saddr + voffset addressing)
+  cplx* phi = walkers + (int64_t)w * N * n;
+  const float* Vr = vhs + (int64_t)w * 2 * N * N;
+  const float* Vi = Vr + (int64_t)N * N;
+  bool prow[RT];
+  int voff[RT], eoff[RT];
+#pragma unroll
+  for (int t = 0; t < RT; ++t) {
+    const int p = rb + 16 * t + col;  // A row of this lane in tile t
+    prow[t] = p < N;
+    voff[t] = (prow[t] ? p : 0) * N;
+    eoff[t] = (prow[t] ? p : 0) * np32;
+  }
+
+  {
+    u32x4_t* z = reinterpret_cast<u32x4_t*>(YB);
+    const int nz = 2 * NC * LDK / 8;
+    for (int e = threadIdx.x; e < nz; e += blockDim.x) z[e] = u32x4_t{0u, 0u, 0u, 0u};
+  }
+  __syncthreads();
+
+  // D layout (rows 4 (l / 16) + r of a tile, column l % 16) -> LDS planes: four
+  // consecutive k of one column, 8-byte stores.  Rows >= N and columns >= n carry zeros
+  // (their A rows / B columns are zero), which keeps the padding zero.
+  auto put = [&](const float4_t (*yr)[CT], const float4_t (*yi)[CT]) {
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          const float4_t v = h ? yi[t][ct] : yr[t][ct];
+          const int off = (h * 16 * CT + ct * 16 + col) * LDK + rb + 16 * t + 4 * q4;
+          const uint32_t h01 = pack_bf16x2(v[0], v[1]);
+          const uint32_t h23 = pack_bf16x2(v[2], v[3]);
+          const uint32_t l01 = pack_bf16x2(v[0] - bf16_lo_val(h01), v[1] - bf16_hi_val(h01));
+          const uint32_t l23 = pack_bf16x2(v[2] - bf16_lo_val(h23), v[3] - bf16_hi_val(h23));
+          *reinterpret_cast<u32x2_t*>(Yh + off) = u32x2_t{h01, h23};
+          *reinterpret_cast<u32x2_t*>(Yl + off) = u32x2_t{l01, l23};
+        }
+  };
+  auto bload = [&](const uint16_t* plane, int c0, int kb) {
+    return __builtin_bit_cast(
+        bf16x8_t, *reinterpret_cast<const u32x4_t*>(plane + (c0 + col) * LDK + kb * 32 + 8 * q4));
+  };
+  auto zero = [&](float4_t (*a)[CT]) {
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) a[t][ct] = float4_t{0.f, 0.f, 0.f, 0.f};
+  };
+  // acc = (E - I) y, a real operator on both halves
+  auto e_product = [&](float4_t (*ar)[CT], float4_t (*ai)[CT]) {
+    zero(ar);
+    zero(ai);
+#pragma unroll 1
+    for (int kb = 0; kb < KBk; ++kb) {
+      bf16x8_t a_h[RT], a_l[RT];
+#pragma unroll
+      for (int t = 0; t < RT; ++t) {
+        u32x4_t ah = {0, 0, 0, 0}, al = {0, 0, 0, 0};
+        if (prow[t]) {
+          ah = *reinterpret_cast<const u32x4_t*>(emi_hi + (eoff[t] + kb * 32 + 8 * q4));
+          al = *reinterpret_cast<const u32x4_t*>(emi_lo + (eoff[t] + kb * 32 + 8 * q4));
+        }
+        a_h[t] = __builtin_bit_cast(bf16x8_t, ah);
+        a_l[t] = __builtin_bit_cast(bf16x8_t, al);
+      }
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) {
+        const bf16x8_t rh = bload(Yh, ct * 16, kb), rl = bload(Yl, ct * 16, kb);
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+          ar[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l[t], rh, ar[t][ct], 0, 0, 0);
+          ar[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h[t], rl, ar[t][ct], 0, 0, 0);
+          ar[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h[t], rh, ar[t][ct], 0, 0, 0);
+        }
+        const bf16x8_t ih = bload(Yh, (CT + ct) * 16, kb), il = bload(Yl, (CT + ct) * 16, kb);
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+          ai[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l[t], ih, ai[t][ct], 0, 0, 0);
+          ai[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h[t], il, ai[t][ct], 0, 0, 0);
+          ai[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h[t], ih, ai[t][ct], 0, 0, 0);
+        }
+      }
+    }
+  };
+  // 8 consecutive k of a V row as two 16-byte loads, for chunks that start inside the
+  // row (others are zero).  A chunk that straddles k = N reads at most 7 floats past the
+  // row end, into the next row or, for the last row of the last walker, into the 8
+  // floats of tail padding of the VHS buffer, and is masked after the load.
+  auto vload = [&](int kb, float (*r)[8], float (*i)[8]) {
+    const int k0 = kb * 32 + 8 * q4;
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+      f4u_t r0 = {0.f, 0.f, 0.f, 0.f}, r1 = r0, i0 = r0, i1 = r0;
+      if (prow[t] && k0 < N) {
+        r0 = *reinterpret_cast<const f4u_t*>(Vr + (voff[t] + k0));
+        r1 = *reinterpret_cast<const f4u_t*>(Vr + (voff[t] + k0 + 4));
+        i0 = *reinterpret_cast<const f4u_t*>(Vi + (voff[t] + k0));
+        i1 = *reinterpret_cast<const f4u_t*>(Vi + (voff[t] + k0 + 4));
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        r[t][e] = r0[e];
+        r[t][4 + e] = r1[e];
+        i[t][e] = i0[e];
+        i[t][4 + e] = i1[e];
+      }
+      if (k0 + 8 > N) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          if (k0 + e >= N) r[t][e] = i[t][e] = 0.f;
+      }
+    }
+  };
+  auto v_product = [&](float4_t (*ur)[CT], float4_t (*ui)[CT]) {
+    zero(ur);
+    zero(ui);
+    float rr[RT][8], ri[RT][8];
+    vload(0, rr, ri);
+#pragma unroll 1
+    for (int kb = 0; kb < KBk; ++kb) {
+      u32x4_t rh[RT], rl[RT], ih[RT], il[RT];
+#pragma unroll
+      for (int t = 0; t < RT; ++t) {
+        split8(rr[t], rh[t], rl[t]);
+        split8(ri[t], ih[t], il[t]);
+      }
+      if (kb + 1 < KBk) vload(kb + 1, rr, ri);  // next block in flight
+      auto bf = [](const u32x4_t& x) { return __builtin_bit_cast(bf16x8_t, x); };
+      // rows of yr: ur += Vr yr, ui += Vi yr
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) {
+        const bf16x8_t yh = bload(Yh, ct * 16, kb), yl = bload(Yl, ct * 16, kb);
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+          ur[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf(rl[t]), yh, ur[t][ct], 0, 0, 0);
+          ur[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf(rh[t]), yl, ur[t][ct], 0, 0, 0);
+          ur[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf(rh[t]), yh, ur[t][ct], 0, 0, 0);
+          ui[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf(il[t]), yh, ui[t][ct], 0, 0, 0);
+          ui[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf(ih[t]), yl, ui[t][ct], 0, 0, 0);
+          ui[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf(ih[t]), yh, ui[t][ct], 0, 0, 0);
+        }
+      }
+      // rows of yi: ur += (-Vi) yi, ui += Vr yi (Vi's planes negated in place)
+      const u32x4_t sg = {0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+#pragma unroll
+      for (int t = 0; t < RT; ++t) {
+        ih[t] ^= sg;
+        il[t] ^= sg;
+      }
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) {
+        const bf16x8_t yh = bload(Yh, (CT + ct) * 16, kb), yl = bload(Yl, (CT + ct) * 16, kb);
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+          ur[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf(il[t]), yh, ur[t][ct], 0, 0, 0);
+          ur[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf(ih[t]), yl, ur[t][ct], 0, 0, 0);
+          ur[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf(ih[t]), yh, ur[t][ct], 0, 0, 0);
+          ui[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf(rl[t]), yh, ui[t][ct], 0, 0, 0);
+          ui[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf(rh[t]), yl, ui[t][ct], 0, 0, 0);
+          ui[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf(rh[t]), yh, ui[t][ct], 0, 0, 0);
+        }
+      }
+    }
+  };
+
+  // walker (float64) -> float32 D layout -> LDS
+  float4_t t0r[RT][CT], t0i[RT][CT], ur[RT][CT], ui[RT][CT];
+#pragma unroll
+  for (int t = 0; t < RT; ++t)
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rb + 16 * t + 4 * q4 + r;
+        const int c = ct * 16 + col;
+        float re = 0.f, im = 0.f;
+        if (row < N && c < n) {
+          const cplx z = phi[row * n + c];
+          re = (float)z.re;
+          im = (float)z.im;
+        }
+        t0r[t][ct][r] = re;
+        t0i[t][ct][r] = im;
+      }
+  put(t0r, t0i);
+  __syncthreads();
+  e_product(ur, ui);
+#pragma unroll
+  for (int t = 0; t < RT; ++t)
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+      t0r[t][ct] += ur[t][ct];
+      t0i[t][ct] += ui[t][ct];
+    }
+  __syncthreads();
+  put(t0r, t0i);
+  __syncthreads();
+  for (int k = n_terms_k; k >= 1; --k) {
+    v_product(ur, ui);
+    __syncthreads();
+    const float rk = 1.0f / (float)k;
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) {
+        ur[t][ct] = t0r[t][ct] + ur[t][ct] * rk;
+        ui[t][ct] = t0i[t][ct] + ui[t][ct] * rk;
+      }
+    put(ur, ui);
+    __syncthreads();
+  }
+  float4_t ar[RT][CT], ai[RT][CT];
+  e_product(ar, ai);
+#pragma unroll
+  for (int t = 0; t < RT; ++t)
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rb + 16 * t + 4 * q4 + r;
+        const int c = ct * 16 + col;
+        if (row < N && c < n)
+          phi[row * n + c] = cplx{(double)(ur[t][ct][r] + ar[t][ct][r]),
+                                  (double)(ui[t][ct][r] + ai[t][ct][r])};
+      }
+}
+
+template <int CT>
+__global__ __launch_bounds__(1024 / bfs_rt(CT)) void taylor_bfs_kernel(
+    int N, int n, int n_terms_k, cplx* __restrict__ walkers, const uint16_t* __restrict__ emi_hi,
+    const uint16_t* __restrict__ emi_lo, const float* __restrict__ vhs) {
+  taylor_bfs_body<CT, bfs_rt(CT)>(N, n, n_terms_k, walkers, emi_hi, emi_lo, vhs, blockIdx.x);
+}
+
+template <int CT>
+__global__ __launch_bounds__(1024 / bfs_rt(CT)) void taylor_bfs_grouped_kernel(
+    const TaylorArgs* __restrict__ args, const int* __restrict__ start, int nprob) {
+  const int p = group_of(start, nprob, blockIdx.x);
+  const TaylorArgs a = args[p];
+  taylor_bfs_body<CT, bfs_rt(CT)>(
+      a.N, a.n, a.K, a.walkers, static_cast<const uint16_t*>(a.E),
+      static_cast<const uint16_t*>(a.E2), static_cast<const float*>(a.vhs), blockIdx.x - start[p]);
+}
+
+template <int CT>
+int launch_bfs(afqmc_ctx* c, int W, int n, cplx* wk, int s) {
+  const int N = c->d.norb;
+  if (N > 256 || n > 16 * CT || 32 * ((N + 31) / 32) != c->Np32)
+    return afqmc_set_error(AFQMC_ERR_ARG, "taylor bf16x3 streaming: N %d n %d column tiles %d",
+                           N, n, CT);
+  const size_t shm = bfs_shm(N, CT);
+  static int lds_limit = 0;
+  const int rc = ensure_dyn_lds((const void*)taylor_bfs_kernel<CT>, shm, &lds_limit);
+  if (rc) return rc;
+  hipLaunchKernelGGL((taylor_bfs_kernel<CT>), dim3(W), dim3(bfs_threads(N, CT)), shm, c->stream,
+                     N, n, c->d.n_exp_terms - 1, wk, c->emi_bf[s][0], c->emi_bf[s][1],
+                     c->vhs_f32);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
 template <int CT>
 int launch_stream_f32(afqmc_ctx* c, int W, int n, cplx* wk, const float* E) {
   const int N = c->d.norb;
@@ -967,6 +1277,11 @@ void taylor_plan(const afqmc_ctx* c, int s, TaylorPlan* pl) {
       pl->nkgv = kb;
       pl->ct = ctp;
       pl->shm = (size_t)2 * ctp * 16 * (32 * kb + 8) * sizeof(uint16_t);
+    } else if (N <= 256 && n <= 64) {
+      pl->kind = TAYLOR_STREAM_BF16X3;
+      pl->ct = (n + 15) / 16;
+      pl->nthreads = bfs_threads(N, pl->ct);
+      pl->shm = bfs_shm(N, pl->ct);
     } else {
       const int ct = (n + 15) / 16 > 4 ? 4 : (n + 15) / 16;
       pl->kind = TAYLOR_STREAM_F32;
@@ -1007,6 +1322,13 @@ int launch_taylor(afqmc_ctx* c, int W, cplx* wup, cplx* wdn) {
           default: rc = launch_bf_ct<8>(c, W, n, wk, s); break;
         }
         (void)E;
+      } else if (N <= 256 && n <= 64) {
+        switch ((n + 15) / 16) {
+          case 1: rc = launch_bfs<1>(c, W, n, wk, s); break;
+          case 2: rc = launch_bfs<2>(c, W, n, wk, s); break;
+          case 3: rc = launch_bfs<3>(c, W, n, wk, s); break;
+          default: rc = launch_bfs<4>(c, W, n, wk, s); break;
+        }
       } else {
         switch ((n + 15) / 16) {
           case 1: rc = launch_stream_f32<1>(c, W, n, wk, E); break;
@@ -1112,6 +1434,16 @@ int launch_taylor_grouped(hipStream_t st, int kind, int nkgv, int ct, const Tayl
 #undef BF_CASES
 #undef BF_CASE
     return afqmc_set_error(AFQMC_ERR_ARG, "grouped taylor: k blocks %d, column tiles %d", nkgv, ct);
+  }
+  if (kind == TAYLOR_STREAM_BF16X3) {
+    static int lim[4] = {};
+    switch (ct) {
+      case 1: return launch_grouped_fn(taylor_bfs_grouped_kernel<1>, st, args, start, nprob, nblocks, nthreads, shm, &lim[0]);
+      case 2: return launch_grouped_fn(taylor_bfs_grouped_kernel<2>, st, args, start, nprob, nblocks, nthreads, shm, &lim[1]);
+      case 3: return launch_grouped_fn(taylor_bfs_grouped_kernel<3>, st, args, start, nprob, nblocks, nthreads, shm, &lim[2]);
+      case 4: return launch_grouped_fn(taylor_bfs_grouped_kernel<4>, st, args, start, nprob, nblocks, nthreads, shm, &lim[3]);
+      default: return afqmc_set_error(AFQMC_ERR_ARG, "grouped taylor: column tiles %d", ct);
+    }
   }
   if (kind == TAYLOR_STREAM_F32) {
     static int lim[4] = {};

@@ -1,5 +1,5 @@
-"""Mixed-precision step (split-bf16 Cholesky contractions, float32 Taylor products in the
-register-resident or the streaming kernel, float64 overlaps / determinants / QR / weights /
+"""Mixed-precision step (split-bf16 Cholesky contractions and Taylor products in the
+register-resident or the streaming kernels, float64 overlaps / determinants / QR / weights /
 energies) against the float64 oracle.
 
 Tolerances: one step agrees to 1e-4 relative (split-bf16 carries 16 mantissa bits);
@@ -26,9 +26,14 @@ CASES = [
     ("rhf", 85, (17, 17), 340, 8),
     ("rhf", 120, (24, 24), 480, 4),
     ("uhf", 36, (7, 5), 90, 24),
-    ("uhf", 136, (34, 30), 160, 8),  # float32 streaming Taylor (N > 128)
-    ("rhf", 96, (40, 40), 200, 8),  # float32 streaming Taylor (n > 32)
+    # streaming split-bf16 Taylor (N > 128 or n > 32): one and two row tiles per wave,
+    # every column-tile count, N not a multiple of 8 (masked row chunks), N = 256
+    ("uhf", 136, (34, 30), 160, 8),
+    ("rhf", 96, (40, 40), 200, 8),
     ("uhf", 200, (40, 40), 300, 4),
+    ("rhf", 150, (12, 12), 120, 6),
+    ("rhf", 250, (50, 50), 200, 4),
+    ("uhf", 256, (64, 57), 120, 3),
 ]
 
 

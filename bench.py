@@ -12,7 +12,7 @@ figure.
 
 Default mode runs the same trajectory twice from the same seed (same device fields,
 same SR draws): first in float64, then in the mixed mode (split-bf16 MFMA Cholesky
-contractions, float32 Taylor products, float64 overlaps / determinants / QR / weights /
+contractions and Taylor products, float64 overlaps / determinants / QR / weights /
 energies).  `value` is the mixed run's walker-steps/s; the line also carries the
 float64 rate, the speed-up and the largest block-energy difference |dE| between the
 two runs.  `--precision fp64|bf16x3` runs one mode only.
@@ -53,7 +53,7 @@ def phase_kernel(precision, phase, norb, nelec):
     if phase == "taylor":
         if precision == "fp64":
             return "taylor_reg_kernel" if reg else "taylor_kernel"
-        return "taylor_bf_kernel" if reg else "taylor_stream_f32_kernel"
+        return "taylor_bf_kernel" if reg else "taylor_bfs_kernel"
     if phase in ("vhs_gemm", "fb_gemm"):
         return "gemm_nt_f64_kernel" if precision == "fp64" else "gemm_nt_bf16x3_kernel"
     if phase in ("overlap", "green"):
@@ -155,9 +155,8 @@ def phase_peak(precision, phase, norb, nelec):
         return PEAK_TFLOPS["fp64"]
     if phase in ("vhs_gemm", "fb_gemm", "energy"):  # split-bf16 contractions
         return PEAK_TFLOPS["bf16"]
-    if phase == "taylor":  # split-bf16 register kernel, or the float32 streaming kernel
-        reg = norb <= 128 and max(nelec) <= 32
-        return PEAK_TFLOPS["bf16"] if reg else PEAK_TFLOPS["fp32"]
+    if phase == "taylor":  # split-bf16 register or streaming kernel
+        return PEAK_TFLOPS["bf16"]
     return PEAK_TFLOPS["fp64"]
 
 

@@ -77,8 +77,7 @@ void free_ctx(afqmc_ctx* c) {
                   c->rot_chol_bf[1][0], c->rot_chol_bf[1][1], c->green[0], c->green[1], c->fb,
                   c->xs, c->vhs, c->terms, c->dets, c->energy_x, c->energy_acc, c->walk_tmp,
                   c->sr_cum, c->sr_idx, c->xs_bf[0], c->xs_bf[1], c->green_bf[0][0],
-                  c->green_bf[0][1], c->green_bf[1][0], c->green_bf[1][1], c->vhs_f32,
-                  c->exp_h1_f32[0], c->exp_h1_f32[1]};
+                  c->green_bf[0][1], c->green_bf[1][0], c->green_bf[1][1], c->vhs_f32};
   if (c->prof_ev) {
     for (int i = 0; i < 2 * c->prof_cap; ++i) (void)hipEventDestroy(c->prof_ev[i]);
     delete[] c->prof_ev;
@@ -235,7 +234,7 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
     c->mixed = d.precision == AFQMC_PREC_BF16X3;
     c->Gp = (G + 31) / 32 * 32;
     c->Np32 = (N + 31) / 32 * 32;
-    c->taylor_f32 = c->mixed;  // float32 Taylor kernels (register or streaming form)
+    c->taylor_bf = c->mixed;  // split-bf16 Taylor kernels (register or streaming form)
     if (c->mixed) {
       const size_t nn = (size_t)N * N;
       for (int h = 0; h < 2; ++h) TRY(dalloc(c, &c->chol_t_bf[h], nn * c->Gp));
@@ -256,14 +255,11 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
         TRY(dalloc(c, &c->xs_bf[h], (size_t)W * 2 * c->Gp));
         TRY(zero_async(c->xs_bf[h], (size_t)W * 2 * c->Gp * sizeof(uint16_t), c->stream));
       }
-      if (c->taylor_f32) {
+      if (c->taylor_bf) {
         // 8 floats of tail padding: the streaming split-bf16 Taylor kernel reads V rows in
         // 8-float chunks, and the chunk that straddles the end of the last row is masked
         TRY(dalloc(c, &c->vhs_f32, (size_t)W * 2 * nn + 8));
-        std::vector<float> ef(nn);
         for (int s = 0; s < c->nspin; ++s) {
-          for (size_t e = 0; e < nn; ++e) ef[e] = (float)ham->exp_h1[(size_t)s * nn + e];
-          TRY(upload(c, &c->exp_h1_f32[s], ef.data(), nn));
           // exp_h1 - I as split-bf16 planes, rows padded to Np32 (bf16 Taylor kernel).  The
           // float64 matrix is staged in the VHS workspace (8 W N^2 bytes, unused until the
           // first step): a temporary allocation's hipFree would wait for the whole device.
@@ -291,7 +287,7 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
     }
     TRY(dalloc(c, &c->fb, (size_t)c->fb_cap * W * 2 * G));
     TRY(dalloc(c, &c->xs, (size_t)W * 2 * G));
-    if (!c->taylor_f32) TRY(dalloc(c, &c->vhs, (size_t)W * 2 * N * N));
+    if (!c->taylor_bf) TRY(dalloc(c, &c->vhs, (size_t)W * 2 * N * N));
     TRY(dalloc(c, &c->terms, (size_t)W * 2));
     TRY(dalloc(c, &c->dets, (size_t)W * 2));
     {

@@ -166,7 +166,7 @@ struct afqmc_ctx {
 
   // reduced-precision copies (mixed mode, AFQMC_PREC_BF16X3)
   int mixed;                 // 1 when the contractions run split-bf16
-  int taylor_f32;            // 1 when the Taylor kernels run in float32 (mixed mode)
+  int taylor_bf;             // 1 when the Taylor kernels run split-bf16 (mixed mode)
   int Gp;                    // nchol rounded up to 32 (bf16 plane leading dimension)
   int Np32;                  // norb rounded up to 32
   uint16_t* chol_t_bf[2];    // hi / lo planes of chol_t, (N*N, Gp)
@@ -174,7 +174,6 @@ struct afqmc_ctx {
   uint16_t* xs_bf[2];        // [hi/lo] (2W, Gp): i sqrt(dt) x~
   uint16_t* green_bf[2][2];  // [spin][hi/lo] (W, 2, n_sigma, Np32)
   float* vhs_f32;            // (W, 2, N*N)
-  float* exp_h1_f32[2];      // (N, N)
   uint16_t* emi_bf[2][2];    // [spin][hi/lo] (N, Np32): split-bf16 exp_h1 - I
 
   // workspace (device)
@@ -364,11 +363,9 @@ struct GemmArgs {
 
 enum TaylorKind {
   TAYLOR_REG_F64 = 0,
-  TAYLOR_REG_F32 = 1,
-  TAYLOR_STREAM_F32 = 2,
-  TAYLOR_STREAM_F64 = 3,
-  TAYLOR_REG_BF16X3 = 4,
-  TAYLOR_STREAM_BF16X3 = 5
+  TAYLOR_STREAM_F64 = 1,
+  TAYLOR_REG_BF16X3 = 2,
+  TAYLOR_STREAM_BF16X3 = 3
 };
 
 struct TaylorPlan {

@@ -268,7 +268,7 @@ int build(afqmc_group* g) {
         a.K = c->d.n_exp_terms - 1;
         a.rows_pad = ((a.N + 15) / 16) * 16;
         a.walkers = s == 0 ? m.wup : m.wdn;
-        a.E = g->mixed ? (const void*)c->exp_h1_f32[s] : (const void*)c->exp_h1[s];
+        a.E = c->exp_h1[s];
         if (pl.kind == TAYLOR_REG_BF16X3 || pl.kind == TAYLOR_STREAM_BF16X3) {
           a.E = c->emi_bf[s][0];
           a.E2 = c->emi_bf[s][1];

@@ -333,164 +333,6 @@ int launch_reg(afqmc_ctx* c, int W, int n, cplx* wk, const double* E) {
 }
 
 // ------------------------------------------------------------------------------------
-// Float32 form of the register-resident kernel for the mixed-precision step: V comes
-// from the bf16x3 VHS contraction in float32, E is float32, products run on
-// v_mfma_f32_16x16x4f32 (same A / B lane layout as the f64 MFMA; D element r of lane l
-// is row 4 (l / 16) + r).  The walker is read and written in float64.
-template <int CTP>
-struct RegTileF {
-  static constexpr int RS = (CTP == 2) ? 48 : (CTP == 4 ? 80 : 16 * CTP);  // 4 RS == 64 (mod 128)
-};
-
-template <int NKGV, int CTP>
-__device__ __forceinline__ void taylor_reg_f32_body(int N, int n, int n_terms_k,
-                                                    cplx* __restrict__ walkers,
-                                                    const float* __restrict__ expE,
-                                                    const float* __restrict__ vhs, const int w) {
-  constexpr int RS = RegTileF<CTP>::RS;
-  extern __shared__ float TF[];  // [2 Np][RS]
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int rb = wave * 16;
-  const int Np = (N + 3) & ~3;
-  const int nkg = Np >> 2;
-  const int nkgv = 2 * nkg;
-  const int s = lane >> 4, col = lane & 15;
-  const int p = rb + col;
-  const bool prow = p < N;
-  const int n2 = 2 * n;
-  cplx* phi = walkers + (int64_t)w * N * n;
-  const float* Vr = vhs + (int64_t)w * 2 * N * N;
-  const float* Vi = Vr + (int64_t)N * N;
-
-  float va[NKGV];
-#pragma unroll
-  for (int kg = 0; kg < NKGV; ++kg) {
-    const int q = 4 * kg + s;
-    float v = 0.f;
-    if (kg < nkgv && prow) {
-      if (q < Np) {
-        if (q < N) v = Vr[(int64_t)p * N + q];
-      } else if (q - Np < N) {
-        v = Vi[(int64_t)p * N + q - Np];
-      }
-    }
-    va[kg] = v;
-  }
-
-  for (int e = threadIdx.x; e < Np * RS; e += blockDim.x) {
-    const int q = e / RS, c = e - (e / RS) * RS;
-    float up = 0.f, lo = 0.f;
-    if (q < N && c < n2) {
-      const int j = c < n ? c : c - n;
-      const cplx z = phi[(int64_t)q * n + j];
-      up = (float)(c < n ? z.re : z.im);
-      lo = (float)(c < n ? -z.im : z.re);
-    }
-    TF[e] = up;
-    TF[Np * RS + e] = lo;
-  }
-  __syncthreads();
-
-  const float* bbase = TF + s * RS + col;
-
-  auto put = [&](const float4_t* v) {
-#pragma unroll
-    for (int ct = 0; ct < CTP; ++ct)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = rb + 4 * s + r;
-        const int cc = ct * 16 + col;
-        if (row < Np) {
-          TF[row * RS + cc] = v[ct][r];
-          if (cc < n)
-            TF[(Np + row) * RS + cc + n] = v[ct][r];
-          else if (cc < n2)
-            TF[(Np + row) * RS + cc - n] = -v[ct][r];
-        }
-      }
-  };
-
-  auto e_product = [&](float4_t* acc) {
-#pragma unroll
-    for (int ct = 0; ct < CTP; ++ct) acc[ct] = float4_t{0.f, 0.f, 0.f, 0.f};
-    float a = (prow && s < N) ? expE[(int64_t)p * N + s] : 0.f;
-    for (int kg = 0; kg < nkg; ++kg) {
-      const int qn = 4 * (kg + 1) + s;
-      const float an = (kg + 1 < nkg && prow && qn < N) ? expE[(int64_t)p * N + qn] : 0.f;
-#pragma unroll
-      for (int ct = 0; ct < CTP; ++ct)
-        acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bbase[kg * 4 * RS + ct * 16], acc[ct],
-                                                       0, 0, 0);
-      a = an;
-    }
-  };
-
-  float4_t t0[CTP], u[CTP];
-  e_product(t0);
-  __syncthreads();
-  put(t0);
-  __syncthreads();
-  for (int k = n_terms_k; k >= 1; --k) {
-#pragma unroll
-    for (int ct = 0; ct < CTP; ++ct) u[ct] = float4_t{0.f, 0.f, 0.f, 0.f};
-    float b[CTP];
-#pragma unroll
-    for (int ct = 0; ct < CTP; ++ct) b[ct] = bbase[ct * 16];
-#pragma unroll
-    for (int kg = 0; kg < NKGV; ++kg) {
-      if (kg < nkgv) {
-        float bn[CTP];
-#pragma unroll
-        for (int ct = 0; ct < CTP; ++ct)
-          bn[ct] = (kg + 1 < nkgv) ? bbase[(kg + 1) * 4 * RS + ct * 16] : 0.f;
-#pragma unroll
-        for (int ct = 0; ct < CTP; ++ct)
-          u[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(va[kg], b[ct], u[ct], 0, 0, 0);
-#pragma unroll
-        for (int ct = 0; ct < CTP; ++ct) b[ct] = bn[ct];
-      }
-      __builtin_amdgcn_sched_barrier(0);
-    }
-    __syncthreads();
-    const float rk = 1.0f / (float)k;
-#pragma unroll
-    for (int ct = 0; ct < CTP; ++ct)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) u[ct][r] = t0[ct][r] + u[ct][r] * rk;
-    put(u);
-    __syncthreads();
-  }
-  e_product(u);
-#pragma unroll
-  for (int ct = 0; ct < CTP; ++ct)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = rb + 4 * s + r;
-      const int cc = ct * 16 + col;
-      if (row < N && cc < n2) {
-        double* dst = reinterpret_cast<double*>(phi + (int64_t)row * n + (cc < n ? cc : cc - n));
-        dst[cc < n ? 0 : 1] = (double)u[ct][r];
-      }
-    }
-}
-
-template <int NKGV, int CTP>
-__global__ __launch_bounds__(512) void taylor_reg_f32_kernel(
-    int N, int n, int n_terms_k, cplx* __restrict__ walkers, const float* __restrict__ expE,
-    const float* __restrict__ vhs) {
-  taylor_reg_f32_body<NKGV, CTP>(N, n, n_terms_k, walkers, expE, vhs, blockIdx.x);
-}
-
-template <int NKGV, int CTP>
-__global__ __launch_bounds__(512) void taylor_reg_f32_grouped_kernel(
-    const TaylorArgs* __restrict__ args, const int* __restrict__ start, int nprob) {
-  const int p = group_of(start, nprob, blockIdx.x);
-  const TaylorArgs a = args[p];
-  taylor_reg_f32_body<NKGV, CTP>(a.N, a.n, a.K, a.walkers, static_cast<const float*>(a.E),
-                                 static_cast<const float*>(a.vhs), blockIdx.x - start[p]);
-}
-
-// ------------------------------------------------------------------------------------
 // Split-bf16 register-resident form for the mixed mode (N <= 128, n <= 32).  The complex
 // product is one real block product [ur | ui] = [Vr | Vi] [[yr, yi], [-yi, yr]] on
 // v_mfma_f32_16x16x32_bf16, with each operand x = hi + lo (bf16 planes) and the three
@@ -736,164 +578,6 @@ int launch_bf_ct(afqmc_ctx* c, int W, int n, cplx* wk, int s) {
     case 3: return launch_bf<KB, 3>(c, W, n, wk, s);
     default: return launch_bf<KB, 4>(c, W, n, wk, s);
   }
-}
-
-// ------------------------------------------------------------------------------------
-// Float32 streaming form for the mixed mode beyond the register kernel's range (N > 128
-// or n > 32).  One workgroup per walker of one spin; wave r owns output rows
-// [16 r, 16 r + 16) and all columns.  The current vector y (N x n complex) lives in LDS
-// as float32 re / im planes; V and E rows stream from global memory (L2) as the A
-// operand, one k-group ahead.  The complex product is four real products,
-//   ur = Vr yr - Vi yi,  ui = Vr yi + Vi yr,
-// on v_mfma_f32_16x16x4f32 (D element r of lane l is row 4 (l / 16) + r).  The series is
-// evaluated in Horner form y <- t0 + V y / k with t0 = E w kept in registers.
-template <int CT>
-struct StreamTileF {
-  // plane row stride in floats: 4 rows of a k-group fall on distinct banks (== 16 or 48
-  // mod 64) except for CT = 4, which keeps 64 so that N = 256 fits the LDS
-  static constexpr int LD = (CT == 2) ? 48 : (CT == 4 ? 64 : 16 * CT);
-};
-
-template <int CT>
-__device__ __forceinline__ void taylor_stream_f32_body(int N, int n, int n_terms_k,
-                                                       cplx* __restrict__ walkers,
-                                                       const float* __restrict__ expE,
-                                                       const float* __restrict__ vhs,
-                                                       const int w) {
-  constexpr int LD = StreamTileF<CT>::LD;
-  extern __shared__ float Y[];  // [2][Np][LD]: re plane, im plane
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int rb = wave * 16;
-  const int Np = (N + 3) & ~3;
-  const int nkg = Np >> 2;
-  const int s = lane >> 4, col = lane & 15;
-  const int p = rb + col;  // A row of this lane
-  const bool prow = p < N;
-  cplx* phi = walkers + (int64_t)w * N * n;
-  const float* Vr = vhs + (int64_t)w * 2 * N * N;
-  const float* Vi = Vr + (int64_t)N * N;
-  float* Yr = Y;
-  float* Yi = Y + Np * LD;
-
-  for (int e = threadIdx.x; e < Np * LD; e += blockDim.x) {
-    const int q = e / LD, j = e - (e / LD) * LD;
-    float re = 0.f, im = 0.f;
-    if (q < N && j < n) {
-      const cplx z = phi[(int64_t)q * n + j];
-      re = (float)z.re;
-      im = (float)z.im;
-    }
-    Yr[e] = re;
-    Yi[e] = im;
-  }
-  __syncthreads();
-  const float* yr_base = Yr + s * LD + col;
-  const float* yi_base = Yi + s * LD + col;
-
-  auto e_product = [&](float4_t* ar, float4_t* ai) {
-#pragma unroll
-    for (int ct = 0; ct < CT; ++ct) {
-      ar[ct] = float4_t{0.f, 0.f, 0.f, 0.f};
-      ai[ct] = float4_t{0.f, 0.f, 0.f, 0.f};
-    }
-    float a = (prow && s < N) ? expE[(int64_t)p * N + s] : 0.f;
-    for (int kg = 0; kg < nkg; ++kg) {
-      const int qn = 4 * (kg + 1) + s;
-      const float an = (kg + 1 < nkg && prow && qn < N) ? expE[(int64_t)p * N + qn] : 0.f;
-#pragma unroll
-      for (int ct = 0; ct < CT; ++ct) {
-        const float yr = yr_base[kg * 4 * LD + ct * 16];
-        const float yi = yi_base[kg * 4 * LD + ct * 16];
-        ar[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, yr, ar[ct], 0, 0, 0);
-        ai[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, yi, ai[ct], 0, 0, 0);
-      }
-      a = an;
-    }
-  };
-  auto v_product = [&](float4_t* ur, float4_t* ui) {
-#pragma unroll
-    for (int ct = 0; ct < CT; ++ct) {
-      ur[ct] = float4_t{0.f, 0.f, 0.f, 0.f};
-      ui[ct] = float4_t{0.f, 0.f, 0.f, 0.f};
-    }
-    const bool ok0 = prow && s < N;
-    float vr = ok0 ? Vr[(int64_t)p * N + s] : 0.f;
-    float vi = ok0 ? Vi[(int64_t)p * N + s] : 0.f;
-    for (int kg = 0; kg < nkg; ++kg) {
-      const int qn = 4 * (kg + 1) + s;
-      const bool okn = kg + 1 < nkg && prow && qn < N;
-      const float vrn = okn ? Vr[(int64_t)p * N + qn] : 0.f;
-      const float vin = okn ? Vi[(int64_t)p * N + qn] : 0.f;
-#pragma unroll
-      for (int ct = 0; ct < CT; ++ct) {
-        const float yr = yr_base[kg * 4 * LD + ct * 16];
-        const float yi = yi_base[kg * 4 * LD + ct * 16];
-        ur[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(vr, yr, ur[ct], 0, 0, 0);
-        ur[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(-vi, yi, ur[ct], 0, 0, 0);
-        ui[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(vr, yi, ui[ct], 0, 0, 0);
-        ui[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(vi, yr, ui[ct], 0, 0, 0);
-      }
-      vr = vrn;
-      vi = vin;
-    }
-  };
-  auto put = [&](const float4_t* vr, const float4_t* vi) {
-#pragma unroll
-    for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = rb + 4 * s + r;
-        if (row < Np) {
-          Yr[row * LD + ct * 16 + col] = vr[ct][r];
-          Yi[row * LD + ct * 16 + col] = vi[ct][r];
-        }
-      }
-  };
-
-  float4_t t0r[CT], t0i[CT], ur[CT], ui[CT];
-  e_product(t0r, t0i);
-  __syncthreads();
-  put(t0r, t0i);
-  __syncthreads();
-  for (int k = n_terms_k; k >= 1; --k) {
-    v_product(ur, ui);
-    __syncthreads();
-    const float rk = 1.0f / (float)k;
-#pragma unroll
-    for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        ur[ct][r] = t0r[ct][r] + ur[ct][r] * rk;
-        ui[ct][r] = t0i[ct][r] + ui[ct][r] * rk;
-      }
-    put(ur, ui);
-    __syncthreads();
-  }
-  e_product(ur, ui);
-#pragma unroll
-  for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = rb + 4 * s + r;
-      const int j = ct * 16 + col;
-      if (row < N && j < n) phi[(int64_t)row * n + j] = cplx{(double)ur[ct][r], (double)ui[ct][r]};
-    }
-}
-
-template <int CT>
-__global__ __launch_bounds__(1024) void taylor_stream_f32_kernel(
-    int N, int n, int n_terms_k, cplx* __restrict__ walkers, const float* __restrict__ expE,
-    const float* __restrict__ vhs) {
-  taylor_stream_f32_body<CT>(N, n, n_terms_k, walkers, expE, vhs, blockIdx.x);
-}
-
-template <int CT>
-__global__ __launch_bounds__(1024) void taylor_stream_f32_grouped_kernel(
-    const TaylorArgs* __restrict__ args, const int* __restrict__ start, int nprob) {
-  const int p = group_of(start, nprob, blockIdx.x);
-  const TaylorArgs a = args[p];
-  taylor_stream_f32_body<CT>(a.N, a.n, a.K, a.walkers, static_cast<const float*>(a.E),
-                             static_cast<const float*>(a.vhs), blockIdx.x - start[p]);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1206,46 +890,6 @@ int launch_bfs(afqmc_ctx* c, int W, int n, cplx* wk, int s) {
   return AFQMC_OK;
 }
 
-template <int CT>
-int launch_stream_f32(afqmc_ctx* c, int W, int n, cplx* wk, const float* E) {
-  const int N = c->d.norb;
-  const int nwaves = (N + 15) / 16;
-  const int Np = (N + 3) & ~3;
-  const size_t shm = (size_t)2 * Np * StreamTileF<CT>::LD * sizeof(float);
-  static int lds_limit = 0;
-  const int rc = ensure_dyn_lds((const void*)taylor_stream_f32_kernel<CT>, shm, &lds_limit);
-  if (rc) return rc;
-  hipLaunchKernelGGL((taylor_stream_f32_kernel<CT>), dim3(W), dim3(nwaves * 64), shm, c->stream,
-                     N, n, c->d.n_exp_terms - 1, wk, E, c->vhs_f32);
-  LAUNCH_CHECK();
-  return AFQMC_OK;
-}
-
-template <int NKGV, int CTP>
-int launch_reg_f32(afqmc_ctx* c, int W, int n, cplx* wk, const float* E) {
-  const int N = c->d.norb;
-  const int nwaves = (N + 15) / 16;
-  const int Np = (N + 3) & ~3;
-  const size_t shm = (size_t)2 * Np * RegTileF<CTP>::RS * sizeof(float);
-  static int lds_limit = 0;
-  const int rc = ensure_dyn_lds((const void*)taylor_reg_f32_kernel<NKGV, CTP>, shm, &lds_limit);
-  if (rc) return rc;
-  hipLaunchKernelGGL((taylor_reg_f32_kernel<NKGV, CTP>), dim3(W), dim3(nwaves * 64), shm,
-                     c->stream, N, n, c->d.n_exp_terms - 1, wk, E, c->vhs_f32);
-  LAUNCH_CHECK();
-  return AFQMC_OK;
-}
-
-template <int NKGV>
-int launch_reg_f32_ct(afqmc_ctx* c, int W, int n, cplx* wk, const float* E) {
-  switch ((2 * n + 15) / 16) {
-    case 1: return launch_reg_f32<NKGV, 1>(c, W, n, wk, E);
-    case 2: return launch_reg_f32<NKGV, 2>(c, W, n, wk, E);
-    case 3: return launch_reg_f32<NKGV, 3>(c, W, n, wk, E);
-    default: return launch_reg_f32<NKGV, 4>(c, W, n, wk, E);
-  }
-}
-
 template <int NKGV>
 int launch_reg_ct(afqmc_ctx* c, int W, int n, cplx* wk, const double* E) {
   const int ctp = (2 * n + 15) / 16;
@@ -1270,24 +914,18 @@ void taylor_plan(const afqmc_ctx* c, int s, TaylorPlan* pl) {
   pl->blocks_per_walker = 1;
   pl->nkgv = 0;
   pl->ct = 0;
-  if (c->taylor_f32) {
+  if (c->taylor_bf) {
     if (N <= 128 && n <= 32) {
       const int kb = bf_kblocks(N);
       pl->kind = TAYLOR_REG_BF16X3;
       pl->nkgv = kb;
       pl->ct = ctp;
       pl->shm = (size_t)2 * ctp * 16 * (32 * kb + 8) * sizeof(uint16_t);
-    } else if (N <= 256 && n <= 64) {
+    } else {  // N <= AFQMC_MAX_NORB = 256, n <= AFQMC_MAX_NOCC = 64
       pl->kind = TAYLOR_STREAM_BF16X3;
       pl->ct = (n + 15) / 16;
       pl->nthreads = bfs_threads(N, pl->ct);
       pl->shm = bfs_shm(N, pl->ct);
-    } else {
-      const int ct = (n + 15) / 16 > 4 ? 4 : (n + 15) / 16;
-      pl->kind = TAYLOR_STREAM_F32;
-      pl->ct = ct;
-      const int ld = (ct == 2) ? 48 : (ct == 4 ? 64 : 16 * ct);
-      pl->shm = (size_t)2 * Np * ld * sizeof(float);
     }
   } else if (N <= 128 && c->nocc[0] <= 32 && c->nocc[1] <= 32) {
     pl->kind = TAYLOR_REG_F64;
@@ -1304,11 +942,10 @@ void taylor_plan(const afqmc_ctx* c, int s, TaylorPlan* pl) {
 
 int launch_taylor(afqmc_ctx* c, int W, cplx* wup, cplx* wdn) {
   const int N = c->d.norb;
-  if (c->taylor_f32) {
+  if (c->taylor_bf) {
     for (int s = 0; s < c->nspin; ++s) {
       const int n = c->nocc[s];
       cplx* wk = (s == 0) ? wup : wdn;
-      const float* E = c->exp_h1_f32[s];
       int rc;
       if (N <= 128 && n <= 32) {
         switch (bf_kblocks(N)) {
@@ -1321,20 +958,12 @@ int launch_taylor(afqmc_ctx* c, int W, cplx* wup, cplx* wdn) {
           case 7: rc = launch_bf_ct<7>(c, W, n, wk, s); break;
           default: rc = launch_bf_ct<8>(c, W, n, wk, s); break;
         }
-        (void)E;
-      } else if (N <= 256 && n <= 64) {
+      } else {
         switch ((n + 15) / 16) {
           case 1: rc = launch_bfs<1>(c, W, n, wk, s); break;
           case 2: rc = launch_bfs<2>(c, W, n, wk, s); break;
           case 3: rc = launch_bfs<3>(c, W, n, wk, s); break;
           default: rc = launch_bfs<4>(c, W, n, wk, s); break;
-        }
-      } else {
-        switch ((n + 15) / 16) {
-          case 1: rc = launch_stream_f32<1>(c, W, n, wk, E); break;
-          case 2: rc = launch_stream_f32<2>(c, W, n, wk, E); break;
-          case 3: rc = launch_stream_f32<3>(c, W, n, wk, E); break;
-          default: rc = launch_stream_f32<4>(c, W, n, wk, E); break;
         }
       }
       if (rc) return rc;
@@ -1385,23 +1014,16 @@ int launch_grouped_fn(KernelFn fn, hipStream_t st, const TaylorArgs* args, const
 }
 
 template <int NKGV>
-int reg_grouped(bool f32, int ct, hipStream_t st, const TaylorArgs* args, const int* start,
-                int nprob, int nblocks, int nthreads, size_t shm) {
-  static int lim[2][4] = {};
+int reg_grouped(int ct, hipStream_t st, const TaylorArgs* args, const int* start, int nprob,
+                int nblocks, int nthreads, size_t shm) {
+  static int lim[4] = {};
   const int ci = ct < 1 ? 0 : (ct > 4 ? 3 : ct - 1);
-#define REG_CASE(C)                                                                          \
-  case C:                                                                                    \
-    return f32 ? launch_grouped_fn(taylor_reg_f32_grouped_kernel<NKGV, C>, st, args, start,  \
-                                   nprob, nblocks, nthreads, shm, &lim[1][ci])               \
-               : launch_grouped_fn(taylor_reg_grouped_kernel<NKGV, C>, st, args, start,      \
-                                   nprob, nblocks, nthreads, shm, &lim[0][ci]);
   switch (ct) {
-    REG_CASE(1)
-    REG_CASE(2)
-    REG_CASE(3)
-    REG_CASE(4)
+    case 1: return launch_grouped_fn(taylor_reg_grouped_kernel<NKGV, 1>, st, args, start, nprob, nblocks, nthreads, shm, &lim[ci]);
+    case 2: return launch_grouped_fn(taylor_reg_grouped_kernel<NKGV, 2>, st, args, start, nprob, nblocks, nthreads, shm, &lim[ci]);
+    case 3: return launch_grouped_fn(taylor_reg_grouped_kernel<NKGV, 3>, st, args, start, nprob, nblocks, nthreads, shm, &lim[ci]);
+    case 4: return launch_grouped_fn(taylor_reg_grouped_kernel<NKGV, 4>, st, args, start, nprob, nblocks, nthreads, shm, &lim[ci]);
   }
-#undef REG_CASE
   return afqmc_set_error(AFQMC_ERR_ARG, "grouped taylor: column tiles %d", ct);
 }
 
@@ -1411,13 +1033,12 @@ int reg_grouped(bool f32, int ct, hipStream_t st, const TaylorArgs* args, const 
 // column tiles ct) and one block size nthreads = 64 * ceil(N / 16).
 int launch_taylor_grouped(hipStream_t st, int kind, int nkgv, int ct, const TaylorArgs* args,
                           const int* start, int nprob, int nblocks, int nthreads, size_t shm) {
-  if (kind == TAYLOR_REG_F64 || kind == TAYLOR_REG_F32) {
-    const bool f32 = kind == TAYLOR_REG_F32;
+  if (kind == TAYLOR_REG_F64) {
     switch (nkgv) {
-      case 32: return reg_grouped<32>(f32, ct, st, args, start, nprob, nblocks, nthreads, shm);
-      case 48: return reg_grouped<48>(f32, ct, st, args, start, nprob, nblocks, nthreads, shm);
-      case 56: return reg_grouped<56>(f32, ct, st, args, start, nprob, nblocks, nthreads, shm);
-      case 64: return reg_grouped<64>(f32, ct, st, args, start, nprob, nblocks, nthreads, shm);
+      case 32: return reg_grouped<32>(ct, st, args, start, nprob, nblocks, nthreads, shm);
+      case 48: return reg_grouped<48>(ct, st, args, start, nprob, nblocks, nthreads, shm);
+      case 56: return reg_grouped<56>(ct, st, args, start, nprob, nblocks, nthreads, shm);
+      case 64: return reg_grouped<64>(ct, st, args, start, nprob, nblocks, nthreads, shm);
       default: return afqmc_set_error(AFQMC_ERR_ARG, "grouped taylor: nkgv %d", nkgv);
     }
   }
@@ -1443,15 +1064,6 @@ int launch_taylor_grouped(hipStream_t st, int kind, int nkgv, int ct, const Tayl
       case 3: return launch_grouped_fn(taylor_bfs_grouped_kernel<3>, st, args, start, nprob, nblocks, nthreads, shm, &lim[2]);
       case 4: return launch_grouped_fn(taylor_bfs_grouped_kernel<4>, st, args, start, nprob, nblocks, nthreads, shm, &lim[3]);
       default: return afqmc_set_error(AFQMC_ERR_ARG, "grouped taylor: column tiles %d", ct);
-    }
-  }
-  if (kind == TAYLOR_STREAM_F32) {
-    static int lim[4] = {};
-    switch (ct) {
-      case 1: return launch_grouped_fn(taylor_stream_f32_grouped_kernel<1>, st, args, start, nprob, nblocks, nthreads, shm, &lim[0]);
-      case 2: return launch_grouped_fn(taylor_stream_f32_grouped_kernel<2>, st, args, start, nprob, nblocks, nthreads, shm, &lim[1]);
-      case 3: return launch_grouped_fn(taylor_stream_f32_grouped_kernel<3>, st, args, start, nprob, nblocks, nthreads, shm, &lim[2]);
-      default: return launch_grouped_fn(taylor_stream_f32_grouped_kernel<4>, st, args, start, nprob, nblocks, nthreads, shm, &lim[3]);
     }
   }
   static int lim64 = 0;

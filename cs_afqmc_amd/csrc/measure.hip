@@ -105,12 +105,14 @@ __device__ __forceinline__ cplx gauss_jordan_registers(cplx* M, int n, int lane)
   return det;
 }
 
-// waves per walker and the LDS the cross-wave overlap reduction needs; the two-wave
-// kernels keep two waves per SIMD (at most 256 registers per lane, AGPRs included)
-constexpr int meas_waves(int t2) { return t2 <= 3 ? 2 : 1; }
-constexpr int meas_ti(int t2) { return t2 > 4 ? 1 : (t2 + 1) / 2; }
+// waves per walker (two for n <= 24, four beyond) and the LDS the two-wave kernel's
+// cross-wave overlap reduction needs
+constexpr int meas_waves(int t2) { return t2 <= 3 ? 2 : 4; }
+// waves per SIMD the register budget is compiled for (n > 48 needs more than 256)
+constexpr int meas_wpe(int t2) { return t2 >= 7 ? 1 : 2; }
+constexpr int meas_ti(int t2) { return t2 > 3 ? 1 : (t2 + 1) / 2; }
 constexpr size_t meas_red_bytes(int t2) {
-  return meas_waves(t2) > 1 ? (size_t)meas_ti(t2) * t2 * 4 * 64 * sizeof(double) : 0;
+  return t2 <= 3 ? (size_t)meas_ti(t2) * t2 * 4 * 64 * sizeof(double) : 0;
 }
 // walkers up to this size are staged in LDS by the two-wave kernel (four blocks per CU)
 constexpr size_t MEAS_STAGE_MAX = 40 * 1024;
@@ -341,17 +343,23 @@ __device__ __forceinline__ void measure_narrow_body(
   for (int tp = wv; tp < tp_n; tp += NW) tile(tp);
 }
 
-// n > 24: one wave per walker; the elimination runs in LDS.
+// n > 24: four waves per walker.  They take alternate row tiles of O (each over the
+// whole K = N), share the Gauss-Jordan elimination on [O | I] in LDS (every wave finds
+// the pivot from the same column with its own wave reduction, so no broadcast is needed;
+// the row updates are split by trailing column and by row half), and take alternate
+// column tiles of the Green's function.
 template <int T2, int TI, bool STAGE>
 __device__ __forceinline__ void measure_wide_body(
     int N, int n, const cplx* __restrict__ walkers, const double* __restrict__ trial,
     cplx* __restrict__ dets, int spin, double* __restrict__ green, uint16_t* __restrict__ ghi,
     uint16_t* __restrict__ glo, int ld16, const int w, const WeightsArgs& tail) {
+  constexpr int NW = 4;
   extern __shared__ double smem[];
   const int n2 = 2 * n;
   cplx* M = reinterpret_cast<cplx*>(smem);  // n x 2n augmented matrix
   cplx* fcol = M + n * n2;                  // pivot column copy (n)
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int s = lane >> 4, col = lane & 15;
   const cplx* phi = walkers + (int64_t)w * N * n;
   // walker operand: staged once into LDS with 16-byte loads (all in flight together)
@@ -359,24 +367,26 @@ __device__ __forceinline__ void measure_wide_body(
   const double* phid = reinterpret_cast<const double*>(phi);
   if (STAGE) {
     cplx* ph = fcol + n;
-    for (int e = lane; e < N * n; e += 64) ph[e] = phi[e];
+    for (int e = tid; e < N * n; e += 64 * NW) ph[e] = phi[e];
     phid = reinterpret_cast<const double*>(ph);
   }
   __syncthreads();
-  // row tiles of O in passes of TI
+  // row tiles of O, one per wave in turn
   const int ti_total = (n + 15) >> 4;
-  for (int tb = 0; tb < ti_total; tb += TI)
-    overlap_tiles<T2, TI, 1>(N, n, tb, trial, phid, M, nullptr, false, lane, 0);
-  for (int e = lane; e < n * n; e += 64) {
+  for (int tb = wv; tb < ti_total; tb += NW)
+    overlap_tiles<T2, 1, 1>(N, n, tb, trial, phid, M, nullptr, false, lane, 0);
+  for (int e = tid; e < n * n; e += 64 * NW) {
     const int i = e / n, j = e - (e / n) * n;
     M[i * n2 + n + j] = cplx{i == j ? 1.0 : 0.0, 0.0};
   }
   __syncthreads();
 
   // ---- Gauss-Jordan with partial pivoting: [O | I] -> [* | X], det from pivots.
+  // Trailing columns k + 1 + (tid % 128) (2n - k - 1 <= 127 of them), rows split in
+  // halves by tid / 128.
   cplx det = cplx{1.0, 0.0};
-  // Wider systems in LDS: lane c' owns column k + 1 + c' of the trailing matrix
-  // (2n - k - 1 <= 127 columns, at most two per lane).
+  const int rh = (n + 1) >> 1;
+  const int r_lo = (tid >> 7) * rh, r_hi = min(n, r_lo + rh);
   for (int k = 0; k < n; ++k) {
     int pr;
     {
@@ -394,13 +404,14 @@ __device__ __forceinline__ void measure_wide_body(
     const cplx pivot = M[pr * n2 + k];
     const double d2 = 1.0 / (pivot.re * pivot.re + pivot.im * pivot.im);
     const cplx inv = cplx{pivot.re * d2, -pivot.im * d2};
-    if (lane == 0) {
+    if (tid == 0) {
       det = cmul(det, pivot);
       if (pr != k) det = cplx{-det.re, -det.im};
     }
+    // (column k is not written again, so the exchange below needs no barrier first)
     // pivot column as seen after the row exchange; exchange + scale the pivot row
-    for (int r = lane; r < n; r += 64) fcol[r] = M[(r == pr ? k : r) * n2 + k];
-    for (int c = k + 1 + lane; c < n2; c += 64) {
+    for (int r = tid; r < n; r += 64 * NW) fcol[r] = M[(r == pr ? k : r) * n2 + k];
+    for (int c = k + 1 + tid; c < n2; c += 64 * NW) {
       double* mk = reinterpret_cast<double*>(&M[k * n2 + c]);
       double* mp = reinterpret_cast<double*>(&M[pr * n2 + c]);
       const double ar = mk[0], ai = mk[1], pr_ = mp[0], pi_ = mp[1];
@@ -412,21 +423,22 @@ __device__ __forceinline__ void measure_wide_body(
       }
     }
     __syncthreads();
-    for (int c = k + 1 + lane; c < n2; c += 64) {
+    const int c = k + 1 + (tid & 127);
+    if (c < n2) {
       const cplx m = M[k * n2 + c];
-      for (int r0 = 0; r0 < n; r0 += 8) {
+      for (int r0 = r_lo; r0 < r_hi; r0 += 8) {
         // eight independent loads, then the updates, then the stores
         cplx x[8], f[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-          const int r = r0 + u < n ? r0 + u : n - 1;
+          const int r = r0 + u < r_hi ? r0 + u : r_hi - 1;
           x[u] = M[r * n2 + c];
           f[u] = fcol[r];
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           const int r = r0 + u;
-          if (r < n && r != k) {
+          if (r < r_hi && r != k) {
             cplx y = x[u];
             y.re -= f[u].re * m.re - f[u].im * m.im;
             y.im -= f[u].re * m.im + f[u].im * m.re;
@@ -437,7 +449,7 @@ __device__ __forceinline__ void measure_wide_body(
     }
     __syncthreads();
   }
-  if (lane == 0) finish_walker(det, dets, w, spin, tail);
+  if (tid == 0) finish_walker(det, dets, w, spin, tail);
   if (!(green || ghi)) return;
 
   // ---- [Gr ; Gi] = [[Xr^T, -Xi^T], [Xi^T, Xr^T]] [wr^T ; wi^T], K = 2n, columns p
@@ -446,7 +458,7 @@ __device__ __forceinline__ void measure_wide_body(
   uint16_t* gl = ghi ? glo + (int64_t)w * n2 * ld16 : nullptr;
   const int tp_n = (N + 15) >> 4;
     const int nkg = (n2 + 3) >> 2;
-    for (int tp = 0; tp < tp_n; ++tp) {
+    for (int tp = wv; tp < tp_n; tp += NW) {
       const int p = tp * 16 + col;  // B column = orbital p
       double4_t acc[T2];
 #pragma unroll
@@ -491,7 +503,7 @@ __device__ __forceinline__ void measure_wide_body(
 }
 
 template <int T2, int TI, bool STAGE>
-__global__ __launch_bounds__(64 * meas_waves(T2), meas_waves(T2)) void measure_kernel(
+__global__ __launch_bounds__(64 * meas_waves(T2), meas_wpe(T2)) void measure_kernel(
     int N, int n, const cplx* __restrict__ walkers, const double* __restrict__ trial,
     cplx* __restrict__ dets, int spin, double* __restrict__ green, uint16_t* __restrict__ ghi,
     uint16_t* __restrict__ glo, int ld16, WeightsArgs tail) {
@@ -504,7 +516,7 @@ __global__ __launch_bounds__(64 * meas_waves(T2), meas_waves(T2)) void measure_k
 }
 
 template <int T2, int TI, bool STAGE>
-__global__ __launch_bounds__(64 * meas_waves(T2), meas_waves(T2)) void measure_grouped_kernel(const MeasureArgs* __restrict__ args,
+__global__ __launch_bounds__(64 * meas_waves(T2), meas_wpe(T2)) void measure_grouped_kernel(const MeasureArgs* __restrict__ args,
                                                              const int* __restrict__ start,
                                                              int nprob) {
   const int p = group_of(start, nprob, blockIdx.x);
@@ -614,7 +626,7 @@ void measure_plan(const afqmc_ctx* c, int s, int* t2, bool* stage, size_t* shm) 
   const int t = (2 * n + 15) / 16;
   *t2 = t > 8 ? 8 : t;
   const size_t walker = (size_t)N * n * sizeof(cplx);
-  if (meas_waves(*t2) > 1) {  // two-wave kernel: [O | I] and partials alias the staged walker
+  if (*t2 <= 3) {  // two-wave kernel: [O | I] and partials alias the staged walker
     const size_t m_red = (size_t)n * 2 * n * sizeof(cplx) + meas_red_bytes(*t2);
     *stage = walker <= MEAS_STAGE_MAX;
     *shm = *stage ? std::max(walker, m_red) : m_red;

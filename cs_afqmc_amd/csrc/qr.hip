@@ -13,7 +13,8 @@
 namespace {
 
 constexpr int QR_NT = 256;
-constexpr size_t QR_LDS_LIMIT = 96 * 1024;
+// dynamic LDS for the walker copy: 160 KiB per workgroup less the static arrays below
+constexpr size_t QR_LDS_LIMIT = 156 * 1024;
 
 __device__ inline double dlapy3(double x, double y, double z) {
   const double xa = fabs(x), ya = fabs(y), za = fabs(z);

@@ -199,6 +199,15 @@ def test_orthonormalize(case):
         assert rel(got[0], ref[0]) < 1e-12 and rel(got[1], ref[1]) < 1e-12
 
 
+@pytest.mark.parametrize("wt,norb,nelec", [("rhf", 200, (40, 40)), ("uhf", 256, (64, 57))],
+                         ids=["rhf-N200-n40", "uhf-N256-n64.57"])
+def test_orthonormalize_large_walkers(gpu, wt, norb, nelec):
+    """QR of walkers copied to LDS (up to 156 KiB: N = 200, n = 40 is 125 KiB) and of
+    walkers factorised in place in global memory (N = 256, n = 64: 256 KiB)."""
+    case = Case(wt, norb, nelec, 8, 4, dev=gpu)
+    test_orthonormalize(case)
+
+
 @pytest.mark.parametrize("global_order", [False, True], ids=["local", "global"])
 def test_sr_indices_bit_exact(case, global_order):
     rng = np.random.default_rng(9)

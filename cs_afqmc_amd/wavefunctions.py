@@ -18,7 +18,7 @@ from typing import Tuple
 import numpy as np
 import torch
 
-from .device import DeviceStep, device_chol
+from .device import DeviceStep, device_chol, drop_cached
 
 
 def _as_np(x):
@@ -47,6 +47,7 @@ def _device_for(trial, ham_data, wave_data, walker_type, n_walkers):
             walker_type, norb, trial.nelec, n_walkers, 0.01, 2, _as_np(ham_data["h0"]), 0.0,
             chol, wave_data["mo_coeff"], ham_data["rot_h1"], ham_data["rot_chol"],
             np.zeros(chol.shape[0], dtype=np.complex128), eye)
+        drop_cached(ham_data)  # the context holds its own copy of the vectors
         ham_data["_afqmc_measure_device"] = dev
         return dev
     # overlap only: the Hamiltonian arrays are placeholders the overlap kernel never reads

@@ -64,16 +64,18 @@ def phase_kernel(precision, phase, norb, nelec):
             "sr": "sr_index_kernel", "rng": "normal_kernel"}.get(phase, phase)
 
 
-def flops_per_walker(norb, nelec, nchol, n_exp_terms, walker_type):
+def flops_per_walker(norb, nelec, nchol, n_exp_terms, walker_type, nv=None):
     """Algorithmic real flops per walker per phase (real x complex MAC = 4 flop,
     complex x complex MAC = 8 flop).  The measurement kernel ("green" at a block's start,
     "overlap" after each step) forms C^T w (4 N n^2), the inverse by elimination (8 n^3)
-    and the Green's function w O^-1 (8 N n^2)."""
+    and the Green's function w O^-1 (8 N n^2).  nv: entries of V the VHS product forms
+    (N^2, or N(N+1)/2 when V is symmetric and kept as its upper triangle)."""
     K = n_exp_terms - 1
     N, G = norb, nchol
+    nv = N * N if nv is None else nv
     ns = [nelec[0]] if walker_type == "rhf" else [nelec[0], nelec[1]]
     f = {
-        "vhs_gemm": 4.0 * G * N * N,
+        "vhs_gemm": 4.0 * G * nv,
         "fb_gemm": sum(4.0 * G * N * n for n in ns),
         "green": sum(12.0 * N * n * n + 8.0 * n**3 for n in ns),
         "taylor": sum((8.0 + 8.0 * K) * N * N * n for n in ns),
@@ -87,22 +89,23 @@ def flops_per_walker(norb, nelec, nchol, n_exp_terms, walker_type):
     return f
 
 
-def bytes_per_walker(norb, nelec, nchol, walker_type, precision, W):
+def bytes_per_walker(norb, nelec, nchol, walker_type, precision, W, nv=None):
     """Algorithmic HBM bytes per walker per phase: each kernel's inputs read once and
     outputs written once in the layouts the library keeps (complex128 walkers; float64
     or split-bf16 Green's function / shifted fields / Cholesky planes; float64 or float32
     VHS), with operands shared by all walkers (Cholesky vectors, trial, one-body factor)
-    counted once per launch and divided by W."""
+    counted once per launch and divided by W.  nv: stored entries of one V plane."""
     N, G = norb, nchol
+    nv = N * N if nv is None else nv
     ns = [nelec[0]] if walker_type == "rhf" else [nelec[0], nelec[1]]
     nsum, nspin = sum(ns), len(ns)
     mixed = precision != "fp64"
     Np32, Gp = -(-N // 32) * 32, -(-G // 32) * 32
     walker = 16.0 * N * nsum
     green = 8.0 * nsum * Np32 if mixed else 16.0 * nsum * N     # hi/lo bf16 or float64
-    vhs = 8.0 * N * N if mixed else 16.0 * N * N                 # float32 or float64 re/im
+    vhs = 8.0 * nv if mixed else 16.0 * nv                       # float32 or float64 re/im
     xs = 8.0 * Gp if mixed else 16.0 * G
-    chol = (4.0 * N * N * Gp if mixed else 8.0 * N * N * G) / W
+    chol = (4.0 * nv * Gp if mixed else 8.0 * nv * G) / W
     rot = (4.0 * G * nsum * Np32 if mixed else 8.0 * G * nsum * N) / W
     one_body = nspin * (4.0 * N * Np32 if mixed else 8.0 * N * N) / W
     trial = 8.0 * N * nsum / W
@@ -120,12 +123,13 @@ def bytes_per_walker(norb, nelec, nchol, walker_type, precision, W):
     }
 
 
-def roofline_of(phase, precision, norb, nelec, nchol, n_exp_terms, walker_type, W, avg_s):
+def roofline_of(phase, precision, norb, nelec, nchol, n_exp_terms, walker_type, W, avg_s,
+                nv=None):
     """Roofline record of one phase's kernel: the bound is whichever of MFMA time and
     HBM time at published peaks is longer for the algorithmic flops and bytes of one
     launch; achieved = those flops (or bytes) / the measured average launch time."""
-    fl = flops_per_walker(norb, nelec, nchol, n_exp_terms, walker_type).get(phase)
-    by = bytes_per_walker(norb, nelec, nchol, walker_type, precision, W).get(phase)
+    fl = flops_per_walker(norb, nelec, nchol, n_exp_terms, walker_type, nv).get(phase)
+    by = bytes_per_walker(norb, nelec, nchol, walker_type, precision, W, nv).get(phase)
     if fl is None or by is None:
         return None
     flop, byt = fl * W, by * W
@@ -345,7 +349,8 @@ def run(args, precision):
 
     # phase breakdown: a short pass with an event pair around every phase (the pairs
     # cost device time, so the timed region below records only the dominant phase)
-    fl = flops_per_walker(norb, nelec, nchol, prop.n_exp_terms, walker_type)
+    nv = dev.vhs_entries()  # V kept whole or as its upper triangle
+    fl = flops_per_walker(norb, nelec, nchol, prop.n_exp_terms, walker_type, nv)
     dev.prof_collect()  # drop anything recorded before
     dev.prof_enable(True)
     for _ in range(min(args.steps, PROFILE_STEPS)):
@@ -382,12 +387,12 @@ def run(args, precision):
     for p, (ms, n) in phases.items():
         if n:
             r = roofline_of(p, precision, norb, nelec, nchol, prop.n_exp_terms, walker_type, W,
-                            ms / n / 1e3)
+                            ms / n / 1e3, nv)
             if r is not None:
                 rooflines[p] = r
     ms_dom, n_dom = timed[dom]
     roofline = roofline_of(dom, precision, norb, nelec, nchol, prop.n_exp_terms, walker_type,
-                           W, ms_dom / n_dom / 1e3)
+                           W, ms_dom / n_dom / 1e3, nv)
     roofline["launches_timed"] = n_dom
     roofline["traffic"] = traffic_from_profiles(roofline["kernel"], args.workload, precision)
     step_flops = sum(v for k, v in fl.items() if k not in ("energy", "qr")) + (

@@ -21,6 +21,7 @@ EXPORTS = (
     "afqmc_ctx_destroy",
     "afqmc_set_stream",
     "afqmc_ctx_device_bytes",
+    "afqmc_ctx_vhs_entries",
     "afqmc_last_error",
     "afqmc_propagate",
     "afqmc_overlap",
@@ -95,6 +96,8 @@ def _declare(lib):
     lib.afqmc_set_stream.argtypes = [_vp, _vp]
     lib.afqmc_ctx_device_bytes.argtypes = [_vp]
     lib.afqmc_ctx_device_bytes.restype = ctypes.c_int64
+    lib.afqmc_ctx_vhs_entries.argtypes = [_vp]
+    lib.afqmc_ctx_vhs_entries.restype = ctypes.c_int64
     lib.afqmc_last_error.argtypes = []
     lib.afqmc_last_error.restype = ctypes.c_char_p
     lib.afqmc_propagate.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32]

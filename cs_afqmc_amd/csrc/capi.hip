@@ -1,5 +1,6 @@
 // C ABI of the gfx950 AFQMC step library: context lifetime, error reporting and the
 // entry points declared in include/cs_afqmc_amd.h.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -77,7 +78,8 @@ void free_ctx(afqmc_ctx* c) {
                   c->rot_chol_bf[1][0], c->rot_chol_bf[1][1], c->green[0], c->green[1], c->fb,
                   c->xs, c->vhs, c->terms, c->dets, c->energy_x, c->energy_acc, c->walk_tmp,
                   c->sr_cum, c->sr_idx, c->xs_bf[0], c->xs_bf[1], c->green_bf[0][0],
-                  c->green_bf[0][1], c->green_bf[1][0], c->green_bf[1][1], c->vhs_f32};
+                  c->green_bf[0][1], c->green_bf[1][0], c->green_bf[1][1], c->vhs_f32,
+                  c->sym_stat};
   if (c->prof_ev) {
     for (int i = 0; i < 2 * c->prof_cap; ++i) (void)hipEventDestroy(c->prof_ev[i]);
     delete[] c->prof_ev;
@@ -232,14 +234,36 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
 
     // mixed mode: split-bf16 planes of the contraction operands (zero padded to 32)
     c->mixed = d.precision == AFQMC_PREC_BF16X3;
+    c->vhs_nv = N * N;
     c->Gp = (G + 31) / 32 * 32;
     c->Np32 = (N + 31) / 32 * 32;
     c->taylor_bf = c->mixed;  // split-bf16 Taylor kernels (register or streaming form)
     if (c->mixed) {
       const size_t nn = (size_t)N * N;
-      for (int h = 0; h < 2; ++h) TRY(dalloc(c, &c->chol_t_bf[h], nn * c->Gp));
-      TRY(launch_split_bf16(c->stream, (int)nn, G, c->chol_t, G, c->chol_t_bf[0], c->chol_t_bf[1],
-                            c->Gp));
+      // V kept as its upper triangle (half the VHS product, its output and the Taylor
+      // kernel's reads) when the Cholesky vectors are symmetric, as they are for real
+      // orbitals, and every spin runs the register-resident Taylor kernel
+      bool reg_all = N <= 128;
+      for (int s = 0; s < c->nspin; ++s) reg_all = reg_all && c->nocc[s] <= 32;
+      if (reg_all) {
+        unsigned long long st[2] = {0ull, 0ull};
+        TRY(dalloc(c, &c->sym_stat, 2));
+        TRY(zero_async(c->sym_stat, 2 * sizeof(unsigned long long), c->stream));
+        TRY(launch_sym_check(c->stream, N, G, c->chol_t, c->sym_stat));
+        TRY(copy_async(st, c->sym_stat, sizeof(st), c->stream));
+        TRY(sync_stream(c->stream));
+        double asym, amax;
+        std::memcpy(&asym, &st[0], sizeof(double));
+        std::memcpy(&amax, &st[1], sizeof(double));
+        if (asym <= 1e-12 * amax) c->vhs_nv = N * (N + 1) / 2;
+      }
+      for (int h = 0; h < 2; ++h) TRY(dalloc(c, &c->chol_t_bf[h], (size_t)c->vhs_nv * c->Gp));
+      if (c->vhs_nv != N * N)
+        TRY(launch_split_bf16_sym(c->stream, N, G, c->chol_t, c->chol_t_bf[0], c->chol_t_bf[1],
+                                  c->Gp));
+      else
+        TRY(launch_split_bf16(c->stream, (int)nn, G, c->chol_t, G, c->chol_t_bf[0],
+                              c->chol_t_bf[1], c->Gp));
       for (int s = 0; s < c->nspin; ++s) {
         const size_t rows = (size_t)G * c->nocc[s];
         for (int h = 0; h < 2; ++h) {
@@ -257,8 +281,9 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
       }
       if (c->taylor_bf) {
         // 8 floats of tail padding: the streaming split-bf16 Taylor kernel reads V rows in
-        // 8-float chunks, and the chunk that straddles the end of the last row is masked
-        TRY(dalloc(c, &c->vhs_f32, (size_t)W * 2 * nn + 8));
+        // 8-float chunks, and the chunk that straddles the end of the last row is masked;
+        // at least N^2 doubles, which stage exp_h1 - I below
+        TRY(dalloc(c, &c->vhs_f32, std::max((size_t)W * 2 * c->vhs_nv + 8, 2 * nn)));
         for (int s = 0; s < c->nspin; ++s) {
           // exp_h1 - I as split-bf16 planes, rows padded to Np32 (bf16 Taylor kernel).  The
           // float64 matrix is staged in the VHS workspace (8 W N^2 bytes, unused until the
@@ -328,6 +353,10 @@ extern "C" int64_t afqmc_ctx_device_bytes(const afqmc_ctx* ctx) {
   return ctx ? ctx->device_bytes : 0;
 }
 
+extern "C" int64_t afqmc_ctx_vhs_entries(const afqmc_ctx* ctx) {
+  return ctx ? ctx->vhs_nv : 0;
+}
+
 extern "C" int afqmc_propagate(afqmc_ctx* c, int32_t W, void* walkers_up, void* walkers_dn,
                                double* weights, void* overlaps, const double* fields,
                                double* scalars, void* force_bias_out, int32_t flags) {
@@ -358,12 +387,13 @@ extern "C" int afqmc_propagate(afqmc_ctx* c, int32_t W, void* walkers_up, void* 
     if ((rc = launch_fields(c, W, fields, static_cast<cplx*>(force_bias_out)))) return rc;
   }
   {
-    // VHS[w] = sum_g (i sqrt(dt) x~_wg) L_g  : (2W, G) x (N*N, G)^T
+    // VHS[w] = sum_g (i sqrt(dt) x~_wg) L_g  : (2W, G) x (vhs_nv, G)^T, all N*N entries
+    // or the upper triangle of the (then symmetric) V
     PhaseTimer t(c, AFQMC_PH_VHS_GEMM);
     if (c->mixed)
-      rc = launch_gemm_nt_bf16x3_f32(c->stream, 2 * W, N * N, c->Gp, 1.0f, c->xs_bf[0],
+      rc = launch_gemm_nt_bf16x3_f32(c->stream, 2 * W, c->vhs_nv, c->Gp, 1.0f, c->xs_bf[0],
                                      c->xs_bf[1], c->Gp, c->chol_t_bf[0], c->chol_t_bf[1], c->Gp,
-                                     c->vhs_f32, N * N, 1, 0);
+                                     c->vhs_f32, c->vhs_nv, 1, 0);
     else
       rc = launch_gemm_nt_f64(c->stream, 2 * W, N * N, c->d.nchol, 1.0, c->xs, c->d.nchol,
                               c->chol_t, c->d.nchol, 0.0, c->vhs, N * N);

@@ -169,11 +169,14 @@ struct afqmc_ctx {
   int taylor_bf;             // 1 when the Taylor kernels run split-bf16 (mixed mode)
   int Gp;                    // nchol rounded up to 32 (bf16 plane leading dimension)
   int Np32;                  // norb rounded up to 32
-  uint16_t* chol_t_bf[2];    // hi / lo planes of chol_t, (N*N, Gp)
+  int vhs_nv;                // stored entries of one V plane: N*N, or N(N+1)/2 when V is
+                             // kept as its upper triangle (symmetric Cholesky vectors)
+  uint16_t* chol_t_bf[2];    // hi / lo planes of chol_t, (vhs_nv, Gp)
   uint16_t* rot_chol_bf[2][2];  // [spin][hi/lo], (G, n_sigma, Np32)
   uint16_t* xs_bf[2];        // [hi/lo] (2W, Gp): i sqrt(dt) x~
   uint16_t* green_bf[2][2];  // [spin][hi/lo] (W, 2, n_sigma, Np32)
-  float* vhs_f32;            // (W, 2, N*N)
+  float* vhs_f32;            // (W, 2, vhs_nv)
+  unsigned long long* sym_stat;  // (2) Cholesky symmetry check at creation
   uint16_t* emi_bf[2][2];    // [spin][hi/lo] (N, Np32): split-bf16 exp_h1 - I
 
   // workspace (device)
@@ -278,6 +281,9 @@ int launch_gemm_nt_bf16x3_f64(hipStream_t s, int M, int N, int K, float alpha,
                               int64_t ldc, int splits, int64_t c_split_stride);
 int launch_split_bf16(hipStream_t s, int rows, int cols, const double* x, int64_t ld_src,
                       uint16_t* hi, uint16_t* lo, int64_t ld_dst);
+int launch_sym_check(hipStream_t s, int N, int G, const double* chol_t, unsigned long long* out);
+int launch_split_bf16_sym(hipStream_t s, int N, int G, const double* chol_t, uint16_t* hi,
+                          uint16_t* lo, int ld_dst);
 // measure.hip: green_f64 -> c->green (float64 split layout); green_bf -> c->green_bf
 // With tail, the last spin's launch also applies the per-walker weight update (thread 0 of
 // each walker's block); the population-control shift then comes from launch_pcs.
@@ -345,6 +351,7 @@ struct FieldsArgs {
 };
 struct TaylorArgs {
   int N, n, K, rows_pad;
+  int nv;           // stored entries of one V plane (N*N, or N(N+1)/2 packed)
   cplx* walkers;
   const void* E;    // float64 / float32 exp_h1, or the hi plane of exp_h1 - I (bf16 kernel)
   const void* E2;   // lo plane of exp_h1 - I (bf16 kernel)

@@ -48,6 +48,53 @@ __global__ void split_bf16_kernel(int rows, int cols, const double* __restrict__
   }
 }
 
+// Symmetry of the Cholesky vectors from their transposed copy chol_t (N*N rows, G):
+// out[0] = max |L_g[p][k] - L_g[k][p]| and out[1] = max |L_g[p][k]| as the bit
+// patterns of non-negative doubles (ordered like the values, so atomicMax applies).
+__global__ void sym_check_kernel(int N, int G, const double* __restrict__ chol_t,
+                                 unsigned long long* __restrict__ out) {
+  double asym = 0.0, amax = 0.0;
+  const int64_t total = (int64_t)N * N * G;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t pk = e / G, g = e - pk * G;
+    const int p = (int)(pk / N), k = (int)(pk - (int64_t)p * N);
+    const double a = chol_t[e], b = chol_t[((int64_t)k * N + p) * G + g];
+    asym = fmax(asym, fabs(a - b));
+    amax = fmax(amax, fabs(a));
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    asym = fmax(asym, __shfl_xor(asym, off, 64));
+    amax = fmax(amax, __shfl_xor(amax, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&out[0], (unsigned long long)__double_as_longlong(asym));
+    atomicMax(&out[1], (unsigned long long)__double_as_longlong(amax));
+  }
+}
+
+// Upper triangle (p <= k, row-major: row p holds k = p .. N-1) of symmetric Cholesky
+// vectors -> hi / lo bf16 planes, one packed row per (p, k) pair, Gp columns (zero
+// padded): the B operand of the VHS product when it forms the packed triangle of V.
+__global__ void split_bf16_sym_kernel(int N, int G, const double* __restrict__ chol_t,
+                                      uint16_t* __restrict__ hi, uint16_t* __restrict__ lo,
+                                      int ld_dst) {
+  const int p = blockIdx.x / N, k = blockIdx.x - (blockIdx.x / N) * N;
+  if (k < p) return;
+  const int64_t t = (int64_t)p * N - (int64_t)p * (p - 1) / 2 + (k - p);
+  const double* src = chol_t + ((int64_t)p * N + k) * G;
+  for (int g = threadIdx.x; g < ld_dst; g += blockDim.x) {
+    uint16_t h = 0, l = 0;
+    if (g < G) {
+      const double v = src[g];
+      h = f32_to_bf16((float)v);
+      l = f32_to_bf16((float)(v - (double)bf16_to_f32(h)));
+    }
+    hi[t * ld_dst + g] = h;
+    lo[t * ld_dst + g] = l;
+  }
+}
+
 template <int BM, int BN, typename OutT>
 __device__ __forceinline__ void gemm_nt_bf16x3_body(
     int M, int N, int K, float alpha, const uint16_t* __restrict__ Ah,
@@ -232,6 +279,23 @@ int launch_split_bf16(hipStream_t s, int rows, int cols, const double* x, int64_
   const int64_t blocks = (total + 255) / 256 < 65536 ? (total + 255) / 256 : 65536;
   hipLaunchKernelGGL(split_bf16_kernel, dim3((unsigned)blocks), dim3(256), 0, s, rows, cols, x,
                      ld_src, hi, lo, ld_dst);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+int launch_sym_check(hipStream_t s, int N, int G, const double* chol_t,
+                     unsigned long long* out) {
+  const int64_t total = (int64_t)N * N * G;
+  const int64_t blocks = (total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096;
+  hipLaunchKernelGGL(sym_check_kernel, dim3((unsigned)blocks), dim3(256), 0, s, N, G, chol_t, out);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+int launch_split_bf16_sym(hipStream_t s, int N, int G, const double* chol_t, uint16_t* hi,
+                          uint16_t* lo, int ld_dst) {
+  hipLaunchKernelGGL(split_bf16_sym_kernel, dim3((unsigned)(N * N)), dim3(256), 0, s, N, G,
+                     chol_t, hi, lo, ld_dst);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }

@@ -222,9 +222,9 @@ int build(afqmc_group* g) {
       const int W = c->d.n_walkers, N = c->d.norb;
       GemmArgs a{};
       a.M = 2 * W;
-      a.N = N * N;
+      a.N = g->mixed ? c->vhs_nv : N * N;
       a.alpha = 1.0f;
-      a.ldc = (int64_t)N * N;
+      a.ldc = a.N;
       if (g->mixed) {
         a.K = c->Gp;
         a.A = c->xs_bf[0];
@@ -267,6 +267,7 @@ int build(afqmc_group* g) {
         a.n = c->nocc[s];
         a.K = c->d.n_exp_terms - 1;
         a.rows_pad = ((a.N + 15) / 16) * 16;
+        a.nv = g->mixed ? c->vhs_nv : a.N * a.N;
         a.walkers = s == 0 ? m.wup : m.wdn;
         a.E = c->exp_h1[s];
         if (pl.kind == TAYLOR_REG_BF16X3 || pl.kind == TAYLOR_STREAM_BF16X3) {

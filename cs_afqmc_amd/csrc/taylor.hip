@@ -374,7 +374,8 @@ __device__ __forceinline__ void taylor_bf_body(int N, int n, int n_terms_k,
                                                cplx* __restrict__ walkers,
                                                const uint16_t* __restrict__ emi_hi,
                                                const uint16_t* __restrict__ emi_lo,
-                                               const float* __restrict__ vhs, const int w) {
+                                               const float* __restrict__ vhs, int nv,
+                                               const int w) {
   constexpr int LDK = BfTile<KB>::LDK;
   constexpr int KH = BfTile<KB>::KH;
   extern __shared__ uint16_t YB[];  // [2 planes][CTP * 16 columns][LDK]
@@ -389,14 +390,16 @@ __device__ __forceinline__ void taylor_bf_body(int N, int n, int n_terms_k,
   const bool prow = p < N;
   const int n2 = 2 * n;
   cplx* phi = walkers + (int64_t)w * N * n;
-  const float* Vr = vhs + (int64_t)w * 2 * N * N;
-  const float* Vi = Vr + (int64_t)N * N;
+  const float* Vr = vhs + (int64_t)w * 2 * nv;
+  const float* Vi = Vr + nv;
+  // V planes hold all N^2 entries row-major, or (nv = N(N+1)/2, symmetric V) the upper
+  // triangle row by row: entry (a, b), a <= b, at a N - a (a - 1) / 2 + b - a
+  const bool packed = nv != N * N;
 
-  // V rows in registers, split once; loads are clamped into the row and masked after,
+  // V rows in registers, split once; loads are clamped into the matrix and masked after,
   // so they issue back to back without branches
   bf16x8_t vh[KB], vl[KB];
-  const float* vrow_r = Vr + (int64_t)(prow ? p : 0) * N;
-  const float* vrow_i = Vi + (int64_t)(prow ? p : 0) * N;
+  const int pc = prow ? p : 0;
 #pragma unroll
   for (int kb = 0; kb < KB; ++kb) {
     float v[8];
@@ -406,7 +409,10 @@ __device__ __forceinline__ void taylor_bf_body(int N, int n, int n_terms_k,
       const bool im = k >= Nh;
       const int kk = im ? k - Nh : k;
       const bool ok = prow && kk < N;
-      const float t = (im ? vrow_i : vrow_r)[ok ? kk : 0];
+      const int kc = ok ? kk : 0;
+      const int a = pc < kc ? pc : kc, b = pc < kc ? kc : pc;
+      const int idx = packed ? a * N - (a * (a - 1)) / 2 + (b - a) : pc * N + kc;
+      const float t = (im ? Vi : Vr)[idx];
       v[e] = ok ? t : 0.f;
     }
     u32x4_t ph, pl;
@@ -537,8 +543,9 @@ __global__ __launch_bounds__(512, bf_waves_per_simd(KB, CTP)) void taylor_bf_ker
                                                            cplx* __restrict__ walkers,
                                                            const uint16_t* __restrict__ emi_hi,
                                                            const uint16_t* __restrict__ emi_lo,
-                                                           const float* __restrict__ vhs) {
-  taylor_bf_body<KB, CTP>(N, n, n_terms_k, walkers, emi_hi, emi_lo, vhs, blockIdx.x);
+                                                           const float* __restrict__ vhs,
+                                                           int nv) {
+  taylor_bf_body<KB, CTP>(N, n, n_terms_k, walkers, emi_hi, emi_lo, vhs, nv, blockIdx.x);
 }
 
 template <int KB, int CTP>
@@ -548,7 +555,7 @@ __global__ __launch_bounds__(512, bf_waves_per_simd(KB, CTP)) void taylor_bf_gro
   const TaylorArgs a = args[pidx];
   taylor_bf_body<KB, CTP>(a.N, a.n, a.K, a.walkers, static_cast<const uint16_t*>(a.E),
                           static_cast<const uint16_t*>(a.E2), static_cast<const float*>(a.vhs),
-                          blockIdx.x - start[pidx]);
+                          a.nv, blockIdx.x - start[pidx]);
 }
 
 // k blocks of the packed [Vr | Vi] axis
@@ -565,7 +572,8 @@ int launch_bf(afqmc_ctx* c, int W, int n, cplx* wk, int s) {
   const int rc = ensure_dyn_lds((const void*)taylor_bf_kernel<KB, CTP>, shm, &lds_limit);
   if (rc) return rc;
   hipLaunchKernelGGL((taylor_bf_kernel<KB, CTP>), dim3(W), dim3(nwaves * 64), shm, c->stream, N,
-                     n, c->d.n_exp_terms - 1, wk, c->emi_bf[s][0], c->emi_bf[s][1], c->vhs_f32);
+                     n, c->d.n_exp_terms - 1, wk, c->emi_bf[s][0], c->emi_bf[s][1], c->vhs_f32,
+                     c->vhs_nv);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }
@@ -876,7 +884,7 @@ __global__ __launch_bounds__(1024 / bfs_rt(CT)) void taylor_bfs_grouped_kernel(
 template <int CT>
 int launch_bfs(afqmc_ctx* c, int W, int n, cplx* wk, int s) {
   const int N = c->d.norb;
-  if (N > 256 || n > 16 * CT || 32 * ((N + 31) / 32) != c->Np32)
+  if (N > 256 || n > 16 * CT || 32 * ((N + 31) / 32) != c->Np32 || c->vhs_nv != N * N)
     return afqmc_set_error(AFQMC_ERR_ARG, "taylor bf16x3 streaming: N %d n %d column tiles %d",
                            N, n, CT);
   const size_t shm = bfs_shm(N, CT);

@@ -167,6 +167,10 @@ class DeviceStep:
     def device_bytes(self):
         return int(self._lib.afqmc_ctx_device_bytes(self._h))
 
+    def vhs_entries(self):
+        """Entries of one stored VHS plane (N*N, or N(N+1)/2 for the upper triangle)."""
+        return int(self._lib.afqmc_ctx_vhs_entries(self._h))
+
     def _prep(self):
         _lib.check(self._lib.afqmc_set_stream(self._h, _stream()))
 

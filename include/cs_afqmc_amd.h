@@ -95,6 +95,10 @@ AFQMC_API int afqmc_ctx_destroy(afqmc_ctx* ctx);
 AFQMC_API int afqmc_set_stream(afqmc_ctx* ctx, void* stream);
 /* Bytes of device memory the context holds (intermediates + workspace). */
 AFQMC_API int64_t afqmc_ctx_device_bytes(const afqmc_ctx* ctx);
+/* Entries of one stored VHS plane: norb*norb, or norb(norb+1)/2 when the mixed mode
+ * keeps V as its upper triangle (symmetric Cholesky vectors, register Taylor kernel).
+ * No counterpart in the reference: a layout query for measurement tools. */
+AFQMC_API int64_t afqmc_ctx_vhs_entries(const afqmc_ctx* ctx);
 AFQMC_API const char* afqmc_last_error(void);
 
 /* One phaseless step, propagator.propagate (propagation.py:100-165) with

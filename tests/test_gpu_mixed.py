@@ -57,7 +57,8 @@ def to_dev(x, dev):
     return torch.as_tensor(np.ascontiguousarray(x), device=dev)
 
 
-def build(walker_type, norb, nelec, nchol, W, precision, seed=7, dt=0.005, chol_std=None):
+def build(walker_type, norb, nelec, nchol, W, precision, seed=7, dt=0.005, chol_std=None,
+          asym=0.0):
     if walker_type == "rhf":
         hd, wd, nelec = problems.synthetic_rhf(norb, nelec[0], nchol, seed=seed, chol_std=chol_std)
         trial = wavefunctions.rhf(norb, nelec)
@@ -67,6 +68,10 @@ def build(walker_type, norb, nelec, nchol, W, precision, seed=7, dt=0.005, chol_
                                                chol_std=chol_std)
         trial = wavefunctions.uhf(norb, nelec)
         prop = propagation.propagator_unrestricted(dt=dt, n_walkers=W, precision=precision)
+    if asym:
+        # non-symmetric Cholesky vectors: V is no longer symmetric and is kept whole
+        r = np.random.default_rng(seed + 99)
+        hd["chol"] = hd["chol"] + asym * r.standard_normal(hd["chol"].shape)
     hd["h0"] = 0.37
     ham = hamiltonian.hamiltonian(norb)
     wave = dict(wd)
@@ -75,13 +80,21 @@ def build(walker_type, norb, nelec, nchol, W, precision, seed=7, dt=0.005, chol_
     return hd, wd, nelec, trial, prop, ham, hdat, wave
 
 
-@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-N{c[1]}" for c in CASES])
-def test_mixed_step_close_to_oracle(gpu, case):
+# symmetric Cholesky vectors (all synthetic problems, and real molecular orbitals) keep
+# V as its upper triangle; these cases break the symmetry so V is kept whole
+ASYM_CASES = [("rhf", 40, (8, 8), 100, 32), ("uhf", 36, (7, 5), 90, 24)]
+
+
+@pytest.mark.parametrize("case,asym",
+                         [(c, 0.0) for c in CASES] + [(c, 0.01) for c in ASYM_CASES],
+                         ids=[f"{c[0]}-N{c[1]}" for c in CASES]
+                         + [f"{c[0]}-N{c[1]}-asym" for c in ASYM_CASES])
+def test_mixed_step_close_to_oracle(gpu, case, asym):
     walker_type, norb, nelec, nchol, W = case
-    print(f"mixed case {case}", flush=True)
+    print(f"mixed case {case} asym {asym}", flush=True)
     cs = 0.5 / np.sqrt(norb)
     hd, wd, nelec, trial, prop, ham, hdat, wave = build(walker_type, norb, nelec, nchol, W,
-                                                        "bf16x3", chol_std=cs)
+                                                        "bf16x3", chol_std=cs, asym=asym * cs)
     dt = prop.dt
     oh = o.build_measurement_intermediates(walker_type, dict(hd), wd, norb)
     oh = o.build_propagation_intermediates(walker_type, oh, wd["rdm1"], norb, nelec, dt)

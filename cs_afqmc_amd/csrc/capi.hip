@@ -241,11 +241,9 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
     if (c->mixed) {
       const size_t nn = (size_t)N * N;
       // V kept as its upper triangle (half the VHS product, its output and the Taylor
-      // kernel's reads) when the Cholesky vectors are symmetric, as they are for real
-      // orbitals, and every spin runs the register-resident Taylor kernel
-      bool reg_all = N <= 128;
-      for (int s = 0; s < c->nspin; ++s) reg_all = reg_all && c->nocc[s] <= 32;
-      if (reg_all) {
+      // kernels' reads) when the Cholesky vectors are symmetric, as they are for real
+      // orbitals
+      {
         unsigned long long st[2] = {0ull, 0ull};
         TRY(dalloc(c, &c->sym_stat, 2));
         TRY(zero_async(c->sym_stat, 2 * sizeof(unsigned long long), c->stream));

@@ -623,7 +623,8 @@ __device__ __forceinline__ void taylor_bfs_body(int N, int n, int n_terms_k,
                                                 cplx* __restrict__ walkers,
                                                 const uint16_t* __restrict__ emi_hi,
                                                 const uint16_t* __restrict__ emi_lo,
-                                                const float* __restrict__ vhs, const int w) {
+                                                const float* __restrict__ vhs, int nv,
+                                                const int w) {
   constexpr int NC = 32 * CT;  // LDS columns: CT tiles of yr, then CT tiles of yi
   const int KBk = (N + 31) >> 5;
   const int LDK = 32 * KBk + 8;
@@ -636,16 +637,21 @@ __device__ __forceinline__ void taylor_bfs_body(int N, int n, int n_terms_k,
   const int q4 = lane >> 4, col = lane & 15;
   // uniform bases with 32-bit lane offsets (saddr + voffset addressing)
   cplx* phi = walkers + (int64_t)w * N * n;
-  const float* Vr = vhs + (int64_t)w * 2 * N * N;
-  const float* Vi = Vr + (int64_t)N * N;
+  // V planes: all N^2 entries row-major, or (nv = N(N+1)/2, symmetric V) the upper
+  // triangle row by row, entry (a, b), a <= b, at a N - a (a - 1) / 2 + b - a
+  const bool packed = nv != N * N;
+  const float* Vr = vhs + (int64_t)w * 2 * nv;
+  const float* Vi = Vr + nv;
   bool prow[RT];
-  int voff[RT], eoff[RT];
+  int prw[RT], voff[RT], eoff[RT];
 #pragma unroll
   for (int t = 0; t < RT; ++t) {
     const int p = rb + 16 * t + col;  // A row of this lane in tile t
     prow[t] = p < N;
-    voff[t] = (prow[t] ? p : 0) * N;
-    eoff[t] = (prow[t] ? p : 0) * np32;
+    prw[t] = prow[t] ? p : 0;
+    // start of row p (packed: of its upper part, entry (p, p))
+    voff[t] = packed ? prw[t] * N - (prw[t] * (prw[t] - 1)) / 2 : prw[t] * N;
+    eoff[t] = prw[t] * np32;
   }
 
   {
@@ -721,20 +727,25 @@ __device__ __forceinline__ void taylor_bfs_body(int N, int n, int n_terms_k,
       }
     }
   };
-  // 8 consecutive k of a V row as two 16-byte loads, for chunks that start inside the
-  // row (others are zero).  A chunk that straddles k = N reads at most 7 floats past the
-  // row end, into the next row or, for the last row of the last walker, into the 8
-  // floats of tail padding of the VHS buffer, and is masked after the load.
+  // 8 consecutive k of a V row, for chunks that start inside the row (others are zero).
+  // A chunk on or right of the diagonal is contiguous in the stored row: two 16-byte
+  // loads.  A chunk that straddles k = N reads at most 7 floats past the row end, into the
+  // next row or, for the last row of the last walker, into the 8 floats of tail padding
+  // of the VHS buffer, and is masked after the load.  In the packed layout a chunk left
+  // of the diagonal reads entries (k, p) of rows k < p one by one; for a fixed k the 16
+  // rows of a lane group are consecutive floats of one stored row.
   auto vload = [&](int kb, float (*r)[8], float (*i)[8]) {
     const int k0 = kb * 32 + 8 * q4;
 #pragma unroll
     for (int t = 0; t < RT; ++t) {
       f4u_t r0 = {0.f, 0.f, 0.f, 0.f}, r1 = r0, i0 = r0, i1 = r0;
-      if (prow[t] && k0 < N) {
-        r0 = *reinterpret_cast<const f4u_t*>(Vr + (voff[t] + k0));
-        r1 = *reinterpret_cast<const f4u_t*>(Vr + (voff[t] + k0 + 4));
-        i0 = *reinterpret_cast<const f4u_t*>(Vi + (voff[t] + k0));
-        i1 = *reinterpret_cast<const f4u_t*>(Vi + (voff[t] + k0 + 4));
+      const int p = prw[t];
+      if (prow[t] && k0 < N && (!packed || k0 >= p)) {
+        const int base = voff[t] + (packed ? k0 - p : k0);
+        r0 = *reinterpret_cast<const f4u_t*>(Vr + base);
+        r1 = *reinterpret_cast<const f4u_t*>(Vr + (base + 4));
+        i0 = *reinterpret_cast<const f4u_t*>(Vi + base);
+        i1 = *reinterpret_cast<const f4u_t*>(Vi + (base + 4));
       }
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
@@ -742,6 +753,16 @@ __device__ __forceinline__ void taylor_bfs_body(int N, int n, int n_terms_k,
         r[t][4 + e] = r1[e];
         i[t][e] = i0[e];
         i[t][4 + e] = i1[e];
+      }
+      if (packed && prow[t] && k0 < p) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int k = k0 + e < N ? k0 + e : N - 1;
+          const int a = k < p ? k : p, b = k < p ? p : k;
+          const int idx = a * N - (a * (a - 1)) / 2 + (b - a);
+          r[t][e] = Vr[idx];
+          i[t][e] = Vi[idx];
+        }
       }
       if (k0 + 8 > N) {
 #pragma unroll
@@ -867,8 +888,9 @@ __device__ __forceinline__ void taylor_bfs_body(int N, int n, int n_terms_k,
 template <int CT>
 __global__ __launch_bounds__(1024 / bfs_rt(CT)) void taylor_bfs_kernel(
     int N, int n, int n_terms_k, cplx* __restrict__ walkers, const uint16_t* __restrict__ emi_hi,
-    const uint16_t* __restrict__ emi_lo, const float* __restrict__ vhs) {
-  taylor_bfs_body<CT, bfs_rt(CT)>(N, n, n_terms_k, walkers, emi_hi, emi_lo, vhs, blockIdx.x);
+    const uint16_t* __restrict__ emi_lo, const float* __restrict__ vhs, int nv) {
+  taylor_bfs_body<CT, bfs_rt(CT)>(N, n, n_terms_k, walkers, emi_hi, emi_lo, vhs, nv,
+                                  blockIdx.x);
 }
 
 template <int CT>
@@ -878,13 +900,14 @@ __global__ __launch_bounds__(1024 / bfs_rt(CT)) void taylor_bfs_grouped_kernel(
   const TaylorArgs a = args[p];
   taylor_bfs_body<CT, bfs_rt(CT)>(
       a.N, a.n, a.K, a.walkers, static_cast<const uint16_t*>(a.E),
-      static_cast<const uint16_t*>(a.E2), static_cast<const float*>(a.vhs), blockIdx.x - start[p]);
+      static_cast<const uint16_t*>(a.E2), static_cast<const float*>(a.vhs), a.nv,
+      blockIdx.x - start[p]);
 }
 
 template <int CT>
 int launch_bfs(afqmc_ctx* c, int W, int n, cplx* wk, int s) {
   const int N = c->d.norb;
-  if (N > 256 || n > 16 * CT || 32 * ((N + 31) / 32) != c->Np32 || c->vhs_nv != N * N)
+  if (N > 256 || n > 16 * CT || 32 * ((N + 31) / 32) != c->Np32)
     return afqmc_set_error(AFQMC_ERR_ARG, "taylor bf16x3 streaming: N %d n %d column tiles %d",
                            N, n, CT);
   const size_t shm = bfs_shm(N, CT);
@@ -893,7 +916,7 @@ int launch_bfs(afqmc_ctx* c, int W, int n, cplx* wk, int s) {
   if (rc) return rc;
   hipLaunchKernelGGL((taylor_bfs_kernel<CT>), dim3(W), dim3(bfs_threads(N, CT)), shm, c->stream,
                      N, n, c->d.n_exp_terms - 1, wk, c->emi_bf[s][0], c->emi_bf[s][1],
-                     c->vhs_f32);
+                     c->vhs_f32, c->vhs_nv);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }

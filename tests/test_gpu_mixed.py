@@ -82,7 +82,8 @@ def build(walker_type, norb, nelec, nchol, W, precision, seed=7, dt=0.005, chol_
 
 # symmetric Cholesky vectors (all synthetic problems, and real molecular orbitals) keep
 # V as its upper triangle; these cases break the symmetry so V is kept whole
-ASYM_CASES = [("rhf", 40, (8, 8), 100, 32), ("uhf", 36, (7, 5), 90, 24)]
+ASYM_CASES = [("rhf", 40, (8, 8), 100, 32), ("uhf", 36, (7, 5), 90, 24),
+              ("uhf", 136, (34, 30), 160, 8), ("rhf", 250, (50, 50), 200, 4)]
 
 
 @pytest.mark.parametrize("case,asym",

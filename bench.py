@@ -449,7 +449,7 @@ def _single_block(f):
 
 
 def run_fragments(args):
-    """Survey config 4: 64 closed-shell fragments, norb uniform in [20, 120], nocc =
+    """Batched-fragment workload: 64 closed-shell fragments, norb uniform in [20, 120], nocc =
     norb / 5, nchol = 4 norb, 256 walkers each.  value = walker-steps/s of the grouped
     mixed-precision step; fp64_value = the plain float64 form (each fragment through
     afqmc_propagate in turn); also the grouped float64 rate."""
@@ -551,7 +551,7 @@ SERVE_WORKLOAD = "serve_poisson200_n20-250"
 
 
 def run_serving(args):
-    """Survey config 5: Poisson arrivals (mean 200 requests/s) of fragment solves with
+    """Serving workload: Poisson arrivals (mean 200 requests/s) of fragment solves with
     norb uniform in [20, 250], nocc = norb / 5, nchol = 4 norb, walkers in {128, 256,
     512, 1024}, 100 steps each; a bounded sample of --requests requests.  Integrals are
     the requests' payload (made before the clock starts); intermediates, context
@@ -639,8 +639,8 @@ def main():
                     help="run a 1 GiB device copy before the timed region (PMC scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs the walker population on one GPU; --gpus {args.gpus} "
-              "is not supported (multi-GPU sharding is a design note only).", file=sys.stderr)
+        print(f"bench.py: the walker population runs on one GPU; --gpus {args.gpus} "
+              "is not supported (sharding across GPUs is not implemented).", file=sys.stderr)
         sys.exit(2)
     if args.workload == FRAG_WORKLOAD:
         print(json.dumps(run_fragments(args)), flush=True)

@@ -30,7 +30,7 @@ OPTION_DEFAULTS = {
     "trial": None,
     "ene0": 0.0,
     "n_batch": 1,
-    # this build: arithmetic of the Cholesky contractions ("fp64" or "bf16x3")
+    # not in the reference: arithmetic of the Cholesky contractions ("fp64" or "bf16x3")
     "precision": "fp64",
 }
 

@@ -6,9 +6,11 @@ state resident on the GPU and advances all of them together, one block of
 n_prop_steps grouped steps per tick (``batch.FragmentBatch``).  Between ticks it
   * retires fragments whose last block has been issued (their result is read once the
     GPU has finished it, without stalling the other fragments),
-  * admits queued requests, oldest first, while the admitted set stays within the HBM
-    budget and the fragment limit: prep threads build each admitted request's host
-    intermediates and device context on its own stream while the GPU keeps ticking,
+  * admits queued requests while the admitted set stays within the HBM budget and the
+    fragment limit, oldest first ("fifo") or smallest estimated solve first ("sjf",
+    which lowers the median latency when requests queue; under sustained overload it
+    can hold large requests back): prep threads build each admitted request's
+    intermediates and device context on their own streams while the GPU keeps ticking,
   * joins the requests whose set-up has finished.
 So requests that arrive while others are on the GPU join at the next tick and their
 work overlaps on the device.  A fragment's result does not depend on what else is in
@@ -79,7 +81,11 @@ def estimate_bytes(norb, nocc, nchol, n_walkers, n_prop_steps, precision):
 
 class FragmentServer:
     def __init__(self, precision="bf16x3", n_prop_steps=50, hbm_budget=64 << 30,
-                 max_active=64, dt=0.005, tune=False, prep_workers=4, max_inflight=2):
+                 max_active=64, dt=0.005, tune=False, prep_workers=4, max_inflight=2,
+                 policy="fifo"):
+        if policy not in ("fifo", "sjf"):
+            raise ValueError(f"policy {policy!r}: 'fifo' or 'sjf'")
+        self.policy = policy
         self.prep_workers = max(1, int(prep_workers))
         self.max_inflight = max(1, int(max_inflight))
         self.precision = precision
@@ -88,6 +94,17 @@ class FragmentServer:
         self.max_active = int(max_active)
         self.dt = dt
         self.tune = tune
+
+    def _priority(self, req):
+        """Admission order among queued requests: arrival ("fifo"), or the estimated
+        work of the whole solve first ("sjf": walkers x steps x N^2 (G + 8 n), the
+        VHS and Taylor products that dominate a step)."""
+        if self.policy == "fifo":
+            return (req.arrival, req.rid)
+        norb = int(np.asarray(req.ham_data["h1"]).shape[-1])
+        nchol = int(np.asarray(req.ham_data["chol"]).shape[0])
+        work = req.n_walkers * req.n_steps * norb * norb * (nchol + 8 * max(req.nelec))
+        return (work, req.arrival, req.rid)
 
     def _build(self, req, side):
         """Host intermediates + device context of one request, on the calling prep
@@ -137,7 +154,7 @@ class FragmentServer:
             while nxt < len(pending) or queue or building or active or retiring:
                 now = time.perf_counter() - t0
                 while nxt < len(pending) and pending[nxt].arrival <= now:
-                    heapq.heappush(queue, pending[nxt])
+                    heapq.heappush(queue, (self._priority(pending[nxt]), pending[nxt]))
                     nxt += 1
                 # retire: results of fragments whose last block the GPU has finished
                 still = []
@@ -154,7 +171,7 @@ class FragmentServer:
                 retiring = still
                 # admit: reserve the budget and build on the prep thread, oldest first
                 while queue and len(active) + len(building) < self.max_active:
-                    req = queue[0]
+                    req = queue[0][1]
                     nchol = int(np.asarray(req.ham_data["chol"]).shape[0])
                     norb = int(np.asarray(req.ham_data["h1"]).shape[-1])
                     need = estimate_bytes(norb, req.nelec[0], nchol, req.n_walkers,

@@ -30,12 +30,37 @@ __device__ inline cplx fb_sum(const double* __restrict__ fb, int nparts, int64_t
   return cplx{re, im};
 }
 
+// Four block sums in one pass, each with block_sum's tree (wave shuffle, then one wave
+// over the per-wave partials: the same order and rounding); valid in thread 0 only.
+template <int NT>
+__device__ inline void block_sum4_first(double (&v)[4], double* red) {
+  constexpr int NWV = NT / 64;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] += __shfl_down(v[c], off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) red[c * NWV + wid] = v[c];
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double t = (lane < NWV) ? red[c * NWV + lane] : 0.0;
+      for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+      v[c] = t;
+    }
+  }
+}
+
 __device__ __forceinline__ void fields_body(
     int G, double sqdt, const double* __restrict__ fb, int nparts, int64_t part_stride,
     const cplx* __restrict__ mf, const double* __restrict__ fields, double* __restrict__ xs,
     uint16_t* __restrict__ xs_hi, uint16_t* __restrict__ xs_lo, int Gp,
     cplx* __restrict__ terms, cplx* __restrict__ fb_out, const int w) {
-  __shared__ double red[FIELDS_NT / 64];
+  __shared__ double red[4 * (FIELDS_NT / 64)];
   const double* x = fields + (int64_t)w * G;
   double* xr = xs + (int64_t)w * 2 * G;
   double* xi = xr + G;
@@ -64,13 +89,11 @@ __device__ __forceinline__ void fields_body(
     }
     if (fb_out) fb_out[(int64_t)w * G + g] = f;
   }
-  const double s0 = block_sum<FIELDS_NT>(st.re, red);
-  const double s1 = block_sum<FIELDS_NT>(st.im, red);
-  const double s2 = block_sum<FIELDS_NT>(ft.re, red);
-  const double s3 = block_sum<FIELDS_NT>(ft.im, red);
+  double v[4] = {st.re, st.im, ft.re, ft.im};
+  block_sum4_first<FIELDS_NT>(v, red);
   if (threadIdx.x == 0) {
-    terms[2 * w] = cplx{s0, s1};
-    terms[2 * w + 1] = cplx{s2, s3};
+    terms[2 * w] = cplx{v[0], v[1]};
+    terms[2 * w + 1] = cplx{v[2], v[3]};
   }
 }
 

@@ -110,17 +110,33 @@ __device__ inline double walker_weight_update(const WeightsArgs& a, double pcs, 
 // contiguous run of tiles.  Within a run the tile index walks the dimension whose
 // operand panels are shared: rows fastest when B (the N side) is the larger operand, so
 // an XCD reads each B panel once; columns fastest when A is larger (m_major = true).
-__device__ inline void xcd_tile(int bx, int tiles_m, int tiles_n, bool m_major, int* tm,
-                                int* tn) {
+// The walked operand is then re-read for every panel of the other one; when it does not
+// fit an XCD's 4 MB L2, the run goes in bands of up to 4 panels of the shared operand
+// (as many as fill half the L2), walked together, so the walked operand streams once
+// per band instead of once per panel.
+constexpr int64_t XCD_L2_BYTES = 4 << 20;
+__device__ inline int xcd_band(int64_t walked_bytes, int64_t shared_panel_bytes) {
+  if (walked_bytes <= XCD_L2_BYTES) return 1;
+  const int64_t fit = (XCD_L2_BYTES / 2) / (shared_panel_bytes > 0 ? shared_panel_bytes : 1);
+  return fit < 1 ? 1 : (fit > 4 ? 4 : (int)fit);
+}
+__device__ inline void xcd_tile(int bx, int tiles_m, int tiles_n, bool m_major, int band,
+                                int* tm, int* tn) {
   const int nblk = tiles_m * tiles_n;
   const int q = nblk >> 3, r = nblk & 7, x = bx & 7, i = bx >> 3;
   const int b = x * q + (x < r ? x : r) + i;
-  if (m_major) {
-    *tm = b / tiles_n;
-    *tn = b - (b / tiles_n) * tiles_n;
-  } else {
-    *tm = b % tiles_m;
-    *tn = b / tiles_m;
+  if (m_major) {  // columns fastest, in bands of `band` row tiles
+    const int bi = b / (band * tiles_n), m0 = bi * band;
+    const int gm = tiles_m - m0 < band ? tiles_m - m0 : band;
+    const int w = b - bi * band * tiles_n;
+    *tn = w / gm;
+    *tm = m0 + w - (w / gm) * gm;
+  } else {  // rows fastest, in bands of `band` column tiles
+    const int bi = b / (band * tiles_m), n0 = bi * band;
+    const int gn = tiles_n - n0 < band ? tiles_n - n0 : band;
+    const int w = b - bi * band * tiles_m;
+    *tm = w / gn;
+    *tn = n0 + w - (w / gn) * gn;
   }
 }
 

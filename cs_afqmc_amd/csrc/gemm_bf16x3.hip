@@ -107,7 +107,11 @@ __device__ __forceinline__ void gemm_nt_bf16x3_body(
   __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][BN * LDSW];
 
   int tile_m, tile_n;
-  xcd_tile(bx, tiles_m, tiles_n, M > N, &tile_m, &tile_n);
+  // operand bytes: hi + lo planes, 4 per element over this split's K
+  const bool m_major = M > N;
+  const int band = xcd_band((int64_t)(m_major ? N : M) * k_per_split * 4,
+                            (int64_t)(m_major ? BM : BN) * k_per_split * 4);
+  xcd_tile(bx, tiles_m, tiles_n, m_major, band, &tile_m, &tile_n);
   const int m0 = tile_m * BM, n0 = tile_n * BN;
   const int kbeg = by * k_per_split;
   const int kend = (kbeg + k_per_split < K) ? kbeg + k_per_split : K;

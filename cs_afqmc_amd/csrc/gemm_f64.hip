@@ -42,7 +42,10 @@ __device__ __forceinline__ void gemm_nt_f64_body(
   // XCD-aware block -> tile map (contiguous tile runs per XCD; the larger operand's
   // panels are the shared ones)
   int tile_m, tile_n;
-  xcd_tile(bx, tiles_m, tiles_n, M > N, &tile_m, &tile_n);
+  const bool m_major = M > N;
+  const int band = xcd_band((int64_t)(m_major ? N : M) * k_per_split * 8,
+                            (int64_t)(m_major ? BM : BN) * k_per_split * 8);
+  xcd_tile(bx, tiles_m, tiles_n, m_major, band, &tile_m, &tile_n);
   const int m0 = tile_m * BM, n0 = tile_n * BN;
   const int kbeg = by * k_per_split;
   const int kend = (kbeg + k_per_split < K) ? kbeg + k_per_split : K;

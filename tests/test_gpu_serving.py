@@ -29,15 +29,19 @@ def requests(specs, steps=20):
     return out
 
 
-@pytest.mark.parametrize("precision", ["bf16x3", "fp64"])
-def test_each_request_same_alone_and_in_mix(gpu, precision):
+@pytest.mark.parametrize("precision,policy,max_active",
+                         [("bf16x3", "fifo", 64), ("fp64", "fifo", 64), ("bf16x3", "sjf", 2)],
+                         ids=["bf16x3-fifo", "fp64-fifo", "bf16x3-sjf2"])
+def test_each_request_same_alone_and_in_mix(gpu, precision, policy, max_active):
     reqs = requests(SPECS)
-    srv = serving.FragmentServer(precision=precision, n_prop_steps=10)
+    srv = serving.FragmentServer(precision=precision, n_prop_steps=10, policy=policy,
+                                 max_active=max_active)
     mixed = {r.rid: r for r in srv.serve(reqs)}
     assert sorted(mixed) == list(range(len(SPECS)))
     stats = serving.latency_stats(list(mixed.values()))
-    print(precision, stats, flush=True)
+    print(precision, policy, stats, flush=True)
     assert stats["p50_s"] > 0
+    assert srv.peak_active <= max_active
     for r in reqs:
         alone = serving.FragmentServer(precision=precision, n_prop_steps=10).serve(
             [serving.Request(arrival=0.0, rid=r.rid, ham_data=r.ham_data,

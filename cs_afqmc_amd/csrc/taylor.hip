@@ -603,16 +603,18 @@ int launch_bf_ct(afqmc_ctx* c, int W, int n, cplx* wk, int s) {
 // float32 accumulation; the planes of -Vi are those of Vi with the sign bits flipped
 // (round-to-nearest-even is symmetric).  E y = y + (E - I) y uses the split planes of
 // E - I (rows of 32 ceil(N / 32) entries) as the register form does, and the Horner
-// recursion y <- t0 + V y / k runs in float32 registers.  Two row tiles per wave (RT = 2,
-// at most 8 waves, 256 registers per lane) hold the wider column counts without spills
-// and share each B operand between both tiles.
+// recursion y <- t0 + V y / k runs in float32 registers.  The body takes RT 16-row tiles
+// per wave (RT = 2 shares each B operand between two tiles at 256 registers per lane);
+// one tile per wave, up to 16 waves at 128 registers, is what runs: with the packed-V
+// gathers it measured faster for three and four column tiles despite spilling 25-48
+// registers around the Horner steps (UHF N = 200: 2.21 -> 2.04 ms per step).
 typedef float f4u_t __attribute__((ext_vector_type(4), aligned(4)));  // dword-aligned rows
 
 inline int bfs_ldk(int N) { return 32 * ((N + 31) / 32) + 8; }
 inline size_t bfs_shm(int N, int ct) {
   return (size_t)2 * 32 * ct * bfs_ldk(N) * sizeof(uint16_t);
 }
-constexpr int bfs_rt(int ct) { return ct >= 3 ? 2 : 1; }
+constexpr int bfs_rt(int) { return 1; }
 inline int bfs_threads(int N, int ct) {
   const int rows = 16 * bfs_rt(ct);
   return 64 * ((N + rows - 1) / rows);

@@ -346,7 +346,7 @@ __device__ __forceinline__ void measure_narrow_body(
 // n > 24: four waves per walker.  They take alternate row tiles of O (each over the
 // whole K = N), share the Gauss-Jordan elimination on [O | I] in LDS (every wave finds
 // the pivot from the same column with its own wave reduction, so no broadcast is needed;
-// the row updates are split by trailing column and by row half), and take alternate
+// the row updates are split by trailing column and row group), and take alternate
 // column tiles of the Green's function.
 template <int T2, int TI, bool STAGE>
 __device__ __forceinline__ void measure_wide_body(
@@ -382,11 +382,7 @@ __device__ __forceinline__ void measure_wide_body(
   __syncthreads();
 
   // ---- Gauss-Jordan with partial pivoting: [O | I] -> [* | X], det from pivots.
-  // Trailing columns k + 1 + (tid % 128) (2n - k - 1 <= 127 of them), rows split in
-  // halves by tid / 128.
   cplx det = cplx{1.0, 0.0};
-  const int rh = (n + 1) >> 1;
-  const int r_lo = (tid >> 7) * rh, r_hi = min(n, r_lo + rh);
   for (int k = 0; k < n; ++k) {
     int pr;
     {
@@ -423,8 +419,15 @@ __device__ __forceinline__ void measure_wide_body(
       }
     }
     __syncthreads();
-    const int c = k + 1 + (tid & 127);
-    if (c < n2) {
+    // trailing columns nc = 2n - k - 1: thread -> (column k + 1 + tid % nc, row group
+    // tid / nc), the rows split into as many groups as 256 threads allow
+    const int nc = n2 - k - 1;
+    const int groups = min(16, (64 * NW) / nc);
+    const int rg = tid / nc;
+    const int c = k + 1 + (tid - rg * nc);
+    const int rh = (n + groups - 1) / groups;
+    const int r_lo = rg * rh, r_hi = min(n, r_lo + rh);
+    if (rg < groups) {
       const cplx m = M[k * n2 + c];
       for (int r0 = r_lo; r0 < r_hi; r0 += 8) {
         // eight independent loads, then the updates, then the stores

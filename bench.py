@@ -1,34 +1,48 @@
-"""Benchmark: walker-steps/s of the phaseless AFQMC step on one MI355X, mixed precision
-against the plain float64 GPU form at the same fields.
+"""Benchmark: walker-steps/s of the mixed-precision batched AFQMC step against the plain
+float64 GPU form on one MI355X, with the energy difference between the two and the
+p50 / p99 latency of fragment solves served under load.
 
-Workload (default): synthetic closed-shell RHF, norb=100, nocc=20, nchol=400, 1024
-complex128 walkers, dt=0.005; a "step" is one phaseless propagation of all walkers
-(force bias, VHS, exp(VHS) Taylor + one-body half steps, overlaps, weights).  Steps
-run in blocks of n_prop_steps=50 as sampler.propagate_phaseless does: auxiliary fields
-are drawn on the device per block, and each block ends with QR re-orthonormalisation,
-overlaps, local energies and block statistics; stochastic reconfiguration follows
-every 4 blocks.  Block work is inside the timed region and amortised into the per-step
+Default run (``python bench.py``, also under ``--steps K --warmup W``), one JSON line:
+
+* headline, ``fragments64_n20-120_w256``: 64 synthetic closed-shell fragments, norb
+  uniform in [20, 120], nocc = norb / 5, nchol = 4 norb, 256 walkers each, dt = 0.005.
+  ``value`` = walker-steps/s of the batched mixed-precision form (every fragment's
+  block -- device field draws, 50 grouped steps, QR, overlaps, local energies, block
+  statistics, stochastic reconfiguration, e_estimate update -- replayed from one
+  captured HIP graph); ``fp64_value`` = the plain float64 form (each fragment's block
+  through its own context, one fragment after another, each block a captured graph).
+* ``single``: the single-problem synthetic RHF workload (norb 100, nocc 20, nchol 400,
+  1024 walkers), float64 and mixed, graph-captured blocks.
+* ``serving``: a Poisson stream (mean 200 requests/s) of fragment solves, norb uniform in
+  [20, 250], walkers in {128, 256, 512, 1024}, 100 steps each: p50 / p99 latency.
+* ``h10``: the linear H10 chain (1.6 bohr, STO-6G, RHF trial, modified Cholesky 1e-5),
+  200 walkers, dt 0.005, QR every 5 steps: mixed against float64 block energies.
+
+Timing.  Steps run in whole blocks of 50 (``--steps`` / ``--warmup`` are rounded up to
+whole blocks): the warm-up blocks include the eager first block and the graph capture;
+then each of 5 repeats replays ceil(steps / 50) blocks between two HIP events on the
+stream the blocks run on, and the median repeat gives the rate.  A block's QR, energy,
+statistics and resampling are inside the timed region, amortised into the per-step
 figure.
 
-Default mode runs the same trajectory twice from the same seed (same device fields,
-same SR draws): first in float64, then in the mixed mode (split-bf16 MFMA Cholesky
-contractions and Taylor products, float64 overlaps / determinants / QR / weights /
-energies).  `value` is the mixed run's walker-steps/s; the line also carries the
-float64 rate, the speed-up and the largest block-energy difference |dE| between the
-two runs.  `--precision fp64|bf16x3` runs one mode only.
+Energy difference.  The mixed and float64 runs start from the same state with the same
+seeds, so they draw the same fields and resampling uniforms.  |dE| is taken per block
+only where both runs' block energies are finite and fewer than 1% of either run's
+walkers had their local energy clamped (|E_L - e_estimate| > sqrt(2 / dt)); clamp and
+zero-weight fractions are reported beside it, and non-finite blocks are counted and
+flagged.
 
-Prints ONE JSON line.  `value` = walker-steps/s over the timed region (inputs already
-resident in HBM).  `phases` / `rooflines` come from a profiled pass of up to 50 steps
-just before the timed region, with a HIP event pair around every phase (each pair costs
-device time); the timed region records events only around the dominant phase, and
-`roofline` describes that kernel from those events.  `cpu_baseline` times
-the numpy restatement of the reference algorithm (the oracle) on a bounded sample on
-the host cores, one single-threaded process per core (the reference's own model: one
-MPI rank per core with one intra-op thread).
+``roofline``: the dominant kernel phase of the headline form, timed with HIP event
+pairs around that phase only, over one eager block after the timed repeats (the pairs
+cost device time, so the repeats run without them).  ``cpu_baseline``: the numpy
+restatement of the reference algorithm (the oracle) on a bounded sample of the same
+fragments, one single-threaded process per host core (the reference's one rank per
+core with one intra-op thread).
 """
 
 import argparse
 import json
+import math
 import multiprocessing as mp
 import os
 import sys
@@ -43,125 +57,79 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 HBM_PEAK_GBS = 8000.0
 
 N_PROP_STEPS = 50
-N_ENE_BLOCKS = 4
-PROFILE_STEPS = 50  # steps of the all-phase profiled pass before the timed region
+REPEATS = 5
+DT = 0.005
+CPU_PROCS_MAX = 16  # the host-core share of one GPU on the measurement boxes
+
+FRAG_WORKLOAD = "fragments64_n20-120_w256"
+SINGLE_WORKLOADS = {
+    "rhf_n100_o20_g400_w1024": ("rhf", 100, 20, 400, 1024),
+    "uhf_n200_o40_g1000_w2048": ("uhf", 200, 40, 1000, 2048),
+}
+SERVE_WORKLOAD = "serve_poisson200_n20-250"
+H10_WORKLOAD = "h10_chain_r1.6_sto6g_w200"
+# admission: shortest estimated solve first among queued requests, at most 8 fragments
+# stepping together
+SERVE_POLICY, SERVE_MAX_ACTIVE = "sjf", 8
+
+from cs_afqmc_amd.work import bytes_per_walker, flops_per_walker, phase_work  # noqa: E402
+
+DTYPE = {"fp64": "f64",
+         "fp32": "mixed: f32 MFMA Cholesky contractions (f32 accumulation), f64 Taylor, "
+                 "overlaps, QR, weights, energy accumulation",
+         "bf16x3": "mixed: split-bf16 (bf16x3) MFMA contractions and Taylor products "
+                   "(f32 accumulation), f64 overlaps/QR/weights/energy accumulation"}
+
+
+def blocks_for(steps):
+    return max(1, -(-int(steps) // N_PROP_STEPS))
+
+
+# ------------------------------------------------------------------ algorithmic work
+
 
 def phase_kernel(precision, phase, norb, nelec):
-    """Device kernel behind a timed phase (symbol prefix as rocprofv3 lists it), by the
-    library's dispatch rules."""
+    """Device kernel behind a timed phase (symbol prefix as rocprofv3 lists it)."""
     reg = norb <= 128 and max(nelec) <= 32
     if phase == "taylor":
-        if precision == "fp64":
+        if precision in ("fp64", "fp32"):
             return "taylor_reg_kernel" if reg else "taylor_kernel"
         return "taylor_bf_kernel" if reg else "taylor_bfs_kernel"
     if phase in ("vhs_gemm", "fb_gemm"):
-        return "gemm_nt_f64_kernel" if precision == "fp64" else "gemm_nt_bf16x3_kernel"
+        return {"fp64": "gemm_nt_f64_kernel", "fp32": "gemm_nt_f32_kernel"}.get(
+            precision, "gemm_nt_bf16x3_kernel")
     if phase in ("overlap", "green"):
         return "measure_kernel"
     if phase == "energy":
-        return "exchange_f64_kernel" if precision == "fp64" else "exchange_bf16x3_kernel"
+        return {"fp64": "exchange_f64_kernel", "fp32": "exchange_f32_kernel"}.get(
+            precision, "exchange_bf16x3_kernel")
     return {"fields": "fields_kernel", "weights": "pcs_kernel", "qr": "qr_kernel",
-            "sr": "sr_index_kernel", "rng": "normal_kernel"}.get(phase, phase)
+            "sr": "sr_index_kernel", "rng": "normal_key_kernel"}.get(phase, phase)
 
 
-def flops_per_walker(norb, nelec, nchol, n_exp_terms, walker_type, nv=None):
-    """Algorithmic real flops per walker per phase (real x complex MAC = 4 flop,
-    complex x complex MAC = 8 flop).  The measurement kernel ("green" at a block's start,
-    "overlap" after each step) forms C^T w (4 N n^2), the inverse by elimination (8 n^3)
-    and the Green's function w O^-1 (8 N n^2).  nv: entries of V the VHS product forms
-    (N^2, or N(N+1)/2 when V is symmetric and kept as its upper triangle)."""
-    K = n_exp_terms - 1
-    N, G = norb, nchol
-    nv = N * N if nv is None else nv
-    ns = [nelec[0]] if walker_type == "rhf" else [nelec[0], nelec[1]]
-    f = {
-        "vhs_gemm": 4.0 * G * nv,
-        "fb_gemm": sum(4.0 * G * N * n for n in ns),
-        "green": sum(12.0 * N * n * n + 8.0 * n**3 for n in ns),
-        "taylor": sum((8.0 + 8.0 * K) * N * N * n for n in ns),
-        "overlap": sum(12.0 * N * n * n + 8.0 * n**3 for n in ns),
-        "energy": sum(4.0 * G * N * n * n + 8.0 * G * n * n + 12.0 * N * n * n + 8.0 * n**3
-                      for n in ns),
-        "qr": sum(16.0 * N * n * n for n in ns),
-        "fields": 40.0 * G,  # shifts, shifted fields and the two field sums, complex
-        "weights": 1.0,      # the population-control sum
-    }
-    return f
+def phase_peak(precision, phase):
+    """Published peak of the arithmetic a phase runs on."""
+    if precision == "fp64":
+        return PEAK_TFLOPS["fp64"]
+    if phase in ("vhs_gemm", "fb_gemm", "energy") or (phase == "taylor" and
+                                                      precision == "bf16x3"):
+        return PEAK_TFLOPS["bf16" if precision == "bf16x3" else "fp32"]
+    return PEAK_TFLOPS["fp64"]
 
 
-def bytes_per_walker(norb, nelec, nchol, walker_type, precision, W, nv=None):
-    """Algorithmic HBM bytes per walker per phase: each kernel's inputs read once and
-    outputs written once in the layouts the library keeps (complex128 walkers; float64
-    or split-bf16 Green's function / shifted fields / Cholesky planes; float64 or float32
-    VHS), with operands shared by all walkers (Cholesky vectors, trial, one-body factor)
-    counted once per launch and divided by W.  nv: stored entries of one V plane."""
-    N, G = norb, nchol
-    nv = N * N if nv is None else nv
-    ns = [nelec[0]] if walker_type == "rhf" else [nelec[0], nelec[1]]
-    nsum, nspin = sum(ns), len(ns)
-    mixed = precision != "fp64"
-    Np32, Gp = -(-N // 32) * 32, -(-G // 32) * 32
-    walker = 16.0 * N * nsum
-    green = 8.0 * nsum * Np32 if mixed else 16.0 * nsum * N     # hi/lo bf16 or float64
-    vhs = 8.0 * nv if mixed else 16.0 * nv                       # float32 or float64 re/im
-    xs = 8.0 * Gp if mixed else 16.0 * G
-    chol = (4.0 * nv * Gp if mixed else 8.0 * nv * G) / W
-    rot = (4.0 * G * nsum * Np32 if mixed else 8.0 * G * nsum * N) / W
-    one_body = nspin * (4.0 * N * Np32 if mixed else 8.0 * N * N) / W
-    trial = 8.0 * N * nsum / W
-    tail = 32.0 + 32.0 + 16.0  # terms read, overlap and weight read + written
-    return {
-        "vhs_gemm": xs + chol + vhs,
-        "fb_gemm": green + rot + 16.0 * G,
-        "fields": 16.0 * G + 8.0 * G + xs + 32.0,
-        "taylor": 2.0 * walker + vhs + one_body,
-        "green": walker + trial + green + 16.0 * nspin,
-        "overlap": walker + trial + green + 16.0 * nspin + tail,
-        "weights": 8.0,
-        "qr": 2.0 * walker,
-        "energy": walker + trial + green + rot,
-    }
-
-
-def roofline_of(phase, precision, norb, nelec, nchol, n_exp_terms, walker_type, W, avg_s,
-                nv=None):
-    """Roofline record of one phase's kernel: the bound is whichever of MFMA time and
-    HBM time at published peaks is longer for the algorithmic flops and bytes of one
-    launch; achieved = those flops (or bytes) / the measured average launch time."""
-    fl = flops_per_walker(norb, nelec, nchol, n_exp_terms, walker_type, nv).get(phase)
-    by = bytes_per_walker(norb, nelec, nchol, walker_type, precision, W, nv).get(phase)
-    if fl is None or by is None:
-        return None
-    flop, byt = fl * W, by * W
-    peak_tf = phase_peak(precision, phase, norb, nelec)
+def roofline_record(phase, precision, kernel, flop, byt, avg_s):
+    """The bound is whichever of MFMA time and HBM time at published peaks is longer for
+    the algorithmic flops and bytes of one launch; achieved = those / launch time."""
+    peak_tf = phase_peak(precision, phase)
     mfma_bound = flop / (peak_tf * 1e12) >= byt / (HBM_PEAK_GBS * 1e9)
     if mfma_bound:
         achieved, peak, unit = flop / avg_s / 1e12, peak_tf, "TFLOP/s"
     else:
         achieved, peak, unit = byt / avg_s / 1e9, HBM_PEAK_GBS, "GB/s"
-    return {
-        "phase": phase,
-        "kernel": phase_kernel(precision, phase, norb, nelec),
-        "bound": "mfma" if mfma_bound else "hbm",
-        "achieved": round(achieved, 3),
-        "peak": peak,
-        "unit": unit,
-        "frac": round(achieved / peak, 4),
-        "flop_per_launch": flop,
-        "bytes_per_launch": byt,
-        "avg_launch_ms": round(avg_s * 1e3, 5),
-    }
-
-
-def phase_peak(precision, phase, norb, nelec):
-    """Published peak of the arithmetic a phase runs on."""
-    if precision == "fp64":
-        return PEAK_TFLOPS["fp64"]
-    if phase in ("vhs_gemm", "fb_gemm", "energy"):  # split-bf16 contractions
-        return PEAK_TFLOPS["bf16"]
-    if phase == "taylor":  # split-bf16 register or streaming kernel
-        return PEAK_TFLOPS["bf16"]
-    return PEAK_TFLOPS["fp64"]
+    return {"phase": phase, "kernel": kernel, "bound": "mfma" if mfma_bound else "hbm",
+            "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+            "frac": round(achieved / peak, 4), "flop_per_launch": flop,
+            "bytes_per_launch": byt, "avg_launch_ms": round(avg_s * 1e3, 5)}
 
 
 def traffic_from_profiles(kernel, workload, precision):
@@ -182,400 +150,439 @@ def traffic_from_profiles(kernel, workload, precision):
     return None
 
 
+# ------------------------------------------------------------------ energy comparison
+
+
+def compare_blocks(stats_a, stats_b, n_walkers, clamp_max=0.01):
+    """Block-by-block energy comparison of two runs with the same draws.  stats_*:
+    (blocks, 4) arrays of block energy, block weight, zero weights, clamped energies.
+    Returns the usable blocks' differences (both finite, both under clamp_max of the
+    walkers clamped) with the bookkeeping that says how many blocks were usable."""
+    import numpy as np
+
+    a, b = np.asarray(stats_a), np.asarray(stats_b)
+    nb = min(len(a), len(b))
+    a, b = a[:nb], b[:nb]
+    finite = np.isfinite(a[:, 0]) & np.isfinite(b[:, 0]) & (a[:, 1] > 0) & (b[:, 1] > 0)
+    unclamped = (a[:, 3] < clamp_max * n_walkers) & (b[:, 3] < clamp_max * n_walkers)
+    ok = finite & unclamped
+    return {
+        "blocks": int(nb),
+        "nonfinite": int(np.sum(~finite)),
+        "clamped_blocks": int(np.sum(finite & ~unclamped)),
+        "usable": int(np.sum(ok)),
+        "diff": (a[ok, 0] - b[ok, 0]).tolist(),
+        "first_usable_is_block0": bool(nb > 0 and ok[0]),
+        "clamp_frac_a": float(np.mean(a[:, 3]) / n_walkers) if nb else None,
+        "clamp_frac_b": float(np.mean(b[:, 3]) / n_walkers) if nb else None,
+        "zero_weight_frac_a": float(np.mean(a[:, 2]) / n_walkers) if nb else None,
+        "zero_weight_frac_b": float(np.mean(b[:, 2]) / n_walkers) if nb else None,
+    }
+
+
+def summarize_delta(comps):
+    """Aggregate compare_blocks results: max |dE| over usable blocks, max |dE| of the
+    first block (before any resampling could separate the runs), the mean difference
+    over usable blocks with its standard error."""
+    import numpy as np
+
+    diffs = [d for c in comps for d in c["diff"]]
+    first = [c["diff"][0] for c in comps if c["first_usable_is_block0"]]
+    nonfinite = sum(c["nonfinite"] for c in comps)
+    out = {
+        "delta_e_max_eh": float(np.max(np.abs(diffs))) if diffs else None,
+        "delta_e_first_block_max_eh": float(np.max(np.abs(first))) if first else None,
+        "delta_e_mean_eh": float(np.mean(diffs)) if diffs else None,
+        "delta_e_mean_stderr_eh": (float(np.std(diffs, ddof=1) / math.sqrt(len(diffs)))
+                                   if len(diffs) > 1 else None),
+        "blocks_usable": len(diffs),
+        "blocks_total": sum(c["blocks"] for c in comps),
+        "blocks_clamped": sum(c["clamped_blocks"] for c in comps),
+        "blocks_nonfinite": nonfinite,
+        "clamp_frac_mean": float(np.mean([c["clamp_frac_a"] for c in comps
+                                          if c["clamp_frac_a"] is not None] or [0.0])),
+        "zero_weight_frac_mean": float(np.mean([c["zero_weight_frac_a"] for c in comps
+                                                if c["zero_weight_frac_a"] is not None]
+                                               or [0.0])),
+    }
+    out["energy_check"] = ("ok" if nonfinite == 0 and diffs else
+                           f"FAILED: {nonfinite} non-finite blocks" if nonfinite else
+                           "FAILED: no usable block")
+    if nonfinite:
+        print(f"# WARNING: {nonfinite} non-finite block energies", file=sys.stderr, flush=True)
+    return out
+
+
 # ------------------------------------------------------------------ CPU baseline
 
 
-def _cpu_worker(args):
-    """One single-threaded process: oracle steps on a slice of the workload."""
-    (walker_type, norb, nocc, nchol, n_walk, budget_s, seed, dt) = args
+def _cpu_fragment_worker(args):
+    """One single-threaded process: oracle steps of each given fragment on a few walkers
+    (time per walker-step, with one QR / overlap / energy amortised over 50 steps)."""
+    specs, n_walk, n_steps = args
     import numpy as np
 
     from cs_afqmc_amd import problems
     from oracle import afqmc_oracle as o
 
-    if walker_type == "rhf":
-        hd, wd, nelec = problems.synthetic_rhf(norb, nocc, nchol, seed=seed)
-    else:
-        hd, wd, nelec = problems.synthetic_uhf(norb, nocc, nocc, nchol, seed=seed)
-    oh = o.build_measurement_intermediates(walker_type, hd, wd, norb)
-    oh = o.build_propagation_intermediates(walker_type, oh, wd["rdm1"], norb, nelec, dt)
-    rng = np.random.default_rng(seed + 17)
-    walkers = o.get_init_walkers(wd["rdm1"], nelec, n_walk, restricted=walker_type == "rhf")
-    prop = {"walkers": walkers, "weights": np.ones(n_walk), "e_estimate": 0.0,
-            "pop_control_ene_shift": 0.0,
-            "overlaps": o.calc_overlap(walker_type, walkers, wd)}
-    steps = 0
-    t_prop = 0.0
-    while True:
+    out = []
+    for wt, norb, nocc, nchol, W, seed in specs:
+        if wt == "rhf":
+            hd, wd, nelec = problems.synthetic_rhf(norb, nocc, nchol, seed=seed)
+        else:
+            hd, wd, nelec = problems.synthetic_uhf(norb, nocc, nocc, nchol, seed=seed)
+        oh = o.build_measurement_intermediates(wt, hd, wd, norb)
+        oh = o.build_propagation_intermediates(wt, oh, wd["rdm1"], norb, nelec, DT)
+        rng = np.random.default_rng(seed + 17)
+        pd = o.init_prop_data(wt, wt, wd, oh, n_walk, nelec)
         t0 = time.perf_counter()
-        fields = rng.standard_normal((n_walk, nchol))
-        prop = o.propagate(walker_type, walker_type, oh, prop, fields, wd, dt, 6, norb,
-                           n_walkers=n_walk)
-        t_prop += time.perf_counter() - t0
-        steps += 1
-        if steps % N_PROP_STEPS == 0:  # keep the walkers orthonormal as the blocks do
-            prop["walkers"] = o.orthonormalize_walkers(walker_type, prop["walkers"])
-            prop["overlaps"] = o.calc_overlap(walker_type, prop["walkers"], wd)
-        if t_prop > budget_s:
-            break
-    t_step = t_prop / steps
-    t1 = time.perf_counter()
-    prop["walkers"] = o.orthonormalize_walkers(walker_type, prop["walkers"])
-    o.calc_overlap(walker_type, prop["walkers"], wd)
-    o.calc_energy(walker_type, prop["walkers"], oh, wd)
-    t_block = time.perf_counter() - t1
-    return n_walk, steps, t_step, t_block
+        for _ in range(n_steps):
+            pd = o.propagate(wt, wt, oh, pd, rng.standard_normal((n_walk, nchol)), wd, DT, 6,
+                             norb, n_walkers=n_walk)
+        t_step = (time.perf_counter() - t0) / (n_steps * n_walk)
+        t1 = time.perf_counter()
+        wk = o.orthonormalize_walkers(wt, pd["walkers"])
+        o.calc_overlap(wt, wk, wd)
+        o.calc_energy(wt, wk, oh, wd)
+        t_block = (time.perf_counter() - t1) / n_walk
+        out.append((W, t_step + t_block / N_PROP_STEPS))
+    return out
 
 
-def cpu_baseline(workload, budget_s):
-    walker_type, norb, nocc, nchol, W = workload
+def cpu_baseline(specs, n_walk, n_steps, label):
+    """Whole-job CPU rate of the oracle for a list of problems (walker_type, norb, nocc,
+    nchol, W, seed): each problem's time per walker-step is measured on n_walk walkers x
+    n_steps steps; the P processes share the job's walker-steps, so the rate is
+    P * sum(W) / sum(W * t_walker_step)."""
     cores = len(os.sched_getaffinity(0))
-    procs = max(1, min(cores, 16))
-    n_walk = max(1, min(16, W // procs))
+    procs = max(1, min(cores, CPU_PROCS_MAX, len(specs)))
     env_before = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS",
                                                  "MKL_NUM_THREADS")}
     for k in env_before:
         os.environ[k] = "1"
+    t0 = time.perf_counter()
     try:
         ctx = mp.get_context("spawn")
+        chunks = [specs[i::procs] for i in range(procs)]
         with ctx.Pool(procs) as pool:
-            res = pool.map(_cpu_worker, [(walker_type, norb, nocc, nchol, n_walk, budget_s,
-                                          11 + i, 0.005) for i in range(procs)])
+            res = pool.map(_cpu_fragment_worker, [(c, n_walk, n_steps) for c in chunks])
     finally:
         for k, v in env_before.items():
             if v is None:
                 os.environ.pop(k, None)
             else:
                 os.environ[k] = v
-    # per-process walker-steps/s with block work (QR, overlap, energy) amortised over
-    # N_PROP_STEPS steps; processes run concurrently, so rates add
-    rate = 0.0
-    for n_w, steps, t_step, t_block in res:
-        rate += n_w / (t_step + t_block / N_PROP_STEPS)
-    steps_done = sum(r[1] for r in res)
+    rows = [r for part in res for r in part]
+    work = sum(W for W, _ in rows)
+    secs = sum(W * t for W, t in rows)
     return {
-        "value": round(rate, 1),
+        "value": round(procs * work / secs, 1),
         "unit": "walker-steps/s",
         "cores": procs,
+        "host_cpus": os.cpu_count(),
         "kind": "port",
-        "sample": (f"{procs} single-threaded processes x {n_walk} walkers of the same workload, "
-                   f"{steps_done} oracle steps total (~{budget_s:.0f} s each) + one "
-                   f"QR/overlap/energy per process amortised over {N_PROP_STEPS} steps"),
+        "sample": (f"{label}: oracle on every listed problem, {n_walk} walkers x {n_steps} "
+                   f"steps each + one QR/overlap/energy amortised over {N_PROP_STEPS} steps, "
+                   f"{procs} single-threaded processes ({time.perf_counter() - t0:.0f} s wall)"),
     }
 
 
-# ------------------------------------------------------------------ GPU run
+# ------------------------------------------------------------------ GPU helpers
 
 
-def copy_probe(nbytes=1 << 30):
-    """A plain device copy of known size, for scaling PMC byte counters."""
+def event_repeats(stream, fn, repeats):
+    """Milliseconds of `fn` (which enqueues work on `stream`) per repeat, from a HIP
+    event pair on that stream."""
     import torch
-    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
+
+    ms = []
+    for _ in range(repeats):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        e1.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    return ms
+
+
+def median(xs):
+    s = sorted(xs)
+    n = len(s)
+    return s[n // 2] if n % 2 else 0.5 * (s[n // 2 - 1] + s[n // 2])
+
+
+def dominant_roofline(precision, phases, fl, by, kernel_of, workload):
+    """Roofline of the phase with the most device time among the step phases."""
+    cands = [p for p, (ms, n) in phases.items() if n and p in fl and p != "weights"]
+    if not cands:
+        return None, None
+    dom = max(cands, key=lambda p: phases[p][0])
+    ms, n = phases[dom]
+    r = roofline_record(dom, precision, kernel_of(dom), fl[dom], by[dom], ms / n / 1e3)
+    r["launches_timed"] = n
+    r["traffic"] = traffic_from_profiles(r["kernel"], workload, precision)
+    return dom, r
+
+
+def all_rooflines(precision, phases, fl, by, kernel_of):
+    out = {}
+    for p, (ms, n) in phases.items():
+        if n and p in fl:
+            r = roofline_record(p, precision, kernel_of(p), fl[p], by[p], ms / n / 1e3)
+            out[p] = {k: r[k] for k in ("kernel", "bound", "achieved", "unit", "frac",
+                                        "avg_launch_ms")}
+    return out
+
+
+# ------------------------------------------------------------------ config 4: fragments
+
+
+def _make_fragments(specs, precision):
+    from cs_afqmc_amd import batch, problems
+    frags = []
+    for wt, norb, nocc, nchol, W, seed in specs:
+        hd, wd, ne = problems.synthetic_rhf(norb, nocc, nchol, seed=seed)
+        frags.append(batch.Fragment(hd, wd, wt, ne, W, dt=DT, seed=seed, precision=precision,
+                                    n_prop_steps=N_PROP_STEPS, tag=(norb, nocc, nchol, W)))
+    return frags
+
+
+def fragment_specs():
+    from cs_afqmc_amd import problems
+    return [("rhf", n, o, g, w, s) for n, o, g, w, s in problems.fragment_specs(64, seed=0)]
+
+
+def run_fragments_mode(specs, mode, warm_blocks, timed_blocks):
+    """One form of the fragment workload.  mode: "<precision>_grouped" (one captured
+    graph per block of every fragment) or "<precision>_plain" (each fragment's own
+    captured block, one fragment after another)."""
+    import torch
+
+    from cs_afqmc_amd import batch
+
+    prec, form = mode.split("_")
+    t0 = time.perf_counter()
+    frags = _make_fragments(specs, prec)
+    setup_s = time.perf_counter() - t0
+    walkers = sum(f.n_walkers for f in frags)
+    stream = torch.cuda.Stream()
+    bt = None
+    if form == "grouped":
+        bt = batch.FragmentBatch(frags, tune=True, use_graph=True, stream=stream)
+
+        def block():
+            bt.run_block()
+    else:
+        def block():
+            with torch.cuda.stream(stream):
+                for f in frags:
+                    f.run_block_single(use_graph=True)
+    for _ in range(warm_blocks):
+        block()
     torch.cuda.synchronize()
-    del a, b
+    print(f"# fragments {mode}: set-up {setup_s:.1f} s, {warm_blocks} warm-up blocks done",
+          file=sys.stderr, flush=True)
+
+    def timed():
+        for _ in range(timed_blocks):
+            block()
+
+    ms = event_repeats(stream, timed, REPEATS)
+    torch.cuda.synchronize()
+    med = median(ms)
+    rate = walkers * N_PROP_STEPS * timed_blocks / (med / 1e3)
+    print(f"# fragments {mode}: {REPEATS} x {timed_blocks} blocks, median {med:.2f} ms "
+          f"-> {rate / 1e6:.3f} M walker-steps/s", file=sys.stderr, flush=True)
+    res = {"value": rate, "ms_per_step": med / (timed_blocks * N_PROP_STEPS),
+           "repeat_ms": [round(x, 3) for x in ms], "set_up_s": round(setup_s, 2),
+           "stats": [f.block_stats() for f in frags]}
+    if bt is not None and bt.tuned:
+        res["gemm_tile_ms"] = bt.tuned
+    if bt is not None:
+        # per-phase device time of one eager block: all phases, then the dominant one only
+        members = [(f.walker_type, f.norb, f.nelec, f.dev.nchol, f.n_walkers,
+                    f.dev.vhs_entries(), f.prop.n_exp_terms) for f in frags]
+        fl, by = phase_work(prec, members)
+        frag_n = [f.norb for f in frags]
+        kern = (lambda p: phase_kernel(prec, p, max(frag_n), (max(f.nelec[0] for f in frags),)
+                                       * 2))
+        bt.use_graph = False
+        bt._drop_replay()
+        with torch.cuda.stream(stream):
+            bt.group.prof_collect()
+            bt.group.prof_enable(True)
+        bt.run_block()
+        with torch.cuda.stream(stream):
+            phases = bt.group.prof_collect()
+            dom, _ = dominant_roofline(prec, phases, fl, by, kern, FRAG_WORKLOAD)
+            bt.group.prof_enable(True, phases=[dom])
+        bt.run_block()
+        with torch.cuda.stream(stream):
+            timed_ph = bt.group.prof_collect()
+            bt.group.prof_enable(False, phases=None)
+        _, roof = dominant_roofline(prec, {dom: timed_ph[dom]}, fl, by, kern, FRAG_WORKLOAD)
+        res["phases"] = {p: {"ms_total": round(v[0], 4), "launches": v[1],
+                             "ms_avg": round(v[0] / v[1], 5)}
+                         for p, v in phases.items() if v[1]}
+        res["roofline"] = roof
+        res["rooflines"] = all_rooflines(prec, phases, fl, by, kern)
+        res["phase_sum_ms_per_step"] = round(
+            sum(v[0] for v in phases.values()) / N_PROP_STEPS, 5)
+    del frags, bt
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return res
 
 
-def run(args, precision):
+def run_fragments(args, modes):
+    specs = fragment_specs()
+    warm = blocks_for(args.warmup) + 1  # the eager block and the capture
+    timed = blocks_for(args.steps)
+    runs = {m: run_fragments_mode(specs, m, warm, timed) for m in modes}
+    return specs, runs
+
+
+# ------------------------------------------------------------------ config 2/3: one problem
+
+
+def run_single(workload, precision, warm_blocks, timed_blocks, n_ene_blocks=4):
+    """One problem, the sampler's graph-captured block (sampling._BlockState): fields,
+    50 steps, QR, overlaps, energies, block statistics; SR and the e_estimate update
+    every n_ene_blocks blocks (one sampler call of driver.afqmc)."""
     import numpy as np
     import torch
 
     from cs_afqmc_amd import hamiltonian, problems, propagation, rng, sampling, wavefunctions
 
-    workload = problems.WORKLOADS[args.workload]
-    walker_type, norb, nocc, nchol, W = workload
-    dt = 0.005
-    torch.cuda.set_device(0)
+    walker_type, norb, nocc, nchol, W = SINGLE_WORKLOADS[workload]
     if walker_type == "rhf":
         hd, wd, nelec = problems.synthetic_rhf(norb, nocc, nchol, seed=0)
         trial = wavefunctions.rhf(norb, nelec)
-        prop = propagation.propagator_restricted(dt=dt, n_walkers=W, precision=precision)
+        prop = propagation.propagator_restricted(dt=DT, n_walkers=W, precision=precision)
     else:
         hd, wd, nelec = problems.synthetic_uhf(norb, nocc, nocc, nchol, seed=0)
         trial = wavefunctions.uhf(norb, nelec)
-        prop = propagation.propagator_unrestricted(dt=dt, n_walkers=W, precision=precision)
+        prop = propagation.propagator_unrestricted(dt=DT, n_walkers=W, precision=precision)
     ham = hamiltonian.hamiltonian(norb)
     ham_data = ham.build_measurement_intermediates(hd, trial, wd)
     ham_data = ham.build_propagation_intermediates(ham_data, prop, trial, wd)
-    prop_data = prop.init_prop_data(trial, wd, ham_data)
-    prop_data["key"] = rng.PRNGKey(0)
+    pd = prop.init_prop_data(trial, wd, ham_data)
+    pd["key"] = rng.PRNGKey(0)
     dev = ham_data["_afqmc_device"]
+    st = sampling._BlockState(dev, pd, N_PROP_STEPS, W, use_graph=True)
+    e0 = pd["e_estimate"].reshape(())
+    st.scalars[0].copy_(e0)
+    st.scalars[1].copy_(e0)
+    rng.device_key(pd["key"], dev.device, out=st.key)
+    stream = torch.cuda.Stream()
+    hist = []
+    since_sr = [0]
 
-    # device state as in sampler.propagate_phaseless
-    ddev = dev.device
-    e0 = prop_data["e_estimate"].reshape(())
-    scalars = torch.stack([e0, e0]).contiguous()
-    prop_data["_scalars"] = scalars
-    prop_data["_energies"] = torch.empty(W, dtype=torch.complex128, device=ddev)
-    fields_buf = torch.empty((N_PROP_STEPS, W, nchol), dtype=torch.float64, device=ddev)
-    total_steps = args.warmup + args.steps
-    stats = torch.zeros((total_steps // N_PROP_STEPS + 1, 3), dtype=torch.float64,
-                        device=ddev)
+    def block():
+        with torch.cuda.stream(stream):
+            st.block()
+            hist.append(st.slot.clone())
+            since_sr[0] += 1
+            if since_sr[0] == n_ene_blocks:
+                since_sr[0] = 0
+                st.sr()
+                dev.estimate_mix(st.scalars, torch.stack(hist[-n_ene_blocks:]))
 
-    state = {"block": 0, "in_block": 0, "blocks_since_sr": 0, "fields": None}
-
-    def one_step():
-        """Advance one propagation step; block and SR work happen at the same points
-        as in sampler.propagate_phaseless."""
-        if state["in_block"] == 0:
-            prop_data["key"], subkey = rng.split(prop_data["key"])
-            state["fields"] = dev.fields_normal(subkey[0], subkey[1], fields_buf.shape,
-                                                out=fields_buf)
-        dev.propagate(prop_data["walkers"], prop_data["weights"], prop_data["overlaps"],
-                      state["fields"][state["in_block"]], scalars,
-                      reuse_green=state["in_block"] > 0)
-        state["in_block"] += 1
-        if state["in_block"] == N_PROP_STEPS:
-            state["in_block"] = 0
-            dev.orthonormalize(prop_data["walkers"])
-            dev.overlap(prop_data["walkers"], out=prop_data["overlaps"])
-            energies = dev.energy(prop_data["walkers"], out=prop_data["_energies"])
-            dev.block_stats(energies, prop_data["weights"], scalars, stats[state["block"]])
-            state["block"] += 1
-            state["blocks_since_sr"] += 1
-            if state["blocks_since_sr"] == N_ENE_BLOCKS:
-                state["blocks_since_sr"] = 0
-                prop.stochastic_reconfiguration_local(prop_data)
-                dev.overlap(prop_data["walkers"], out=prop_data["overlaps"])
-
-    for _ in range(args.warmup):
-        one_step()
+    # the warm-up covers two resamplings too (the eager one and the capture)
+    total_warm = max(warm_blocks, 2 * n_ene_blocks)
+    for _ in range(total_warm):
+        block()
     torch.cuda.synchronize()
-    print(f"# {precision}: warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
-    if args.copy_probe:
-        copy_probe()
 
-    # phase breakdown: a short pass with an event pair around every phase (the pairs
-    # cost device time, so the timed region below records only the dominant phase)
-    nv = dev.vhs_entries()  # V kept whole or as its upper triangle
-    fl = flops_per_walker(norb, nelec, nchol, prop.n_exp_terms, walker_type, nv)
-    dev.prof_collect()  # drop anything recorded before
-    dev.prof_enable(True)
-    for _ in range(min(args.steps, PROFILE_STEPS)):
-        one_step()
-    torch.cuda.synchronize()
-    dev.prof_enable(False)
-    phases = dev.prof_collect()
-    # the population-control sum ("weights") is one workgroup reducing W doubles; its
-    # event pair directly follows the measurement kernel's and has been seen to take
-    # that kernel's start as its own, so it is not a dominant-kernel candidate
-    dom = max((p for p, (ms, n) in phases.items() if n and p in fl and p != "weights"),
-              key=lambda p: phases[p][0])
+    def timed():
+        for _ in range(timed_blocks):
+            block()
 
-    dev.prof_enable(True, phases=[dom])
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        one_step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    dev.prof_enable(False)
-    timed = dev.prof_collect()
-    dev.prof_enable(False, phases=None)
-    print(f"# {precision}: timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr,
-          flush=True)
-
-    nb = state["block"]
-    st = stats[:nb].cpu().numpy() if nb else np.zeros((0, 3))
-    e_mixed = float(np.sum(st[:, 0] * st[:, 1]) / np.sum(st[:, 1])) if nb else float("nan")
-
-    value = W * args.steps / elapsed
-    per_phase = {p: {"ms_total": round(ms, 4), "launches": n,
-                     "ms_avg": round(ms / n, 5) if n else None}
-                 for p, (ms, n) in phases.items() if n}
-    # every phase's roofline from the profiled pass; the dominant kernel's from its
-    # events over the timed region
-    rooflines = {}
-    for p, (ms, n) in phases.items():
-        if n:
-            r = roofline_of(p, precision, norb, nelec, nchol, prop.n_exp_terms, walker_type, W,
-                            ms / n / 1e3, nv)
-            if r is not None:
-                rooflines[p] = r
-    ms_dom, n_dom = timed[dom]
-    roofline = roofline_of(dom, precision, norb, nelec, nchol, prop.n_exp_terms, walker_type,
-                           W, ms_dom / n_dom / 1e3, nv)
-    roofline["launches_timed"] = n_dom
-    roofline["traffic"] = traffic_from_profiles(roofline["kernel"], args.workload, precision)
-    step_flops = sum(v for k, v in fl.items() if k not in ("energy", "qr")) + (
-        fl["energy"] + fl["qr"]) / N_PROP_STEPS
-    res = {
-        "value": value,
-        "elapsed": elapsed,
-        "block_energies": st[:, 0].tolist(),
-        "block_weights": st[:, 1].tolist(),
-        "mixed_energy": e_mixed,
-        "phases": per_phase,
-        "roofline": roofline,
-        "rooflines": {p: {k: r[k] for k in ("kernel", "bound", "achieved", "unit", "frac")}
-                      for p, r in rooflines.items()},
-        "achieved_tflops_step": round(step_flops * value / 1e12, 3),
-        "config": {
-            "workload": args.workload,
-            "walker_type": walker_type, "norb": norb, "nocc": nocc, "nchol": nchol,
-            "n_walkers": W, "dt": dt, "n_exp_terms": prop.n_exp_terms,
-            "n_prop_steps": N_PROP_STEPS, "n_ene_blocks": N_ENE_BLOCKS,
-        },
-    }
-    del dev, ham_data, prop_data, fields_buf
+    ms = event_repeats(stream, timed, REPEATS)
+    med = median(ms)
+    rate = W * N_PROP_STEPS * timed_blocks / (med / 1e3)
+    print(f"# single {workload} {precision}: median {med:.2f} ms / {timed_blocks} blocks -> "
+          f"{rate / 1e6:.3f} M walker-steps/s", file=sys.stderr, flush=True)
+    stats = torch.stack(hist).cpu().numpy()
+    # phases: one eager block with all event pairs, one with the dominant phase only
+    nv = dev.vhs_entries()
+    fl, by = phase_work(precision, [(walker_type, norb, nelec, nchol, W, nv,
+                                     prop.n_exp_terms)])
+    kern = lambda p: phase_kernel(precision, p, norb, nelec)  # noqa: E731
+    with torch.cuda.stream(stream):
+        dev.prof_collect()
+        dev.prof_enable(True)
+        st._block()
+        phases = dev.prof_collect()
+        dom, _ = dominant_roofline(precision, phases, fl, by, kern, workload)
+        dev.prof_enable(True, phases=[dom])
+        st._block()
+        timed_ph = dev.prof_collect()
+        dev.prof_enable(False, phases=None)
+    _, roof = dominant_roofline(precision, {dom: timed_ph[dom]}, fl, by, kern, workload)
+    step_phases = ("green", "fb_gemm", "fields", "vhs_gemm", "taylor", "overlap", "weights")
+    block_phases = ("qr", "energy", "rng")
+    phase_sum = (sum(phases[p][0] for p in step_phases) +
+                 sum(phases[p][0] for p in block_phases)) / N_PROP_STEPS
+    res = {"value": rate, "ms_per_step": med / (timed_blocks * N_PROP_STEPS),
+           "repeat_ms": [round(x, 3) for x in ms], "stats": stats,
+           "phases": {p: {"ms_total": round(v[0], 4), "launches": v[1],
+                          "ms_avg": round(v[0] / v[1], 5)} for p, v in phases.items() if v[1]},
+           "phase_sum_ms_per_step": round(phase_sum, 5),
+           "roofline": roof, "rooflines": all_rooflines(precision, phases, fl, by, kern),
+           "n_walkers": W}
+    del st, dev, ham_data, pd
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
-    return res, workload
+    _ = np
+    return res
 
 
-# ------------------------------------------------------------------ fragment batch
+# ------------------------------------------------------------------ H10 (config 1)
 
 
-FRAG_WORKLOAD = "fragments64_n20-120_w256"
-
-
-def _make_fragments(specs, precision, n_prop_steps):
-    from cs_afqmc_amd import batch, problems
-    frags = []
-    for norb, nocc, nchol, W, seed in specs:
-        hd, wd, ne = problems.synthetic_rhf(norb, nocc, nchol, seed=seed)
-        frags.append(batch.Fragment(hd, wd, "rhf", ne, W, dt=0.005, seed=seed,
-                                    precision=precision, n_prop_steps=n_prop_steps,
-                                    tag=(norb, nocc, nchol, W)))
-    return frags
-
-
-def _single_block(f):
-    """The plain (ungrouped) form: one fragment's block through afqmc_propagate."""
-    pd = f.prop_data
-    f.begin_block()
-    for s in range(f.n_prop_steps):
-        f.dev.propagate(pd["walkers"], pd["weights"], pd["overlaps"], f.fields[s], f.scalars,
-                        reuse_green=s > 0)
-    f.end_block()
-
-
-def run_fragments(args):
-    """Batched-fragment workload: 64 closed-shell fragments, norb uniform in [20, 120], nocc =
-    norb / 5, nchol = 4 norb, 256 walkers each.  value = walker-steps/s of the grouped
-    mixed-precision step; fp64_value = the plain float64 form (each fragment through
-    afqmc_propagate in turn); also the grouped float64 rate."""
-    import numpy as np
+def run_h10(precision, n_blocks):
+    """Linear H10, 1.6 bohr, STO-6G, RHF trial, modified Cholesky 1e-5 (own integrals),
+    200 walkers, dt 0.005, QR every 5 steps (n_prop_steps 5, n_ene_blocks 10: 50 steps
+    per sampler call), seed 0; n_blocks sampler calls with SR and the driver's
+    e_estimate update after each."""
     import torch
 
-    from cs_afqmc_amd import batch, problems
+    from cs_afqmc_amd import hchain, rng, sampling
 
-    torch.cuda.set_device(0)
-    n_prop = N_PROP_STEPS
-    blocks = max(1, args.steps // n_prop)
-    wblocks = max(1, args.warmup // n_prop)
-    specs = problems.fragment_specs(64, seed=0)
-    total_w = sum(s[3] for s in specs)
-    res = {}
-    energies = {}
-    for mode in ("bf16x3", "fp64", "fp64_plain"):
-        prec = "fp64" if mode.startswith("fp64") else mode
-        frags = _make_fragments(specs, prec, n_prop)
-        bt = None if mode == "fp64_plain" else batch.FragmentBatch(frags, tune=True,
-                                                                   use_graph=True)
-        for _ in range(wblocks):
-            if bt is None:
-                for f in frags:
-                    _single_block(f)
-            else:
-                bt.run_block()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(blocks):
-            if bt is None:
-                for f in frags:
-                    _single_block(f)
-            else:
-                bt.run_block()
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
-        res[mode] = total_w * n_prop * blocks / el
-        energies[mode] = [f.block_energies()[0] for f in frags]
-        print(f"# {mode}: {blocks} blocks in {el:.3f} s", file=sys.stderr, flush=True)
-        if bt is not None and bt.tuned:
-            res[mode + "_tuned"] = bt.tuned
-        del frags, bt
-        torch.cuda.synchronize()
-        torch.cuda.empty_cache()
-    # |dE| of the first block (before the first resampling: the precision's own effect)
-    # and of every block (later blocks can also differ by a discrete event, a walker's
-    # weight clipped or resampled in one precision and not the other)
-    d1 = [float(abs(a[0] - b[0])) for a, b in zip(energies["bf16x3"], energies["fp64_plain"])
-          if a.size]
-    d = [float(np.max(np.abs(a - b))) for a, b in zip(energies["bf16x3"], energies["fp64_plain"])
-         if a.size]
-    dg = [float(np.max(np.abs(a - b))) for a, b in zip(energies["fp64"], energies["fp64_plain"])
-          if a.size]
-    finite = [bool(np.all(np.isfinite(a))) for a in energies["bf16x3"]]
-    # per fragment: block-by-block |dE| and the energy scale, largest first
-    per = sorted(((float(np.max(np.abs(a - b))), i) for i, (a, b) in
-                  enumerate(zip(energies["bf16x3"], energies["fp64_plain"])) if a.size),
-                 reverse=True)
-    worst = [{"fragment": i, "norb": specs[i][0], "n_walkers": specs[i][3],
-              "delta_e_blocks_eh": np.abs(energies["bf16x3"][i] - energies["fp64_plain"][i]).tolist(),
-              "energy_scale_eh": float(np.max(np.abs(energies["fp64_plain"][i])))}
-             for _, i in per[:4]]
-    return {
-        "metric": "walker-steps/s mixed vs FP64 GPU at |dE|<0.1 mEh (batched fragments)",
-        "value": round(res["bf16x3"], 1),
-        "unit": "walker-steps/s",
-        "n_gpus": 1,
-        "steps": blocks * n_prop,
-        "warmup": wblocks * n_prop,
-        "ms_per_step": round(total_w / res["bf16x3"] * 1e3, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": DTYPE["bf16x3"],
-        "data": "synthetic fragments (seeded Cholesky vectors, h1, Haar trial; device Philox "
-                "fields)",
-        "config": {"workload": FRAG_WORKLOAD, "n_fragments": len(specs),
-                   "norb": [s[0] for s in specs], "n_walkers_total": total_w,
-                   "n_prop_steps": n_prop, "dt": 0.005, "precision": "bf16x3"},
-        "fp64_value": round(res["fp64_plain"], 1),
-        "fp64_grouped_value": round(res["fp64"], 1),
-        "speedup_vs_fp64": round(res["bf16x3"] / res["fp64_plain"], 3),
-        "delta_e_max_eh": max(d1) if d1 else None,
-        "delta_e_max_all_blocks_eh": max(d) if d else None,
-        "delta_e_fp64_grouped_vs_plain_eh": max(dg) if dg else None,
-        "delta_e_worst_fragments": worst,
-        "fragments_finite": sum(finite),
-        "gemm_tuning_ms": {k: v for k, v in res.items() if k.endswith("_tuned")},
-        "roofline": None,
-        "cpu_baseline": None,
-    }
-
-
-# ------------------------------------------------------------------ serving
-
-
-SERVE_WORKLOAD = "serve_poisson200_n20-250"
-# shortest estimated solve first among queued requests, at most 8 fragments stepping
-# together (measured on this workload: p50 0.69 s, p99 1.84 s against 1.68 s / 2.03 s
-# for arrival order with every request admitted at once)
-SERVE_POLICY, SERVE_MAX_ACTIVE = "sjf", 8
-
-
-def run_serving(args):
-    """Serving workload: Poisson arrivals (mean 200 requests/s) of fragment solves with
-    norb uniform in [20, 250], nocc = norb / 5, nchol = 4 norb, walkers in {128, 256,
-    512, 1024}, 100 steps each; a bounded sample of --requests requests.  Integrals are
-    the requests' payload (made before the clock starts); intermediates, context
-    creation and the solve are the server's work.  value = walker-steps/s of the whole
-    run (completed walker-steps / makespan); p50 / p99 latency from arrival to GPU
-    completion."""
+    setup = hchain.h10_config1(precision=precision)
+    smp = sampling.sampler(n_prop_steps=5, n_ene_blocks=10, n_sr_blocks=1)
+    pd = setup["prop_data"]
+    pd["key"] = rng.PRNGKey(0)
+    rows = []
+    t0 = time.perf_counter()
+    for _ in range(n_blocks):
+        e, pd = smp.propagate_phaseless(setup["ham"], setup["ham_data"], setup["prop"], pd,
+                                        setup["trial"], setup["wave_data"])
+        rows.append(pd["block_stats"].cpu().numpy())
+        pd["e_estimate"] = 0.9 * pd["e_estimate"] + 0.1 * e
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
     import numpy as np
-    import torch
+    stats = np.concatenate(rows)
+    return {"stats": stats, "wall_s": wall, "nchol": setup["nchol"],
+            "e_hf": setup["e_hf"], "walker_steps": 200 * 50 * n_blocks}
+
+
+# ------------------------------------------------------------------ config 5: serving
+
+
+def serving_requests(n_requests):
+    import numpy as np
 
     from cs_afqmc_amd import problems, serving
 
-    torch.cuda.set_device(0)
     rng = np.random.default_rng(2024)
     t = 0.0
     reqs = []
-    for i in range(args.requests):
+    for i in range(n_requests):
         t += rng.exponential(1.0 / 200.0)
         norb = int(rng.integers(20, 251))
         nocc = max(1, norb // 5)
@@ -583,126 +590,225 @@ def run_serving(args):
         hd, wd, ne = problems.synthetic_rhf(norb, nocc, 4 * norb, seed=5000 + i)
         reqs.append(serving.Request(arrival=t, rid=i, ham_data=hd, wave_data=wd, nelec=ne,
                                     n_walkers=W, n_steps=100, seed=17 + i))
-    print(f"# serving: {len(reqs)} requests generated", file=sys.stderr, flush=True)
+    return reqs
+
+
+def run_serving(reqs):
+    import numpy as np
+    import torch
+
+    from cs_afqmc_amd import serving
+
+    torch.cuda.set_device(0)
     srv = serving.FragmentServer(precision="bf16x3", n_prop_steps=N_PROP_STEPS,
                                  hbm_budget=200 << 30, max_active=SERVE_MAX_ACTIVE,
-                                 policy=SERVE_POLICY)
+                                 policy=SERVE_POLICY, profile=True)
     t0 = time.perf_counter()
     res = srv.serve(reqs)
     wall = time.perf_counter() - t0
     ws = sum(r.walker_steps for r in res)
     makespan = max(r.done for r in res)
     st = serving.latency_stats(res)
-    return {
-        "metric": "p50/p99 fragment latency under load, with walker-steps/s (serving)",
+    last_arrival = max(r.arrival for r in reqs)
+    out = {
         "value": round(ws / makespan, 1),
         "unit": "walker-steps/s",
-        "n_gpus": 1,
-        "steps": 100,
-        "warmup": 0,
-        "ms_per_step": None,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": DTYPE["bf16x3"],
-        "data": "synthetic fragment requests (seeded integrals, Haar trial), Poisson arrivals",
-        "config": {"workload": SERVE_WORKLOAD, "n_requests": len(reqs),
-                   "arrival_rate_per_s": 200.0, "n_prop_steps": N_PROP_STEPS,
-                   "steps_per_request": 100, "hbm_budget_gib": 200, "precision": "bf16x3",
-                   "admission": SERVE_POLICY, "max_active_fragments": SERVE_MAX_ACTIVE},
-        "latency_p50_s": st["p50_s"],
-        "latency_p99_s": st["p99_s"],
-        "latency_mean_s": st["mean_s"],
-        "latency_max_s": st.get("max_s"),
-        "makespan_s": makespan,
-        "wall_s": wall,
+        "latency_p50_s": st["p50_s"], "latency_p99_s": st["p99_s"],
+        "latency_mean_s": st["mean_s"], "latency_max_s": st.get("max_s"),
+        "n_requests": len(reqs), "completed": len(res),
+        "arrival_window_s": round(last_arrival, 3),
+        "offered_walker_steps_per_s": round(
+            sum(r.n_walkers * r.n_steps for r in reqs) / max(last_arrival, 1e-9), 1),
+        "makespan_s": round(makespan, 3), "wall_s": round(wall, 3),
         "build_s_mean": float(np.mean([r.build_s for r in res])),
-        "build_s_max": float(np.max([r.build_s for r in res])),
-        "admit_to_done_s_mean": float(np.mean([r.done - r.admitted for r in res])),
         "arrival_to_admit_s_mean": float(np.mean([r.admitted - r.arrival for r in res])),
+        "admit_to_done_s_mean": float(np.mean([r.done - r.admitted for r in res])),
         "peak_active_fragments": srv.peak_active,
-        "peak_reserved_fragments": srv.peak_reserved,
-        "completed": len(res),
-        "roofline": None,
-        "cpu_baseline": None,
+        "peak_device_bytes_counted": srv.peak_used,
+        "admission": SERVE_POLICY, "max_active_fragments": SERVE_MAX_ACTIVE,
     }
+    if srv.phase_ms:
+        fl, by, ms = srv.phase_flops, srv.phase_bytes, srv.phase_ms
+        kern = lambda p: phase_kernel("bf16x3", p, 250, (50, 50))  # noqa: E731
+        cands = [p for p in ms if ms[p][1] and p in fl and p != "weights"]
+        if cands:
+            dom = max(cands, key=lambda p: ms[p][0])
+            # work and time summed over every tick: achieved = total work / total time
+            tot_ms, n = ms[dom]
+            r = roofline_record(dom, "bf16x3", kern(dom), fl[dom] / n, by[dom] / n,
+                                tot_ms / n / 1e3)
+            r["launches_timed"] = n
+            r["traffic"] = None
+            out["roofline"] = r
+    return out
 
 
-DTYPE = {"fp64": "f64",
-         "bf16x3": "mixed: bf16x3 MFMA contractions and Taylor products (f32 accumulation), "
-                   "f64 overlaps/QR/weights/energy accumulation"}
+# ------------------------------------------------------------------ main
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="rhf_n100_o20_g400_w1024")
-    ap.add_argument("--precision", default="compare", choices=["compare", "fp64", "bf16x3"],
-                    help="compare: float64 then mixed at the same fields (default)")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--steps", type=int, default=50,
+                    help="timed steps per repeat, rounded up to whole 50-step blocks")
+    ap.add_argument("--warmup", type=int, default=50,
+                    help="warm-up steps, rounded up to whole blocks (plus the capture)")
+    ap.add_argument("--workload", default="all",
+                    choices=["all", FRAG_WORKLOAD, SERVE_WORKLOAD, H10_WORKLOAD,
+                             *SINGLE_WORKLOADS])
+    ap.add_argument("--precision", default="compare",
+                    choices=["compare", "fp64", "fp32", "bf16x3"])
+    ap.add_argument("--requests", type=int, default=64,
+                    help="requests of the serving workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--requests", type=int, default=32,
-                    help="number of requests of the serving workload")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="run a 1 GiB device copy before the timed region (PMC scaling)")
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: the walker population runs on one GPU; --gpus {args.gpus} "
-              "is not supported (sharding across GPUs is not implemented).", file=sys.stderr)
+        print(f"bench.py: the walker population runs on one GPU; --gpus {args.gpus} is not "
+              "supported (sharding across GPUs is not implemented).", file=sys.stderr)
         sys.exit(2)
-    if args.workload == FRAG_WORKLOAD:
-        print(json.dumps(run_fragments(args)), flush=True)
-        return
-    if args.workload == SERVE_WORKLOAD:
-        print(json.dumps(run_serving(args)), flush=True)
-        return
-    modes = ["fp64", "bf16x3"] if args.precision == "compare" else [args.precision]
-    cpu = None
+
+    warm, timed = blocks_for(args.warmup) + 1, blocks_for(args.steps)
+    mixed = "bf16x3" if args.precision in ("compare", "bf16x3") else args.precision
+    want = args.workload
+
+    # CPU baselines first: the worker processes are spawned before anything touches the GPU
+    cpu = {}
     if not args.no_cpu_baseline:
-        # before anything touches the GPU: the worker processes are started by spawn
-        from cs_afqmc_amd import problems
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(problems.WORKLOADS[args.workload], args.cpu_seconds)
-    runs = {}
-    for m in modes:
-        runs[m], _ = run(args, m)
-    main_mode = modes[-1]
-    r = runs[main_mode]
-    out = {
-        "metric": "walker-steps/s mixed vs FP64 GPU at |dE|<0.1 mEh (phaseless AFQMC step)",
-        "value": round(r["value"], 1),
-        "unit": "walker-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(r["elapsed"] / args.steps * 1e3, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": DTYPE[main_mode],
-        "data": "synthetic (seeded Cholesky vectors, h1, Haar trial; device Philox fields)",
-        "config": dict(r["config"], precision=main_mode),
-    }
-    if "fp64" in runs and "bf16x3" in runs:
-        b64 = runs["fp64"]["block_energies"]
-        bmx = runs["bf16x3"]["block_energies"]
-        d = [abs(a - b) for a, b in zip(bmx, b64)]
-        out["fp64_value"] = round(runs["fp64"]["value"], 1)
-        out["speedup_vs_fp64"] = round(runs["bf16x3"]["value"] / runs["fp64"]["value"], 3)
-        out["delta_e_max_eh"] = max(d) if d else None
-        out["delta_e_mean_eh"] = abs(runs["bf16x3"]["mixed_energy"] - runs["fp64"]["mixed_energy"])
-        out["fp64_phases"] = runs["fp64"]["phases"]
-        out["fp64_roofline"] = runs["fp64"]["roofline"]
-    out["mixed_energy"] = r["mixed_energy"]
-    out["block_energies"] = r["block_energies"]
-    out["achieved_tflops_step"] = r["achieved_tflops_step"]
-    out["phases"] = r["phases"]
-    out["roofline"] = r["roofline"]
-    out["rooflines"] = r["rooflines"]
-    out["cpu_baseline"] = cpu
+        if want in ("all", FRAG_WORKLOAD):
+            cpu["fragments"] = cpu_baseline(fragment_specs(), 2, 3, FRAG_WORKLOAD)
+        if want in SINGLE_WORKLOADS:
+            wt, n, o, g, W = SINGLE_WORKLOADS[want]
+            cpu["single"] = cpu_baseline([(wt, n, o, g, W, 11 + i) for i in range(16)], 4, 4,
+                                         want)
+    import torch
+    torch.cuda.set_device(0)
+
+    out = {}
+    if want in ("all", FRAG_WORKLOAD):
+        modes = ([f"{mixed}_grouped", "fp64_grouped", "fp64_plain"]
+                 if args.precision == "compare" else [f"{mixed}_grouped"])
+        specs, runs = run_fragments(args, modes)
+        m = runs[f"{mixed}_grouped"]
+        total_w = sum(s[4] for s in specs)
+        out.update({
+            "metric": "walker-steps/s mixed vs FP64 GPU at |dE|<0.1 mEh; p50/p99 fragment "
+                      "latency",
+            "value": round(m["value"], 1),
+            "unit": "walker-steps/s",
+            "n_gpus": 1,
+            "steps": timed * N_PROP_STEPS,
+            "warmup": warm * N_PROP_STEPS,
+            "ms_per_step": round(m["ms_per_step"], 5),
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": DTYPE[mixed],
+            "data": "synthetic fragments (seeded Cholesky vectors N(0, 1/norb^2) x exp(-g/80), "
+                    "h1, Haar trial; device Philox fields)",
+            "config": {"workload": FRAG_WORKLOAD, "n_fragments": len(specs),
+                       "norb_range": [20, 120], "nocc": "norb/5", "nchol": "4 norb",
+                       "n_walkers_total": total_w, "walkers_per_fragment": 256,
+                       "n_prop_steps": N_PROP_STEPS, "dt": DT, "precision": mixed,
+                       "form": "grouped step, whole block replayed from one HIP graph",
+                       "repeats": REPEATS, "blocks_per_repeat": timed},
+            "repeat_ms": m["repeat_ms"],
+        })
+        if "fp64_plain" in runs:
+            out["fp64_value"] = round(runs["fp64_plain"]["value"], 1)
+            out["fp64_form"] = "plain: each fragment's block through its own context in turn"
+            out["speedup_vs_fp64"] = round(m["value"] / runs["fp64_plain"]["value"], 3)
+            out["fp64_grouped_value"] = round(runs["fp64_grouped"]["value"], 1)
+            out["speedup_vs_fp64_grouped"] = round(m["value"] / runs["fp64_grouped"]["value"], 3)
+            comps = [compare_blocks(a, b, s[4]) for a, b, s in
+                     zip(m["stats"], runs["fp64_plain"]["stats"], specs)]
+            out.update(summarize_delta(comps))
+            gp = [compare_blocks(a, b, s[4]) for a, b, s in
+                  zip(runs["fp64_grouped"]["stats"], runs["fp64_plain"]["stats"], specs)]
+            out["delta_e_fp64_grouped_vs_plain_eh"] = summarize_delta(gp)["delta_e_max_eh"]
+            out["fp64_phases"] = runs["fp64_grouped"].get("phases")
+            out["fp64_roofline"] = runs["fp64_grouped"].get("roofline")
+        out["phases"] = m.get("phases")
+        out["phase_sum_ms_per_step"] = m.get("phase_sum_ms_per_step")
+        out["gemm_tile_ms"] = m.get("gemm_tile_ms")
+        out["roofline"] = m.get("roofline")
+        out["rooflines"] = m.get("rooflines")
+        out["cpu_baseline"] = cpu.get("fragments")
+
+    if want == "all" or want in SINGLE_WORKLOADS:
+        wl = want if want in SINGLE_WORKLOADS else "rhf_n100_o20_g400_w1024"
+        precs = ["fp64", mixed] if args.precision == "compare" else [mixed]
+        sr = {p: run_single(wl, p, warm, timed) for p in precs}
+        W = SINGLE_WORKLOADS[wl][4]
+        single = {"workload": wl,
+                  "value": round(sr[mixed]["value"], 1),
+                  "ms_per_step": round(sr[mixed]["ms_per_step"], 5),
+                  "phase_sum_ms_per_step": sr[mixed]["phase_sum_ms_per_step"],
+                  "repeat_ms": sr[mixed]["repeat_ms"],
+                  "phases": sr[mixed]["phases"], "roofline": sr[mixed]["roofline"],
+                  "rooflines": sr[mixed]["rooflines"]}
+        if "fp64" in sr and mixed != "fp64":
+            single["fp64_value"] = round(sr["fp64"]["value"], 1)
+            single["fp64_ms_per_step"] = round(sr["fp64"]["ms_per_step"], 5)
+            single["fp64_phase_sum_ms_per_step"] = sr["fp64"]["phase_sum_ms_per_step"]
+            single["speedup_vs_fp64"] = round(sr[mixed]["value"] / sr["fp64"]["value"], 3)
+            single["fp64_roofline"] = sr["fp64"]["roofline"]
+            single.update(summarize_delta([compare_blocks(sr[mixed]["stats"],
+                                                          sr["fp64"]["stats"], W)]))
+        if want == "all":
+            out["single"] = single
+        else:
+            out.update({"metric": "walker-steps/s mixed vs FP64 GPU (one problem)",
+                        "value": single.pop("value"), "unit": "walker-steps/s", "n_gpus": 1,
+                        "steps": timed * N_PROP_STEPS, "warmup": warm * N_PROP_STEPS,
+                        "ms_per_step": single.pop("ms_per_step"), "higher_is_better": True,
+                        "scaling": "weak", "vs_baseline": None, "dtype": DTYPE[mixed],
+                        "data": "synthetic", "config": {"workload": wl, "precision": mixed}})
+            out.update(single)
+            out["cpu_baseline"] = cpu.get("single")
+
+    if want in ("all", H10_WORKLOAD):
+        try:
+            h = {p: run_h10(p, 6) for p in ("fp64", mixed)}
+            hd = summarize_delta([compare_blocks(h[mixed]["stats"], h["fp64"]["stats"], 200)])
+            h10 = {"workload": H10_WORKLOAD, "nchol": h["fp64"]["nchol"],
+                   "e_hf": h["fp64"]["e_hf"],
+                   "energy_fp64": float(_wmean(h["fp64"]["stats"])),
+                   "energy_mixed": float(_wmean(h[mixed]["stats"]))}
+            h10.update(hd)
+            out["h10"] = h10
+        except ImportError as e:  # config-1 integrals not available
+            out["h10"] = {"error": str(e)}
+
+    if want in ("all", SERVE_WORKLOAD) and not args.no_serving:
+        t0 = time.perf_counter()
+        reqs = serving_requests(args.requests)
+        print(f"# serving: {len(reqs)} requests generated in {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+        sv = run_serving(reqs)
+        print(f"# serving: p50 {sv['latency_p50_s']:.3f} s, p99 {sv['latency_p99_s']:.3f} s",
+              file=sys.stderr, flush=True)
+        if want == "all":
+            out["serving"] = dict(sv, workload=SERVE_WORKLOAD)
+            out["latency_p50_s"] = sv["latency_p50_s"]
+            out["latency_p99_s"] = sv["latency_p99_s"]
+        else:
+            out.update({"metric": "p50/p99 fragment latency under load (serving)",
+                        "value": sv.pop("value"), "unit": "walker-steps/s", "n_gpus": 1,
+                        "steps": 100, "warmup": 0, "ms_per_step": None,
+                        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                        "dtype": DTYPE["bf16x3"], "data": "synthetic fragment requests",
+                        "config": {"workload": SERVE_WORKLOAD}})
+            out.update(sv)
+            out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
+
+
+def _wmean(stats):
+    import numpy as np
+    ok = np.isfinite(stats[:, 0]) & (stats[:, 1] > 0)
+    return np.sum(stats[ok, 0] * stats[ok, 1]) / np.sum(stats[ok, 1])
 
 
 if __name__ == "__main__":

@@ -13,6 +13,7 @@ AFQMC_RHF = 1
 AFQMC_UHF = 2
 AFQMC_STEP_REUSE_GREEN = 1
 AFQMC_SR_GLOBAL_ORDER = 1
+BLOCK_STATS_LEN = 4  # AFQMC_BLOCK_STATS_LEN: block energy, weight, zero weights, clamped
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 
 # every symbol declared in include/cs_afqmc_amd.h
@@ -29,11 +30,14 @@ EXPORTS = (
     "afqmc_energy",
     "afqmc_orthonormalize",
     "afqmc_sr",
+    "afqmc_sr_key",
     "afqmc_block_stats",
+    "afqmc_estimate_mix",
     "afqmc_prof_enable",
     "afqmc_prof_collect",
     "afqmc_prof_phases",
     "afqmc_fields_normal",
+    "afqmc_fields_normal_key",
     "afqmc_cs_outliers",
     "afqmc_group_create",
     "afqmc_group_destroy",
@@ -42,6 +46,9 @@ EXPORTS = (
     "afqmc_group_add",
     "afqmc_group_size",
     "afqmc_group_propagate",
+    "afqmc_group_prof_enable",
+    "afqmc_group_prof_phases",
+    "afqmc_group_prof_collect",
     "afqmc_group_tune",
 )
 
@@ -106,12 +113,15 @@ def _declare(lib):
     lib.afqmc_energy.argtypes = [_vp, _i32, _vp, _vp, _vp]
     lib.afqmc_orthonormalize.argtypes = [_vp, _i32, _vp, _vp]
     lib.afqmc_sr.argtypes = [_vp, _i32, _vp, _vp, _vp, ctypes.c_double, _vp, _i32]
+    lib.afqmc_sr_key.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _i32]
     lib.afqmc_block_stats.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp]
+    lib.afqmc_estimate_mix.argtypes = [_vp, _vp, _vp, _i32]
     lib.afqmc_prof_enable.argtypes = [_vp, _i32]
     lib.afqmc_prof_collect.argtypes = [_vp, _vp, _vp]
     lib.afqmc_prof_phases.argtypes = [_vp, ctypes.c_uint32]
     lib.afqmc_fields_normal.argtypes = [
         _vp, ctypes.c_uint64, ctypes.c_uint64, _vp, ctypes.c_int64]
+    lib.afqmc_fields_normal_key.argtypes = [_vp, _vp, _vp, ctypes.c_int64]
     lib.afqmc_cs_outliers.argtypes = [_vp, _i32, _vp, _vp, ctypes.c_double]
     lib.afqmc_group_create.argtypes = [ctypes.POINTER(_vp), _i32, _vp]
     lib.afqmc_group_destroy.argtypes = [_vp]
@@ -121,6 +131,9 @@ def _declare(lib):
     lib.afqmc_group_size.argtypes = [_vp]
     lib.afqmc_group_propagate.argtypes = [_vp, _i32, _i32]
     lib.afqmc_group_tune.argtypes = [_vp, _vp]
+    lib.afqmc_group_prof_enable.argtypes = [_vp, _i32]
+    lib.afqmc_group_prof_phases.argtypes = [_vp, ctypes.c_uint32]
+    lib.afqmc_group_prof_collect.argtypes = [_vp, _vp, _vp]
     for name in EXPORTS:
         fn = getattr(lib, name)
         if fn.restype is ctypes.c_int:  # default

@@ -7,19 +7,25 @@ fragment keeps its own device context and resident state, and one grouped step
 the step is a single launch over every fragment's walkers, so small fragments fill the
 GPU together.
 
-A fragment's block follows sampler.propagate_phaseless (sampling.py:319-341) with
-n_ene_blocks = 1: fields for n_prop_steps steps are drawn on the device from the
-fragment's key, the steps run grouped, then QR, overlaps, local energies and the block
-statistics run per fragment, and stochastic reconfiguration closes the block.  Each
-fragment's result depends only on its own inputs and seed: a fragment gives bit-identical
-results alone and in any mix (no split-K, fixed summation order in every kernel).
+A fragment's block is one sampler call of the fragment driver
+(lno_afqmc/ccsd_pt2/run_afqmc_nompi.py:83-90 around sampling.py:319-341 with one
+energy block): fields for n_prop_steps steps are drawn on the device from the
+fragment's device-resident key, the steps run grouped, then QR, overlaps, local
+energies and the block statistics, stochastic reconfiguration with the key's next
+uniform draw, and the driver's e_estimate <- 0.9 e_estimate + 0.1 E_block.  Nothing in
+a block reads back to the host, so the whole block of every fragment -- draws, grouped
+steps, block ends -- is captured once into a HIP graph and replayed while the member
+set is unchanged.  Each fragment's result depends only on its own inputs and seed: a
+fragment gives bit-identical results alone and in any mix (no split-K, fixed summation
+order in every kernel).
 """
 
 import numpy as np
 import torch
 
-from . import hamiltonian, propagation, rng, wavefunctions
+from . import _lib, hamiltonian, propagation, rng, wavefunctions
 from .device import DeviceGroup
+from .graphs import Replay
 
 
 class Fragment:
@@ -32,6 +38,7 @@ class Fragment:
                  precision="bf16x3", n_prop_steps=50, n_exp_terms=6, tag=None):
         norb = int(np.asarray(ham_data["h1"]).shape[-1])
         self.tag = tag
+        self.walker_type = walker_type
         self.norb, self.nelec, self.n_walkers = norb, tuple(nelec), int(n_walkers)
         self.n_prop_steps = int(n_prop_steps)
         if walker_type == "rhf":
@@ -55,11 +62,14 @@ class Fragment:
         e0 = pd["e_estimate"].reshape(()).to(torch.float64)
         self.scalars = torch.stack([e0, e0]).contiguous()
         pd["_scalars"] = self.scalars
+        self.key = rng.device_key(pd["key"], ddev)
         self.fields = torch.empty((self.n_prop_steps, self.n_walkers, self.dev.nchol),
                                   dtype=torch.float64, device=ddev)
         self.energies = torch.empty(self.n_walkers, dtype=torch.complex128, device=ddev)
-        self.stats = []  # one (3,) device tensor per finished block
+        self.slot = torch.zeros(_lib.BLOCK_STATS_LEN, dtype=torch.float64, device=ddev)
+        self.stats = []  # one (BLOCK_STATS_LEN,) device tensor per finished block
         self.blocks_done = 0
+        self._single = None  # captured single-context block (the plain form)
         self.dev.overlap(pd["walkers"], out=pd["overlaps"])
 
     @property
@@ -74,30 +84,60 @@ class Fragment:
         return int(self.dev.device_bytes()) + fields + walkers
 
     def begin_block(self):
-        """sampling.py:60-69: split the key and draw this block's fields."""
-        pd = self.prop_data
-        pd["key"], subkey = rng.split(pd["key"])
-        self.dev.fields_normal(subkey[0], subkey[1], self.fields.shape, out=self.fields)
+        """sampling.py:60-69: split the (device) key and draw this block's fields."""
+        self.dev.fields_normal_key(self.key, self.fields)
 
     def end_block(self):
-        """sampling.py:74-91 and :142: QR, overlaps, energies, block statistics, SR."""
+        """sampling.py:74-91 and :142-143, then the driver's estimate update
+        (run_afqmc_nompi.py:89): QR, overlaps, energies, block statistics into
+        self.slot, SR, overlaps, e_estimate mix.  Host bookkeeping is in record()."""
         pd = self.prop_data
         self.dev.orthonormalize(pd["walkers"])
         self.dev.overlap(pd["walkers"], out=pd["overlaps"])
         en = self.dev.energy(pd["walkers"], out=self.energies)
-        st = torch.zeros(3, dtype=torch.float64, device=self.dev.device)
-        self.dev.block_stats(en, pd["weights"], self.scalars, st)
-        self.stats.append(st)
-        self.prop.stochastic_reconfiguration_local(pd)
+        self.dev.block_stats(en, pd["weights"], self.scalars, self.slot)
+        self.dev.sr_key(pd["walkers"], pd["weights"], self.key)
         self.dev.overlap(pd["walkers"], out=pd["overlaps"])
+        self.dev.estimate_mix(self.scalars, self.slot)
+
+    def record(self, stats_row):
+        """Host side of a finished block: keep its statistics row (a device tensor),
+        advance the host copy of the key by the block's two draws."""
+        self.stats.append(stats_row)
+        self.prop_data["key"] = rng.advance(self.prop_data["key"], 2)
         self.blocks_done += 1
+
+    def _steps_single(self):
+        pd = self.prop_data
+        for s in range(self.n_prop_steps):
+            self.dev.propagate(pd["walkers"], pd["weights"], pd["overlaps"], self.fields[s],
+                               self.scalars, reuse_green=s > 0)
+
+    def _block_single(self):
+        self.begin_block()
+        self._steps_single()
+        self.end_block()
+
+    def run_block_single(self, use_graph=True):
+        """The plain (ungrouped) form: this fragment's block through its own context
+        (afqmc_propagate per step), captured once and replayed."""
+        if self._single is None or self._single.use_graph != use_graph:
+            if self._single is not None:
+                self._single.release()
+            self._single = Replay(self._block_single, use_graph)
+        self._single()
+        self.record(self.slot.clone())
 
     def block_energies(self):
         """(block energies, block weights) as numpy arrays (synchronises)."""
-        if not self.stats:
-            return np.zeros(0), np.zeros(0)
-        st = torch.stack(self.stats).cpu().numpy()
+        st = self.block_stats()
         return st[:, 0], st[:, 1]
+
+    def block_stats(self):
+        """(blocks, BLOCK_STATS_LEN) numpy array: energy, weight, zero weights, clamped."""
+        if not self.stats:
+            return np.zeros((0, _lib.BLOCK_STATS_LEN))
+        return torch.stack(self.stats).cpu().numpy()
 
     def energy(self):
         be, bw = self.block_energies()
@@ -107,11 +147,11 @@ class Fragment:
 class FragmentBatch:
     """A set of fragments advanced by grouped steps (one group, one stream).
 
-    With use_graph, the n_prop_steps grouped steps of a block are captured once into a
-    HIP graph (torch.cuda.CUDAGraph over the batch's stream, which is then a stream of its
-    own unless one is given) and replayed every block while the member set is unchanged;
-    the first block after a change runs eagerly (it rebuilds the group's tables), the
-    next one captures.  The caller's current stream waits for each block."""
+    With use_graph, a block of every fragment (fields, n_prop_steps grouped steps, block
+    ends) is captured once into a HIP graph and replayed every block while the member
+    set is unchanged; the first block after a change runs eagerly (it rebuilds the
+    group's tables), the next one captures.  The caller's current stream waits for
+    each block."""
 
     def __init__(self, fragments=(), max_fragments=1024, tune=False, use_graph=False,
                  stream=None):
@@ -123,16 +163,28 @@ class FragmentBatch:
             self.group = DeviceGroup(max_fragments)
         self.fragments = []
         self.use_graph = use_graph
-        self._version = 0        # bumped on every membership change
-        self._warm_version = -1  # version whose tables an eager block has built
-        self._graph = None
-        self._graph_version = -1
+        self._replay = None
+        self.tuned = None
         for f in fragments:
             self.add(f)
-        self.tuned = None
         if tune and self.fragments:
-            with torch.cuda.stream(self.stream):
-                self.tuned = self.group.tune()
+            self.tune()
+
+    @property
+    def _graph(self):
+        return None if self._replay is None else self._replay.graph
+
+    def tune(self):
+        """Pick GEMM tiles for the current members by timing (afqmc_group_tune)."""
+        self._drop_replay()  # re-capture with the chosen tiles
+        with torch.cuda.stream(self.stream):
+            self.tuned = self.group.tune()
+        return self.tuned
+
+    def _drop_replay(self):
+        if self._replay is not None:
+            self._replay.release()
+        self._replay = None
 
     def add(self, frag):
         if self.fragments and frag.n_prop_steps != self.fragments[0].n_prop_steps:
@@ -140,42 +192,39 @@ class FragmentBatch:
         pd = frag.prop_data
         # work that set the fragment up on the caller's stream comes first
         self.stream.wait_stream(torch.cuda.current_stream())
+        self._drop_replay()
         self.group.add(frag.dev, pd["walkers"], pd["weights"], pd["overlaps"], frag.fields,
                        frag.scalars)
         self.fragments.append(frag)
-        self._version += 1
 
     def remove(self, frags):
         """Drop finished fragments (the group's tables are rebuilt on the next step)."""
         gone = {id(f) for f in frags}
         keep = [f for f in self.fragments if id(f) not in gone]
+        self._drop_replay()
         self.group.clear()
         self.fragments = []
         for f in keep:
             self.add(f)
 
-    def _steps(self):
+    def _block(self):
+        for f in self.fragments:
+            f.begin_block()
         for s in range(self.fragments[0].n_prop_steps):
             self.group.propagate(s, reuse_green=s > 0)
+        for f in self.fragments:
+            f.end_block()
 
     def run_block(self):
-        """One block of every fragment: fields, n_prop_steps grouped steps, block end."""
+        """One block of every fragment: fields, n_prop_steps grouped steps, block ends."""
         if not self.fragments:
             return
         with torch.cuda.stream(self.stream):
-            for f in self.fragments:
-                f.begin_block()
-            if not self.use_graph or self._warm_version != self._version:
-                self._steps()
-                self._warm_version = self._version
-            else:
-                if self._graph_version != self._version:
-                    self._graph = torch.cuda.CUDAGraph()
-                    with torch.cuda.graph(self._graph, stream=self.stream):
-                        self._steps()
-                    self._graph_version = self._version
-                self._graph.replay()
-            for f in self.fragments:
-                f.end_block()
+            if self._replay is None:
+                self._replay = Replay(self._block, self.use_graph)
+            self._replay()
+            rows = torch.stack([f.slot for f in self.fragments])
+            for i, f in enumerate(self.fragments):
+                f.record(rows[i])
         # later work on the caller's stream (reading results) follows the block
         torch.cuda.current_stream().wait_stream(self.stream)

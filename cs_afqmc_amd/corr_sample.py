@@ -15,7 +15,7 @@ contexts bound to the same fields block), or through their own contexts in turn.
 import numpy as np
 import torch
 
-from . import rng
+from . import _lib, rng
 from .device import DeviceGroup
 
 
@@ -40,7 +40,7 @@ def _field_block_end(sy):
     sy.dev.orthonormalize(pd["walkers"])
     sy.dev.overlap(pd["walkers"], out=pd["overlaps"])
     en = sy.dev.energy(pd["walkers"], out=sy.energies)
-    st = torch.zeros(3, dtype=torch.float64, device=sy.dev.device)
+    st = torch.zeros(_lib.BLOCK_STATS_LEN, dtype=torch.float64, device=sy.dev.device)
     sy.dev.block_stats(en, pd["weights"], sy.scalars, st)  # pcs <- 0.9 pcs + 0.1 E_block
     return en
 
@@ -48,7 +48,7 @@ def _field_block_end(sy):
 def _local_sums(sy, en):
     """corr_sample.py:672-684: sum(clamped E_L * w) and sum(w) (device scalars)."""
     scratch = torch.stack([sy.scalars[0], sy.scalars[1]]).contiguous()  # keeps pcs untouched
-    st = torch.zeros(3, dtype=torch.float64, device=sy.dev.device)
+    st = torch.zeros(_lib.BLOCK_STATS_LEN, dtype=torch.float64, device=sy.dev.device)
     sy.dev.block_stats(en, sy.pd["weights"], scratch, st)
     w = st[1]
     return torch.where(w > 0, st[0] * w, torch.zeros_like(w)), w

@@ -80,11 +80,7 @@ void free_ctx(afqmc_ctx* c) {
                   c->sr_cum, c->sr_idx, c->xs_bf[0], c->xs_bf[1], c->green_bf[0][0],
                   c->green_bf[0][1], c->green_bf[1][0], c->green_bf[1][1], c->vhs_f32,
                   c->sym_stat};
-  if (c->prof_ev) {
-    for (int i = 0; i < 2 * c->prof_cap; ++i) (void)hipEventDestroy(c->prof_ev[i]);
-    delete[] c->prof_ev;
-    delete[] c->prof_phase;
-  }
+  prof_free(&c->prof);
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete c;
@@ -434,37 +430,55 @@ extern "C" int afqmc_cs_outliers(afqmc_ctx* c, int32_t W, double* weights1, doub
   return launch_cs_outliers(c, W, weights1, weights2, z_thresh);
 }
 
+int prof_enable(ProfRec* p, int on) {
+  if (on && !p->ev) {
+    const int cap = 1 << 14;
+    p->ev = new hipEvent_t[2 * cap];
+    p->phase = new int[cap];
+    for (int i = 0; i < 2 * cap; ++i) HIP_TRY(hipEventCreate(&p->ev[i]));
+    p->cap = cap;
+    p->n = 0;
+  }
+  p->on = on ? 1 : 0;
+  return AFQMC_OK;
+}
+
+int prof_collect(ProfRec* p, double* ms, int64_t* counts) {
+  for (int i = 0; i < p->n; ++i) {
+    HIP_TRY(hipEventSynchronize(p->ev[2 * i + 1]));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, p->ev[2 * i], p->ev[2 * i + 1]));
+    ms[p->phase[i]] += (double)t;
+    counts[p->phase[i]] += 1;
+  }
+  p->n = 0;
+  return AFQMC_OK;
+}
+
+void prof_free(ProfRec* p) {
+  if (!p->ev) return;
+  for (int i = 0; i < 2 * p->cap; ++i) (void)hipEventDestroy(p->ev[i]);
+  delete[] p->ev;
+  delete[] p->phase;
+  p->ev = nullptr;
+  p->phase = nullptr;
+  p->cap = p->n = p->on = 0;
+}
+
 extern "C" int afqmc_prof_enable(afqmc_ctx* c, int32_t on) {
   if (!c) return afqmc_set_error(AFQMC_ERR_ARG, "null context");
-  if (on && !c->prof_ev) {
-    const int cap = 1 << 14;
-    c->prof_ev = new hipEvent_t[2 * cap];
-    c->prof_phase = new int[cap];
-    for (int i = 0; i < 2 * cap; ++i) HIP_TRY(hipEventCreate(&c->prof_ev[i]));
-    c->prof_cap = cap;
-    c->prof_n = 0;
-  }
-  c->prof_on = on ? 1 : 0;
-  return AFQMC_OK;
+  return prof_enable(&c->prof, on);
 }
 
 extern "C" int afqmc_prof_phases(afqmc_ctx* c, uint32_t mask) {
   if (!c) return afqmc_set_error(AFQMC_ERR_ARG, "null context");
-  c->prof_mask = mask;
+  c->prof.mask = mask;
   return AFQMC_OK;
 }
 
 extern "C" int afqmc_prof_collect(afqmc_ctx* c, double* ms, int64_t* counts) {
   if (!c || !ms || !counts) return afqmc_set_error(AFQMC_ERR_ARG, "null argument");
-  for (int i = 0; i < c->prof_n; ++i) {
-    HIP_TRY(hipEventSynchronize(c->prof_ev[2 * i + 1]));
-    float t = 0.f;
-    HIP_TRY(hipEventElapsedTime(&t, c->prof_ev[2 * i], c->prof_ev[2 * i + 1]));
-    ms[c->prof_phase[i]] += (double)t;
-    counts[c->prof_phase[i]] += 1;
-  }
-  c->prof_n = 0;
-  return AFQMC_OK;
+  return prof_collect(&c->prof, ms, counts);
 }
 
 extern "C" int afqmc_overlap(afqmc_ctx* c, int32_t W, const void* walkers_up,
@@ -540,6 +554,41 @@ extern "C" int afqmc_sr(afqmc_ctx* c, int32_t W, void* walkers_up, void* walkers
   if (flags & ~AFQMC_SR_GLOBAL_ORDER) return afqmc_set_error(AFQMC_ERR_ARG, "unknown sr flags");
   return launch_sr(c, W, static_cast<cplx*>(walkers_up), static_cast<cplx*>(walkers_dn), weights,
                    zeta, idx_out, (flags & AFQMC_SR_GLOBAL_ORDER) != 0);
+}
+
+extern "C" int afqmc_sr_key(afqmc_ctx* c, int32_t W, void* walkers_up, void* walkers_dn,
+                            double* weights, uint64_t* key, int32_t* idx_out, int32_t flags) {
+  int rc = check_w(c, W);
+  if (rc) return rc;
+  if ((rc = check_walkers(c, walkers_up, walkers_dn))) return rc;
+  if (!weights || !key) return afqmc_set_error(AFQMC_ERR_ARG, "null weights or key");
+  if (flags & ~AFQMC_SR_GLOBAL_ORDER) return afqmc_set_error(AFQMC_ERR_ARG, "unknown sr flags");
+  if ((rc = set_device(c))) return rc;
+  c->green_src = nullptr;
+  PhaseTimer t(c, AFQMC_PH_SR);
+  return launch_sr(c, W, static_cast<cplx*>(walkers_up), static_cast<cplx*>(walkers_dn), weights,
+                   0.0, idx_out, (flags & AFQMC_SR_GLOBAL_ORDER) != 0, key);
+}
+
+extern "C" int afqmc_estimate_mix(afqmc_ctx* c, double* scalars, const double* stats,
+                                  int32_t nblocks) {
+  if (!c) return afqmc_set_error(AFQMC_ERR_ARG, "null context");
+  if (!scalars || !stats || nblocks < 1)
+    return afqmc_set_error(AFQMC_ERR_ARG, "null scalars/stats or nblocks < 1");
+  int rc = set_device(c);
+  if (rc) return rc;
+  return launch_estimate_mix(c, nblocks, AFQMC_BLOCK_STATS_LEN, stats, scalars);
+}
+
+extern "C" int afqmc_fields_normal_key(afqmc_ctx* c, uint64_t* key, double* fields,
+                                       int64_t count) {
+  if (!c) return afqmc_set_error(AFQMC_ERR_ARG, "null context");
+  if (!key || !fields || count < 0) return afqmc_set_error(AFQMC_ERR_ARG, "bad key or fields");
+  if (count == 0) return AFQMC_OK;
+  int rc = set_device(c);
+  if (rc) return rc;
+  PhaseTimer t(c, AFQMC_PH_RNG);
+  return launch_fields_normal_key(c, key, fields, count);
 }
 
 extern "C" int afqmc_fields_normal(afqmc_ctx* c, uint64_t seed, uint64_t counter,

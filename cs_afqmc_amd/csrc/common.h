@@ -161,6 +161,21 @@ __device__ inline double block_sum(double v, double* red) {
   return s;
 }
 
+// ---------------------------------------------------------------- phase timing
+// Event pairs around the phases of a step (afqmc_prof_*, afqmc_group_prof_*): while
+// `on`, each phase whose bit is set in `mask` (0: all) records a pair on its stream.
+struct ProfRec {
+  int on;
+  uint32_t mask;
+  int n;               // recorded event pairs
+  int cap;
+  hipEvent_t* ev;      // 2 * cap events
+  int* phase;          // phase of each pair
+};
+int prof_enable(ProfRec* p, int on);
+int prof_collect(ProfRec* p, double* ms, int64_t* counts);
+void prof_free(ProfRec* p);
+
 // ---------------------------------------------------------------- context
 struct afqmc_ctx {
   int device;
@@ -215,28 +230,24 @@ struct afqmc_ctx {
   int green_w;
 
   // phase timing (afqmc_prof_*)
-  int prof_on;
-  uint32_t prof_mask;         // phases that record (0: all)
-  int prof_n;                 // recorded event pairs
-  int prof_cap;
-  hipEvent_t* prof_ev;        // 2 * prof_cap events
-  int* prof_phase;            // phase of each pair
+  ProfRec prof;
 };
 
 // phase timer: records an event pair around a phase when profiling is on
 struct PhaseTimer {
-  afqmc_ctx* c;
+  ProfRec* p;
+  hipStream_t s;
   int slot;
-  PhaseTimer(afqmc_ctx* ctx, int phase) : c(ctx), slot(-1) {
-    if (c->prof_on && c->prof_n < c->prof_cap &&
-        (c->prof_mask == 0 || ((c->prof_mask >> phase) & 1u))) {
-      slot = c->prof_n++;
-      c->prof_phase[slot] = phase;
-      (void)hipEventRecord(c->prof_ev[2 * slot], c->stream);
+  PhaseTimer(ProfRec* rec, hipStream_t st, int phase) : p(rec), s(st), slot(-1) {
+    if (p->on && p->n < p->cap && (p->mask == 0 || ((p->mask >> phase) & 1u))) {
+      slot = p->n++;
+      p->phase[slot] = phase;
+      (void)hipEventRecord(p->ev[2 * slot], s);
     }
   }
+  PhaseTimer(afqmc_ctx* c, int phase) : PhaseTimer(&c->prof, c->stream, phase) {}
   ~PhaseTimer() {
-    if (slot >= 0) (void)hipEventRecord(c->prof_ev[2 * slot + 1], c->stream);
+    if (slot >= 0) (void)hipEventRecord(p->ev[2 * slot + 1], s);
   }
 };
 
@@ -324,9 +335,12 @@ int energy_g_tiles(const afqmc_ctx* c, int s);
 int launch_qr(afqmc_ctx* c, int W, cplx* wup, cplx* wdn);
 // sr.hip
 int launch_sr(afqmc_ctx* c, int W, cplx* wup, cplx* wdn, double* weights, double zeta,
-              int32_t* idx_out, bool global_order);
+              int32_t* idx_out, bool global_order, uint64_t* key = nullptr);
 int launch_fields_normal(afqmc_ctx* c, uint64_t seed, uint64_t counter, double* out,
                          int64_t count);
+int launch_fields_normal_key(afqmc_ctx* c, uint64_t* key, double* out, int64_t count);
+int launch_estimate_mix(afqmc_ctx* c, int nblocks, int stride, const double* stats,
+                        double* scalars);
 
 // ---------------------------------------------------------------- grouped launches
 // Several independent problems (fragments x spins) share one grid: problem p owns the

@@ -56,6 +56,7 @@ struct afqmc_group {
   unsigned char* pinned = nullptr;  // staging of the table upload
   size_t pinned_cap = 0;
   hipEvent_t upload_done = nullptr;  // the last upload has read `pinned`
+  ProfRec prof{};                    // afqmc_group_prof_*
 };
 
 namespace {
@@ -410,6 +411,7 @@ extern "C" int afqmc_group_destroy(afqmc_group* g) {
   if (g->dev_tab) (void)hipFree(g->dev_tab);
   if (g->pinned) (void)hipHostFree(g->pinned);
   if (g->upload_done) (void)hipEventDestroy(g->upload_done);
+  prof_free(&g->prof);
   delete g;
   return AFQMC_OK;
 }
@@ -418,10 +420,16 @@ extern "C" int afqmc_group_set_stream(afqmc_group* g, void* stream) {
   if (!g) return afqmc_set_error(AFQMC_ERR_ARG, "null group");
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (st != g->stream && g->dev_tab) {
-    // launches queued on the old stream may still read the table
+    // launches queued on the old stream may still read the table, so it is not replaced
+    // before they finish.  A stream that is capturing a graph runs nothing now and may
+    // not be synchronised against: the captured launches read the table as it stands
+    // (a rebuild would synchronise, which fails the capture), and ordering the graph's
+    // replays after earlier work is the caller's, as for any graph.
     int rc = set_group_device(g);
     if (rc) return rc;
-    HIP_TRY(hipStreamSynchronize(g->stream));
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusActive) HIP_TRY(hipStreamSynchronize(g->stream));
   }
   g->stream = st;
   return AFQMC_OK;
@@ -475,25 +483,60 @@ extern "C" int afqmc_group_propagate(afqmc_group* g, int32_t step, int32_t flags
   for (const Member& m : g->mem)
     if (m.c->green_src != m.wup || m.c->green_w != m.c->d.n_walkers) reuse = false;
   for (const Member& m : g->mem) m.c->green_src = nullptr;
-  if (!reuse && (rc = run_measure(g))) return rc;
-  if ((rc = run_gemm(g, g->fb, 0))) return rc;
-  for (const Member& m : g->mem) m.c->fb_parts = m.c->nspin;
-  if ((rc = launch_fields_grouped(g->stream, targs<FieldsArgs>(g, g->fields),
-                                  tstart(g, g->fields), g->fields.nprob, g->fields.nblocks,
-                                  step)))
-    return rc;
-  if ((rc = run_gemm(g, g->vhs, 1))) return rc;
-  for (const Launch& L : g->taylor) {
-    rc = launch_taylor_grouped(g->stream, L.k0, L.k1, L.k2, targs<TaylorArgs>(g, L),
-                               tstart(g, L), L.nprob, L.nblocks, L.k3, L.shm);
-    if (rc) return rc;
+  if (!reuse) {
+    PhaseTimer t(&g->prof, g->stream, AFQMC_PH_GREEN);
+    if ((rc = run_measure(g))) return rc;
   }
-  if ((rc = run_measure(g))) return rc;
+  {
+    PhaseTimer t(&g->prof, g->stream, AFQMC_PH_FB_GEMM);
+    if ((rc = run_gemm(g, g->fb, 0))) return rc;
+  }
+  for (const Member& m : g->mem) m.c->fb_parts = m.c->nspin;
+  {
+    PhaseTimer t(&g->prof, g->stream, AFQMC_PH_FIELDS);
+    if ((rc = launch_fields_grouped(g->stream, targs<FieldsArgs>(g, g->fields),
+                                    tstart(g, g->fields), g->fields.nprob, g->fields.nblocks,
+                                    step)))
+      return rc;
+  }
+  {
+    PhaseTimer t(&g->prof, g->stream, AFQMC_PH_VHS_GEMM);
+    if ((rc = run_gemm(g, g->vhs, 1))) return rc;
+  }
+  {
+    PhaseTimer t(&g->prof, g->stream, AFQMC_PH_TAYLOR);
+    for (const Launch& L : g->taylor) {
+      rc = launch_taylor_grouped(g->stream, L.k0, L.k1, L.k2, targs<TaylorArgs>(g, L),
+                                 tstart(g, L), L.nprob, L.nblocks, L.k3, L.shm);
+      if (rc) return rc;
+    }
+  }
+  {
+    PhaseTimer t(&g->prof, g->stream, AFQMC_PH_OVERLAP);
+    if ((rc = run_measure(g))) return rc;
+  }
   for (const Member& m : g->mem) {
     m.c->green_src = m.wup;
     m.c->green_w = m.c->d.n_walkers;
   }
+  PhaseTimer t(&g->prof, g->stream, AFQMC_PH_WEIGHTS);
   return launch_weights_grouped(g->stream, targs<WeightsArgs>(g, g->weights), g->weights.nprob);
+}
+
+extern "C" int afqmc_group_prof_enable(afqmc_group* g, int32_t on) {
+  if (!g) return afqmc_set_error(AFQMC_ERR_ARG, "null group");
+  return prof_enable(&g->prof, on);
+}
+
+extern "C" int afqmc_group_prof_phases(afqmc_group* g, uint32_t mask) {
+  if (!g) return afqmc_set_error(AFQMC_ERR_ARG, "null group");
+  g->prof.mask = mask;
+  return AFQMC_OK;
+}
+
+extern "C" int afqmc_group_prof_collect(afqmc_group* g, double* ms, int64_t* counts) {
+  if (!g || !ms || !counts) return afqmc_set_error(AFQMC_ERR_ARG, "null argument");
+  return prof_collect(&g->prof, ms, counts);
 }
 
 // Times both GEMM tile edges (64, 128) for the force-bias and VHS launches of the current

@@ -13,6 +13,32 @@
 
 namespace {
 
+// ---- Philox4x32-10
+struct u32x4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ inline void mulhilo(uint32_t a, uint32_t b, uint32_t& hi, uint32_t& lo) {
+  const uint64_t p = (uint64_t)a * (uint64_t)b;
+  hi = (uint32_t)(p >> 32);
+  lo = (uint32_t)p;
+}
+
+__device__ inline u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi0, lo0, hi1, lo1;
+    mulhilo(M0, c.x, hi0, lo0);
+    mulhilo(M1, c.z, hi1, lo1);
+    c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    k0 += W0;
+    k1 += W1;
+  }
+  return c;
+}
+
 constexpr int SR_NT = 1024;
 constexpr int SR_CHUNK = 4096;  // |w| staged through LDS in chunks of this many
 
@@ -21,13 +47,27 @@ constexpr int SR_CHUNK = 4096;  // |w| staged through LDS in chunks of this many
 // the other threads only stage |w| into LDS ahead of it and run the searches.
 // global_order selects z_i = ((i + zeta) / W) * total (the gather/scatter form of
 // sr.py:66-106 on one rank) instead of z_i = total * (i + zeta) / W (sr.py:24-34).
+// key (device, may be NULL): {seed, counter}; zeta is then the uniform draw of that key
+// (the host's rng.uniform: Philox4x32-10 of counter (0, 0xFFFFFFFF, counter) under the
+// seed, top 53 bits of the first two words) and the counter advances by one, as
+// rng.split does before the draw.
 __global__ __launch_bounds__(SR_NT) void sr_index_kernel(int W, double zeta, int global_order,
                                                          double* __restrict__ weights,
                                                          double* __restrict__ cum,
                                                          int32_t* __restrict__ idx,
-                                                         int32_t* __restrict__ idx_out) {
+                                                         int32_t* __restrict__ idx_out,
+                                                         uint64_t* __restrict__ key) {
   __shared__ double absw[SR_CHUNK];
   __shared__ double total_s;
+  __shared__ double zeta_s;
+  if (key && threadIdx.x == 0) {
+    const uint64_t seed = key[0], ctr = key[1];
+    const u32x4 r = philox4x32_10(u32x4{0u, 0xFFFFFFFFu, (uint32_t)ctr, (uint32_t)(ctr >> 32)},
+                                  (uint32_t)seed, (uint32_t)(seed >> 32));
+    const uint64_t a = ((uint64_t)r.y << 32) | r.x;
+    zeta_s = (double)(a >> 11) * 0x1.0p-53;
+    key[1] = ctr + 1;
+  }
   double run = 0.0;
   for (int base = 0; base < W; base += SR_CHUNK) {
     const int len = W - base < SR_CHUNK ? W - base : SR_CHUNK;
@@ -44,6 +84,7 @@ __global__ __launch_bounds__(SR_NT) void sr_index_kernel(int W, double zeta, int
   }
   const double total = total_s;
   const double avg = total / (double)W;
+  if (key) zeta = zeta_s;  // written before the first barrier above
   for (int i = threadIdx.x; i < W; i += SR_NT) {
     const double z = global_order ? (((double)i + zeta) / (double)W) * total
                                   : total * ((double)i + zeta) / (double)W;
@@ -73,32 +114,6 @@ __global__ void gather_kernel(int W, int elems, const int32_t* __restrict__ idx,
     d[e] = s[e];
 }
 
-// ---- Philox4x32-10
-struct u32x4 {
-  uint32_t x, y, z, w;
-};
-
-__device__ inline void mulhilo(uint32_t a, uint32_t b, uint32_t& hi, uint32_t& lo) {
-  const uint64_t p = (uint64_t)a * (uint64_t)b;
-  hi = (uint32_t)(p >> 32);
-  lo = (uint32_t)p;
-}
-
-__device__ inline u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
-  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
-  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
-#pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    uint32_t hi0, lo0, hi1, lo1;
-    mulhilo(M0, c.x, hi0, lo0);
-    mulhilo(M1, c.z, hi1, lo1);
-    c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
-    k0 += W0;
-    k1 += W1;
-  }
-  return c;
-}
-
 __global__ void normal_kernel(uint64_t seed, uint64_t counter, double* __restrict__ out,
                               int64_t count) {
   const int64_t pair = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -117,12 +132,37 @@ __global__ void normal_kernel(uint64_t seed, uint64_t counter, double* __restric
   if (2 * pair + 1 < count) out[2 * pair + 1] = rad * s;
 }
 
+// The same draws keyed by a device-resident {seed, counter} (read, not modified: the
+// counter advances in key_advance_kernel, stream-ordered after this launch).
+__global__ void normal_key_kernel(const uint64_t* __restrict__ key, double* __restrict__ out,
+                                  int64_t count) {
+  const int64_t pair = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (2 * pair >= count) return;
+  const uint64_t seed = key[0], counter = key[1];
+  const u32x4 c = u32x4{(uint32_t)pair, (uint32_t)((uint64_t)pair >> 32), (uint32_t)counter,
+                        (uint32_t)(counter >> 32)};
+  const u32x4 r = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const uint64_t a = ((uint64_t)r.y << 32) | r.x;
+  const uint64_t b = ((uint64_t)r.w << 32) | r.z;
+  const double u1 = ((double)(a >> 11) + 1.0) * 0x1.0p-53;
+  const double u2 = (double)(b >> 11) * 0x1.0p-53;
+  const double rad = sqrt(-2.0 * log(u1));
+  double s, co;
+  sincos(2.0 * M_PI * u2, &s, &co);
+  out[2 * pair] = rad * co;
+  if (2 * pair + 1 < count) out[2 * pair + 1] = rad * s;
+}
+
+__global__ void key_advance_kernel(uint64_t* __restrict__ key) {
+  if (threadIdx.x == 0) key[1] += 1;
+}
+
 }  // namespace
 
 int launch_sr(afqmc_ctx* c, int W, cplx* wup, cplx* wdn, double* weights, double zeta,
-              int32_t* idx_out, bool global_order) {
+              int32_t* idx_out, bool global_order, uint64_t* key) {
   hipLaunchKernelGGL(sr_index_kernel, dim3(1), dim3(SR_NT), 0, c->stream, W, zeta,
-                     global_order ? 1 : 0, weights, c->sr_cum, c->sr_idx, idx_out);
+                     global_order ? 1 : 0, weights, c->sr_cum, c->sr_idx, idx_out, key);
   LAUNCH_CHECK();
   const int N = c->d.norb;
   for (int s = 0; s < c->nspin; ++s) {
@@ -143,6 +183,16 @@ int launch_fields_normal(afqmc_ctx* c, uint64_t seed, uint64_t counter, double* 
   const int64_t pairs = (count + 1) / 2;
   hipLaunchKernelGGL(normal_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0,
                      c->stream, seed, counter, out, count);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+int launch_fields_normal_key(afqmc_ctx* c, uint64_t* key, double* out, int64_t count) {
+  const int64_t pairs = (count + 1) / 2;
+  hipLaunchKernelGGL(normal_key_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0,
+                     c->stream, key, out, count);
+  LAUNCH_CHECK();
+  hipLaunchKernelGGL(key_advance_kernel, dim3(1), dim3(64), 0, c->stream, key);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }

@@ -159,10 +159,13 @@ __global__ __launch_bounds__(WEIGHTS_NT) void block_stats_kernel(
   __shared__ double red[WEIGHTS_NT / 64];
   const double e_est = scalars[0];
   const double cap = sqrt(2.0 / dt);
-  double sw = 0.0, sew = 0.0, nz = 0.0;
+  double sw = 0.0, sew = 0.0, nz = 0.0, ncl = 0.0;
   for (int w = threadIdx.x; w < W; w += WEIGHTS_NT) {
     double e = energies[w].re;
-    if (fabs(e - e_est) > cap) e = e_est;
+    if (fabs(e - e_est) > cap) {
+      e = e_est;
+      ncl += 1.0;
+    }
     const double wt = weights[w];
     sw += wt;
     sew += e * wt;
@@ -171,13 +174,33 @@ __global__ __launch_bounds__(WEIGHTS_NT) void block_stats_kernel(
   const double bw = block_sum<WEIGHTS_NT>(sw, red);
   const double bew = block_sum<WEIGHTS_NT>(sew, red);
   const double bz = block_sum<WEIGHTS_NT>(nz, red);
+  const double bc = block_sum<WEIGHTS_NT>(ncl, red);
   if (threadIdx.x == 0) {
     const double be = bew / bw;
     out[0] = be;
     out[1] = bw;
     out[2] = bz;
+    out[3] = bc;
     scalars[1] = 0.9 * scalars[1] + 0.1 * be;
   }
+}
+
+// The driver's estimate update after a sampler call (driver.py:308,
+// lno_afqmc/ccsd_pt2/run_afqmc_nompi.py:89): e_estimate <- 0.9 e_estimate + 0.1 E with
+// E = sum_b E_b w_b / sum_b w_b over the call's nblocks rows of block statistics
+// (rows of `stride` doubles: block energy, block weight, ...), summed in row order;
+// then the next call's start, pop_control_ene_shift <- e_estimate (sampling.py:329).
+__global__ void estimate_mix_kernel(int nblocks, int stride, const double* __restrict__ stats,
+                                    double* __restrict__ scalars) {
+  if (threadIdx.x != 0) return;
+  double sew = 0.0, sw = 0.0;
+  for (int b = 0; b < nblocks; ++b) {
+    sew += stats[(int64_t)b * stride] * stats[(int64_t)b * stride + 1];
+    sw += stats[(int64_t)b * stride + 1];
+  }
+  const double e = 0.9 * scalars[0] + 0.1 * (sew / sw);
+  scalars[0] = e;
+  scalars[1] = e;
 }
 
 // corr_sample.py:551-585 (corr_otler_rm): z-scores of the weights of two correlated
@@ -255,6 +278,14 @@ int launch_block_stats(afqmc_ctx* c, int W, const cplx* energies, const double* 
                        double* scalars, double* out) {
   hipLaunchKernelGGL(block_stats_kernel, dim3(1), dim3(WEIGHTS_NT), 0, c->stream, W, c->d.dt,
                      energies, weights, scalars, out);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+int launch_estimate_mix(afqmc_ctx* c, int nblocks, int stride, const double* stats,
+                        double* scalars) {
+  hipLaunchKernelGGL(estimate_mix_kernel, dim3(1), dim3(64), 0, c->stream, nblocks, stride, stats,
+                     scalars);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }

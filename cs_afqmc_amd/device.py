@@ -274,14 +274,60 @@ class DeviceStep:
             out.numel()))
         return out
 
+    def _check_key(self, key):
+        if (not isinstance(key, torch.Tensor) or key.device.type != "cuda"
+                or key.dtype != torch.int64 or key.numel() != 2 or not key.is_contiguous()):
+            raise TypeError("key must be a contiguous cuda int64 tensor {seed, counter}")
+
+    def fields_normal_key(self, key, out):
+        """Fields from a device-resident key (int64[2] = {seed, counter} as two's
+        complement uint64): draws with (seed, counter), then counter += 1."""
+        self._check_key(key)
+        if out.dtype != torch.float64 or not out.is_cuda or not out.is_contiguous():
+            raise TypeError("out must be a contiguous cuda float64 tensor")
+        self._prep()
+        _lib.check(self._lib.afqmc_fields_normal_key(self._h, _dptr(key), _dptr(out),
+                                                     out.numel()))
+        return out
+
+    def sr_key(self, walkers, weights, key, idx_out=None, global_order=False):
+        """Systematic resampling with zeta drawn on the device from `key` (see
+        fields_normal_key); nothing is read back to the host."""
+        up, dn, W = self._walkers(walkers)
+        self._vec(weights, W, torch.float64, "weights")
+        self._check_key(key)
+        if idx_out is not None:
+            self._vec(idx_out, W, torch.int32, "idx_out")
+        self._prep()
+        flags = _lib.AFQMC_SR_GLOBAL_ORDER if global_order else 0
+        _lib.check(self._lib.afqmc_sr_key(self._h, W, _dptr(up), _dptr(dn), _dptr(weights),
+                                          _dptr(key), _dptr(idx_out), flags))
+        return idx_out
+
     def block_stats(self, energies, weights, scalars, out):
+        """out: (BLOCK_STATS_LEN,) float64 = block energy, block weight, zero weights,
+        clamped local energies."""
         W = weights.shape[0]
         self._vec(energies, W, torch.complex128, "energies")
         self._vec(weights, W, torch.float64, "weights")
+        if (out.dtype != torch.float64 or not out.is_cuda or not out.is_contiguous()
+                or out.numel() != _lib.BLOCK_STATS_LEN):
+            raise TypeError(f"out must be a contiguous cuda float64 tensor of "
+                            f"{_lib.BLOCK_STATS_LEN} elements")
         self._prep()
         _lib.check(self._lib.afqmc_block_stats(self._h, W, _dptr(energies), _dptr(weights),
                                                _dptr(scalars), _dptr(out)))
         return out
+
+    def estimate_mix(self, scalars, stats):
+        """scalars[0] <- 0.9 scalars[0] + 0.1 (weighted mean block energy of `stats`,
+        rows of BLOCK_STATS_LEN), the driver's update after a sampler call."""
+        if (stats.dtype != torch.float64 or not stats.is_cuda or not stats.is_contiguous()
+                or stats.numel() % _lib.BLOCK_STATS_LEN or stats.numel() == 0):
+            raise TypeError("stats must be contiguous cuda float64 rows of block statistics")
+        self._prep()
+        _lib.check(self._lib.afqmc_estimate_mix(self._h, _dptr(scalars), _dptr(stats),
+                                                stats.numel() // _lib.BLOCK_STATS_LEN))
 
     def cs_outliers(self, weights1, weights2, z_thresh=5.0):
         """corr_sample.py:551-585: zero the weights of joint z-score outliers."""
@@ -361,6 +407,23 @@ class DeviceGroup:
         self._prep()
         flags = _lib.AFQMC_STEP_REUSE_GREEN if reuse_green else 0
         _lib.check(self._lib.afqmc_group_propagate(self._h, int(step), flags))
+
+    def prof_enable(self, on=True, phases=None):
+        """Event pairs around each phase of the grouped step (all grouped launches of a
+        phase inside one pair); `phases` (names) restricts them, None = all."""
+        mask = 0
+        for p in phases or ():
+            mask |= 1 << _lib.PHASES.index(p)
+        _lib.check(self._lib.afqmc_group_prof_phases(self._h, mask))
+        _lib.check(self._lib.afqmc_group_prof_enable(self._h, 1 if on else 0))
+
+    def prof_collect(self):
+        """{phase: (total ms, launches)} since the last collect (synchronises)."""
+        ms = np.zeros(len(_lib.PHASES), dtype=np.float64)
+        cnt = np.zeros(len(_lib.PHASES), dtype=np.int64)
+        _lib.check(self._lib.afqmc_group_prof_collect(
+            self._h, ms.ctypes.data_as(ctypes.c_void_p), cnt.ctypes.data_as(ctypes.c_void_p)))
+        return {p: (float(ms[i]), int(cnt[i])) for i, p in enumerate(_lib.PHASES)}
 
     def tune(self):
         """Pick GEMM tiles by timing; returns {fb: (ms64, ms128), vhs: (ms64, ms128)}."""

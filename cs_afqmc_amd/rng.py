@@ -38,6 +38,29 @@ def split(key):
     return (seed, counter + 1), (seed, counter)
 
 
+def _to_i64(u):
+    u &= 2**64 - 1
+    return u - 2**64 if u >= 2**63 else u
+
+
+def device_key(key, device, out=None):
+    """The key as a device int64[2] tensor {seed, counter} (uint64 bit patterns), the
+    form afqmc_fields_normal_key / afqmc_sr_key read and advance on the device."""
+    import torch
+
+    host = torch.tensor([_to_i64(key[0]), _to_i64(key[1])], dtype=torch.int64)
+    if out is None:
+        return host.to(device)
+    out.copy_(host)
+    return out
+
+
+def advance(key, n):
+    """The key after n device draws (each fields block and each SR advances the
+    counter by one, as one split does on the host)."""
+    return (key[0], key[1] + int(n))
+
+
 def uniform(key):
     """One float64 uniform in [0, 1) from the 64 high-quality bits of a Philox draw;
     stream 0xFFFFFFFF keeps it apart from the field draws of the same counter."""

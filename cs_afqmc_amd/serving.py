@@ -29,7 +29,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from . import batch
+from . import batch, work
 
 
 @dataclass(order=True)
@@ -62,27 +62,51 @@ class Result:
         return self.done - self.arrival
 
 
-def estimate_bytes(norb, nocc, nchol, n_walkers, n_prop_steps, precision):
-    """Device bytes of one fragment (context, workspace, walker state, fields block, and
-    the rotated Cholesky vectors ham_data keeps on the device)."""
-    N, n, G, W = norb, nocc, nchol, n_walkers
-    b = 8 * N * N * G + 2 * 8 * G * n * N + 16 * G  # chol_t, rot_chol (ctx, ham_data), mf
-    if precision != "fp64":
-        gp, np32 = -(-G // 32) * 32, -(-N // 32) * 32
-        b += 4 * N * N * gp + 4 * G * n * np32 + 4 * W * 2 * n * np32 + 4 * W * 2 * gp
-        b += 4 * W * 2 * N * N  # float32 VHS
+def estimate_bytes(norb, nelec, nchol, n_walkers, n_prop_steps, precision,
+                   walker_type="rhf"):
+    """Upper estimate of one fragment's device bytes before it is built: context
+    intermediates and workspace (as afqmc_ctx_create sizes them), walker state, fields
+    block, and the rotated Cholesky vectors ham_data keeps on the device.  nelec:
+    (n_up, n_dn) or one count for closed shells.  Once built, the server replaces this
+    with the fragment's measured bytes (Fragment.device_bytes)."""
+    if np.ndim(nelec) == 0:
+        nelec = (int(nelec), int(nelec))
+    N, G, W = norb, nchol, n_walkers
+    ns = [nelec[0]] if walker_type == "rhf" else [nelec[0], nelec[1]]
+    mixed = precision != "fp64"
+    gp, np32 = -(-G // 32) * 32, -(-N // 32) * 32
+    b = 8 * N * N * G + 16 * G              # chol_t, mf_shifts
+    b += 8 * N * N * G                      # the caller's chol upload, freed at creation
+    fb_slices = 0
+    for n in ns:
+        b += 2 * 8 * G * n * N              # rot_chol in the context and in ham_data
+        b += 8 * N * n + 8 * n * N + 8 * N * N  # trial, rot_h1, exp_h1
+        b += 16 * W * n * N                 # float64 Green's function
+        b += 2 * 16 * W * N * n             # walkers and the SR gather buffer
+        kd = n * np32 if mixed else n * N
+        fb_slices += min(8, max(1, kd // 256))
+        b += 16 * W * -(-G // max(1, (128 if mixed else 64) // n))  # exchange partials
+        if mixed:
+            b += 2 * 2 * G * n * np32 + 2 * 2 * W * 2 * n * np32  # rot_chol, Green planes
+            b += 2 * 2 * N * np32           # exp_h1 - I planes
+    b += 8 * W * 2 * G * fb_slices + 8 * W * 2 * G  # force-bias slices, shifted fields
+    if mixed:
+        b += 2 * 2 * N * N * gp + 2 * 2 * W * 2 * gp  # Cholesky, shifted-field planes
+        b += 4 * W * 2 * N * N + 32         # float32 VHS
     else:
-        b += 16 * W * N * N
-    b += 16 * W * n * N * 3  # walkers, Green, temporaries
-    b += 8 * W * 2 * G * 9   # force-bias slices, shifted fields
-    b += 8 * n_prop_steps * W * G  # fields block
+        b += 16 * W * N * N                 # float64 VHS
+    b += 8 * n_prop_steps * W * G          # fields block
+    b += 64 * W + (1 << 20)                 # scalars, energies, SR, exchange partials, slack
     return int(b)
 
 
 class FragmentServer:
     def __init__(self, precision="bf16x3", n_prop_steps=50, hbm_budget=64 << 30,
                  max_active=64, dt=0.005, tune=False, prep_workers=4, max_inflight=2,
-                 policy="fifo"):
+                 policy="fifo", profile=False, profile_ticks=48):
+        """profile: record HIP event pairs around every phase of the grouped step for the
+        first `profile_ticks` ticks (the event pool holds 16384 pairs, about 300 per tick)
+        and the algorithmic work of those ticks: phase_ms / phase_flops / phase_bytes."""
         if policy not in ("fifo", "sjf"):
             raise ValueError(f"policy {policy!r}: 'fifo' or 'sjf'")
         self.policy = policy
@@ -94,6 +118,9 @@ class FragmentServer:
         self.max_active = int(max_active)
         self.dt = dt
         self.tune = tune
+        self.profile = bool(profile)
+        self.profile_ticks = int(profile_ticks)
+        self.phase_ms, self.phase_flops, self.phase_bytes = {}, {}, {}
 
     def _priority(self, req):
         """Admission order among queued requests: arrival ("fifo"), or the estimated
@@ -126,12 +153,17 @@ class FragmentServer:
         queue = []
         building = []  # (future, request, bytes)
         active = {}    # id(frag) -> (frag, request, admitted time, bytes)
-        retiring = []  # (frag, request, admitted, event, host stats)
+        retiring = []  # (frag, request, admitted, event, host stats, bytes)
         results = []
-        graveyard = []  # finished fragments, freed when the GPU is idle
+        # finished fragments and their bytes: freeing a context waits for the whole device
+        # (hipFree), so they are freed when the GPU is idle, or when admission needs their
+        # memory; until then their bytes stay counted in `used`
+        graveyard = []
         used = 0
         self.peak_active = 0    # fragments stepping together
         self.peak_reserved = 0  # fragments holding budget (stepping or being built)
+        self.peak_used = 0      # largest device bytes counted against the budget
+        self.graveyard_frees = 0  # times admission freed retired fragments early
         dev = torch.cuda.current_device()
         tick = torch.cuda.Stream()  # the grouped steps
         local = threading.local()   # one set-up stream per prep thread
@@ -148,6 +180,7 @@ class FragmentServer:
         ev0 = torch.cuda.Event(enable_timing=True)
         ev0.record(tick)
         nxt = 0
+        prof_ticks = 0
         inflight = []  # events after the queued ticks
         with torch.cuda.stream(tick):
             bt = batch.FragmentBatch(max_fragments=max(self.max_active, 1))
@@ -158,25 +191,30 @@ class FragmentServer:
                     nxt += 1
                 # retire: results of fragments whose last block the GPU has finished
                 still = []
-                for frag, req, adm, ev, host in retiring:
+                for frag, req, adm, ev, host, nb in retiring:
                     if ev.query():
                         be, bw = host[:, 0].numpy().copy(), host[:, 1].numpy().copy()
                         done = ev0.elapsed_time(ev) / 1e3
                         energy = float(np.sum(be * bw) / np.sum(bw))
                         results.append(Result(req.rid, energy, be, bw, req.arrival, adm, done,
                                               frag.walker_steps, frag.build_s))
-                        graveyard.append(frag)
+                        graveyard.append((frag, nb))
                     else:
-                        still.append((frag, req, adm, ev, host))
+                        still.append((frag, req, adm, ev, host, nb))
                 retiring = still
                 # admit: reserve the budget and build on the prep thread, oldest first
                 while queue and len(active) + len(building) < self.max_active:
                     req = queue[0][1]
                     nchol = int(np.asarray(req.ham_data["chol"]).shape[0])
                     norb = int(np.asarray(req.ham_data["h1"]).shape[-1])
-                    need = estimate_bytes(norb, req.nelec[0], nchol, req.n_walkers,
-                                          self.n_prop_steps, self.precision)
-                    if (active or building) and used + need > self.hbm_budget:
+                    need = estimate_bytes(norb, req.nelec, nchol, req.n_walkers,
+                                          self.n_prop_steps, self.precision, req.walker_type)
+                    if used + need > self.hbm_budget and graveyard:
+                        tick.synchronize()
+                        used -= sum(nb for _, nb in graveyard)
+                        graveyard.clear()
+                        self.graveyard_frees += 1
+                    if (active or building or retiring) and used + need > self.hbm_budget:
                         break
                     heapq.heappop(queue)
                     used += need
@@ -189,16 +227,21 @@ class FragmentServer:
                         frag, ev = fut.result()
                         tick.wait_event(ev)
                         frag.n_blocks_total = max(1, -(-req.n_steps // self.n_prop_steps))
-                        active[id(frag)] = (frag, req, time.perf_counter() - t0, need)
+                        # the measured footprint replaces the estimate
+                        real = frag.device_bytes() + _tensor_bytes(frag.ham_data)
+                        used += real - need
+                        self.peak_used = max(self.peak_used, used)
+                        active[id(frag)] = (frag, req, time.perf_counter() - t0, real)
                         self.peak_active = max(self.peak_active, len(active))
                         bt.add(frag)
                         joined.append(fut)
                 building = [b for b in building if b[0] not in joined]
                 if joined and self.tune and len(bt.fragments) > 1:
-                    bt.group.tune()
+                    bt.tune()
                 if not active:
                     if graveyard and not retiring and not building:
                         torch.cuda.synchronize()
+                        used -= sum(nb for _, nb in graveyard)
                         graveyard.clear()
                     if building or retiring:
                         time.sleep(poll_s)
@@ -215,26 +258,50 @@ class FragmentServer:
                 if len(inflight) >= self.max_inflight:
                     time.sleep(poll_s)
                     continue
-                bt.run_block()
+                if self.profile and prof_ticks < self.profile_ticks:
+                    members = [(f.walker_type, f.norb, f.nelec, f.dev.nchol, f.n_walkers,
+                                f.dev.vhs_entries(), f.prop.n_exp_terms) for f in bt.fragments]
+                    fl, by = work.phase_work(self.precision, members)
+                    for ph in fl:  # the first step computes the Green's function, later ones reuse it
+                        k = 1 if ph == "green" else self.n_prop_steps
+                        self.phase_flops[ph] = self.phase_flops.get(ph, 0.0) + k * fl[ph]
+                        self.phase_bytes[ph] = self.phase_bytes.get(ph, 0.0) + k * by[ph]
+                    bt.group.prof_enable(True)
+                    bt.run_block()
+                    bt.group.prof_enable(False)
+                    prof_ticks += 1
+                else:
+                    bt.run_block()
                 tev = torch.cuda.Event()
                 tev.record(tick)
                 inflight.append(tev)
                 done = [f for f, *_ in active.values() if f.blocks_done >= f.n_blocks_total]
                 if done:
                     for f in done:
-                        _, req, adm, need = active.pop(id(f))
-                        used -= need
+                        _, req, adm, nb = active.pop(id(f))
                         st = torch.stack(f.stats)
                         host = torch.empty(st.shape, dtype=st.dtype, pin_memory=True)
                         host.copy_(st, non_blocking=True)
                         ev = torch.cuda.Event(enable_timing=True)
                         ev.record(tick)
-                        retiring.append((f, req, adm, ev, host))
+                        retiring.append((f, req, adm, ev, host, nb))
                     bt.remove(done)
         pool.shutdown(wait=True)
         torch.cuda.synchronize()
+        if prof_ticks:
+            self.phase_ms = {k: v for k, v in bt.group.prof_collect().items() if v[1]}
         graveyard.clear()
         return results
+
+
+def _tensor_bytes(d):
+    """Bytes of the device tensors a ham_data dict keeps (rotated Cholesky vectors)."""
+    n = 0
+    for v in d.values():
+        for t in (v if isinstance(v, (list, tuple)) else [v]):
+            if isinstance(t, torch.Tensor) and t.is_cuda:
+                n += t.numel() * t.element_size()
+    return n
 
 
 def latency_stats(results):

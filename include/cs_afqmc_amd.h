@@ -153,15 +153,36 @@ enum { AFQMC_SR_GLOBAL_ORDER = 1 };
 AFQMC_API int afqmc_sr(afqmc_ctx* ctx, int32_t n_walkers, void* walkers_up, void* walkers_dn,
              double* weights, double zeta, int32_t* idx_out, int32_t flags);
 
+/* afqmc_sr with the uniform draw taken on the device from a device-resident key
+ * (uint64[2] = {seed, counter}), as stochastic_reconfiguration_local does with
+ * random.split + random.uniform (propagation.py:234-241): zeta is the draw of
+ * (seed, counter) and counter advances by one.  Nothing is read back to the host, so a
+ * block that ends with SR can be captured into a HIP graph and replayed. */
+AFQMC_API int afqmc_sr_key(afqmc_ctx* ctx, int32_t n_walkers, void* walkers_up,
+                           void* walkers_dn, double* weights, uint64_t* key, int32_t* idx_out,
+                           int32_t flags);
+
 /* Block statistics of sampler._block_scan (sampling.py:74-91), on the device:
  *   energies dev (W) complex128 local energies; the real part is clamped to
  *            e_estimate when |E - e_estimate| > sqrt(2 / dt)
  *   weights  dev (W) float64
  *   scalars  dev float64[2] = {e_estimate, pop_control_ene_shift};
  *            [1] <- 0.9 [1] + 0.1 block_energy
- *   out      dev float64[3] = {block_energy, block_weight, number of zero weights} */
+ *   out      dev float64[AFQMC_BLOCK_STATS_LEN] = {block_energy, block_weight,
+ *            number of zero weights, number of clamped local energies} */
+enum { AFQMC_BLOCK_STATS_LEN = 4 };
 AFQMC_API int afqmc_block_stats(afqmc_ctx* ctx, int32_t n_walkers, const void* energies,
                                 const double* weights, double* scalars, double* out);
+
+/* The driver's estimate update after each sampler call (driver.py:308; the fragment
+ * driver lno_afqmc/ccsd_pt2/run_afqmc_nompi.py:89): scalars[0] = e_estimate <-
+ * 0.9 e_estimate + 0.1 E, E = sum_b E_b w_b / sum_b w_b over nblocks consecutive rows
+ * of afqmc_block_stats output (stats dev, nblocks * AFQMC_BLOCK_STATS_LEN float64),
+ * summed in row order; then scalars[1] = pop_control_ene_shift <- the new e_estimate,
+ * as the next sampler call starts (sampling.py:329).  Stream-ordered; no host
+ * synchronisation. */
+AFQMC_API int afqmc_estimate_mix(afqmc_ctx* ctx, double* scalars, const double* stats,
+                                 int32_t nblocks);
 
 /* Per-phase device timing with HIP events on the context's stream (rocprof-free
  * measurement for benchmarks).  While enabled, each phase of afqmc_propagate and the
@@ -202,6 +223,12 @@ AFQMC_API int afqmc_cs_outliers(afqmc_ctx* ctx, int32_t n_walkers, double* weigh
  * (seed, counter).  Stands in for jax.random.normal in sampling.py:61-69. */
 AFQMC_API int afqmc_fields_normal(afqmc_ctx* ctx, uint64_t seed, uint64_t counter,
                         double* fields, int64_t count);
+/* The same draws keyed by a device-resident key (uint64[2] = {seed, counter}): the
+ * block's random.split + random.normal of sampling.py:61-69 with the key kept in device
+ * memory, so the draw can sit inside a captured graph.  Uses (seed, counter), then
+ * advances counter by one (stream-ordered). */
+AFQMC_API int afqmc_fields_normal_key(afqmc_ctx* ctx, uint64_t* key, double* fields,
+                                      int64_t count);
 
 /* ---- Fragment groups: many independent problems advanced by one grouped step.
  *
@@ -228,6 +255,12 @@ AFQMC_API int afqmc_group_add(afqmc_group* g, afqmc_ctx* ctx, void* walkers_up,
 AFQMC_API int afqmc_group_size(const afqmc_group* g);
 /* One phaseless step of every member (flags: 0 or AFQMC_STEP_REUSE_GREEN). */
 AFQMC_API int afqmc_group_propagate(afqmc_group* g, int32_t step, int32_t flags);
+/* Per-phase event timing of the grouped step, as afqmc_prof_enable / _phases / _collect
+ * for one context (phases AFQMC_PH_GREEN .. AFQMC_PH_WEIGHTS, each phase covering all
+ * of its grouped launches).  No counterpart in the reference: measurement only. */
+AFQMC_API int afqmc_group_prof_enable(afqmc_group* g, int32_t on);
+AFQMC_API int afqmc_group_prof_phases(afqmc_group* g, uint32_t mask);
+AFQMC_API int afqmc_group_prof_collect(afqmc_group* g, double* ms, int64_t* counts);
 /* Picks the GEMM tile edge (64 or 128) of the force-bias and VHS launches by timing
  * both on the current members; ms_out (4 floats64, may be NULL) receives the times
  * [fb 64, fb 128, vhs 64, vhs 128].  Synchronises. */

@@ -6,7 +6,8 @@ against.  Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` 
 never as the code path that is shipped or measured on the GPU.
 
 Every function follows one reference function and cites it (file:line, paths relative
-to the reference repository root).  The reference is pure JAX; this restatement keeps
+to the reference repository root).  Contractions that XLA lowers to BLAS dot products in
+the reference run through numpy's BLAS here too (einsum with optimize=True).  The reference is pure JAX; this restatement keeps
 its operation order (e.g. ``exp(a) * o_new / o_old``) so that float64 results agree to
 rounding.  Batching (``n_batch`` scans, ``vmap``) does not change results and is
 dropped.
@@ -70,7 +71,7 @@ def rhf_energy_restricted(walker, h0, rot_h1, rot_chol, mo_coeff):
     ene0 = h0
     green = rhf_green(walker, mo_coeff)
     ene1 = 2.0 * np.sum(green * rot_h1)
-    f = np.einsum("gij,jk->gik", rot_chol, green.T)
+    f = np.einsum("gij,jk->gik", rot_chol, green.T, optimize=True)
     c = np.trace(f, axis1=1, axis2=2)
     exc = np.sum(f * np.transpose(f, (0, 2, 1)))
     ene2 = 2.0 * np.sum(c * c) - exc
@@ -109,8 +110,8 @@ def uhf_energy(walker_up, walker_dn, h0, rot_h1, rot_chol, mo_coeff):
     ene0 = h0
     g = uhf_green(walker_up, walker_dn, mo_coeff)
     ene1 = np.sum(g[0] * rot_h1[0]) + np.sum(g[1] * rot_h1[1])
-    f_up = np.einsum("gij,jk->gik", rot_chol[0], g[0].T)
-    f_dn = np.einsum("gij,jk->gik", rot_chol[1], g[1].T)
+    f_up = np.einsum("gij,jk->gik", rot_chol[0], g[0].T, optimize=True)
+    f_dn = np.einsum("gij,jk->gik", rot_chol[1], g[1].T, optimize=True)
     c_up = np.trace(f_up, axis1=1, axis2=2)
     c_dn = np.trace(f_dn, axis1=1, axis2=2)
     exc_up = np.sum(f_up * np.transpose(f_up, (0, 2, 1)))
@@ -199,13 +200,13 @@ def build_measurement_intermediates(trial, ham_data, wave_data, norb):
     if trial == "rhf":
         mo = wave_data["mo_coeff"]
         ham_data["rot_h1"] = mo.T.conj() @ ((h1[0] + h1[1]) / 2.0)
-        ham_data["rot_chol"] = np.einsum("pi,gij->gpj", mo.T.conj(), chol3)
+        ham_data["rot_chol"] = np.einsum("pi,gij->gpj", mo.T.conj(), chol3, optimize=True)
     else:
         mo = wave_data["mo_coeff"]
         ham_data["rot_h1"] = [mo[0].T.conj() @ h1[0], mo[1].T.conj() @ h1[1]]
         ham_data["rot_chol"] = [
-            np.einsum("pi,gij->gpj", mo[0].T.conj(), chol3),
-            np.einsum("pi,gij->gpj", mo[1].T.conj(), chol3),
+            np.einsum("pi,gij->gpj", mo[0].T.conj(), chol3, optimize=True),
+            np.einsum("pi,gij->gpj", mo[1].T.conj(), chol3, optimize=True),
         ]
     return ham_data
 
@@ -243,8 +244,8 @@ def build_propagation_intermediates(walker_type, ham_data, rdm1, norb, nelec, dt
     mf_shifts = 1.0j * np.array([np.sum(x.reshape(norb, norb) * rdm1t) for x in chol])
     ham_data["mf_shifts"] = mf_shifts
     ham_data["h0_prop"] = -ham_data["h0"] - np.sum(mf_shifts**2) / 2.0
-    v0 = 0.5 * np.einsum("gik,gjk->ij", chol3, chol3)
-    v1 = np.real(1.0j * np.einsum("g,gik->ik", mf_shifts, chol3))
+    v0 = 0.5 * np.einsum("gik,gjk->ij", chol3, chol3, optimize=True)
+    v1 = np.real(1.0j * np.einsum("g,gik->ik", mf_shifts, chol3, optimize=True))
     h1 = np.asarray(ham_data["h1"])
     if walker_type == "rhf":
         h1_mod = (h1[0] + h1[1]) / 2.0 - v0

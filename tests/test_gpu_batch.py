@@ -56,10 +56,17 @@ def test_group_matches_single_context_sampler(gpu, precision, tol):
     for spec, f in zip(SPECS, frags):
         print(f"{precision} fragment {spec}", flush=True)
         ref = fragment(spec, precision, steps)
-        smp = sampling.sampler(n_prop_steps=steps, n_ene_blocks=1, n_sr_blocks=blocks)
-        e_ref, pd = smp.propagate_phaseless(None, ref.ham_data, ref.prop, ref.prop_data,
-                                            ref.trial, ref.wave_data)
-        be_ref = pd["block_energies"].cpu().numpy()
+        # the fragment driver: one-block sampler calls, e_estimate <- 0.9 e + 0.1 E after
+        # each (lno_afqmc/ccsd_pt2/run_afqmc_nompi.py:83-89)
+        smp = sampling.sampler(n_prop_steps=steps, n_ene_blocks=1, n_sr_blocks=1)
+        pd = ref.prop_data
+        be_ref = []
+        for _ in range(blocks):
+            e_ref, pd = smp.propagate_phaseless(None, ref.ham_data, ref.prop, pd, ref.trial,
+                                                ref.wave_data)
+            be_ref.append(pd["block_energies"].cpu().numpy()[0])
+            pd["e_estimate"] = 0.9 * pd["e_estimate"] + 0.1 * e_ref
+        be_ref = np.array(be_ref)
         be, _ = f.block_energies()
         assert np.all(np.isfinite(be))
         scale = np.max(np.abs(be_ref))

@@ -102,3 +102,98 @@ def test_device_normals_match_philox(gpu):
             ref[2 * pair + 1] = rad * np.sin(2 * np.pi * u2)
     assert np.max(np.abs(got - ref)) < 1e-13
     assert abs(np.mean(got)) < 0.2 and abs(np.std(got) - 1) < 0.15
+
+
+def _sampler_problem(seed=5, W=24):
+    norb, nchol = 16, 40
+    cs = 0.5 / np.sqrt(norb)
+    hd, wd, nelec = problems.synthetic_rhf(norb, 5, nchol, seed=seed, chol_std=cs)
+    trial = wavefunctions.rhf(norb, nelec)
+    prop = propagation.propagator_restricted(dt=0.01, n_walkers=W)
+    ham = hamiltonian.hamiltonian(norb)
+    wave = dict(wd)
+    hdat = ham.build_measurement_intermediates(dict(hd), trial, wave)
+    hdat = ham.build_propagation_intermediates(hdat, prop, trial, wave)
+    pd = prop.init_prop_data(trial, wave, hdat)
+    pd["key"] = rng.PRNGKey(77)
+    return ham, hdat, prop, pd, trial, wave
+
+
+def test_graph_sampler_equals_eager_sampler(gpu):
+    """Graph-replayed blocks equal eager blocks bit for bit over several calls, with the
+    host's own work (the driver's QR, global SR and estimate update) between calls."""
+    runs = {}
+    for use_graph in (False, True):
+        ham, hdat, prop, pd, trial, wave = _sampler_problem()
+        smp = sampling.sampler(n_prop_steps=4, n_ene_blocks=3, n_sr_blocks=2,
+                               use_graph=use_graph)
+        es = []
+        for _ in range(3):
+            e, pd = smp.propagate_phaseless(ham, hdat, prop, pd, trial, wave)
+            torch.cuda.synchronize()
+            es.append(e)
+            pd = prop.orthonormalize_walkers(pd)
+            pd = prop.stochastic_reconfiguration_global(pd)
+            pd["e_estimate"] = 0.9 * pd["e_estimate"] + 0.1 * e
+        runs[use_graph] = (es, to_np(pd["walkers"]), to_np(pd["weights"]), pd["key"])
+    assert runs[True][0] == runs[False][0]
+    assert np.array_equal(runs[True][1], runs[False][1])
+    assert np.array_equal(runs[True][2], runs[False][2])
+    assert runs[True][3] == runs[False][3]
+
+
+def test_device_key_draws_match_host_keys(gpu):
+    """afqmc_fields_normal_key / afqmc_sr_key: the draws of the host key sequence, and the
+    device counter advances by one per draw."""
+    ham, hdat, prop, pd, trial, wave = _sampler_problem(W=32)
+    dev = hdat["_afqmc_device"]
+    key = (0xFEDCBA9876543210, 41)
+    dkey = rng.device_key(key, dev.device)
+    got = dev.fields_normal_key(dkey, torch.empty((3, 32, dev.nchol), dtype=torch.float64,
+                                                  device=dev.device))
+    ref = dev.fields_normal(key[0], key[1], (3, 32, dev.nchol))
+    assert torch.equal(got, ref)
+    w = torch.as_tensor(np.random.default_rng(3).uniform(0.1, 2.0, 32), device=dev.device)
+    w2 = w.clone()
+    walk = pd["walkers"].clone()
+    walk2 = pd["walkers"].clone()
+    idx_dev = torch.empty(32, dtype=torch.int32, device=dev.device)
+    idx_host = torch.empty(32, dtype=torch.int32, device=dev.device)
+    dev.sr_key(walk, w, dkey, idx_out=idx_dev)  # counter 42
+    dev.sr(walk2, w2, rng.uniform((key[0], 42)), idx_out=idx_host)
+    assert torch.equal(idx_dev, idx_host) and torch.equal(w, w2) and torch.equal(walk, walk2)
+    k = dkey.cpu().numpy().view(np.uint64)
+    assert int(k[0]) == key[0] and int(k[1]) == 43
+
+
+def test_block_stats_clamp_count_and_estimate_mix(gpu):
+    """afqmc_block_stats counts zero weights and clamped local energies as the oracle's
+    clamp rule (sampling.py:82-86) selects them; afqmc_estimate_mix is driver.py:308 with
+    the next call's population-control reset (sampling.py:329)."""
+    ham, hdat, prop, pd, trial, wave = _sampler_problem(W=40)
+    dev = hdat["_afqmc_device"]
+    rs = np.random.default_rng(9)
+    e_est, dt = -3.0, 0.01
+    cap = np.sqrt(2.0 / dt)
+    e = e_est + rs.uniform(-2 * cap, 2 * cap, 40)
+    w = rs.uniform(0.0, 2.0, 40)
+    w[[3, 17]] = 0.0
+    en = torch.as_tensor(e + 0.0j, device=dev.device)
+    wt = torch.as_tensor(w, device=dev.device)
+    sc = torch.tensor([e_est, -2.5], dtype=torch.float64, device=dev.device)
+    out = torch.zeros(4, dtype=torch.float64, device=dev.device)
+    dev.block_stats(en, wt, sc, out)
+    ec = np.where(np.abs(e - e_est) > cap, e_est, e)
+    be = np.sum(ec * w) / np.sum(w)
+    o = out.cpu().numpy()
+    assert abs(o[0] - be) < 1e-12 * abs(be)
+    assert abs(o[1] - np.sum(w)) < 1e-12 * np.sum(w)
+    assert o[2] == 2 and o[3] == np.sum(np.abs(e - e_est) > cap)
+    assert abs(float(sc[1]) - (0.9 * -2.5 + 0.1 * be)) < 1e-12
+    rows = torch.stack([out, out * torch.tensor([1.0, 2.0, 1.0, 1.0], dtype=torch.float64,
+                                                device=dev.device)])
+    dev.estimate_mix(sc, rows.contiguous())
+    r = rows.cpu().numpy()
+    mixed = 0.9 * e_est + 0.1 * (np.sum(r[:, 0] * r[:, 1]) / np.sum(r[:, 1]))
+    s = sc.cpu().numpy()
+    assert abs(s[0] - mixed) < 1e-12 and s[1] == s[0]

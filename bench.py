@@ -307,6 +307,17 @@ def event_repeats(stream, fn, repeats):
     return ms
 
 
+def copy_probe(nbytes=1 << 30):
+    """A plain device copy of known size, for scaling PMC byte counters."""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
 def median(xs):
     s = sorted(xs)
     n = len(s)
@@ -662,6 +673,8 @@ def main():
                     help="requests of the serving workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 1 GiB device copy first (scales PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: the walker population runs on one GPU; --gpus {args.gpus} is not "
@@ -684,6 +697,8 @@ def main():
                                          want)
     import torch
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        copy_probe()
 
     out = {}
     if want in ("all", FRAG_WORKLOAD):

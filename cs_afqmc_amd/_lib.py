@@ -31,6 +31,7 @@ EXPORTS = (
     "afqmc_orthonormalize",
     "afqmc_sr",
     "afqmc_sr_key",
+    "afqmc_sr_static",
     "afqmc_block_stats",
     "afqmc_estimate_mix",
     "afqmc_prof_enable",
@@ -114,6 +115,8 @@ def _declare(lib):
     lib.afqmc_orthonormalize.argtypes = [_vp, _i32, _vp, _vp]
     lib.afqmc_sr.argtypes = [_vp, _i32, _vp, _vp, _vp, ctypes.c_double, _vp, _i32]
     lib.afqmc_sr_key.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _i32]
+    lib.afqmc_sr_static.argtypes = [_i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp,
+                                    ctypes.c_double, _vp, _i32, _vp]
     lib.afqmc_block_stats.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp]
     lib.afqmc_estimate_mix.argtypes = [_vp, _vp, _vp, _i32]
     lib.afqmc_prof_enable.argtypes = [_vp, _i32]

@@ -9,6 +9,8 @@
 // Fields: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) keyed by the 64-bit seed,
 // counter = (pair index, stream counter); two 53-bit uniforms per draw feed a
 // Box-Muller transform (float64).
+#include <algorithm>
+
 #include "common.h"
 
 namespace {
@@ -194,5 +196,105 @@ int launch_fields_normal_key(afqmc_ctx* c, uint64_t* key, double* out, int64_t c
   LAUNCH_CHECK();
   hipLaunchKernelGGL(key_advance_kernel, dim3(1), dim3(64), 0, c->stream, key);
   LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+// ---------------------------------------------------------------- allocation-free SR
+// afqmc_sr_static: the resampling of afqmc_sr for callers that cannot allocate (code
+// under graph capture, tight serving loops).  It needs no context: the cumulative
+// weights and the indices -- the scratch that a context's workspace holds for afqmc_sr --
+// live in the file-scope array below, which the runtime places on every device the
+// library runs on for the life of the process, and the walkers are gathered out of place
+// into the caller's output buffers.
+namespace {
+
+constexpr int64_t SR_STATIC_MAX_WALKERS = int64_t(1) << 20;
+// cumulative weights (W float64), then indices (W int32): 12 MiB
+constexpr int64_t SR_STATIC_BYTES = SR_STATIC_MAX_WALKERS * (sizeof(double) + sizeof(int32_t));
+static __device__ __align__(256) unsigned char scratch[SR_STATIC_BYTES];
+
+constexpr int SR_STATIC_MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[SR_STATIC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device, looked up once per device with
+// that device current (the array's host-side name is not a device address).
+int scratch_for_current_device(void** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= SR_STATIC_MAX_DEVICES)
+    return afqmc_set_error(AFQMC_ERR_ARG, "device %d outside [0, %d)", dev,
+                           SR_STATIC_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return afqmc_set_error(AFQMC_ERR_HIP, "hipGetSymbolAddress returned NULL");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return AFQMC_OK;
+}
+
+// dst[i * elems + k] = src[idx[i] * elems + k] over all W * elems entries
+__global__ void gather_flat_kernel(int64_t total, int elems, const int32_t* __restrict__ idx,
+                                   const cplx* __restrict__ src, cplx* __restrict__ dst) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / elems;
+    dst[e] = src[(int64_t)idx[i] * elems + (e - i * elems)];
+  }
+}
+
+}  // namespace
+
+extern "C" int afqmc_sr_static(int32_t n_walkers, int32_t elems_up, int32_t elems_dn,
+                               const void* walkers_up, const void* walkers_dn, void* out_up,
+                               void* out_dn, double* weights, double zeta, int32_t* idx_out,
+                               int32_t flags, void* stream) {
+  if (n_walkers < 1 || n_walkers > SR_STATIC_MAX_WALKERS)
+    return afqmc_set_error(AFQMC_ERR_SIZE,
+                           "afqmc_sr_static: n_walkers=%d outside [1, %lld] (its fixed "
+                           "scratch holds %lld walkers)",
+                           n_walkers, (long long)SR_STATIC_MAX_WALKERS,
+                           (long long)SR_STATIC_MAX_WALKERS);
+  if (elems_up < 1 || !walkers_up || !out_up || !weights)
+    return afqmc_set_error(AFQMC_ERR_ARG, "afqmc_sr_static: null walkers/out/weights or "
+                                          "elems_up < 1");
+  if (elems_dn < 0 || (elems_dn > 0 && (!walkers_dn || !out_dn)))
+    return afqmc_set_error(AFQMC_ERR_ARG, "afqmc_sr_static: elems_dn > 0 needs both "
+                                          "walkers_dn and out_dn");
+  if (walkers_up == out_up || (elems_dn > 0 && walkers_dn == out_dn))
+    return afqmc_set_error(AFQMC_ERR_ARG, "afqmc_sr_static gathers out of place: the "
+                                          "output buffers must differ from the inputs");
+  if (!(zeta >= 0.0 && zeta < 1.0))
+    return afqmc_set_error(AFQMC_ERR_ARG, "zeta must lie in [0, 1)");
+  if (flags & ~AFQMC_SR_GLOBAL_ORDER)
+    return afqmc_set_error(AFQMC_ERR_ARG, "unknown sr flags");
+  void* base = nullptr;
+  int rc = scratch_for_current_device(&base);
+  if (rc) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // both scratch arrays are written in full before they are read (the running sum by the
+  // index kernel's sequential pass, the indices by its searches): nothing to clear
+  double* cum = static_cast<double*>(base);
+  int32_t* idx = reinterpret_cast<int32_t*>(static_cast<unsigned char*>(base) +
+                                            SR_STATIC_MAX_WALKERS * sizeof(double));
+  hipLaunchKernelGGL(sr_index_kernel, dim3(1), dim3(SR_NT), 0, st, (int)n_walkers, zeta,
+                     (flags & AFQMC_SR_GLOBAL_ORDER) ? 1 : 0, weights, cum, idx, idx_out,
+                     (uint64_t*)nullptr);
+  LAUNCH_CHECK();
+  const int elems[2] = {elems_up, elems_dn};
+  const cplx* src[2] = {static_cast<const cplx*>(walkers_up),
+                        static_cast<const cplx*>(walkers_dn)};
+  cplx* dst[2] = {static_cast<cplx*>(out_up), static_cast<cplx*>(out_dn)};
+  for (int s = 0; s < 2; ++s) {
+    if (elems[s] == 0) continue;
+    const int64_t total = (int64_t)n_walkers * elems[s];
+    const int64_t blocks = std::min<int64_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(gather_flat_kernel, dim3((unsigned)blocks), dim3(256), 0, st, total,
+                       elems[s], idx, src[s], dst[s]);
+    LAUNCH_CHECK();
+  }
   return AFQMC_OK;
 }

@@ -162,6 +162,27 @@ AFQMC_API int afqmc_sr_key(afqmc_ctx* ctx, int32_t n_walkers, void* walkers_up,
                            void* walkers_dn, double* weights, uint64_t* key, int32_t* idx_out,
                            int32_t flags);
 
+/* The resampling of afqmc_sr without a context and without allocating: no counterpart in
+ * the reference.  It is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, foreign runtimes that own all buffers).  Same
+ * indices, weights and walker copies as afqmc_sr on the same weights and zeta, but the
+ * walkers are gathered out of place:
+ *   walkers_up / walkers_dn  dev  inputs, n_walkers x elems_up / elems_dn complex128
+ *                                 (elems = norb * nocc of that spin; elems_dn = 0 and
+ *                                 walkers_dn = out_dn = NULL for restricted walkers)
+ *   out_up / out_dn          dev  outputs of the same sizes, distinct from the inputs
+ *   weights                  dev  (W) float64, in place: sum|w| / W
+ *   idx_out                  dev  (W) int32 or NULL
+ *   stream                       hipStream_t of the caller (NULL = default stream)
+ * The cumulative weights and indices live in a fixed array the library keeps on every
+ * device it runs on (12 MiB), so n_walkers is at most 2^20 (AFQMC_ERR_SIZE otherwise).
+ * Every call on one device shares that array: calls that overlap in time on one device
+ * (different streams or threads) race and must be ordered by the caller. */
+AFQMC_API int afqmc_sr_static(int32_t n_walkers, int32_t elems_up, int32_t elems_dn,
+                              const void* walkers_up, const void* walkers_dn, void* out_up,
+                              void* out_dn, double* weights, double zeta, int32_t* idx_out,
+                              int32_t flags, void* stream);
+
 /* Block statistics of sampler._block_scan (sampling.py:74-91), on the device:
  *   energies dev (W) complex128 local energies; the real part is clamped to
  *            e_estimate when |E - e_estimate| > sqrt(2 / dt)

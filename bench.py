@@ -68,6 +68,7 @@ SINGLE_WORKLOADS = {
 }
 SERVE_WORKLOAD = "serve_poisson200_n20-250"
 H10_WORKLOAD = "h10_chain_r1.6_sto6g_w200"
+H10_CALLS = 40  # sampler calls of 50 steps (10 blocks) per precision
 # admission: shortest estimated solve first among queued requests, at most 8 fragments
 # stepping together
 SERVE_POLICY, SERVE_MAX_ACTIVE = "sjf", 8
@@ -153,11 +154,15 @@ def traffic_from_profiles(kernel, workload, precision):
 # ------------------------------------------------------------------ energy comparison
 
 
-def compare_blocks(stats_a, stats_b, n_walkers, clamp_max=0.01):
+def compare_blocks(stats_a, stats_b, n_walkers, clamp_max=0.01, idx_a=None, idx_b=None,
+                   sr_every=1):
     """Block-by-block energy comparison of two runs with the same draws.  stats_*:
     (blocks, 4) arrays of block energy, block weight, zero weights, clamped energies.
     Returns the usable blocks' differences (both finite, both under clamp_max of the
-    walkers clamped) with the bookkeeping that says how many blocks were usable."""
+    walkers clamped) with the bookkeeping that says how many blocks were usable.
+    idx_a / idx_b: the resampling indices of each SR of the two runs (an SR after every
+    sr_every blocks); blocks before the first SR whose indices differ hold the same
+    walker populations in both runs, so their difference is the arithmetic's alone."""
     import numpy as np
 
     a, b = np.asarray(stats_a), np.asarray(stats_b)
@@ -166,7 +171,16 @@ def compare_blocks(stats_a, stats_b, n_walkers, clamp_max=0.01):
     finite = np.isfinite(a[:, 0]) & np.isfinite(b[:, 0]) & (a[:, 1] > 0) & (b[:, 1] > 0)
     unclamped = (a[:, 3] < clamp_max * n_walkers) & (b[:, 3] < clamp_max * n_walkers)
     ok = finite & unclamped
+    same = np.zeros(nb, dtype=bool)
+    if idx_a is not None and idx_b is not None:
+        first = 0
+        while (first < min(len(idx_a), len(idx_b))
+               and np.array_equal(np.asarray(idx_a[first]), np.asarray(idx_b[first]))):
+            first += 1
+        same = np.array([bi // sr_every <= first for bi in range(nb)], dtype=bool)
     return {
+        "same_history": int(np.sum(same)),
+        "diff_same": (a[ok & same, 0] - b[ok & same, 0]).tolist(),
         "blocks": int(nb),
         "nonfinite": int(np.sum(~finite)),
         "clamped_blocks": int(np.sum(finite & ~unclamped)),
@@ -188,8 +202,12 @@ def summarize_delta(comps):
 
     diffs = [d for c in comps for d in c["diff"]]
     first = [c["diff"][0] for c in comps if c["first_usable_is_block0"]]
+    same = [d for c in comps for d in c.get("diff_same", [])]
     nonfinite = sum(c["nonfinite"] for c in comps)
     out = {
+        "delta_e_same_history_max_eh": float(np.max(np.abs(same))) if same else None,
+        "blocks_same_history": sum(c.get("same_history", 0) for c in comps),
+        "blocks_same_history_usable": len(same),
         "delta_e_max_eh": float(np.max(np.abs(diffs))) if diffs else None,
         "delta_e_first_block_max_eh": float(np.max(np.abs(first))) if first else None,
         "delta_e_mean_eh": float(np.mean(diffs)) if diffs else None,
@@ -227,7 +245,11 @@ def _cpu_fragment_worker(args):
 
     out = []
     for wt, norb, nocc, nchol, W, seed in specs:
-        if wt == "rhf":
+        if wt == "h10":
+            from cs_afqmc_amd import hchain
+            hd, wd, nelec, info = hchain.prep(10, 1.6, 1e-5)
+            wt, norb, nchol = "rhf", 10, info["nchol"]
+        elif wt == "rhf":
             hd, wd, nelec = problems.synthetic_rhf(norb, nocc, nchol, seed=seed)
         else:
             hd, wd, nelec = problems.synthetic_uhf(norb, nocc, nocc, nchol, seed=seed)
@@ -365,7 +387,7 @@ def fragment_specs():
     return [("rhf", n, o, g, w, s) for n, o, g, w, s in problems.fragment_specs(64, seed=0)]
 
 
-def run_fragments_mode(specs, mode, warm_blocks, timed_blocks):
+def run_fragments_mode(specs, mode, warm_blocks, timed_blocks, side_streams=4):
     """One form of the fragment workload.  mode: "<precision>_grouped" (one captured
     graph per block of every fragment) or "<precision>_plain" (each fragment's own
     captured block, one fragment after another)."""
@@ -381,7 +403,8 @@ def run_fragments_mode(specs, mode, warm_blocks, timed_blocks):
     stream = torch.cuda.Stream()
     bt = None
     if form == "grouped":
-        bt = batch.FragmentBatch(frags, tune=True, use_graph=True, stream=stream)
+        bt = batch.FragmentBatch(frags, tune=True, use_graph=True, stream=stream,
+                                 keep_sr_indices=True, side_streams=side_streams)
 
         def block():
             bt.run_block()
@@ -408,7 +431,8 @@ def run_fragments_mode(specs, mode, warm_blocks, timed_blocks):
           f"-> {rate / 1e6:.3f} M walker-steps/s", file=sys.stderr, flush=True)
     res = {"value": rate, "ms_per_step": med / (timed_blocks * N_PROP_STEPS),
            "repeat_ms": [round(x, 3) for x in ms], "set_up_s": round(setup_s, 2),
-           "stats": [f.block_stats() for f in frags]}
+           "stats": [f.block_stats() for f in frags],
+           "sr_idx": [[i.cpu().numpy() for i in f.sr_indices] for f in frags]}
     if bt is not None and bt.tuned:
         res["gemm_tile_ms"] = bt.tuned
     if bt is not None:
@@ -451,7 +475,7 @@ def run_fragments(args, modes):
     specs = fragment_specs()
     warm = blocks_for(args.warmup) + 1  # the eager block and the capture
     timed = blocks_for(args.steps)
-    runs = {m: run_fragments_mode(specs, m, warm, timed) for m in modes}
+    runs = {m: run_fragments_mode(specs, m, warm, timed, args.side_streams) for m in modes}
     return specs, runs
 
 
@@ -489,6 +513,7 @@ def run_single(workload, precision, warm_blocks, timed_blocks, n_ene_blocks=4):
     rng.device_key(pd["key"], dev.device, out=st.key)
     stream = torch.cuda.Stream()
     hist = []
+    sr_hist = []
     since_sr = [0]
 
     def block():
@@ -499,6 +524,7 @@ def run_single(workload, precision, warm_blocks, timed_blocks, n_ene_blocks=4):
             if since_sr[0] == n_ene_blocks:
                 since_sr[0] = 0
                 st.sr()
+                sr_hist.append(st.idx.clone())
                 dev.estimate_mix(st.scalars, torch.stack(hist[-n_ene_blocks:]))
 
     # the warm-up covers two resamplings too (the eager one and the capture)
@@ -539,6 +565,7 @@ def run_single(workload, precision, warm_blocks, timed_blocks, n_ene_blocks=4):
                  sum(phases[p][0] for p in block_phases)) / N_PROP_STEPS
     res = {"value": rate, "ms_per_step": med / (timed_blocks * N_PROP_STEPS),
            "repeat_ms": [round(x, 3) for x in ms], "stats": stats,
+           "sr_idx": [i.cpu().numpy() for i in sr_hist], "sr_every": n_ene_blocks,
            "phases": {p: {"ms_total": round(v[0], 4), "launches": v[1],
                           "ms_avg": round(v[0] / v[1], 5)} for p, v in phases.items() if v[1]},
            "phase_sum_ms_per_step": round(phase_sum, 5),
@@ -554,11 +581,14 @@ def run_single(workload, precision, warm_blocks, timed_blocks, n_ene_blocks=4):
 # ------------------------------------------------------------------ H10 (config 1)
 
 
-def run_h10(precision, n_blocks):
+def run_h10(precision, n_calls):
     """Linear H10, 1.6 bohr, STO-6G, RHF trial, modified Cholesky 1e-5 (own integrals),
     200 walkers, dt 0.005, QR every 5 steps (n_prop_steps 5, n_ene_blocks 10: 50 steps
-    per sampler call), seed 0; n_blocks sampler calls with SR and the driver's
-    e_estimate update after each."""
+    per sampler call), seed 0; n_calls sampler calls with the driver's e_estimate update
+    after each.  Returns the block statistics, the SR indices of each call, and the
+    walker-steps/s of the calls (events around the whole loop, host synchronisation of
+    each call's energy included)."""
+    import numpy as np
     import torch
 
     from cs_afqmc_amd import hchain, rng, sampling
@@ -567,19 +597,24 @@ def run_h10(precision, n_blocks):
     smp = sampling.sampler(n_prop_steps=5, n_ene_blocks=10, n_sr_blocks=1)
     pd = setup["prop_data"]
     pd["key"] = rng.PRNGKey(0)
-    rows = []
-    t0 = time.perf_counter()
-    for _ in range(n_blocks):
+    rows, idx = [], []
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    for c in range(n_calls):
+        if c == 2:  # calls 0 and 1 run eagerly and capture the graphs
+            e0.record(stream)
         e, pd = smp.propagate_phaseless(setup["ham"], setup["ham_data"], setup["prop"], pd,
                                         setup["trial"], setup["wave_data"])
         rows.append(pd["block_stats"].cpu().numpy())
+        idx.append(pd["_block_state"].idx.cpu().numpy())
         pd["e_estimate"] = 0.9 * pd["e_estimate"] + 0.1 * e
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    import numpy as np
-    stats = np.concatenate(rows)
-    return {"stats": stats, "wall_s": wall, "nchol": setup["nchol"],
-            "e_hf": setup["e_hf"], "walker_steps": 200 * 50 * n_blocks}
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1)
+    return {"stats": np.concatenate(rows), "sr_idx": idx, "nchol": setup["nchol"],
+            "e_hf": setup["e_hf"],
+            "value": 200 * 50 * (n_calls - 2) / (ms / 1e3)}
 
 
 # ------------------------------------------------------------------ config 5: serving
@@ -640,7 +675,10 @@ def run_serving(reqs):
     }
     if srv.phase_ms:
         fl, by, ms = srv.phase_flops, srv.phase_bytes, srv.phase_ms
-        kern = lambda p: phase_kernel("bf16x3", p, 250, (50, 50))  # noqa: E731
+        # requests up to norb 128 run the register kernels, larger ones the streaming ones
+        kern = lambda p: (phase_kernel("bf16x3", p, 100, (20, 20)) + " / "  # noqa: E731
+                          + phase_kernel("bf16x3", p, 250, (50, 50))
+                          if p in ("taylor",) else phase_kernel("bf16x3", p, 250, (50, 50)))
         cands = [p for p in ms if ms[p][1] and p in fl and p != "weights"]
         if cands:
             dom = max(cands, key=lambda p: ms[p][0])
@@ -673,6 +711,8 @@ def main():
                     help="requests of the serving workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--side-streams", type=int, default=4,
+                    help="side streams for the fragments' draws and block ends")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (scales PMC byte counters)")
     args = ap.parse_args()
@@ -690,11 +730,16 @@ def main():
     if not args.no_cpu_baseline:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
         if want in ("all", FRAG_WORKLOAD):
-            cpu["fragments"] = cpu_baseline(fragment_specs(), 2, 3, FRAG_WORKLOAD)
+            cpu["fragments"] = cpu_baseline(fragment_specs(), 16, 8, FRAG_WORKLOAD)
+        if want in ("all", H10_WORKLOAD):
+            # 200 walkers over the processes, as the reference's ranks share them
+            cpu["h10"] = cpu_baseline([("h10", 10, 5, 27, 200, i) for i in range(16)], 13, 50,
+                                      H10_WORKLOAD)
         if want in SINGLE_WORKLOADS:
             wt, n, o, g, W = SINGLE_WORKLOADS[want]
             cpu["single"] = cpu_baseline([(wt, n, o, g, W, 11 + i) for i in range(16)], 4, 4,
                                          want)
+    import numpy as np
     import torch
     torch.cuda.set_device(0)
     if args.copy_probe:
@@ -727,6 +772,7 @@ def main():
                        "n_walkers_total": total_w, "walkers_per_fragment": 256,
                        "n_prop_steps": N_PROP_STEPS, "dt": DT, "precision": mixed,
                        "form": "grouped step, whole block replayed from one HIP graph",
+                       "side_streams": args.side_streams,
                        "repeats": REPEATS, "blocks_per_repeat": timed},
             "repeat_ms": m["repeat_ms"],
         })
@@ -736,8 +782,9 @@ def main():
             out["speedup_vs_fp64"] = round(m["value"] / runs["fp64_plain"]["value"], 3)
             out["fp64_grouped_value"] = round(runs["fp64_grouped"]["value"], 1)
             out["speedup_vs_fp64_grouped"] = round(m["value"] / runs["fp64_grouped"]["value"], 3)
-            comps = [compare_blocks(a, b, s[4]) for a, b, s in
-                     zip(m["stats"], runs["fp64_plain"]["stats"], specs)]
+            comps = [compare_blocks(a, b, s[4], idx_a=ia, idx_b=ib) for a, b, s, ia, ib in
+                     zip(m["stats"], runs["fp64_plain"]["stats"], specs, m["sr_idx"],
+                         runs["fp64_plain"]["sr_idx"])]
             out.update(summarize_delta(comps))
             gp = [compare_blocks(a, b, s[4]) for a, b, s in
                   zip(runs["fp64_grouped"]["stats"], runs["fp64_plain"]["stats"], specs)]
@@ -769,8 +816,9 @@ def main():
             single["fp64_phase_sum_ms_per_step"] = sr["fp64"]["phase_sum_ms_per_step"]
             single["speedup_vs_fp64"] = round(sr[mixed]["value"] / sr["fp64"]["value"], 3)
             single["fp64_roofline"] = sr["fp64"]["roofline"]
-            single.update(summarize_delta([compare_blocks(sr[mixed]["stats"],
-                                                          sr["fp64"]["stats"], W)]))
+            single.update(summarize_delta([compare_blocks(
+                sr[mixed]["stats"], sr["fp64"]["stats"], W, idx_a=sr[mixed]["sr_idx"],
+                idx_b=sr["fp64"]["sr_idx"], sr_every=sr[mixed]["sr_every"])]))
         if want == "all":
             out["single"] = single
         else:
@@ -784,17 +832,36 @@ def main():
             out["cpu_baseline"] = cpu.get("single")
 
     if want in ("all", H10_WORKLOAD):
-        try:
-            h = {p: run_h10(p, 6) for p in ("fp64", mixed)}
-            hd = summarize_delta([compare_blocks(h[mixed]["stats"], h["fp64"]["stats"], 200)])
-            h10 = {"workload": H10_WORKLOAD, "nchol": h["fp64"]["nchol"],
-                   "e_hf": h["fp64"]["e_hf"],
-                   "energy_fp64": float(_wmean(h["fp64"]["stats"])),
-                   "energy_mixed": float(_wmean(h[mixed]["stats"]))}
-            h10.update(hd)
+        from cs_afqmc_amd import stat_utils
+        h = {p: run_h10(p, H10_CALLS) for p in ("fp64", mixed)}
+        hd = summarize_delta([compare_blocks(h[mixed]["stats"], h["fp64"]["stats"], 200,
+                                             idx_a=h[mixed]["sr_idx"],
+                                             idx_b=h["fp64"]["sr_idx"], sr_every=10)])
+        h10 = {"workload": H10_WORKLOAD, "nchol": h["fp64"]["nchol"],
+               "e_rhf": h["fp64"]["e_hf"], "calls": H10_CALLS, "blocks_per_call": 10,
+               "value": round(h[mixed]["value"], 1), "fp64_value": round(h["fp64"]["value"], 1)}
+        for p, tag in (("fp64", "fp64"), (mixed, "mixed")):
+            st = h[p]["stats"][10 * 2:]  # the first two calls equilibrate
+            e, err = stat_utils.blocking_analysis(st[:, 1], st[:, 0], block_dtype=np.float64)
+            h10[f"energy_{tag}"] = float(e)
+            h10[f"energy_{tag}_err"] = float(err) if err is not None else None
+        if h10["energy_fp64_err"] and h10["energy_mixed_err"]:
+            h10["delta_e_run_eh"] = h10["energy_mixed"] - h10["energy_fp64"]
+            h10["delta_e_run_sigma"] = abs(h10["delta_e_run_eh"]) / math.hypot(
+                h10["energy_fp64_err"], h10["energy_mixed_err"])
+        h10.update(hd)
+        h10["cpu_baseline"] = cpu.get("h10")
+        if want == "all":
             out["h10"] = h10
-        except ImportError as e:  # config-1 integrals not available
-            out["h10"] = {"error": str(e)}
+        else:
+            out.update({"metric": "walker-steps/s mixed vs FP64 GPU (H10 chain)",
+                        "value": h10["value"], "unit": "walker-steps/s", "n_gpus": 1,
+                        "steps": 50 * (H10_CALLS - 2), "warmup": 100, "ms_per_step": None,
+                        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                        "dtype": DTYPE[mixed], "data": "H10 chain, own STO-6G integrals",
+                        "config": {"workload": H10_WORKLOAD}})
+            out.update(h10)
+            out["cpu_baseline"] = cpu.get("h10")
 
     if want in ("all", SERVE_WORKLOAD) and not args.no_serving:
         t0 = time.perf_counter()
@@ -818,12 +885,6 @@ def main():
             out.update(sv)
             out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
-
-
-def _wmean(stats):
-    import numpy as np
-    ok = np.isfinite(stats[:, 0]) & (stats[:, 1] > 0)
-    return np.sum(stats[ok, 0] * stats[ok, 1]) / np.sum(stats[ok, 1])
 
 
 if __name__ == "__main__":

@@ -67,7 +67,9 @@ class Fragment:
                                   dtype=torch.float64, device=ddev)
         self.energies = torch.empty(self.n_walkers, dtype=torch.complex128, device=ddev)
         self.slot = torch.zeros(_lib.BLOCK_STATS_LEN, dtype=torch.float64, device=ddev)
+        self.idx = torch.zeros(self.n_walkers, dtype=torch.int32, device=ddev)
         self.stats = []  # one (BLOCK_STATS_LEN,) device tensor per finished block
+        self.sr_indices = []  # the resampling indices of each finished block (device)
         self.blocks_done = 0
         self._single = None  # captured single-context block (the plain form)
         self.dev.overlap(pd["walkers"], out=pd["overlaps"])
@@ -96,14 +98,16 @@ class Fragment:
         self.dev.overlap(pd["walkers"], out=pd["overlaps"])
         en = self.dev.energy(pd["walkers"], out=self.energies)
         self.dev.block_stats(en, pd["weights"], self.scalars, self.slot)
-        self.dev.sr_key(pd["walkers"], pd["weights"], self.key)
+        self.dev.sr_key(pd["walkers"], pd["weights"], self.key, idx_out=self.idx)
         self.dev.overlap(pd["walkers"], out=pd["overlaps"])
         self.dev.estimate_mix(self.scalars, self.slot)
 
-    def record(self, stats_row):
-        """Host side of a finished block: keep its statistics row (a device tensor),
-        advance the host copy of the key by the block's two draws."""
+    def record(self, stats_row, sr_idx=None):
+        """Host side of a finished block: keep its statistics row and resampling indices
+        (device tensors), advance the host copy of the key by the block's two draws."""
         self.stats.append(stats_row)
+        if sr_idx is not None:
+            self.sr_indices.append(sr_idx)
         self.prop_data["key"] = rng.advance(self.prop_data["key"], 2)
         self.blocks_done += 1
 
@@ -126,7 +130,7 @@ class Fragment:
                 self._single.release()
             self._single = Replay(self._block_single, use_graph)
         self._single()
-        self.record(self.slot.clone())
+        self.record(self.slot.clone(), self.idx.clone())
 
     def block_energies(self):
         """(block energies, block weights) as numpy arrays (synchronises)."""
@@ -154,15 +158,20 @@ class FragmentBatch:
     each block."""
 
     def __init__(self, fragments=(), max_fragments=1024, tune=False, use_graph=False,
-                 stream=None):
+                 stream=None, keep_sr_indices=False, side_streams=4):
         # graph capture needs a stream other than the default one (default priority)
         if stream is None:
             stream = torch.cuda.Stream() if use_graph else torch.cuda.current_stream()
         self.stream = stream
+        # the per-fragment draws and block ends are independent of one another: they are
+        # spread over side streams (default priority) that fork from and join back into
+        # the batch stream, so the small launches of different fragments can overlap
+        self._side = [torch.cuda.Stream() for _ in range(max(0, int(side_streams)))]
         with torch.cuda.stream(self.stream):
             self.group = DeviceGroup(max_fragments)
         self.fragments = []
         self.use_graph = use_graph
+        self.keep_sr_indices = keep_sr_indices  # each block's SR indices, per fragment
         self._replay = None
         self.tuned = None
         for f in fragments:
@@ -207,13 +216,32 @@ class FragmentBatch:
         for f in keep:
             self.add(f)
 
+    def _per_fragment(self, fn):
+        """fn(fragment) for every fragment, round robin over the side streams, which
+        wait for the work queued so far and are joined back before returning."""
+        if not self._side:
+            for f in self.fragments:
+                fn(f)
+            return
+        main = torch.cuda.current_stream()
+        fork = torch.cuda.Event()
+        fork.record(main)
+        used = self._side[:len(self.fragments)]
+        for st in used:
+            st.wait_event(fork)
+        for i, f in enumerate(self.fragments):
+            with torch.cuda.stream(used[i % len(used)]):
+                fn(f)
+        for st in used:
+            join = torch.cuda.Event()
+            join.record(st)
+            main.wait_event(join)
+
     def _block(self):
-        for f in self.fragments:
-            f.begin_block()
+        self._per_fragment(lambda f: f.begin_block())
         for s in range(self.fragments[0].n_prop_steps):
             self.group.propagate(s, reuse_green=s > 0)
-        for f in self.fragments:
-            f.end_block()
+        self._per_fragment(lambda f: f.end_block())
 
     def run_block(self):
         """One block of every fragment: fields, n_prop_steps grouped steps, block ends."""
@@ -224,7 +252,8 @@ class FragmentBatch:
                 self._replay = Replay(self._block, self.use_graph)
             self._replay()
             rows = torch.stack([f.slot for f in self.fragments])
+            idx = [f.idx.clone() for f in self.fragments] if self.keep_sr_indices else None
             for i, f in enumerate(self.fragments):
-                f.record(rows[i])
+                f.record(rows[i], None if idx is None else idx[i])
         # later work on the caller's stream (reading results) follows the block
         torch.cuda.current_stream().wait_stream(self.stream)

@@ -34,6 +34,7 @@ class _BlockState:
         self.fields = torch.empty((n_prop_steps, W, dev.nchol), dtype=torch.float64, device=ddev)
         self.key = torch.zeros(2, dtype=torch.int64, device=ddev)
         self.slot = torch.zeros(_lib.BLOCK_STATS_LEN, dtype=torch.float64, device=ddev)
+        self.idx = torch.zeros(W, dtype=torch.int32, device=ddev)  # the last SR's indices
         self.walkers = prop_data["walkers"]
         self.weights = prop_data["weights"]
         self.overlaps = prop_data["overlaps"]
@@ -65,7 +66,7 @@ class _BlockState:
     def _sr(self):
         """sampling.py:142-143 (propagation.py:234-241): local SR with the key's next
         uniform draw, then the overlaps of the resampled walkers."""
-        self.dev.sr_key(self.walkers, self.weights, self.key)
+        self.dev.sr_key(self.walkers, self.weights, self.key, idx_out=self.idx)
         self.dev.overlap(self.walkers, out=self.overlaps)
 
 

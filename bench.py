@@ -102,7 +102,7 @@ def phase_kernel(precision, phase, norb, nelec):
     if phase in ("overlap", "green"):
         return "measure_kernel"
     if phase == "energy":
-        return {"fp64": "exchange_f64_kernel", "fp32": "exchange_f32_kernel"}.get(
+        return {"fp64": "exchange_f64_kernel", "fp32": "exchange_f64_kernel<F32>"}.get(
             precision, "exchange_bf16x3_kernel")
     return {"fields": "fields_kernel", "weights": "pcs_kernel", "qr": "qr_kernel",
             "sr": "sr_index_kernel", "rng": "normal_key_kernel"}.get(phase, phase)
@@ -747,7 +747,7 @@ def main():
 
     out = {}
     if want in ("all", FRAG_WORKLOAD):
-        modes = ([f"{mixed}_grouped", "fp64_grouped", "fp64_plain"]
+        modes = ([f"{mixed}_grouped", "fp32_grouped", "fp64_grouped", "fp64_plain"]
                  if args.precision == "compare" else [f"{mixed}_grouped"])
         specs, runs = run_fragments(args, modes)
         m = runs[f"{mixed}_grouped"]
@@ -791,6 +791,19 @@ def main():
             out["delta_e_fp64_grouped_vs_plain_eh"] = summarize_delta(gp)["delta_e_max_eh"]
             out["fp64_phases"] = runs["fp64_grouped"].get("phases")
             out["fp64_roofline"] = runs["fp64_grouped"].get("roofline")
+            f32 = runs["fp32_grouped"]
+            c32 = [compare_blocks(a, b, s[4], idx_a=ia, idx_b=ib) for a, b, s, ia, ib in
+                   zip(f32["stats"], runs["fp64_plain"]["stats"], specs, f32["sr_idx"],
+                       runs["fp64_plain"]["sr_idx"])]
+            d32 = summarize_delta(c32)
+            out["fp32"] = {"form": "grouped step, FP32-MFMA Cholesky contractions",
+                           "dtype": DTYPE["fp32"], "value": round(f32["value"], 1),
+                           "speedup_vs_fp64": round(f32["value"] / runs["fp64_plain"]["value"], 3),
+                           "roofline": f32.get("roofline"), "phases": f32.get("phases"),
+                           **{k: d32[k] for k in ("delta_e_same_history_max_eh",
+                                                  "blocks_same_history_usable",
+                                                  "delta_e_first_block_max_eh",
+                                                  "delta_e_max_eh", "energy_check")}}
         out["phases"] = m.get("phases")
         out["phase_sum_ms_per_step"] = m.get("phase_sum_ms_per_step")
         out["gemm_tile_ms"] = m.get("gemm_tile_ms")
@@ -800,7 +813,7 @@ def main():
 
     if want == "all" or want in SINGLE_WORKLOADS:
         wl = want if want in SINGLE_WORKLOADS else "rhf_n100_o20_g400_w1024"
-        precs = ["fp64", mixed] if args.precision == "compare" else [mixed]
+        precs = ["fp64", "fp32", mixed] if args.precision == "compare" else [mixed]
         sr = {p: run_single(wl, p, warm, timed) for p in precs}
         W = SINGLE_WORKLOADS[wl][4]
         single = {"workload": wl,
@@ -819,6 +832,18 @@ def main():
             single.update(summarize_delta([compare_blocks(
                 sr[mixed]["stats"], sr["fp64"]["stats"], W, idx_a=sr[mixed]["sr_idx"],
                 idx_b=sr["fp64"]["sr_idx"], sr_every=sr[mixed]["sr_every"])]))
+        if "fp64" in sr and "fp32" in sr and mixed != "fp32":
+            d32 = summarize_delta([compare_blocks(
+                sr["fp32"]["stats"], sr["fp64"]["stats"], W, idx_a=sr["fp32"]["sr_idx"],
+                idx_b=sr["fp64"]["sr_idx"], sr_every=sr["fp32"]["sr_every"])])
+            single["fp32"] = {"dtype": DTYPE["fp32"], "value": round(sr["fp32"]["value"], 1),
+                              "ms_per_step": round(sr["fp32"]["ms_per_step"], 5),
+                              "speedup_vs_fp64": round(sr["fp32"]["value"] / sr["fp64"]["value"],
+                                                       3),
+                              "roofline": sr["fp32"]["roofline"],
+                              **{k: d32[k] for k in ("delta_e_same_history_max_eh",
+                                                     "delta_e_first_block_max_eh",
+                                                     "blocks_usable", "energy_check")}}
         if want == "all":
             out["single"] = single
         else:

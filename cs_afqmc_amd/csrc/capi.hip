@@ -107,8 +107,12 @@ int set_device(const afqmc_ctx* c) {
 
 // fb[w][c][g] = mult * sum_s green_s[w][c][:] . rot_chol_s[g][:], as c->fb_parts split-K
 // slices of (2W, G) that the consumers add in slice order
-// bf16: split-bf16 planes of the mixed mode; otherwise the float64 Green's function
-int force_bias_gemm(afqmc_ctx* c, int W, bool bf16) {
+// arithmetic: FB_BF16 the split-bf16 planes of the mixed mode; otherwise the float64
+// Green's function, on float64 MFMA (FB_F64) or float32 MFMA (FB_F32, the FP32 mode)
+enum { FB_F64 = 0, FB_BF16 = 1, FB_F32 = 2 };
+int step_fb_mode(const afqmc_ctx* c) { return c->mixed ? FB_BF16 : (c->f32 ? FB_F32 : FB_F64); }
+int force_bias_gemm(afqmc_ctx* c, int W, int mode) {
+  const bool bf16 = mode == FB_BF16;
   const int G = c->d.nchol;
   const int64_t slice = (int64_t)2 * W * G;
   int part = 0;
@@ -129,8 +133,12 @@ int force_bias_gemm(afqmc_ctx* c, int W, bool bf16) {
       sp = gemm_f64_splits(2 * W, G, kd);
       if (part + sp > c->fb_cap)
         return afqmc_set_error(AFQMC_ERR_STATE, "force-bias split slices exceed workspace");
-      rc = launch_gemm_nt_f64_split(c->stream, 2 * W, G, kd, (double)c->mult, c->green[s], kd,
-                                    c->rot_chol[s], kd, c->fb + part * slice, G, sp, slice);
+      if (mode == FB_F32)
+        rc = launch_gemm_nt_f32_split(c->stream, 2 * W, G, kd, (double)c->mult, c->green[s], kd,
+                                      c->rot_chol[s], kd, c->fb + part * slice, G, sp, slice);
+      else
+        rc = launch_gemm_nt_f64_split(c->stream, 2 * W, G, kd, (double)c->mult, c->green[s], kd,
+                                      c->rot_chol[s], kd, c->fb + part * slice, G, sp, slice);
     }
     if (rc) return rc;
     part += sp;
@@ -164,7 +172,8 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
   if (d.n_exp_terms < 1 || d.n_exp_terms > 18)
     return afqmc_set_error(AFQMC_ERR_ARG, "n_exp_terms=%d outside [1, 18]", d.n_exp_terms);
   if (!(d.dt > 0.0)) return afqmc_set_error(AFQMC_ERR_ARG, "dt must be > 0");
-  if (d.precision != AFQMC_PREC_FP64 && d.precision != AFQMC_PREC_BF16X3)
+  if (d.precision != AFQMC_PREC_FP64 && d.precision != AFQMC_PREC_FP32 &&
+      d.precision != AFQMC_PREC_BF16X3)
     return afqmc_set_error(AFQMC_ERR_ARG, "precision mode %d not supported", d.precision);
   if (!ham->chol || !ham->mo_coeff || !ham->rot_h1 || !ham->rot_chol || !ham->mf_shifts ||
       !ham->exp_h1)
@@ -230,6 +239,7 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
 
     // mixed mode: split-bf16 planes of the contraction operands (zero padded to 32)
     c->mixed = d.precision == AFQMC_PREC_BF16X3;
+    c->f32 = d.precision == AFQMC_PREC_FP32;
     c->vhs_nv = N * N;
     c->Gp = (G + 31) / 32 * 32;
     c->Np32 = (N + 31) / 32 * 32;
@@ -374,7 +384,7 @@ extern "C" int afqmc_propagate(afqmc_ctx* c, int32_t W, void* walkers_up, void* 
   }
   {
     PhaseTimer t(c, AFQMC_PH_FB_GEMM);
-    if ((rc = force_bias_gemm(c, W, c->mixed))) return rc;
+    if ((rc = force_bias_gemm(c, W, step_fb_mode(c)))) return rc;
   }
   {
     PhaseTimer t(c, AFQMC_PH_FIELDS);
@@ -388,6 +398,9 @@ extern "C" int afqmc_propagate(afqmc_ctx* c, int32_t W, void* walkers_up, void* 
       rc = launch_gemm_nt_bf16x3_f32(c->stream, 2 * W, c->vhs_nv, c->Gp, 1.0f, c->xs_bf[0],
                                      c->xs_bf[1], c->Gp, c->chol_t_bf[0], c->chol_t_bf[1], c->Gp,
                                      c->vhs_f32, c->vhs_nv, 1, 0);
+    else if (c->f32)
+      rc = launch_gemm_nt_f32(c->stream, 2 * W, N * N, c->d.nchol, 1.0, c->xs, c->d.nchol,
+                              c->chol_t, c->d.nchol, 0.0, c->vhs, N * N);
     else
       rc = launch_gemm_nt_f64(c->stream, 2 * W, N * N, c->d.nchol, 1.0, c->xs, c->d.nchol,
                               c->chol_t, c->d.nchol, 0.0, c->vhs, N * N);
@@ -506,7 +519,7 @@ extern "C" int afqmc_force_bias(afqmc_ctx* c, int32_t W, const void* walkers_up,
   if ((rc = launch_measure(c, W, static_cast<const cplx*>(walkers_up),
                            static_cast<const cplx*>(walkers_dn), !c->mixed, c->mixed)))
     return rc;
-  if ((rc = force_bias_gemm(c, W, c->mixed))) return rc;
+  if ((rc = force_bias_gemm(c, W, step_fb_mode(c)))) return rc;
   return launch_fb_out(c, W, static_cast<cplx*>(force_bias));
 }
 
@@ -525,7 +538,7 @@ extern "C" int afqmc_energy(afqmc_ctx* c, int32_t W, const void* walkers_up,
     return rc;
   PhaseTimer t(c, AFQMC_PH_ENERGY);
   // c_g = force bias, always contracted in float64
-  if ((rc = force_bias_gemm(c, W, false))) return rc;
+  if ((rc = force_bias_gemm(c, W, FB_F64))) return rc;
   return launch_energy(c, W, static_cast<cplx*>(energies));
 }
 

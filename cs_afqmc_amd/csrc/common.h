@@ -197,6 +197,7 @@ struct afqmc_ctx {
 
   // reduced-precision copies (mixed mode, AFQMC_PREC_BF16X3)
   int mixed;                 // 1 when the contractions run split-bf16
+  int f32;                   // 1 when the contractions run on float32 MFMA (AFQMC_PREC_FP32)
   int taylor_bf;             // 1 when the Taylor kernels run split-bf16 (mixed mode)
   int Gp;                    // nchol rounded up to 32 (bf16 plane leading dimension)
   int Np32;                  // norb rounded up to 32
@@ -295,6 +296,13 @@ int launch_gemm_nt_f64_split(hipStream_t s, int M, int N, int K, double alpha, c
                              int lda, const double* B, int ldb, double* C, int ldc, int splits,
                              int64_t c_split_stride);
 int gemm_f64_splits(int M, int N, int K);
+// gemm_f32.hip: the same products on float32 MFMA (float64 operands rounded as staged,
+// float32 accumulation, float64 C)
+int launch_gemm_nt_f32(hipStream_t s, int M, int N, int K, double alpha, const double* A,
+                       int lda, const double* B, int ldb, double beta, double* C, int ldc);
+int launch_gemm_nt_f32_split(hipStream_t s, int M, int N, int K, double alpha, const double* A,
+                             int lda, const double* B, int ldb, double* C, int ldc, int splits,
+                             int64_t c_split_stride);
 // dst (cols x rows) = src (rows x cols)^T, float64
 int launch_transpose_f64(hipStream_t s, int rows, int cols, const double* src, double* dst);
 // gemm_bf16x3.hip: split-bf16 NT GEMM (K, lda, ldb multiples of 32 / 8), float / double out
@@ -423,4 +431,6 @@ int launch_taylor_grouped(hipStream_t s, int kind, int nkgv, int ct, const Taylo
 int launch_gemm_bf16x3_grouped(hipStream_t s, int bm, int out, const GemmArgs* args,
                                const int* start, int nprob, int nblocks);
 int launch_gemm_f64_grouped(hipStream_t s, int bm, bool vec2, const GemmArgs* args,
+                            const int* start, int nprob, int nblocks);
+int launch_gemm_f32_grouped(hipStream_t s, int bm, bool vec2, const GemmArgs* args,
                             const int* start, int nprob, int nblocks);

@@ -15,6 +15,8 @@
 // sums are added in tile order by the finishing kernel, so results are deterministic.
 // The tile holds whole (g, w) blocks for n <= 64 occupied orbitals per spin, the
 // library's limit (AFQMC_MAX_NOCC).
+#include <type_traits>
+
 #include "common.h"
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
@@ -61,26 +63,31 @@ __device__ inline void exchange_tile(const T* __restrict__ Fs, int ldf, int n, i
 }
 
 // float64 MFMA tile 64 x 128 (4 waves as 2 x 2, each 32 x 64), BK = 16, operands k-major
-// in LDS, double buffered.
+// in LDS, double buffered.  With F32 (the FP32 precision mode) the same tile runs on
+// v_mfma_f32_16x16x4f32: operands rounded to float32 as they are staged, float32
+// accumulation and F tile (its D rows are 4 (l / 16) + r instead of l / 16 + 4 r); the
+// exchange sums still accumulate in float64.
 constexpr int EF_BM = 64, EF_BN = 128, EF_BK = 16, EF_PAD = 16;
 constexpr int EF_LDF = 129;  // F tile row stride (odd: transposed reads spread over banks)
 constexpr int EF_OPS_BYTES = 2 * EF_BK * (EF_BM + EF_PAD) * 8 + 2 * EF_BK * (EF_BN + EF_PAD) * 8;
 constexpr int EF_F_BYTES = EF_BM * EF_LDF * 8;
 constexpr int EF_LDS = EF_OPS_BYTES > EF_F_BYTES ? EF_OPS_BYTES : EF_F_BYTES;
 
-template <int VEC>
+template <int VEC, bool F32>
 __global__ __launch_bounds__(ENER_NT) void exchange_f64_kernel(
     int G, int n, int N, int W, int gpt, int wpt, int g_tiles, const double* __restrict__ A,
     const double* __restrict__ B, cplx* __restrict__ xpart) {
+  typedef typename std::conditional<F32, float, double>::type T;
+  typedef typename std::conditional<F32, float4_t, double4_t>::type AccT;
   constexpr int BM = EF_BM, BN = EF_BN, BK = EF_BK;
   constexpr int TM = BM / 2 / 16, TN = BN / 2 / 16;
   constexpr int KT = BK / VEC, RSTEP = ENER_NT / KT, A_LD = BM / RSTEP, B_LD = BN / RSTEP;
   __shared__ __attribute__((aligned(16))) unsigned char lds[EF_LDS];
-  typedef double AsT[BK][BM + EF_PAD];
-  typedef double BsT[BK][BN + EF_PAD];
+  typedef T AsT[BK][BM + EF_PAD];
+  typedef T BsT[BK][BN + EF_PAD];
   AsT* As = reinterpret_cast<AsT*>(lds);
   BsT* Bs = reinterpret_cast<BsT*>(lds + 2 * sizeof(AsT));
-  double* Fs = reinterpret_cast<double*>(lds);
+  T* Fs = reinterpret_cast<T*>(lds);
 
   const int gt = blockIdx.x % g_tiles, wt = blockIdx.x / g_tiles;
   const int g0 = gt * gpt, w0 = wt * wpt;
@@ -90,7 +97,7 @@ __global__ __launch_bounds__(ENER_NT) void exchange_f64_kernel(
   const int wm = wave & 1, wn = wave >> 1;
   const int lk = (t % KT) * VEC, lr = t / KT;
 
-  double ra[A_LD][VEC], rb[B_LD][VEC];
+  T ra[A_LD][VEC], rb[B_LD][VEC];
   auto load_tile = [&](int k0) {
     const int k = k0 + lk;
     const bool kok = k < N;  // VEC == 2 needs N even (checked at launch)
@@ -100,10 +107,10 @@ __global__ __launch_bounds__(ENER_NT) void exchange_f64_kernel(
       if (VEC == 2) {
         double2 v = make_double2(0.0, 0.0);
         if (r < M && kok) v = *reinterpret_cast<const double2*>(A + (int64_t)r * N + k);
-        ra[i][0] = v.x;
-        ra[i][VEC - 1] = v.y;
+        ra[i][0] = (T)v.x;
+        ra[i][VEC - 1] = (T)v.y;
       } else {
-        ra[i][0] = (r < M && kok) ? A[(int64_t)r * N + k] : 0.0;
+        ra[i][0] = (r < M && kok) ? (T)A[(int64_t)r * N + k] : (T)0;
       }
     }
 #pragma unroll
@@ -112,10 +119,10 @@ __global__ __launch_bounds__(ENER_NT) void exchange_f64_kernel(
       if (VEC == 2) {
         double2 v = make_double2(0.0, 0.0);
         if (r < NB && kok) v = *reinterpret_cast<const double2*>(B + (int64_t)r * N + k);
-        rb[i][0] = v.x;
-        rb[i][VEC - 1] = v.y;
+        rb[i][0] = (T)v.x;
+        rb[i][VEC - 1] = (T)v.y;
       } else {
-        rb[i][0] = (r < NB && kok) ? B[(int64_t)r * N + k] : 0.0;
+        rb[i][0] = (r < NB && kok) ? (T)B[(int64_t)r * N + k] : (T)0;
       }
     }
   };
@@ -130,11 +137,11 @@ __global__ __launch_bounds__(ENER_NT) void exchange_f64_kernel(
       for (int v = 0; v < VEC; ++v) Bs[buf][lk + v][lr + i * RSTEP] = rb[i][v];
   };
 
-  double4_t acc[TM][TN];
+  AccT acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < TN; ++j) acc[i][j] = AccT{0, 0, 0, 0};
 
   const int nk = (N + BK - 1) / BK;
   load_tile(0);
@@ -148,7 +155,7 @@ __global__ __launch_bounds__(ENER_NT) void exchange_f64_kernel(
     if (kt + 1 < nk) load_tile((kt + 1) * BK);
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
-      double af[TM], bf[TN];
+      T af[TM], bf[TN];
 #pragma unroll
       for (int i = 0; i < TM; ++i) af[i] = As[cur][ks * 4 + kq][am + i * 16];
 #pragma unroll
@@ -156,8 +163,12 @@ __global__ __launch_bounds__(ENER_NT) void exchange_f64_kernel(
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < TN; ++j) {
+          if constexpr (F32)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i], bf[j], acc[i][j], 0, 0, 0);
+          else
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
     }
     if (kt + 1 < nk) store_tile(cur ^ 1);
     __syncthreads();
@@ -170,14 +181,14 @@ __global__ __launch_bounds__(ENER_NT) void exchange_f64_kernel(
       const int col = wn * (BN / 2) + j * 16 + (lane & 15);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = wm * (BM / 2) + i * 16 + (lane >> 4) + 4 * r;
+        const int row = wm * (BM / 2) + i * 16 + (F32 ? 4 * (lane >> 4) + r : (lane >> 4) + 4 * r);
         Fs[row * EF_LDF + col] = acc[i][j][r];
       }
     }
   __syncthreads();
   const int g_used = (G - g0 < gpt) ? G - g0 : gpt;
   const int w_used = (W - w0 < wpt) ? W - w0 : wpt;
-  exchange_tile<double>(Fs, EF_LDF, n, g_used, w_used, w0, xpart + (int64_t)gt * W);
+  exchange_tile<T>(Fs, EF_LDF, n, g_used, w_used, w0, xpart + (int64_t)gt * W);
 }
 
 // split-bf16 tile 128 x 128 on v_mfma_f32_16x16x32_bf16 (4 waves 2 x 2, each 64 x 64),
@@ -398,14 +409,21 @@ int launch_energy(afqmc_ctx* c, int W, cplx* energies) {
       const int g_tiles = (G + gpt - 1) / gpt, w_tiles = (W + wpt - 1) / wpt;
       gts[s] = g_tiles;
       const bool vec2 = (N % 2 == 0);
-      if (vec2)
-        hipLaunchKernelGGL(exchange_f64_kernel<2>, dim3(g_tiles * w_tiles), dim3(ENER_NT), 0,
-                           c->stream, G, n, N, W, gpt, wpt, g_tiles, c->rot_chol[s], c->green[s],
-                           xp);
+      const dim3 grid(g_tiles * w_tiles), block(ENER_NT);
+      const double* A = c->rot_chol[s];
+      const double* B = c->green[s];
+      if (c->f32 && vec2)
+        hipLaunchKernelGGL((exchange_f64_kernel<2, true>), grid, block, 0, c->stream, G, n, N,
+                           W, gpt, wpt, g_tiles, A, B, xp);
+      else if (c->f32)
+        hipLaunchKernelGGL((exchange_f64_kernel<1, true>), grid, block, 0, c->stream, G, n, N,
+                           W, gpt, wpt, g_tiles, A, B, xp);
+      else if (vec2)
+        hipLaunchKernelGGL((exchange_f64_kernel<2, false>), grid, block, 0, c->stream, G, n, N,
+                           W, gpt, wpt, g_tiles, A, B, xp);
       else
-        hipLaunchKernelGGL(exchange_f64_kernel<1>, dim3(g_tiles * w_tiles), dim3(ENER_NT), 0,
-                           c->stream, G, n, N, W, gpt, wpt, g_tiles, c->rot_chol[s], c->green[s],
-                           xp);
+        hipLaunchKernelGGL((exchange_f64_kernel<1, false>), grid, block, 0, c->stream, G, n, N,
+                           W, gpt, wpt, g_tiles, A, B, xp);
     }
     LAUNCH_CHECK();
     xp += (int64_t)gts[s] * W;

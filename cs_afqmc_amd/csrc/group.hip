@@ -45,6 +45,7 @@ struct afqmc_group {
   std::vector<Member> mem;
   bool dirty = true;
   bool mixed = false;
+  bool f32 = false;           // FP32 mode: the float64 operands on float32 MFMA
   int fb_bm = 0, vhs_bm = 0;  // GEMM tile edge (0: heuristic)
   bool f64_vec2 = true;
   // built tables
@@ -378,6 +379,9 @@ int run_gemm(afqmc_group* g, const Launch& L, int out_f32) {
   if (g->mixed)
     return launch_gemm_bf16x3_grouped(g->stream, L.k0, out_f32, targs<GemmArgs>(g, L),
                                       tstart(g, L), L.nprob, L.nblocks);
+  if (g->f32)
+    return launch_gemm_f32_grouped(g->stream, L.k0, g->f64_vec2, targs<GemmArgs>(g, L),
+                                   tstart(g, L), L.nprob, L.nblocks);
   return launch_gemm_f64_grouped(g->stream, L.k0, g->f64_vec2, targs<GemmArgs>(g, L),
                                  tstart(g, L), L.nprob, L.nblocks);
 }
@@ -452,14 +456,18 @@ extern "C" int afqmc_group_add(afqmc_group* g, afqmc_ctx* c, void* walkers_up, v
       !scalars || n_steps < 1)
     return afqmc_set_error(AFQMC_ERR_ARG, "null buffer or n_steps < 1");
   const bool mixed = c->mixed != 0;
-  if (!g->mem.empty() && mixed != g->mixed)
+  const bool f32 = c->f32 != 0;
+  if (!g->mem.empty() && (mixed != g->mixed || f32 != g->f32))
     return afqmc_set_error(AFQMC_ERR_ARG, "all fragments of a group share one precision mode");
   if (c->device != g->device)
     return afqmc_set_error(AFQMC_ERR_ARG, "context on device %d, group on device %d", c->device,
                            g->device);
   for (const Member& m : g->mem)
     if (m.c == c) return afqmc_set_error(AFQMC_ERR_ARG, "context already in the group");
-  if (g->mem.empty()) g->mixed = mixed;
+  if (g->mem.empty()) {
+    g->mixed = mixed;
+    g->f32 = f32;
+  }
   Member m{c, static_cast<cplx*>(walkers_up), static_cast<cplx*>(walkers_dn), weights,
            static_cast<cplx*>(overlaps), fields_block, n_steps, scalars};
   g->mem.push_back(m);

@@ -41,18 +41,13 @@ def bytes_per_walker(norb, nelec, nchol, walker_type, precision, W, nv=None):
     ns = [nelec[0]] if walker_type == "rhf" else [nelec[0], nelec[1]]
     nsum, nspin = sum(ns), len(ns)
     bf = precision == "bf16x3"
-    f32 = precision == "fp32"
     Np32, Gp = -(-N // 32) * 32, -(-G // 32) * 32
     walker = 16.0 * N * nsum
     if bf:
         green, vhs, xs = 8.0 * nsum * Np32, 8.0 * nv, 8.0 * Gp
         chol, rot = 4.0 * nv * Gp / W, 4.0 * G * nsum * Np32 / W
         one_body = nspin * 4.0 * N * Np32 / W
-    elif f32:
-        green, vhs, xs = 8.0 * nsum * N, 16.0 * nv, 8.0 * G
-        chol, rot = 4.0 * nv * G / W, 4.0 * G * nsum * N / W
-        one_body = nspin * 8.0 * N * N / W
-    else:
+    else:  # float64 operands (the FP32 mode rounds them to float32 on chip)
         green, vhs, xs = 16.0 * nsum * N, 16.0 * nv, 16.0 * G
         chol, rot = 8.0 * nv * G / W, 8.0 * G * nsum * N / W
         one_body = nspin * 8.0 * N * N / W

@@ -11,6 +11,7 @@ from . import _build
 
 AFQMC_RHF = 1
 AFQMC_UHF = 2
+AFQMC_UHF_SPIN_CHOL = 3
 AFQMC_STEP_REUSE_GREEN = 1
 AFQMC_SR_GLOBAL_ORDER = 1
 BLOCK_STATS_LEN = 4  # AFQMC_BLOCK_STATS_LEN: block energy, weight, zero weights, clamped

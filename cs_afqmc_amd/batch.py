@@ -26,13 +26,17 @@ import torch
 from . import _lib, hamiltonian, propagation, rng, wavefunctions
 from .device import DeviceGroup
 from .graphs import Replay
+from .prop_unrestricted import propagation as pu_prop
+from .prop_unrestricted import wavefunctions_unrestricted as pu_wf
 
 
 class Fragment:
     """One AFQMC problem with resident device state.
 
     ham_data / wave_data: the reference's dicts (h0, h1 (2,N,N), chol (G,N*N); mo_coeff,
-    rdm1); walker_type "rhf" or "uhf"; nelec (n_up, n_dn)."""
+    rdm1); walker_type "rhf" or "uhf"; nelec (n_up, n_dn).  Unrestricted fragments whose
+    chol is (2, G, N*N) -- one set of vectors per spin, the layout of the reference's
+    prop_unrestricted package -- step with that package's trial and propagator."""
 
     def __init__(self, ham_data, wave_data, walker_type, nelec, n_walkers, dt=0.005, seed=0,
                  precision="bf16x3", n_prop_steps=50, n_exp_terms=6, tag=None):
@@ -45,6 +49,10 @@ class Fragment:
             self.trial = wavefunctions.rhf(norb, tuple(nelec))
             self.prop = propagation.propagator_restricted(dt, n_walkers, n_exp_terms,
                                                           precision=precision)
+        elif len(getattr(ham_data["chol"], "shape", np.shape(ham_data["chol"]))) == 3:
+            self.trial = pu_wf.uhf(norb, tuple(nelec))
+            self.prop = pu_prop.propagator_unrestricted(dt, n_walkers, n_exp_terms,
+                                                        precision=precision)
         else:
             self.trial = wavefunctions.uhf(norb, tuple(nelec))
             self.prop = propagation.propagator_unrestricted(dt, n_walkers, n_exp_terms,

@@ -154,14 +154,15 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
   if (!out || !dims || !ham) return afqmc_set_error(AFQMC_ERR_ARG, "null argument");
   *out = nullptr;
   const afqmc_dims d = *dims;
-  if (d.walker_type != AFQMC_RHF && d.walker_type != AFQMC_UHF)
+  if (d.walker_type != AFQMC_RHF && d.walker_type != AFQMC_UHF &&
+      d.walker_type != AFQMC_UHF_SPIN_CHOL)
     return afqmc_set_error(AFQMC_ERR_ARG, "walker_type=%d", d.walker_type);
   if (d.norb < 1 || d.norb > AFQMC_MAX_NORB)
     return afqmc_set_error(AFQMC_ERR_SIZE, "norb=%d outside [1, %d]", d.norb, AFQMC_MAX_NORB);
   if (d.nocc_up < 1 || d.nocc_up > AFQMC_MAX_NOCC || d.nocc_up > d.norb)
     return afqmc_set_error(AFQMC_ERR_SIZE, "nocc_up=%d outside [1, min(norb, %d)]", d.nocc_up,
                            AFQMC_MAX_NOCC);
-  if (d.walker_type == AFQMC_UHF &&
+  if (d.walker_type != AFQMC_RHF &&
       (d.nocc_dn < 1 || d.nocc_dn > AFQMC_MAX_NOCC || d.nocc_dn > d.norb))
     return afqmc_set_error(AFQMC_ERR_SIZE, "nocc_dn=%d outside [1, min(norb, %d)]", d.nocc_dn,
                            AFQMC_MAX_NOCC);
@@ -189,6 +190,7 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
     return afqmc_set_error(AFQMC_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
   }
   c->nspin = d.walker_type == AFQMC_RHF ? 1 : 2;
+  c->chol_spins = d.walker_type == AFQMC_UHF_SPIN_CHOL ? 2 : 1;
   c->mult = d.walker_type == AFQMC_RHF ? 2 : 1;
   c->nocc[0] = d.nocc_up;
   c->nocc[1] = d.walker_type == AFQMC_RHF ? d.nocc_up : d.nocc_dn;
@@ -220,18 +222,20 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
     // transposed on the device from the (G, N*N) array (read in place when the caller
     // passes device memory, else from a temporary upload)
     {
+      const size_t per = (size_t)G * N * N;
       double* raw = nullptr;
       const double* src = ham->chol;
       if (!is_device_ptr(src, c->device)) {
-        rc = upload(c, &raw, ham->chol, (size_t)G * N * N);
+        rc = upload(c, &raw, ham->chol, c->chol_spins * per);
         src = raw;
       }
-      if (!rc) rc = dalloc(c, &c->chol_t, (size_t)G * N * N);
-      if (!rc) rc = launch_transpose_f64(c->stream, G, N * N, src, c->chol_t);
+      if (!rc) rc = dalloc(c, &c->chol_t, c->chol_spins * per);
+      for (int cs = 0; cs < c->chol_spins && !rc; ++cs)
+        rc = launch_transpose_f64(c->stream, G, N * N, src + cs * per, c->chol_t + cs * per);
       (void)hipStreamSynchronize(c->stream);
       if (raw) {
         (void)hipFree(raw);
-        c->device_bytes -= (int64_t)G * N * N * (int64_t)sizeof(double);
+        c->device_bytes -= (int64_t)(c->chol_spins * per * sizeof(double));
       }
       if (rc) goto fail;
     }
@@ -253,7 +257,8 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
         unsigned long long st[2] = {0ull, 0ull};
         TRY(dalloc(c, &c->sym_stat, 2));
         TRY(zero_async(c->sym_stat, 2 * sizeof(unsigned long long), c->stream));
-        TRY(launch_sym_check(c->stream, N, G, c->chol_t, c->sym_stat));
+        for (int cs = 0; cs < c->chol_spins; ++cs)  // maxima accumulate over the spins
+          TRY(launch_sym_check(c->stream, N, G, chol_t_of(c, cs), c->sym_stat));
         TRY(copy_async(st, c->sym_stat, sizeof(st), c->stream));
         TRY(sync_stream(c->stream));
         double asym, amax;
@@ -261,13 +266,16 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
         std::memcpy(&amax, &st[1], sizeof(double));
         if (asym <= 1e-12 * amax) c->vhs_nv = N * (N + 1) / 2;
       }
-      for (int h = 0; h < 2; ++h) TRY(dalloc(c, &c->chol_t_bf[h], (size_t)c->vhs_nv * c->Gp));
-      if (c->vhs_nv != N * N)
-        TRY(launch_split_bf16_sym(c->stream, N, G, c->chol_t, c->chol_t_bf[0], c->chol_t_bf[1],
-                                  c->Gp));
-      else
-        TRY(launch_split_bf16(c->stream, (int)nn, G, c->chol_t, G, c->chol_t_bf[0],
-                              c->chol_t_bf[1], c->Gp));
+      for (int h = 0; h < 2; ++h)
+        TRY(dalloc(c, &c->chol_t_bf[h], (size_t)c->chol_spins * c->vhs_nv * c->Gp));
+      for (int cs = 0; cs < c->chol_spins; ++cs) {
+        if (c->vhs_nv != N * N)
+          TRY(launch_split_bf16_sym(c->stream, N, G, chol_t_of(c, cs), chol_t_bf_of(c, 0, cs),
+                                    chol_t_bf_of(c, 1, cs), c->Gp));
+        else
+          TRY(launch_split_bf16(c->stream, (int)nn, G, chol_t_of(c, cs), G,
+                                chol_t_bf_of(c, 0, cs), chol_t_bf_of(c, 1, cs), c->Gp));
+      }
       for (int s = 0; s < c->nspin; ++s) {
         const size_t rows = (size_t)G * c->nocc[s];
         for (int h = 0; h < 2; ++h) {
@@ -287,7 +295,9 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
         // 8 floats of tail padding: the streaming split-bf16 Taylor kernel reads V rows in
         // 8-float chunks, and the chunk that straddles the end of the last row is masked;
         // at least N^2 doubles, which stage exp_h1 - I below
-        TRY(dalloc(c, &c->vhs_f32, std::max((size_t)W * 2 * c->vhs_nv + 8, 2 * nn)));
+        c->vhs_f32_stride = (int64_t)W * 2 * c->vhs_nv + 8;
+        TRY(dalloc(c, &c->vhs_f32,
+                   std::max((size_t)c->chol_spins * c->vhs_f32_stride, 2 * nn)));
         for (int s = 0; s < c->nspin; ++s) {
           // exp_h1 - I as split-bf16 planes, rows padded to Np32 (bf16 Taylor kernel).  The
           // float64 matrix is staged in the VHS workspace (8 W N^2 bytes, unused until the
@@ -316,7 +326,7 @@ extern "C" int afqmc_ctx_create(afqmc_ctx** out, const afqmc_dims* dims,
     }
     TRY(dalloc(c, &c->fb, (size_t)c->fb_cap * W * 2 * G));
     TRY(dalloc(c, &c->xs, (size_t)W * 2 * G));
-    if (!c->taylor_bf) TRY(dalloc(c, &c->vhs, (size_t)W * 2 * N * N));
+    if (!c->taylor_bf) TRY(dalloc(c, &c->vhs, (size_t)c->chol_spins * W * 2 * N * N));
     TRY(dalloc(c, &c->terms, (size_t)W * 2));
     TRY(dalloc(c, &c->dets, (size_t)W * 2));
     {
@@ -394,16 +404,19 @@ extern "C" int afqmc_propagate(afqmc_ctx* c, int32_t W, void* walkers_up, void* 
     // VHS[w] = sum_g (i sqrt(dt) x~_wg) L_g  : (2W, G) x (vhs_nv, G)^T, all N*N entries
     // or the upper triangle of the (then symmetric) V
     PhaseTimer t(c, AFQMC_PH_VHS_GEMM);
-    if (c->mixed)
-      rc = launch_gemm_nt_bf16x3_f32(c->stream, 2 * W, c->vhs_nv, c->Gp, 1.0f, c->xs_bf[0],
-                                     c->xs_bf[1], c->Gp, c->chol_t_bf[0], c->chol_t_bf[1], c->Gp,
-                                     c->vhs_f32, c->vhs_nv, 1, 0);
-    else if (c->f32)
-      rc = launch_gemm_nt_f32(c->stream, 2 * W, N * N, c->d.nchol, 1.0, c->xs, c->d.nchol,
-                              c->chol_t, c->d.nchol, 0.0, c->vhs, N * N);
-    else
-      rc = launch_gemm_nt_f64(c->stream, 2 * W, N * N, c->d.nchol, 1.0, c->xs, c->d.nchol,
-                              c->chol_t, c->d.nchol, 0.0, c->vhs, N * N);
+    for (int cs = 0; cs < c->chol_spins && !rc; ++cs) {
+      if (c->mixed)
+        rc = launch_gemm_nt_bf16x3_f32(c->stream, 2 * W, c->vhs_nv, c->Gp, 1.0f, c->xs_bf[0],
+                                       c->xs_bf[1], c->Gp, chol_t_bf_of(c, 0, cs),
+                                       chol_t_bf_of(c, 1, cs), c->Gp, vhs_f32_of(c, cs),
+                                       c->vhs_nv, 1, 0);
+      else if (c->f32)
+        rc = launch_gemm_nt_f32(c->stream, 2 * W, N * N, c->d.nchol, 1.0, c->xs, c->d.nchol,
+                                chol_t_of(c, cs), c->d.nchol, 0.0, vhs_of(c, cs), N * N);
+      else
+        rc = launch_gemm_nt_f64(c->stream, 2 * W, N * N, c->d.nchol, 1.0, c->xs, c->d.nchol,
+                                chol_t_of(c, cs), c->d.nchol, 0.0, vhs_of(c, cs), N * N);
+    }
     if (rc) return rc;
   }
   {

@@ -182,6 +182,7 @@ struct afqmc_ctx {
   hipStream_t stream;
   afqmc_dims d;
   int nspin;          // 1 (restricted) or 2
+  int chol_spins;     // 1, or 2 for separate alpha / beta Cholesky vectors
   int nocc[2];
   int mult;           // 2 for restricted (both spins share the walker), 1 otherwise
   int kdim[2];        // n_sigma * N
@@ -207,7 +208,8 @@ struct afqmc_ctx {
   uint16_t* rot_chol_bf[2][2];  // [spin][hi/lo], (G, n_sigma, Np32)
   uint16_t* xs_bf[2];        // [hi/lo] (2W, Gp): i sqrt(dt) x~
   uint16_t* green_bf[2][2];  // [spin][hi/lo] (W, 2, n_sigma, Np32)
-  float* vhs_f32;            // (W, 2, vhs_nv)
+  float* vhs_f32;            // chol_spins x [(W, 2, vhs_nv) + 8 floats of padding]
+  int64_t vhs_f32_stride;    // floats between the spins' V blocks
   unsigned long long* sym_stat;  // (2) Cholesky symmetry check at creation
   uint16_t* emi_bf[2][2];    // [spin][hi/lo] (N, Np32): split-bf16 exp_h1 - I
 
@@ -233,6 +235,22 @@ struct afqmc_ctx {
   // phase timing (afqmc_prof_*)
   ProfRec prof;
 };
+
+// Spin s's transposed Cholesky vectors, their split-bf16 planes and its V buffers: one
+// set shared by both spins unless the vectors are spin resolved.
+inline int chol_spin(const afqmc_ctx* c, int s) { return c->chol_spins == 2 ? s : 0; }
+inline double* chol_t_of(const afqmc_ctx* c, int s) {
+  return c->chol_t + (size_t)chol_spin(c, s) * c->d.norb * c->d.norb * c->d.nchol;
+}
+inline uint16_t* chol_t_bf_of(const afqmc_ctx* c, int h, int s) {
+  return c->chol_t_bf[h] + (size_t)chol_spin(c, s) * c->vhs_nv * c->Gp;
+}
+inline double* vhs_of(const afqmc_ctx* c, int s) {
+  return c->vhs + (size_t)chol_spin(c, s) * c->d.n_walkers * 2 * c->d.norb * c->d.norb;
+}
+inline float* vhs_f32_of(const afqmc_ctx* c, int s) {
+  return c->vhs_f32 + (size_t)chol_spin(c, s) * c->vhs_f32_stride;
+}
 
 // phase timer: records an event pair around a phase when profiling is on
 struct PhaseTimer {

@@ -209,44 +209,46 @@ int build(afqmc_group* g) {
     put_table(tab, g->fields, args, start);
   }
 
-  // ---- VHS GEMMs: one problem per fragment
+  // ---- VHS GEMMs: one problem per fragment (per spin with spin-resolved vectors)
   {
     std::vector<GemmArgs> args;
     std::vector<int> start;
     int64_t t128 = 0;
     for (const Member& m : g->mem)
-      t128 += (int64_t)((2 * m.c->d.n_walkers + 127) / 128) *
+      t128 += (int64_t)m.c->chol_spins * ((2 * m.c->d.n_walkers + 127) / 128) *
               (((int64_t)m.c->d.norb * m.c->d.norb + 127) / 128);
     const int bm = g->vhs_bm ? g->vhs_bm : heuristic_bm(t128);
     int b = 0;
     for (const Member& m : g->mem) {
       afqmc_ctx* c = m.c;
       const int W = c->d.n_walkers, N = c->d.norb;
-      GemmArgs a{};
-      a.M = 2 * W;
-      a.N = g->mixed ? c->vhs_nv : N * N;
-      a.alpha = 1.0f;
-      a.ldc = a.N;
-      if (g->mixed) {
-        a.K = c->Gp;
-        a.A = c->xs_bf[0];
-        a.Al = c->xs_bf[1];
-        a.B = c->chol_t_bf[0];
-        a.Bl = c->chol_t_bf[1];
-        a.C = c->vhs_f32;
-      } else {
-        a.K = c->d.nchol;
-        a.A = c->xs;
-        a.B = c->chol_t;
-        a.C = c->vhs;
-        if (a.K % 2) g->f64_vec2 = false;
+      for (int cs = 0; cs < c->chol_spins; ++cs) {
+        GemmArgs a{};
+        a.M = 2 * W;
+        a.N = g->mixed ? c->vhs_nv : N * N;
+        a.alpha = 1.0f;
+        a.ldc = a.N;
+        if (g->mixed) {
+          a.K = c->Gp;
+          a.A = c->xs_bf[0];
+          a.Al = c->xs_bf[1];
+          a.B = chol_t_bf_of(c, 0, cs);
+          a.Bl = chol_t_bf_of(c, 1, cs);
+          a.C = vhs_f32_of(c, cs);
+        } else {
+          a.K = c->d.nchol;
+          a.A = c->xs;
+          a.B = chol_t_of(c, cs);
+          a.C = vhs_of(c, cs);
+          if (a.K % 2) g->f64_vec2 = false;
+        }
+        a.lda = a.ldb = a.K;
+        a.tiles_m = (a.M + bm - 1) / bm;
+        a.tiles_n = (a.N + bm - 1) / bm;
+        args.push_back(a);
+        start.push_back(b);
+        b += a.tiles_m * a.tiles_n;
       }
-      a.lda = a.ldb = a.K;
-      a.tiles_m = (a.M + bm - 1) / bm;
-      a.tiles_n = (a.N + bm - 1) / bm;
-      args.push_back(a);
-      start.push_back(b);
-      b += a.tiles_m * a.tiles_n;
     }
     g->vhs = Launch{};
     g->vhs.k0 = bm;
@@ -276,7 +278,7 @@ int build(afqmc_group* g) {
           a.E = c->emi_bf[s][0];
           a.E2 = c->emi_bf[s][1];
         }
-        a.vhs = g->mixed ? (const void*)c->vhs_f32 : (const void*)c->vhs;
+        a.vhs = g->mixed ? (const void*)vhs_f32_of(c, s) : (const void*)vhs_of(c, s);
         const auto key = std::make_tuple(pl.kind, pl.nkgv, pl.ct, pl.nthreads);
         by_key[key].push_back({a, c->d.n_walkers * pl.blocks_per_walker});
         shm[key] = std::max(shm[key], pl.shm);

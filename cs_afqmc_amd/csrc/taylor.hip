@@ -318,7 +318,7 @@ __global__ __launch_bounds__(512) void taylor_reg_grouped_kernel(
 }
 
 template <int NKGV, int CTP>
-int launch_reg(afqmc_ctx* c, int W, int n, cplx* wk, const double* E) {
+int launch_reg(afqmc_ctx* c, int W, int n, cplx* wk, const double* E, const double* V) {
   const int N = c->d.norb;
   const int nwaves = (N + 15) / 16;
   const int Np = (N + 3) & ~3;
@@ -327,7 +327,7 @@ int launch_reg(afqmc_ctx* c, int W, int n, cplx* wk, const double* E) {
   const int rc = ensure_dyn_lds((const void*)taylor_reg_kernel<NKGV, CTP>, shm, &lds_limit);
   if (rc) return rc;
   hipLaunchKernelGGL((taylor_reg_kernel<NKGV, CTP>), dim3(W), dim3(nwaves * 64), shm, c->stream,
-                     N, n, c->d.n_exp_terms - 1, wk, E, c->vhs);
+                     N, n, c->d.n_exp_terms - 1, wk, E, V);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }
@@ -572,8 +572,8 @@ int launch_bf(afqmc_ctx* c, int W, int n, cplx* wk, int s) {
   const int rc = ensure_dyn_lds((const void*)taylor_bf_kernel<KB, CTP>, shm, &lds_limit);
   if (rc) return rc;
   hipLaunchKernelGGL((taylor_bf_kernel<KB, CTP>), dim3(W), dim3(nwaves * 64), shm, c->stream, N,
-                     n, c->d.n_exp_terms - 1, wk, c->emi_bf[s][0], c->emi_bf[s][1], c->vhs_f32,
-                     c->vhs_nv);
+                     n, c->d.n_exp_terms - 1, wk, c->emi_bf[s][0], c->emi_bf[s][1],
+                     vhs_f32_of(c, s), c->vhs_nv);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }
@@ -918,19 +918,19 @@ int launch_bfs(afqmc_ctx* c, int W, int n, cplx* wk, int s) {
   if (rc) return rc;
   hipLaunchKernelGGL((taylor_bfs_kernel<CT>), dim3(W), dim3(bfs_threads(N, CT)), shm, c->stream,
                      N, n, c->d.n_exp_terms - 1, wk, c->emi_bf[s][0], c->emi_bf[s][1],
-                     c->vhs_f32, c->vhs_nv);
+                     vhs_f32_of(c, s), c->vhs_nv);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }
 
 template <int NKGV>
-int launch_reg_ct(afqmc_ctx* c, int W, int n, cplx* wk, const double* E) {
+int launch_reg_ct(afqmc_ctx* c, int W, int n, cplx* wk, const double* E, const double* V) {
   const int ctp = (2 * n + 15) / 16;
   switch (ctp) {
-    case 1: return launch_reg<NKGV, 1>(c, W, n, wk, E);
-    case 2: return launch_reg<NKGV, 2>(c, W, n, wk, E);
-    case 3: return launch_reg<NKGV, 3>(c, W, n, wk, E);
-    default: return launch_reg<NKGV, 4>(c, W, n, wk, E);
+    case 1: return launch_reg<NKGV, 1>(c, W, n, wk, E, V);
+    case 2: return launch_reg<NKGV, 2>(c, W, n, wk, E, V);
+    case 3: return launch_reg<NKGV, 3>(c, W, n, wk, E, V);
+    default: return launch_reg<NKGV, 4>(c, W, n, wk, E, V);
   }
 }
 
@@ -1009,10 +1009,11 @@ int launch_taylor(afqmc_ctx* c, int W, cplx* wup, cplx* wdn) {
       cplx* wk = (s == 0) ? wup : wdn;
       // register budget: 2 ceil(N/4) k-groups of [Vr | Vi] per lane, bucketed
       const int nkgv = 2 * ((N + 3) / 4);
-      const int rc = nkgv <= 32   ? launch_reg_ct<32>(c, W, n, wk, c->exp_h1[s])
-                     : nkgv <= 48 ? launch_reg_ct<48>(c, W, n, wk, c->exp_h1[s])
-                     : nkgv <= 56 ? launch_reg_ct<56>(c, W, n, wk, c->exp_h1[s])
-                                  : launch_reg_ct<64>(c, W, n, wk, c->exp_h1[s]);
+      const double* V = vhs_of(c, s);
+      const int rc = nkgv <= 32   ? launch_reg_ct<32>(c, W, n, wk, c->exp_h1[s], V)
+                     : nkgv <= 48 ? launch_reg_ct<48>(c, W, n, wk, c->exp_h1[s], V)
+                     : nkgv <= 56 ? launch_reg_ct<56>(c, W, n, wk, c->exp_h1[s], V)
+                                  : launch_reg_ct<64>(c, W, n, wk, c->exp_h1[s], V);
       if (rc) return rc;
     }
     return AFQMC_OK;
@@ -1028,7 +1029,7 @@ int launch_taylor(afqmc_ctx* c, int W, cplx* wup, cplx* wdn) {
     cplx* wk = (s == 0) ? wup : wdn;
     dim3 grid(W, (n + 15) / 16);
     hipLaunchKernelGGL(taylor_kernel, grid, dim3(nwaves * 64), shm, c->stream, N, n,
-                       c->d.n_exp_terms - 1, wk, c->exp_h1[s], c->vhs, rows_pad);
+                       c->d.n_exp_terms - 1, wk, c->exp_h1[s], vhs_of(c, s), rows_pad);
     LAUNCH_CHECK();
   }
   return AFQMC_OK;

@@ -98,7 +98,8 @@ class DeviceStep:
     """
 
     def __init__(self, walker_type, norb, nelec, n_walkers, dt, n_exp_terms, h0, h0_prop,
-                 chol, mo_coeff, rot_h1, rot_chol, mf_shifts, exp_h1, precision="fp64"):
+                 chol, mo_coeff, rot_h1, rot_chol, mf_shifts, exp_h1, precision="fp64",
+                 spin_chol=False):
         if not torch.cuda.is_available():
             raise _lib.AfqmcError("cs_afqmc_amd needs a ROCm GPU (torch.cuda is unavailable)")
         self.walker_type = walker_type
@@ -108,12 +109,20 @@ class DeviceStep:
         self.dt = float(dt)
         self.n_exp_terms = int(n_exp_terms)
         self.precision = precision
+        # spin_chol: unrestricted walkers with separate alpha / beta Cholesky vectors,
+        # chol (2, G, N*N) alpha then beta (prop_unrestricted/propagation.py:361-400)
+        self.spin_chol = bool(spin_chol)
+        if self.spin_chol and walker_type != "uhf":
+            raise ValueError("spin-resolved Cholesky vectors need unrestricted walkers")
+        ncs = 2 if self.spin_chol else 1
         if isinstance(chol, torch.Tensor):
-            self.nchol = chol.numel() // (norb * norb)
-            chol = _dev_f64(chol, self.nchol * norb * norb, "chol")
+            self.nchol = chol.numel() // (ncs * norb * norb)
+            chol = _dev_f64(chol, ncs * self.nchol * norb * norb, "chol")
         else:
             chol = np.ascontiguousarray(np.asarray(chol, dtype=_f64).reshape(-1, norb * norb))
-            self.nchol = chol.shape[0]
+            if chol.shape[0] % ncs:
+                raise ValueError("spin-resolved chol needs (2, G, N*N)")
+            self.nchol = chol.shape[0] // ncs
         if walker_type == "rhf":
             mo = np.ascontiguousarray(np.asarray(mo_coeff, dtype=_f64))
             rh1 = np.ascontiguousarray(np.asarray(rot_h1, dtype=_f64))
@@ -126,7 +135,7 @@ class DeviceStep:
             rh1 = np.ascontiguousarray(np.concatenate(
                 [np.asarray(m, dtype=_f64).ravel() for m in rot_h1]))
             eh1 = np.ascontiguousarray(np.asarray(exp_h1, dtype=_f64).reshape(2, norb, norb))
-            wt = _lib.AFQMC_UHF
+            wt = _lib.AFQMC_UHF_SPIN_CHOL if self.spin_chol else _lib.AFQMC_UHF
         else:
             raise ValueError(f"walker_type {walker_type!r}")
         if any(isinstance(m, torch.Tensor) for m in rot_chol):

@@ -67,6 +67,26 @@ def synthetic_uhf(norb, nocc_up, nocc_dn, nchol, seed=0, decay=80.0, chol_std=No
     return ham_data, wave_data, (nocc_up, nocc_dn)
 
 
+def synthetic_uhf_spin(norb, nocc_up, nocc_dn, nchol, seed=0, decay=80.0, chol_std=None,
+                       h1_std=0.1, mix=0.6):
+    """Unrestricted problem with spin-resolved Cholesky vectors, in the layout of the
+    reference's prop_unrestricted package: chol (2, G, N*N), h1 (2, N, N).
+    L^a_g as in synthetic_uhf; L^b_g = sqrt(1 - mix^2) L^a_g + mix B_g with B_g an
+    independent draw of the same scale and decay; h1[1] likewise from h1[0]."""
+    rng = np.random.default_rng(seed)
+    chol_a, h1_a = _chol_h1(norb, nchol, rng, decay, chol_std, h1_std)
+    chol_n, h1_n = _chol_h1(norb, nchol, rng, decay, chol_std, h1_std)
+    c = np.sqrt(1.0 - mix * mix)
+    chol = np.array([chol_a, c * chol_a + mix * chol_n])
+    h1 = np.array([h1_a, c * h1_a + mix * h1_n])
+    mo_a = haar_orthogonal(norb, rng)[:, :nocc_up]
+    mo_b = haar_orthogonal(norb, rng)[:, :nocc_dn]
+    ham_data = {"h0": 0.0, "h1": h1, "chol": chol, "ene0": 0.0}
+    wave_data = {"mo_coeff": [mo_a, mo_b],
+                 "rdm1": np.array([mo_a @ mo_a.T, mo_b @ mo_b.T])}
+    return ham_data, wave_data, (nocc_up, nocc_dn)
+
+
 def random_walkers(walker_type, norb, nelec, n_walkers, rng, near=None, noise=0.1):
     """Complex walkers: ``near`` (trial coefficients) plus complex Gaussian noise, or
     purely random ones.  Used for parity tests away from the trial state."""

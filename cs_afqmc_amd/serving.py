@@ -62,21 +62,32 @@ class Result:
         return self.done - self.arrival
 
 
+def _spin_chol(ham_data):
+    return len(getattr(ham_data["chol"], "shape", np.shape(ham_data["chol"]))) == 3
+
+
+def _nchol(ham_data):
+    shape = getattr(ham_data["chol"], "shape", np.shape(ham_data["chol"]))
+    return int(shape[1] if len(shape) == 3 else shape[0])
+
+
 def estimate_bytes(norb, nelec, nchol, n_walkers, n_prop_steps, precision,
-                   walker_type="rhf"):
+                   walker_type="rhf", spin_chol=False):
     """Upper estimate of one fragment's device bytes before it is built: context
     intermediates and workspace (as afqmc_ctx_create sizes them), walker state, fields
     block, and the rotated Cholesky vectors ham_data keeps on the device.  nelec:
     (n_up, n_dn) or one count for closed shells.  Once built, the server replaces this
-    with the fragment's measured bytes (Fragment.device_bytes)."""
+    with the fragment's measured bytes (Fragment.device_bytes).  spin_chol: one set of
+    Cholesky vectors per spin (chol (2, G, N*N)), which doubles the vectors and the VHS."""
     if np.ndim(nelec) == 0:
         nelec = (int(nelec), int(nelec))
     N, G, W = norb, nchol, n_walkers
     ns = [nelec[0]] if walker_type == "rhf" else [nelec[0], nelec[1]]
     mixed = precision != "fp64"
     gp, np32 = -(-G // 32) * 32, -(-N // 32) * 32
-    b = 8 * N * N * G + 16 * G              # chol_t, mf_shifts
-    b += 8 * N * N * G                      # the caller's chol upload, freed at creation
+    cs = 2 if spin_chol else 1
+    b = cs * 8 * N * N * G + 16 * G         # chol_t, mf_shifts
+    b += cs * 8 * N * N * G                 # the caller's chol upload, freed at creation
     fb_slices = 0
     for n in ns:
         b += 2 * 8 * G * n * N              # rot_chol in the context and in ham_data
@@ -91,10 +102,10 @@ def estimate_bytes(norb, nelec, nchol, n_walkers, n_prop_steps, precision,
             b += 2 * 2 * N * np32           # exp_h1 - I planes
     b += 8 * W * 2 * G * fb_slices + 8 * W * 2 * G  # force-bias slices, shifted fields
     if mixed:
-        b += 2 * 2 * N * N * gp + 2 * 2 * W * 2 * gp  # Cholesky, shifted-field planes
-        b += 4 * W * 2 * N * N + 32         # float32 VHS
+        b += cs * 2 * 2 * N * N * gp + 2 * 2 * W * 2 * gp  # Cholesky, shifted-field planes
+        b += cs * 4 * W * 2 * N * N + 32    # float32 VHS
     else:
-        b += 16 * W * N * N                 # float64 VHS
+        b += cs * 16 * W * N * N            # float64 VHS
     b += 8 * n_prop_steps * W * G          # fields block
     b += 64 * W + (1 << 20)                 # scalars, energies, SR, exchange partials, slack
     return int(b)
@@ -129,7 +140,7 @@ class FragmentServer:
         if self.policy == "fifo":
             return (req.arrival, req.rid)
         norb = int(np.asarray(req.ham_data["h1"]).shape[-1])
-        nchol = int(np.asarray(req.ham_data["chol"]).shape[0])
+        nchol = _nchol(req.ham_data)
         work = req.n_walkers * req.n_steps * norb * norb * (nchol + 8 * max(req.nelec))
         return (work, req.arrival, req.rid)
 
@@ -205,10 +216,11 @@ class FragmentServer:
                 # admit: reserve the budget and build on the prep thread, oldest first
                 while queue and len(active) + len(building) < self.max_active:
                     req = queue[0][1]
-                    nchol = int(np.asarray(req.ham_data["chol"]).shape[0])
+                    nchol = _nchol(req.ham_data)
                     norb = int(np.asarray(req.ham_data["h1"]).shape[-1])
                     need = estimate_bytes(norb, req.nelec, nchol, req.n_walkers,
-                                          self.n_prop_steps, self.precision, req.walker_type)
+                                          self.n_prop_steps, self.precision, req.walker_type,
+                                          spin_chol=_spin_chol(req.ham_data))
                     if used + need > self.hbm_budget and graveyard:
                         tick.synchronize()
                         used -= sum(nb for _, nb in graveyard)
@@ -260,7 +272,8 @@ class FragmentServer:
                     continue
                 if self.profile and prof_ticks < self.profile_ticks:
                     members = [(f.walker_type, f.norb, f.nelec, f.dev.nchol, f.n_walkers,
-                                f.dev.vhs_entries(), f.prop.n_exp_terms) for f in bt.fragments]
+                                f.dev.vhs_entries(), f.prop.n_exp_terms, f.dev.spin_chol)
+                               for f in bt.fragments]
                     fl, by = work.phase_work(self.precision, members)
                     for ph in fl:  # the first step computes the Green's function, later ones reuse it
                         k = 1 if ph == "green" else self.n_prop_steps

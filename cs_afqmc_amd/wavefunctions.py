@@ -37,16 +37,22 @@ def _device_for(trial, ham_data, wave_data, walker_type, n_walkers):
     for holder in holders:
         for key in ("_afqmc_device", "_afqmc_measure_device"):
             dev = holder.get(key)
-            if dev is not None and dev.walker_type == walker_type and dev.n_walkers >= n_walkers:
+            # overlaps (ham_data None) do not read the Cholesky vectors of either kind
+            if (dev is not None and dev.walker_type == walker_type
+                    and (ham_data is None
+                         or dev.spin_chol == getattr(trial, "spin_chol", False))
+                    and dev.n_walkers >= n_walkers):
                 return dev
     norb = trial.norb
     eye = np.eye(norb) if walker_type == "rhf" else np.array([np.eye(norb)] * 2)
     if ham_data is not None:
         chol = device_chol(ham_data, norb)
+        spin = getattr(trial, "spin_chol", False)
         dev = DeviceStep(
             walker_type, norb, trial.nelec, n_walkers, 0.01, 2, _as_np(ham_data["h0"]), 0.0,
             chol, wave_data["mo_coeff"], ham_data["rot_h1"], ham_data["rot_chol"],
-            np.zeros(chol.shape[0], dtype=np.complex128), eye)
+            np.zeros(chol.shape[0] // (2 if spin else 1), dtype=np.complex128), eye,
+            spin_chol=spin)
         drop_cached(ham_data)  # the context holds its own copy of the vectors
         ham_data["_afqmc_measure_device"] = dev
         return dev

@@ -46,8 +46,13 @@ enum {
 };
 
 /* walker_type: propagator_restricted + rhf trial, or propagator_unrestricted + uhf
- * trial (mpi_jax.py:314-352 picks the pair from options["walker_type"/"trial"]). */
-enum { AFQMC_RHF = 1, AFQMC_UHF = 2 };
+ * trial (mpi_jax.py:314-352 picks the pair from options["walker_type"/"trial"]), or the
+ * unrestricted propagator and trial of prop_unrestricted/ with separate alpha and beta
+ * Cholesky vectors (prop_unrestricted/propagation.py:361-400, :461-510,
+ * wavefunctions_unrestricted.py:453-475): AFQMC_UHF_SPIN_CHOL, with ham->chol holding
+ * (2, G, N*N), alpha then beta, and rot_chol / exp_h1 of each spin formed from that
+ * spin's vectors.  Each spin's VHS is then built from its own vectors. */
+enum { AFQMC_RHF = 1, AFQMC_UHF = 2, AFQMC_UHF_SPIN_CHOL = 3 };
 
 /* Precision of the Cholesky contractions (force bias, VHS, energy) and of the
  * Taylor/one-body products.  Overlaps, determinants, QR, weights and energy
@@ -74,7 +79,8 @@ typedef struct afqmc_dims {
 typedef struct afqmc_ham_host {
   double h0;               /* ham_data["h0"]                                        */
   double h0_prop;          /* Re ham_data["h0_prop"]                                */
-  const double* chol;      /* ham_data["chol"]      (G, N*N)                         */
+  const double* chol;      /* ham_data["chol"]      (G, N*N); AFQMC_UHF_SPIN_CHOL:
+                              (2, G, N*N), alpha then beta                          */
   const double* mo_coeff;  /* wave_data["mo_coeff"] rhf (N,n); uhf up (N,n_up)
                               followed by dn (N,n_dn)                               */
   const double* rot_h1;    /* ham_data["rot_h1"]    rhf (n,N); uhf up then dn       */

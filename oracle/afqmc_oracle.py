@@ -262,6 +262,47 @@ def build_propagation_intermediates(walker_type, ham_data, rdm1, norb, nelec, dt
     return ham_data
 
 
+def build_measurement_intermediates_spin(ham_data, wave_data, norb):
+    """prop_unrestricted/wavefunctions_unrestricted.py:453-475 (uhf with spin-resolved
+    Cholesky vectors, chol (2, G, N*N)): symmetrised h1 per spin,
+    rot_h1[s] = C_s^H h1[s], rot_chol[s] = C_s^H L^s_g."""
+    ham_data = dict(ham_data)
+    h1 = np.array(ham_data["h1"], dtype=np.float64)
+    h1 = np.array([(h1[0] + h1[0].T) / 2.0, (h1[1] + h1[1].T) / 2.0])
+    ham_data["h1"] = h1
+    chol = np.asarray(ham_data["chol"])
+    mo = wave_data["mo_coeff"]
+    ham_data["rot_h1"] = [mo[0].T.conj() @ h1[0], mo[1].T.conj() @ h1[1]]
+    ham_data["rot_chol"] = [
+        np.einsum("pi,gij->gpj", mo[s].T.conj(), chol[s].reshape(-1, norb, norb),
+                  optimize=True)
+        for s in range(2)
+    ]
+    return ham_data
+
+
+def build_propagation_intermediates_spin(ham_data, rdm1, norb, dt):
+    """prop_unrestricted/propagation.py:461-510: mf_shifts summed over both spins' vectors,
+    v0 / v1 / exp_h1 per spin from that spin's vectors."""
+    ham_data = dict(ham_data)
+    chol = np.asarray(ham_data["chol"])
+    mf_a = 1.0j * np.array([np.sum(x.reshape(norb, norb) * rdm1[0]) for x in chol[0]])
+    mf_b = 1.0j * np.array([np.sum(x.reshape(norb, norb) * rdm1[1]) for x in chol[1]])
+    ham_data["mf_shifts"] = mf_a + mf_b
+    ham_data["h0_prop"] = -ham_data["h0"] - np.sum(ham_data["mf_shifts"] ** 2) / 2.0
+    v0, v1 = [], []
+    for s in range(2):
+        c3 = chol[s].reshape(-1, norb, norb)
+        v0.append(0.5 * np.einsum("gik,gjk->ij", c3, c3, optimize=True))
+        v1.append(np.real(1.0j * np.einsum("g,gik->ik", ham_data["mf_shifts"], c3,
+                                           optimize=True)))
+    h1_mod = np.asarray(ham_data["h1"]) - np.array([v0[0] + v1[0], v0[1] + v1[1]])
+    ham_data["exp_h1"] = np.array(
+        [scipy.linalg.expm(-dt * h1_mod[0] / 2.0), scipy.linalg.expm(-dt * h1_mod[1] / 2.0)]
+    )
+    return ham_data
+
+
 def apply_trotprop_det(exp_h1, vhs, walker, n_exp_terms):
     """propagation.py:67-92 (propagator._apply_trotprop_det):
     E (w + sum_{n=1}^{K} V^n w / n!) with E applied before and after."""
@@ -287,7 +328,19 @@ def build_vhs(chol, fields, dt, norb):
 
 
 def apply_trotprop(walker_type, ham_data, walkers, fields, dt, n_exp_terms, norb):
-    """restricted: propagation.py:255-290; unrestricted: propagation.py:359-400."""
+    """restricted: propagation.py:255-290; unrestricted: propagation.py:359-400;
+    "uhf_spin" (spin-resolved vectors chol (2, G, N*N)):
+    prop_unrestricted/propagation.py:361-411, one VHS per spin."""
+    if walker_type == "uhf_spin":
+        chol = np.asarray(ham_data["chol"])
+        e = ham_data["exp_h1"]
+        out = []
+        for s in range(2):
+            vhs = build_vhs(chol[s].reshape(-1, norb * norb), fields, dt, norb)
+            out.append(np.array(
+                [apply_trotprop_det(e[s], vhs[i], walkers[s][i], n_exp_terms)
+                 for i in range(walkers[s].shape[0])]))
+        return out
     chol = np.asarray(ham_data["chol"]).reshape(-1, norb * norb)
     vhs = build_vhs(chol, fields, dt, norb)
     if walker_type == "rhf":
@@ -312,7 +365,9 @@ def apply_trotprop(walker_type, ham_data, walkers, fields, dt, n_exp_terms, norb
 
 def propagate(walker_type, trial, ham_data, prop_data, fields, wave_data, dt,
               n_exp_terms, norb, n_walkers=None, return_force_bias=False):
-    """propagation.py:100-165 (propagator.propagate), phaseless step."""
+    """propagation.py:100-165 (propagator.propagate), phaseless step; walker_type
+    "uhf_spin" is the same step of prop_unrestricted/propagation.py:101-165 over
+    spin-resolved Cholesky vectors (trial "uhf" with per-spin rot_chol)."""
     prop_data = dict(prop_data)
     if n_walkers is None:
         n_walkers = fields.shape[0]

@@ -1,27 +1,30 @@
-// Trial measurements per walker and spin: O = C^T w (n x n), det(O) by Gauss-Jordan
-// elimination with partial pivoting (pivots as in LAPACK getrf: first max of
-// |re| + |im|), X = O^-1, and the half-rotated Green's function G = (w X)^T (n x N).
+// Trial measurements per walker and spin: O = C^T w (n x n), O^-1 and det(O) by a blocked
+// in-place Gauss-Jordan inversion with partial pivoting (pivot: the largest |re| + |im|
+// over the remaining rows, LAPACK's izamax measure), and the half-rotated Green's
+// function G = (w O^-1)^T (n x N).
 //
 // Follows rhf._calc_overlap_restricted / _calc_green (wavefunctions.py:478-501) and
 // uhf._calc_overlap / _calc_green (wavefunctions.py:658-692).  The trial C is real.
 //
-// One workgroup per (walker, spin).  For n <= 24 it has two waves: they split the k
-// chunks of the overlap product (partials summed in a fixed order through LDS) and the
-// column tiles of the Green's function, and wave 0 runs the register Gauss-Jordan while
-// both waves' first Green operands are in flight; with one walker per SIMD (W = 4 x CUs)
-// this halves the chains of dependent global loads each walker waits on.  Wider systems
-// use one wave and the LDS elimination.  Both products run on
-// v_mfma_f64_16x16x4f64 in real block form:
+// One workgroup per (walker, spin): two waves for n <= 24, four beyond.  The overlap and
+// Green products run on v_mfma_f64_16x16x4f64 in real block form:
 //   [Or | Oi]  = C^T [wr | wi]                         (n x 2n,  K = N)
 //   [Gr ; Gi]  = [[Xr^T, -Xi^T], [Xi^T, Xr^T]] [wr^T ; wi^T]   (2n x N, K = 2n)
 // The second product's rows are exactly the rows of the split Green layout
 // green[w][c][i][p] (c = re / im), so results go straight to global memory.
-// [O | I] lives in LDS (n x 2n complex).
+//
+// Inversion: O is overwritten in LDS (n x n complex) by a blocked in-place Gauss-Jordan
+// elimination with implicit pivoting (rows stay in place), in panels of GJ_PB = 4 pivot
+// columns.  The wave that owns a panel's 16-column block factors it with a lane per row
+// (pivot search by a wave reduction, the panel's other columns updated in registers); the
+// panel columns then hold the panel's transform, and every other column is updated by a
+// complex rank-4 product on MFMA (four real MFMAs per 16 x 16 tile), each wave over its
+// own column block.  With r_k the pivot row of column k, O^-1[i][j] = S[r_i][r^-1(j)] for
+// the stored S; the Green product reads X through these two maps.
 #include <algorithm>
 
 #include "common.h"
 
-extern "C" __device__ unsigned long long __ockl_wfred_max_u64(unsigned long long);
 
 namespace {
 
@@ -34,75 +37,197 @@ __device__ inline double readlane_f64(double x, int l) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Gauss-Jordan on [O | I] (n x 2n, n <= 8 T2 <= 24) held in registers: lane c keeps
-// column c.  Pivot k: lane k finds the first maximum of |re| + |im| in its column
-// (rows >= k), the row exchange and the pivot-row scaling are per-lane register
-// operations, and each other row r is updated with the multiplier M[r][k] read from
-// lane k (v_readlane).  No LDS traffic and no barriers inside the elimination.
-// Returns det(O); on exit M (LDS) holds [* | O^-1].
-template <int T2>
-__device__ __forceinline__ cplx gauss_jordan_registers(cplx* M, int n, int lane) {
-  constexpr int NR = 8 * T2;
-  const int n2 = 2 * n;
-  double vr[NR], vi[NR];
+__device__ inline cplx readlane_c(cplx x, int l) {
+  return cplx{readlane_f64(x.re, l), readlane_f64(x.im, l)};
+}
+
+constexpr int GJ_PB = 4;  // panel width: the K of one f64 MFMA
+
+// LDS of the inversion: B (n x ld complex), then the state below (256 ints)
+__host__ __device__ constexpr int gj_ld(int n) { return n | 1; }
+__host__ __device__ constexpr size_t gj_bytes(int n) {
+  return (size_t)n * gj_ld(n) * sizeof(cplx) + 256 * sizeof(int);
+}
+struct GjState {
+  int rho[64];     // rho[k]: the pivot row of column k
+  int rhoinv[64];  // its inverse (filled at the end)
+  int vrow[64];    // row order of the equivalent row-exchanging elimination (for the sign)
+  unsigned long long used;      // rows pivoted so far
+  unsigned long long pmask[2];  // pivot rows of the current panel (double-buffered)
+  unsigned long long pad;
+  cplx det;
+};
+static_assert(sizeof(GjState) <= 256 * sizeof(int), "GjState exceeds its LDS slot");
+
+// Barrier over the workgroup's LDS traffic only: waits for this wave's LDS operations,
+// not for its global loads in flight (the Green product's first operands are loaded
+// before the inversion and stay outstanding across it)
+__device__ inline void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// 1 / x for x > 0 of moderate range: hardware estimate plus two Newton steps
+__device__ inline double rcp_f64(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+
+// Panel [k0, k0 + pb) on one wave, lane = row r < n (n <= 64), rows kept in place
+// (implicit pivoting).  Pivot of column k: the largest |re| + |im| over the rows not yet
+// pivoted (compared to 2^-17 relative, the first row among near-equal ones).  The pivot row r_k is scaled by 1 / piv (its column-k entry becomes
+// 1 / piv) and every other row r loses f_r = A[r][k] times it (its column-k entry
+// becomes -f_r / piv), over the panel's columns only.  Updates the state (pivot rows,
+// used rows, row order, det).
+__device__ __forceinline__ void gj_panel(cplx* B, int ld, int n, int k0, int pb, int panel,
+                                         GjState* st, int lane) {
+  const bool live = lane < n;
+  unsigned long long used = st->used;
+  int vr = st->vrow[lane];
+  cplx det = st->det;
+  unsigned long long pm = 0ull;
+  int rk[GJ_PB];
+  cplx p[GJ_PB];
+  // loads clamped into the matrix and masked after, so they issue back to back
 #pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    cplx x = cplx{0.0, 0.0};
-    if (r < n && lane < n2) x = M[r * n2 + lane];
-    vr[r] = x.re;
-    vi[r] = x.im;
+  for (int j = 0; j < GJ_PB; ++j) {
+    const cplx v = B[(live ? lane : 0) * ld + k0 + (j < pb ? j : 0)];
+    p[j] = (live && j < pb) ? v : cplx{0.0, 0.0};
+    rk[j] = 0;
   }
-  cplx det = cplx{1.0, 0.0};
 #pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    if (k < n) {
-      int best = k;
-      double bv = -1.0;
+  for (int j = 0; j < GJ_PB; ++j) {
+    if (j >= pb) continue;
+    const int k = k0 + j;
+    // one 32-bit key per candidate row: |re| + |im| as float32 with its low 6 bits
+    // replaced by 63 - row, so an integer wave maximum picks the largest magnitude and,
+    // among magnitudes equal to 2^-17 relative, the first row
+    const bool cand = live && !((used >> lane) & 1ull);
+    const float amag = (float)(fabs(p[j].re) + fabs(p[j].im));
+    unsigned key = 0u;
+    if (cand) key = (amag == amag ? (__float_as_uint(amag) & ~63u) : 0u) | (unsigned)(63 - lane);
+    const unsigned kmax = __ockl_wfred_max_u32(key);
+    // (a column of NaN or zeros only: the first remaining row)
+    const int pr = kmax ? 63 - (int)(kmax & 63u) : __builtin_ctzll(__ballot(cand));
+    used |= 1ull << pr;
+    pm |= 1ull << pr;
+    rk[j] = pr;
+    // the row-exchanging elimination would swap positions k and q (row pr's position)
+    const int q = __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(__ballot(vr == pr)));
+    const int vk = __builtin_amdgcn_readlane(vr, k);
+    vr = lane == k ? pr : (lane == q ? vk : vr);
+    const cplx piv = readlane_c(p[j], pr);
+    det = cmul(det, piv);
+    if (q != k) det = cplx{-det.re, -det.im};
+    const double d2 = rcp_f64(piv.re * piv.re + piv.im * piv.im);
+    const cplx ipiv = cplx{piv.re * d2, -piv.im * d2};
+    cplx qv[GJ_PB];
 #pragma unroll
-      for (int r = k; r < NR; ++r) {
-        const double a = fabs(vr[r]) + fabs(vi[r]);
-        if (r < n && a > bv) {
-          bv = a;
-          best = r;
-        }
-      }
-      const int pr = __builtin_amdgcn_readlane(best, k);
-      if (pr != k) {
+    for (int jj = 0; jj < GJ_PB; ++jj) qv[jj] = jj == j ? ipiv : cmul(readlane_c(p[jj], pr), ipiv);
+    const cplx f = p[j];
+    const bool pivrow = lane == pr;
 #pragma unroll
-        for (int r = k + 1; r < NR; ++r) {
-          if (r == pr) {
-            const double tr = vr[r], ti = vi[r];
-            vr[r] = vr[k];
-            vi[r] = vi[k];
-            vr[k] = tr;
-            vi[k] = ti;
-          }
-        }
-      }
-      const cplx piv = cplx{readlane_f64(vr[k], k), readlane_f64(vi[k], k)};
-      det = cmul(det, piv);
-      if (pr != k) det = cplx{-det.re, -det.im};
-      const double d2 = 1.0 / (piv.re * piv.re + piv.im * piv.im);
-      const cplx inv = cplx{piv.re * d2, -piv.im * d2};
-      const double mr = vr[k] * inv.re - vi[k] * inv.im;
-      const double mi = vr[k] * inv.im + vi[k] * inv.re;
-      vr[k] = mr;
-      vi[k] = mi;
-#pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        if (r != k && r < n) {
-          // x -= f m as four fused multiply-adds
-          const double fr = readlane_f64(vr[r], k), fi = readlane_f64(vi[r], k);
-          vr[r] = fma(-fr, mr, fma(fi, mi, vr[r]));
-          vi[r] = fma(-fr, mi, fma(-fi, mr, vi[r]));
-        }
+    for (int jj = 0; jj < GJ_PB; ++jj) {
+      if (jj == j) {
+        const cplx e = cmul(f, ipiv);
+        p[jj] = pivrow ? ipiv : cplx{-e.re, -e.im};
+      } else {
+        const cplx e = cmul(f, qv[jj]);
+        p[jj] = pivrow ? qv[jj] : cplx{p[jj].re - e.re, p[jj].im - e.im};
       }
     }
   }
 #pragma unroll
-  for (int r = 0; r < NR; ++r)
-    if (r < n && lane < n2) M[r * n2 + lane] = cplx{vr[r], vi[r]};
-  return det;
+  for (int j = 0; j < GJ_PB; ++j)
+    if (live && j < pb) B[lane * ld + k0 + j] = p[j];
+  st->vrow[lane] = vr;
+  if (lane < pb) st->rho[k0 + lane] = lane == 0 ? rk[0] : lane == 1 ? rk[1] : lane == 2 ? rk[2] : rk[3];
+  if (lane == 0) {
+    st->used = used;
+    st->pmask[panel & 1] = pm;
+    st->det = det;
+  }
+}
+
+// In-place inversion of the n x n complex matrix B (row stride ld, LDS; st right after
+// it) by NW >= ceil(n / 16) waves; TT >= ceil(n / 16) row tiles.  Wave w owns column block w (all its row tiles): it
+// factors the panels that fall in its block and applies every panel's transform to its
+// block, so one barrier per panel orders the factorization before the other blocks'
+// updates.  For panel K (pivot rows R_K) and a column x outside it:
+//   x <- T x[R_K] + (x with the rows R_K cleared),  T = the panel's columns,
+// four real MFMAs per 16 x 16 tile.  On return B holds S with
+//   O^-1[i][j] = S[rho[i]][rhoinv[j]],
+// and st->det = det(O).  Ends with a barrier.
+template <int NW, int TT>
+__device__ __forceinline__ void gj_invert(cplx* B, int ld, int n, GjState* st, int lane,
+                                          int wv) {
+  const int tid = threadIdx.x;
+  if (tid < 64) st->vrow[tid] = tid;
+  if (tid == 0) {
+    st->used = 0ull;
+    st->det = cplx{1.0, 0.0};
+  }
+  const int tt = (n + 15) >> 4;
+  const int kk = lane >> 4, c16 = lane & 15;
+  const int cb = 16 * wv + c16;  // this wave's column
+  const bool mine = wv < tt;
+  lds_barrier();
+  for (int k0 = 0, panel = 0; k0 < n; k0 += GJ_PB, ++panel) {
+    const int pb = min(GJ_PB, n - k0);
+    if (wv == (k0 >> 4)) gj_panel(B, ld, n, k0, pb, panel, st, lane);
+    lds_barrier();
+    if (!mine) continue;
+    // loads clamped into the matrix and masked after, so they issue back to back
+    const unsigned long long pm = st->pmask[panel & 1];
+    const bool kok = kk < pb, cok = cb < n;
+    const int kc = k0 + (kok ? kk : 0), cbc = cok ? cb : 0;
+    const cplx xv = B[st->rho[kc] * ld + cbc];
+    const double br = (kok && cok) ? xv.re : 0.0, bi = (kok && cok) ? xv.im : 0.0;
+    double4_t dr[TT], di[TT];
+#pragma unroll
+    for (int tr = 0; tr < TT; ++tr) {
+      dr[tr] = di[tr] = double4_t{0.0, 0.0, 0.0, 0.0};
+      if (tr >= tt) continue;
+      const int ra = 16 * tr + c16;
+      const cplx tv = B[(ra < n ? ra : 0) * ld + kc];
+      const double ar = (kok && ra < n) ? tv.re : 0.0, ai = (kok && ra < n) ? tv.im : 0.0;
+      cplx dv[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = 16 * tr + kk + 4 * q;
+        dv[q] = B[(row < n ? row : 0) * ld + cbc];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = 16 * tr + kk + 4 * q;
+        const bool ok = row < n && cok && !((pm >> row) & 1ull);
+        dr[tr][q] = ok ? dv[q].re : 0.0;
+        di[tr][q] = ok ? dv[q].im : 0.0;
+      }
+      dr[tr] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, dr[tr], 0, 0, 0);
+      dr[tr] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, dr[tr], 0, 0, 0);
+      di[tr] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, di[tr], 0, 0, 0);
+      di[tr] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, di[tr], 0, 0, 0);
+    }
+    // every read of this block precedes the first write (the wave's LDS operations stay
+    // in order); the panel's own columns keep T
+    if (cok && (cb < k0 || cb >= k0 + pb)) {
+#pragma unroll
+      for (int tr = 0; tr < TT; ++tr) {
+        if (tr >= tt) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int row = 16 * tr + kk + 4 * q;
+          if (row < n) B[row * ld + cb] = cplx{dr[tr][q], di[tr][q]};
+        }
+      }
+    }
+  }
+  lds_barrier();
+  if (tid < n) st->rhoinv[st->rho[tid]] = tid;
+  lds_barrier();
 }
 
 // waves per walker (two for n <= 24, four beyond) and the LDS the two-wave kernel's
@@ -117,15 +242,15 @@ constexpr size_t meas_red_bytes(int t2) {
 // walkers up to this size are staged in LDS by the two-wave kernel (four blocks per CU)
 constexpr size_t MEAS_STAGE_MAX = 40 * 1024;
 
-// [Or | Oi] = C^T [wr | wi] for the row tiles [tb, tb + TI) of O, into M (rows i, columns
-// c < 2n, K = N).  All TI x T2 accumulators are live, so each B value feeds TI MFMAs; the
+// [Or | Oi] = C^T [wr | wi] for the row tiles [tb, tb + TI) of O, into M (complex n x n,
+// row stride ldm; K = N).  All TI x T2 accumulators are live, so each B value feeds TI MFMAs; the
 // operands of k-group chunk j + NW are loaded while chunk j runs.  With NW = 2 the waves
 // take alternate chunks, wave 1's partial sums reach wave 0 through red (summed in the
 // order wave 0 + wave 1) and wave 0 writes M; `aliased` adds the barrier that keeps red's
 // writes after both waves' last read of a staged walker sharing that LDS.
 template <int T2, int TI, int NW>
 __device__ __forceinline__ void overlap_tiles(int N, int n, int tb, const double* __restrict__ trial,
-                                              const double* __restrict__ wsrc, cplx* M,
+                                              const double* __restrict__ wsrc, cplx* M, int ldm,
                                               double* red, bool aliased, int lane, int wv) {
   const int n2 = 2 * n;
   const int s = lane >> 4, col = lane & 15;
@@ -205,7 +330,7 @@ __device__ __forceinline__ void overlap_tiles(int N, int n, int tb, const double
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = (tb + ti) * 16 + s + 4 * r;
-        if (row < n) reinterpret_cast<double*>(&M[row * n2 + cj])[ci] = acc[ti][tc][r];
+        if (row < n) reinterpret_cast<double*>(&M[row * ldm + cj])[ci] = acc[ti][tc][r];
       }
   }
 }
@@ -221,10 +346,10 @@ __device__ inline void finish_walker(cplx det, cplx* dets, int w, int spin,
 }
 
 // n <= 24 (T2 <= 3): two waves per walker.  A walker of at most MEAS_STAGE_MAX bytes is
-// staged in LDS first (one round of loads), and the region is then reused for [O | I]
-// and the overlap partials.  Wave 0 runs the register Gauss-Jordan while each wave's
-// first Green tile of the walker operand loads; the Green product's A operand (from X) is
-// then read into registers once and the walker operand runs two tiles ahead.
+// staged in LDS first (one round of loads), and the region is then reused for O, the
+// inversion's maps and the overlap partials.  Each wave's first Green tile of the walker
+// operand loads while the waves invert O; the Green product's A operand (from X) is then
+// read into registers once and the walker operand runs two tiles ahead.
 template <int T2, bool STAGE>
 __device__ __forceinline__ void measure_narrow_body(
     int N, int n, const cplx* __restrict__ walkers, const double* __restrict__ trial,
@@ -233,9 +358,10 @@ __device__ __forceinline__ void measure_narrow_body(
   constexpr int NW = 2, TI = meas_ti(T2);
   constexpr int NKG = 4 * T2;  // k groups of 2n <= 16 T2
   extern __shared__ double smem[];
-  const int n2 = 2 * n;
-  cplx* M = reinterpret_cast<cplx*>(smem);              // n x 2n augmented matrix
-  double* red = reinterpret_cast<double*>(M + n * n2);  // wave 1's overlap partials
+  const int n2 = 2 * n, ld = gj_ld(n);
+  cplx* M = reinterpret_cast<cplx*>(smem);                    // O, then its inverse: n x ld
+  GjState* st = reinterpret_cast<GjState*>(M + n * ld);       // the inversion's state
+  double* red = reinterpret_cast<double*>(reinterpret_cast<int*>(st) + 256);  // overlap partials
   // wave index as a scalar: the waves' paths below are real branches, so registers that
   // live on one wave's path only can share storage with the other's
   const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -247,20 +373,15 @@ __device__ __forceinline__ void measure_narrow_body(
     cplx* ph = reinterpret_cast<cplx*>(smem);
     for (int e = threadIdx.x; e < N * n; e += 64 * NW) ph[e] = phi[e];
     __syncthreads();
-    overlap_tiles<T2, TI, NW>(N, n, 0, trial, reinterpret_cast<const double*>(ph), M, red,
-                              true, lane, wv);
+    overlap_tiles<T2, TI, NW>(N, n, 0, trial, reinterpret_cast<const double*>(ph), M, ld,
+                              red, true, lane, wv);
   } else {
-    overlap_tiles<T2, TI, NW>(N, n, 0, trial, phid, M, red, false, lane, wv);
+    overlap_tiles<T2, TI, NW>(N, n, 0, trial, phid, M, ld, red, false, lane, wv);
   }
-  for (int e = threadIdx.x; e < n * n; e += 64 * NW) {
-    const int i = e / n, j = e - (e / n) * n;
-    M[i * n2 + n + j] = cplx{i == j ? 1.0 : 0.0, 0.0};
-  }
-  __syncthreads();
 
-  // ---- Gauss-Jordan with partial pivoting on wave 0: [O | I] -> [* | X], det from
-  // pivots; then [Gr ; Gi] = [[Xr^T, -Xi^T], [Xi^T, Xr^T]] [wr^T ; wi^T], K = 2n,
-  // columns p in tiles of 16, tiles split between the waves.
+  // ---- O^-1 and det(O) (blocked in-place Gauss-Jordan, both waves); then
+  // [Gr ; Gi] = [[Xr^T, -Xi^T], [Xi^T, Xr^T]] [wr^T ; wi^T], K = 2n, columns p in tiles
+  // of 16, tiles split between the waves.
   const bool want_green = green || ghi;
   const int tp_n = (N + 15) >> 4;
   // B[k][p] = re / im of w[p][k mod n]: 32-bit offsets within a walker row (tile
@@ -280,17 +401,12 @@ __device__ __forceinline__ void measure_narrow_body(
       b[kg] = (boff[kg] >= 0 && p < N) ? v : 0.0;
     }
   };
-  // wave 1 loads its first tile while wave 0 eliminates
+  // each wave's first tile loads while the waves invert O
   double b[NKG];
-  cplx det = cplx{1.0, 0.0};
-  if (wv == 0)
-    det = gauss_jordan_registers<T2>(M, n, lane);
-  else if (want_green && wv < tp_n)
-    load_b(b, wv);
-  __syncthreads();
-  if (threadIdx.x == 0) finish_walker(det, dets, w, spin, tail);
+  if (want_green && wv < tp_n) load_b(b, wv);
+  gj_invert<NW, 2>(M, ld, n, st, lane, wv);
+  if (threadIdx.x == 0) finish_walker(st->det, dets, w, spin, tail);
   if (!want_green) return;
-  if (wv == 0) load_b(b, 0);
 
   // A operand of every tile, from X in LDS, read once
   double a[NKG][T2];
@@ -303,7 +419,8 @@ __device__ __forceinline__ void measure_narrow_body(
       double v = 0.0;
       if (k < n2 && r < n2) {
         const int kj = k < n ? k : k - n;
-        const double* x = reinterpret_cast<const double*>(&M[kj * n2 + n + (r < n ? r : r - n)]);
+        const double* x =
+            reinterpret_cast<const double*>(&M[st->rho[kj] * ld + st->rhoinv[r < n ? r : r - n]]);
         // X[kj][ri]: (r < n, k < n): Xr; (r < n, k >= n): -Xi; (r >= n, k < n): Xi; else Xr
         v = (r < n) == (k < n) ? x[0] : ((r < n) ? -x[1] : x[1]);
       }
@@ -344,10 +461,8 @@ __device__ __forceinline__ void measure_narrow_body(
 }
 
 // n > 24: four waves per walker.  They take alternate row tiles of O (each over the
-// whole K = N), share the Gauss-Jordan elimination on [O | I] in LDS (every wave finds
-// the pivot from the same column with its own wave reduction, so no broadcast is needed;
-// the row updates are split by trailing column and row group), and take alternate
-// column tiles of the Green's function.
+// whole K = N), share the blocked inversion of O, and take alternate column tiles of the
+// Green's function.
 template <int T2, int TI, bool STAGE>
 __device__ __forceinline__ void measure_wide_body(
     int N, int n, const cplx* __restrict__ walkers, const double* __restrict__ trial,
@@ -355,9 +470,9 @@ __device__ __forceinline__ void measure_wide_body(
     uint16_t* __restrict__ glo, int ld16, const int w, const WeightsArgs& tail) {
   constexpr int NW = 4;
   extern __shared__ double smem[];
-  const int n2 = 2 * n;
-  cplx* M = reinterpret_cast<cplx*>(smem);  // n x 2n augmented matrix
-  cplx* fcol = M + n * n2;                  // pivot column copy (n)
+  const int n2 = 2 * n, ld = gj_ld(n);
+  cplx* M = reinterpret_cast<cplx*>(smem);               // O, then its inverse: n x ld
+  GjState* st = reinterpret_cast<GjState*>(M + n * ld);  // the inversion's state
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int s = lane >> 4, col = lane & 15;
@@ -366,7 +481,7 @@ __device__ __forceinline__ void measure_wide_body(
   // when it fits, else read from global memory (L2)
   const double* phid = reinterpret_cast<const double*>(phi);
   if (STAGE) {
-    cplx* ph = fcol + n;
+    cplx* ph = reinterpret_cast<cplx*>(reinterpret_cast<int*>(st) + 256);
     for (int e = tid; e < N * n; e += 64 * NW) ph[e] = phi[e];
     phid = reinterpret_cast<const double*>(ph);
   }
@@ -374,84 +489,11 @@ __device__ __forceinline__ void measure_wide_body(
   // row tiles of O, one per wave in turn
   const int ti_total = (n + 15) >> 4;
   for (int tb = wv; tb < ti_total; tb += NW)
-    overlap_tiles<T2, 1, 1>(N, n, tb, trial, phid, M, nullptr, false, lane, 0);
-  for (int e = tid; e < n * n; e += 64 * NW) {
-    const int i = e / n, j = e - (e / n) * n;
-    M[i * n2 + n + j] = cplx{i == j ? 1.0 : 0.0, 0.0};
-  }
-  __syncthreads();
+    overlap_tiles<T2, 1, 1>(N, n, tb, trial, phid, M, ld, nullptr, false, lane, 0);
 
-  // ---- Gauss-Jordan with partial pivoting: [O | I] -> [* | X], det from pivots.
-  cplx det = cplx{1.0, 0.0};
-  for (int k = 0; k < n; ++k) {
-    int pr;
-    {
-      // first maximum of |re| + |im| in column k (rows >= k): one 64-bit key per lane,
-      // float32 magnitude in the high word, 63 - row in the low word
-      const int r = k + lane;
-      unsigned long long key = 0ull;
-      if (r < n) {
-        const float v = (float)cabs1(M[r * n2 + k]);
-        key = (((unsigned long long)__float_as_uint(v)) << 32) | (unsigned)(63 - lane);
-      }
-      key = __ockl_wfred_max_u64(key);  // DPP wave reduction, result in every lane
-      pr = k + 63 - (int)__builtin_amdgcn_readfirstlane((unsigned)(key & 0xFFFFFFFFull));
-    }
-    const cplx pivot = M[pr * n2 + k];
-    const double d2 = 1.0 / (pivot.re * pivot.re + pivot.im * pivot.im);
-    const cplx inv = cplx{pivot.re * d2, -pivot.im * d2};
-    if (tid == 0) {
-      det = cmul(det, pivot);
-      if (pr != k) det = cplx{-det.re, -det.im};
-    }
-    // (column k is not written again, so the exchange below needs no barrier first)
-    // pivot column as seen after the row exchange; exchange + scale the pivot row
-    for (int r = tid; r < n; r += 64 * NW) fcol[r] = M[(r == pr ? k : r) * n2 + k];
-    for (int c = k + 1 + tid; c < n2; c += 64 * NW) {
-      double* mk = reinterpret_cast<double*>(&M[k * n2 + c]);
-      double* mp = reinterpret_cast<double*>(&M[pr * n2 + c]);
-      const double ar = mk[0], ai = mk[1], pr_ = mp[0], pi_ = mp[1];
-      mk[0] = pr_ * inv.re - pi_ * inv.im;
-      mk[1] = pr_ * inv.im + pi_ * inv.re;
-      if (pr != k) {
-        mp[0] = ar;
-        mp[1] = ai;
-      }
-    }
-    __syncthreads();
-    // trailing columns nc = 2n - k - 1: thread -> (column k + 1 + tid % nc, row group
-    // tid / nc), the rows split into as many groups as 256 threads allow
-    const int nc = n2 - k - 1;
-    const int groups = min(16, (64 * NW) / nc);
-    const int rg = tid / nc;
-    const int c = k + 1 + (tid - rg * nc);
-    const int rh = (n + groups - 1) / groups;
-    const int r_lo = rg * rh, r_hi = min(n, r_lo + rh);
-    if (rg < groups) {
-      const cplx m = M[k * n2 + c];
-      for (int r0 = r_lo; r0 < r_hi; r0 += 8) {
-        // eight independent loads, then the updates, then the stores
-        cplx x[8], f[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int r = r0 + u < r_hi ? r0 + u : r_hi - 1;
-          x[u] = M[r * n2 + c];
-          f[u] = fcol[r];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int r = r0 + u;
-          if (r < r_hi && r != k) {
-            cplx y = x[u];
-            y.re -= f[u].re * m.re - f[u].im * m.im;
-            y.im -= f[u].re * m.im + f[u].im * m.re;
-            M[r * n2 + c] = y;
-          }
-        }
-      }
-    }
-    __syncthreads();
-  }
+  // ---- O^-1 and det(O): blocked in-place Gauss-Jordan on all four waves
+  gj_invert<NW, 4>(M, ld, n, st, lane, wv);
+  const cplx det = st->det;
   if (tid == 0) finish_walker(det, dets, w, spin, tail);
   if (!(green || ghi)) return;
 
@@ -479,7 +521,7 @@ __device__ __forceinline__ void measure_wide_body(
             const int r = tr * 16 + col;  // A row
             double v = 0.0;
             if (kok && r < n2) {
-              const cplx x = M[kj * n2 + n + (r < n ? r : r - n)];  // X[kj][ri]
+              const cplx x = M[st->rho[kj] * ld + st->rhoinv[r < n ? r : r - n]];  // X[kj][ri]
               // (r < n, k < n): Xr; (r < n, k >= n): -Xi; (r >= n, k < n): Xi; else Xr
               v = (r < n) == (k < n) ? x.re : ((r < n) ? -x.im : x.im);
             }
@@ -629,13 +671,13 @@ void measure_plan(const afqmc_ctx* c, int s, int* t2, bool* stage, size_t* shm) 
   const int t = (2 * n + 15) / 16;
   *t2 = t > 8 ? 8 : t;
   const size_t walker = (size_t)N * n * sizeof(cplx);
-  if (*t2 <= 3) {  // two-wave kernel: [O | I] and partials alias the staged walker
-    const size_t m_red = (size_t)n * 2 * n * sizeof(cplx) + meas_red_bytes(*t2);
+  if (*t2 <= 3) {  // two-wave kernel: O, state and partials alias the staged walker
+    const size_t m_red = gj_bytes(n) + meas_red_bytes(*t2);
     *stage = walker <= MEAS_STAGE_MAX;
     *shm = *stage ? std::max(walker, m_red) : m_red;
     return;
   }
-  const size_t base = (size_t)(n * 2 * n + n) * sizeof(cplx);
+  const size_t base = gj_bytes(n);
   *stage = base + walker <= 24 * 1024;  // small walkers only: staging costs occupancy
   *shm = *stage ? base + walker : base;
 }

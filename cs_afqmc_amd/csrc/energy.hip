@@ -29,36 +29,69 @@ constexpr int ENER_NT = 256;
 // ---------------------------------------------------------------- fused exchange
 
 // Exchange sums of the (g, w) blocks of an F tile held in LDS (row stride ldf):
-// F row gl*n + i, column wl*2n + c*n + j.  Wave v takes walkers wl = v, v+4, ...; each
-// lane owns entries (i, j) of the n x n block and runs down the tile's g; the wave sum
-// is written to xpart[w].
-template <typename T>
+// F row gl*n + i, column wl*2n + c*n + j.  x_w = sum_g sum_ij f(i,j) f(j,i) is taken over
+// the pairs i <= j (weight 2 off the diagonal): each lane owns pairs and runs down the
+// tile's g.  With at least as many walkers as waves each wave takes walkers wl = v, v+NWV,
+// ...; with fewer, the waves split each walker's g range and their partial sums (parts,
+// one entry per wave in LDS) are added in wave order.  Results go to xpart[w].
+template <typename T, int NT = ENER_NT>
 __device__ inline void exchange_tile(const T* __restrict__ Fs, int ldf, int n, int g_used,
-                                     int w_used, int w0, cplx* __restrict__ xpart) {
+                                     int w_used, int w0, cplx* __restrict__ xpart,
+                                     cplx* parts) {
+  constexpr int NWV = NT / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int nn = n * n;
+  const int npairs = n * (n + 1) / 2;
   const int gstride = n * ldf;
-  for (int wl = wave; wl < w_used; wl += ENER_NT / 64) {
-    const T* base = Fs + wl * 2 * n;
+  const int wpw = w_used >= NWV ? 1 : NWV / w_used;  // waves per walker
+  const int items = w_used * wpw;
+  const int gc = (g_used + wpw - 1) / wpw;
+  for (int it = wave; it < items; it += NWV) {
+    const int wl = it / wpw, part = it - wl * wpw;
+    const int gb = part * gc, ge = min(g_used, gb + gc);
+    const T* base = Fs + wl * 2 * n + gb * gstride;
     double xr = 0.0, xi = 0.0;
-    for (int e = lane; e < nn; e += 64) {
-      const int i = e / n, j = e - i * n;
+    for (int t = lane; t < npairs; t += 64) {
+      // pair t = a (a + 1) / 2 + b, b <= a: (i, j) = (b, a)
+      int a = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+      if (a * (a + 1) / 2 > t) --a;
+      if ((a + 1) * (a + 2) / 2 <= t) ++a;
+      const int i = t - a * (a + 1) / 2, j = a;
+      const double wgt = i == j ? 1.0 : 2.0;
       const T* pa = base + i * ldf + j;  // f(i, j): re pa[0], im pa[n]
       const T* pb = base + j * ldf + i;  // f(j, i)
-      for (int gl = 0; gl < g_used; ++gl) {
+      double sr = 0.0, si = 0.0;
+      for (int gl = gb; gl < ge; ++gl) {
         const double ar = (double)pa[0], ai = (double)pa[n];
         const double br = (double)pb[0], bi = (double)pb[n];
-        xr += ar * br - ai * bi;
-        xi += ar * bi + ai * br;
+        sr += ar * br - ai * bi;
+        si += ar * bi + ai * br;
         pa += gstride;
         pb += gstride;
       }
+      xr += wgt * sr;
+      xi += wgt * si;
     }
     for (int off = 32; off > 0; off >>= 1) {
       xr += __shfl_down(xr, off, 64);
       xi += __shfl_down(xi, off, 64);
     }
-    if (lane == 0) xpart[w0 + wl] = cplx{xr, xi};
+    if (lane == 0) {
+      if (wpw == 1)
+        xpart[w0 + wl] = cplx{xr, xi};
+      else
+        parts[it] = cplx{xr, xi};
+    }
+  }
+  if (wpw > 1) {  // (uniform in the workgroup)
+    __syncthreads();
+    if (threadIdx.x < w_used) {
+      double xr = 0.0, xi = 0.0;
+      for (int q = 0; q < wpw; ++q) {
+        xr += parts[threadIdx.x * wpw + q].re;
+        xi += parts[threadIdx.x * wpw + q].im;
+      }
+      xpart[w0 + threadIdx.x] = cplx{xr, xi};
+    }
   }
 }
 
@@ -71,7 +104,8 @@ constexpr int EF_BM = 64, EF_BN = 128, EF_BK = 16, EF_PAD = 16;
 constexpr int EF_LDF = 129;  // F tile row stride (odd: transposed reads spread over banks)
 constexpr int EF_OPS_BYTES = 2 * EF_BK * (EF_BM + EF_PAD) * 8 + 2 * EF_BK * (EF_BN + EF_PAD) * 8;
 constexpr int EF_F_BYTES = EF_BM * EF_LDF * 8;
-constexpr int EF_LDS = EF_OPS_BYTES > EF_F_BYTES ? EF_OPS_BYTES : EF_F_BYTES;
+constexpr int EF_F_PARTS = EF_F_BYTES + (ENER_NT / 64) * 16;
+constexpr int EF_LDS = EF_OPS_BYTES > EF_F_PARTS ? EF_OPS_BYTES : EF_F_PARTS;
 
 template <int VEC, bool F32>
 __global__ __launch_bounds__(ENER_NT) void exchange_f64_kernel(
@@ -188,25 +222,36 @@ __global__ __launch_bounds__(ENER_NT) void exchange_f64_kernel(
   __syncthreads();
   const int g_used = (G - g0 < gpt) ? G - g0 : gpt;
   const int w_used = (W - w0 < wpt) ? W - w0 : wpt;
-  exchange_tile<T>(Fs, EF_LDF, n, g_used, w_used, w0, xpart + (int64_t)gt * W);
+  exchange_tile<T>(Fs, EF_LDF, n, g_used, w_used, w0, xpart + (int64_t)gt * W,
+                   reinterpret_cast<cplx*>(lds + EF_F_BYTES));
 }
 
-// split-bf16 tile 128 x 128 on v_mfma_f32_16x16x32_bf16 (4 waves 2 x 2, each 64 x 64),
-// BK = 32; operand planes have K = Np32 (zero padded), so there is no k tail.
-constexpr int EB_BM = 128, EB_BN = 128, EB_BK = 32, EB_LDSW = EB_BK + 8;
-constexpr int EB_LDF = 129;
-constexpr int EB_OPS_BYTES = 2 * 2 * EB_BM * EB_LDSW * 2 + 2 * 2 * EB_BN * EB_LDSW * 2;
-constexpr int EB_F_BYTES = EB_BM * EB_LDF * 4;
-constexpr int EB_LDS = EB_OPS_BYTES > EB_F_BYTES ? EB_OPS_BYTES : EB_F_BYTES;
+// split-bf16 tile 128 x 64 WN on v_mfma_f32_16x16x32_bf16 (2 x WN waves, each 64 x 64),
+// BK = 32; operand planes have K = Np32 (zero padded), so there is no k tail.  WN = 2
+// (128 columns, 4 waves) or, where it holds more whole walkers (2n columns each) per
+// tile, WN = 4 (256 columns, 8 waves): the Cholesky tile is then read once per more
+// walkers.
+constexpr int EB_BM = 128, EB_BK = 32, EB_LDSW = EB_BK + 8;
+template <int WN>
+struct EbTile {
+  static constexpr int NT = 128 * WN, BN = 64 * WN, LDF = BN + 1;
+  static constexpr int OPS = 2 * 2 * EB_BM * EB_LDSW * 2 + 2 * 2 * BN * EB_LDSW * 2;
+  static constexpr int F = EB_BM * LDF * 4;
+  static constexpr int F_PARTS = F + (NT / 64) * 16;
+  static constexpr int LDS = OPS > F_PARTS ? OPS : F_PARTS;
+};
 
-__global__ __launch_bounds__(ENER_NT) void exchange_bf16x3_kernel(
+template <int WN>
+__global__ __launch_bounds__(128 * WN) void exchange_bf16x3_kernel(
     int G, int n, int K, int W, int gpt, int wpt, int g_tiles, const uint16_t* __restrict__ Ah,
     const uint16_t* __restrict__ Al, const uint16_t* __restrict__ Bh,
     const uint16_t* __restrict__ Bl, cplx* __restrict__ xpart) {
-  constexpr int BM = EB_BM, BN = EB_BN, LDSW = EB_LDSW;
-  constexpr int TM = BM / 2 / 16, TN = BN / 2 / 16;
-  constexpr int A_LD = BM * 4 / ENER_NT, B_LD = BN * 4 / ENER_NT;
-  __shared__ __attribute__((aligned(16))) unsigned char lds[EB_LDS];
+  typedef EbTile<WN> Tile;
+  constexpr int NT = Tile::NT, BM = EB_BM, BN = Tile::BN, LDSW = EB_LDSW;
+  constexpr int TM = BM / 2 / 16, TN = BN / WN / 16;
+  constexpr int A_LD = BM * 4 / NT, B_LD = BN * 4 / NT;
+  static_assert(A_LD >= 1 && B_LD >= 1, "tile loads");
+  __shared__ __attribute__((aligned(16))) unsigned char lds[Tile::LDS];
   typedef uint16_t PlaneA[2][BM * LDSW];  // [hi/lo]
   typedef uint16_t PlaneB[2][BN * LDSW];
   PlaneA* sA = reinterpret_cast<PlaneA*>(lds);
@@ -218,13 +263,13 @@ __global__ __launch_bounds__(ENER_NT) void exchange_bf16x3_kernel(
   const int M = G * n, NB = 2 * W * n;
   const int m0 = g0 * n, n0 = w0 * 2 * n;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave & 1, wn = wave >> 1;  // wn < WN
 
   u32x4_t ra[2][A_LD], rb[2][B_LD];
   auto load_tile = [&](int k0) {
 #pragma unroll
     for (int j = 0; j < A_LD; ++j) {
-      const int q = t + ENER_NT * j, row = q >> 2, ch = q & 3;
+      const int q = t + NT * j, row = q >> 2, ch = q & 3;
       const int r = m0 + row;
       u32x4_t vh = {0, 0, 0, 0}, vl = {0, 0, 0, 0};
       if (r < M) {
@@ -236,7 +281,7 @@ __global__ __launch_bounds__(ENER_NT) void exchange_bf16x3_kernel(
     }
 #pragma unroll
     for (int j = 0; j < B_LD; ++j) {
-      const int q = t + ENER_NT * j, row = q >> 2, ch = q & 3;
+      const int q = t + NT * j, row = q >> 2, ch = q & 3;
       const int r = n0 + row;
       u32x4_t vh = {0, 0, 0, 0}, vl = {0, 0, 0, 0};
       if (r < NB) {
@@ -250,14 +295,14 @@ __global__ __launch_bounds__(ENER_NT) void exchange_bf16x3_kernel(
   auto store_tile = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < A_LD; ++j) {
-      const int q = t + ENER_NT * j, row = q >> 2, ch = q & 3;
+      const int q = t + NT * j, row = q >> 2, ch = q & 3;
 #pragma unroll
       for (int h = 0; h < 2; ++h)
         *reinterpret_cast<u32x4_t*>(&sA[buf][h][row * LDSW + ch * 8]) = ra[h][j];
     }
 #pragma unroll
     for (int j = 0; j < B_LD; ++j) {
-      const int q = t + ENER_NT * j, row = q >> 2, ch = q & 3;
+      const int q = t + NT * j, row = q >> 2, ch = q & 3;
 #pragma unroll
       for (int h = 0; h < 2; ++h)
         *reinterpret_cast<u32x4_t*>(&sB[buf][h][row * LDSW + ch * 8]) = rb[h][j];
@@ -275,7 +320,7 @@ __global__ __launch_bounds__(ENER_NT) void exchange_bf16x3_kernel(
   store_tile(0);
   __syncthreads();
   const int arow = wm * (BM / 2) + (lane & 15);
-  const int brow = wn * (BN / 2) + (lane & 15);
+  const int brow = wn * (BN / WN) + (lane & 15);
   const int koff = (lane >> 4) * 8;
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
@@ -308,17 +353,18 @@ __global__ __launch_bounds__(ENER_NT) void exchange_bf16x3_kernel(
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
-      const int col = wn * (BN / 2) + j * 16 + (lane & 15);
+      const int col = wn * (BN / WN) + j * 16 + (lane & 15);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = wm * (BM / 2) + i * 16 + 4 * (lane >> 4) + r;
-        Fs[row * EB_LDF + col] = acc[i][j][r];
+        Fs[row * Tile::LDF + col] = acc[i][j][r];
       }
     }
   __syncthreads();
   const int g_used = (G - g0 < gpt) ? G - g0 : gpt;
   const int w_used = (W - w0 < wpt) ? W - w0 : wpt;
-  exchange_tile<float>(Fs, EB_LDF, n, g_used, w_used, w0, xpart + (int64_t)gt * W);
+  exchange_tile<float, NT>(Fs, Tile::LDF, n, g_used, w_used, w0, xpart + (int64_t)gt * W,
+                           reinterpret_cast<cplx*>(lds + Tile::F));
 }
 
 // E_w = sum_g (c_g^2 - x_g) / 2 + sum_s m_s sum G_s rot_h1_s + h0 with c_g the summed
@@ -378,6 +424,14 @@ __global__ __launch_bounds__(ENER_NT) void energy_finish_kernel(
 
 }  // namespace
 
+// Waves across the split-bf16 exchange tile for n occupied orbitals: 4 (256 columns)
+// where that tile holds at least 1.2 times the walker columns per Cholesky row of the
+// 128-column one, else 2.
+int exchange_wn(int n) {
+  const int c2 = (128 / (2 * n)) * 2, c4 = (256 / (2 * n)) * 2;
+  return 5 * c4 >= 12 * c2 ? 4 : 2;
+}
+
 int energy_g_tiles(const afqmc_ctx* c, int s) {
   const int n = c->nocc[s];
   if (n < 1) return 0;
@@ -390,7 +444,7 @@ int energy_g_tiles(const afqmc_ctx* c, int s) {
 int launch_energy(afqmc_ctx* c, int W, cplx* energies) {
   static_assert(AFQMC_MAX_NOCC <= EF_BM && 2 * AFQMC_MAX_NOCC <= EF_BN,
                 "a (g, w) block must fit one tile");
-  static_assert(AFQMC_MAX_NOCC <= EB_BM && 2 * AFQMC_MAX_NOCC <= EB_BN,
+  static_assert(AFQMC_MAX_NOCC <= EB_BM && 2 * AFQMC_MAX_NOCC <= EbTile<2>::BN,
                 "a (g, w) block must fit one tile");
   const int N = c->d.norb, G = c->d.nchol;
   int gts[2] = {0, 0};
@@ -398,12 +452,19 @@ int launch_energy(afqmc_ctx* c, int W, cplx* energies) {
   for (int s = 0; s < c->nspin; ++s) {
     const int n = c->nocc[s];
     if (c->mixed) {
-      const int gpt = EB_BM / n, wpt = EB_BN / (2 * n);
+      const int wn = exchange_wn(n);
+      const int gpt = EB_BM / n, wpt = 64 * wn / (2 * n);
       const int g_tiles = (G + gpt - 1) / gpt, w_tiles = (W + wpt - 1) / wpt;
       gts[s] = g_tiles;
-      hipLaunchKernelGGL(exchange_bf16x3_kernel, dim3(g_tiles * w_tiles), dim3(ENER_NT), 0,
-                         c->stream, G, n, c->Np32, W, gpt, wpt, g_tiles, c->rot_chol_bf[s][0],
-                         c->rot_chol_bf[s][1], c->green_bf[s][0], c->green_bf[s][1], xp);
+      const dim3 grid(g_tiles * w_tiles);
+      if (wn == 4)
+        hipLaunchKernelGGL(exchange_bf16x3_kernel<4>, grid, dim3(EbTile<4>::NT), 0, c->stream,
+                           G, n, c->Np32, W, gpt, wpt, g_tiles, c->rot_chol_bf[s][0],
+                           c->rot_chol_bf[s][1], c->green_bf[s][0], c->green_bf[s][1], xp);
+      else
+        hipLaunchKernelGGL(exchange_bf16x3_kernel<2>, grid, dim3(EbTile<2>::NT), 0, c->stream,
+                           G, n, c->Np32, W, gpt, wpt, g_tiles, c->rot_chol_bf[s][0],
+                           c->rot_chol_bf[s][1], c->green_bf[s][0], c->green_bf[s][1], xp);
     } else {
       const int gpt = EF_BM / n, wpt = EF_BN / (2 * n);
       const int g_tiles = (G + gpt - 1) / gpt, w_tiles = (W + wpt - 1) / wpt;

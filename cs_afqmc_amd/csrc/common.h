@@ -436,9 +436,9 @@ struct TaylorPlan {
   size_t shm;
 };
 void taylor_plan(const afqmc_ctx* c, int s, TaylorPlan* pl);
-void measure_plan(const afqmc_ctx* c, int s, int* t2, bool* stage, size_t* shm);
+void measure_plan(const afqmc_ctx* c, int s, int* t2, int* stage, size_t* shm);
 
-int launch_measure_grouped(hipStream_t s, int t2, bool stage, const MeasureArgs* args,
+int launch_measure_grouped(hipStream_t s, int t2, int stage, const MeasureArgs* args,
                            const int* start, int nprob, int nblocks, size_t shm);
 int launch_fields_grouped(hipStream_t s, const FieldsArgs* args, const int* start, int nprob,
                           int nblocks, int step);

@@ -91,8 +91,7 @@ int build(afqmc_group* g) {
     for (const Member& m : g->mem) {
       afqmc_ctx* c = m.c;
       for (int s = 0; s < c->nspin; ++s) {
-        int t2;
-        bool stage;
+        int t2, stage;
         size_t sh;
         measure_plan(c, s, &t2, &stage, &sh);
         MeasureArgs a{};
@@ -106,7 +105,7 @@ int build(afqmc_group* g) {
         a.green = g->mixed ? nullptr : c->green[s];
         a.ghi = g->mixed ? c->green_bf[s][0] : nullptr;
         a.glo = g->mixed ? c->green_bf[s][1] : nullptr;
-        const auto key = std::make_pair(t2, stage ? 1 : 0);
+        const auto key = std::make_pair(t2, stage);
         by_key[key].push_back({a, c->d.n_walkers});
         shm[key] = std::max(shm[key], sh);
       }
@@ -370,7 +369,7 @@ const int* tstart(const afqmc_group* g, const Launch& L) {
 
 int run_measure(afqmc_group* g) {
   for (const Launch& L : g->meas) {
-    const int rc = launch_measure_grouped(g->stream, L.k0, L.k1 != 0, targs<MeasureArgs>(g, L),
+    const int rc = launch_measure_grouped(g->stream, L.k0, L.k1, targs<MeasureArgs>(g, L),
                                           tstart(g, L), L.nprob, L.nblocks, L.shm);
     if (rc) return rc;
   }

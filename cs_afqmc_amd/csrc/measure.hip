@@ -345,12 +345,16 @@ __device__ inline void finish_walker(cplx det, cplx* dets, int w, int spin,
   }
 }
 
-// n <= 24 (T2 <= 3): two waves per walker.  A walker of at most MEAS_STAGE_MAX bytes is
-// staged in LDS first (one round of loads), and the region is then reused for O, the
-// inversion's maps and the overlap partials.  Each wave's first Green tile of the walker
+// n <= 24 (T2 <= 3): two waves per walker.  STAGE 2: the walker is staged in LDS (one
+// round of loads) and stays there for the Green product, with O and the inversion's
+// state after it; with two row tiles of O each wave then forms one of them over the whole
+// K, else the waves split K and wave 1's partial sums reach wave 0 through LDS.  STAGE 1
+// (walker too large to keep beside O): staged for the overlap product only, the region
+// then reused for O, the state and the partials, the Green product reading the walker
+// from global memory.  STAGE 0: no staging.  Each wave's first Green tile of the walker
 // operand loads while the waves invert O; the Green product's A operand (from X) is then
 // read into registers once and the walker operand runs two tiles ahead.
-template <int T2, bool STAGE>
+template <int T2, int STAGE>
 __device__ __forceinline__ void measure_narrow_body(
     int N, int n, const cplx* __restrict__ walkers, const double* __restrict__ trial,
     cplx* __restrict__ dets, int spin, double* __restrict__ green, uint16_t* __restrict__ ghi,
@@ -359,7 +363,8 @@ __device__ __forceinline__ void measure_narrow_body(
   constexpr int NKG = 4 * T2;  // k groups of 2n <= 16 T2
   extern __shared__ double smem[];
   const int n2 = 2 * n, ld = gj_ld(n);
-  cplx* M = reinterpret_cast<cplx*>(smem);                    // O, then its inverse: n x ld
+  cplx* ph = reinterpret_cast<cplx*>(smem);                   // staged walker (STAGE > 0)
+  cplx* M = STAGE == 2 ? ph + N * n : ph;                     // O, then its inverse: n x ld
   GjState* st = reinterpret_cast<GjState*>(M + n * ld);       // the inversion's state
   double* red = reinterpret_cast<double*>(reinterpret_cast<int*>(st) + 256);  // overlap partials
   // wave index as a scalar: the waves' paths below are real branches, so registers that
@@ -369,8 +374,15 @@ __device__ __forceinline__ void measure_narrow_body(
   const cplx* phi = walkers + (int64_t)w * N * n;
   const double* phid = reinterpret_cast<const double*>(phi);
 
-  if constexpr (STAGE) {
-    cplx* ph = reinterpret_cast<cplx*>(smem);
+  if constexpr (STAGE == 2) {
+    for (int e = threadIdx.x; e < N * n; e += 64 * NW) ph[e] = phi[e];
+    __syncthreads();
+    phid = reinterpret_cast<const double*>(ph);
+    if constexpr (TI == 2)
+      overlap_tiles<T2, 1, 1>(N, n, wv, trial, phid, M, ld, nullptr, false, lane, 0);
+    else
+      overlap_tiles<T2, TI, NW>(N, n, 0, trial, phid, M, ld, red, false, lane, wv);
+  } else if constexpr (STAGE == 1) {
     for (int e = threadIdx.x; e < N * n; e += 64 * NW) ph[e] = phi[e];
     __syncthreads();
     overlap_tiles<T2, TI, NW>(N, n, 0, trial, reinterpret_cast<const double*>(ph), M, ld,
@@ -547,7 +559,7 @@ __device__ __forceinline__ void measure_wide_body(
     }
 }
 
-template <int T2, int TI, bool STAGE>
+template <int T2, int TI, int STAGE>
 __global__ __launch_bounds__(64 * meas_waves(T2), meas_wpe(T2)) void measure_kernel(
     int N, int n, const cplx* __restrict__ walkers, const double* __restrict__ trial,
     cplx* __restrict__ dets, int spin, double* __restrict__ green, uint16_t* __restrict__ ghi,
@@ -556,11 +568,11 @@ __global__ __launch_bounds__(64 * meas_waves(T2), meas_wpe(T2)) void measure_ker
     measure_narrow_body<T2, STAGE>(N, n, walkers, trial, dets, spin, green, ghi, glo, ld16,
                                    blockIdx.x, tail);
   else
-    measure_wide_body<T2, TI, STAGE>(N, n, walkers, trial, dets, spin, green, ghi, glo, ld16,
-                                     blockIdx.x, tail);
+    measure_wide_body<T2, TI, STAGE != 0>(N, n, walkers, trial, dets, spin, green, ghi, glo,
+                                          ld16, blockIdx.x, tail);
 }
 
-template <int T2, int TI, bool STAGE>
+template <int T2, int TI, int STAGE>
 __global__ __launch_bounds__(64 * meas_waves(T2), meas_wpe(T2)) void measure_grouped_kernel(const MeasureArgs* __restrict__ args,
                                                              const int* __restrict__ start,
                                                              int nprob) {
@@ -571,26 +583,45 @@ __global__ __launch_bounds__(64 * meas_waves(T2), meas_wpe(T2)) void measure_gro
     measure_narrow_body<T2, STAGE>(a.N, a.n, a.walkers, a.trial, a.dets, a.spin, a.green,
                                    a.ghi, a.glo, a.ld16, blockIdx.x - start[p], none);
   else
-    measure_wide_body<T2, TI, STAGE>(a.N, a.n, a.walkers, a.trial, a.dets, a.spin, a.green,
-                                     a.ghi, a.glo, a.ld16, blockIdx.x - start[p], none);
+    measure_wide_body<T2, TI, STAGE != 0>(a.N, a.n, a.walkers, a.trial, a.dets, a.spin,
+                                          a.green, a.ghi, a.glo, a.ld16, blockIdx.x - start[p],
+                                          none);
+}
+
+template <int T2, int STAGE>
+int launch_measure_grouped_s(hipStream_t st, const MeasureArgs* args, const int* start,
+                             int nprob, int nblocks, size_t shm) {
+  constexpr int TI = meas_ti(T2);
+  static int lds_limit = 0;
+  const int rc =
+      ensure_dyn_lds((const void*)measure_grouped_kernel<T2, TI, STAGE>, shm, &lds_limit);
+  if (rc) return rc;
+  hipLaunchKernelGGL((measure_grouped_kernel<T2, TI, STAGE>), dim3(nblocks),
+                     dim3(64 * meas_waves(T2)), shm, st, args, start, nprob);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
 }
 
 template <int T2>
-int launch_measure_grouped_t(hipStream_t st, bool stage, const MeasureArgs* args,
+int launch_measure_grouped_t(hipStream_t st, int stage, const MeasureArgs* args,
                              const int* start, int nprob, int nblocks, size_t shm) {
+  if constexpr (T2 <= 3)
+    if (stage == 2) return launch_measure_grouped_s<T2, 2>(st, args, start, nprob, nblocks, shm);
+  if (stage != 0) return launch_measure_grouped_s<T2, 1>(st, args, start, nprob, nblocks, shm);
+  return launch_measure_grouped_s<T2, 0>(st, args, start, nprob, nblocks, shm);
+}
+
+template <int T2, int STAGE>
+int launch_measure_s(afqmc_ctx* c, int W, int n, const cplx* wk, int s, bool g64, bool gbf,
+                     const WeightsArgs& tail, size_t shm) {
   constexpr int TI = meas_ti(T2);
-  const dim3 block(64 * meas_waves(T2));
-  if (stage) {
-    hipLaunchKernelGGL((measure_grouped_kernel<T2, TI, true>), dim3(nblocks), block, shm, st,
-                       args, start, nprob);
-  } else {
-    static int lds_limit = 0;
-    const int rc =
-        ensure_dyn_lds((const void*)measure_grouped_kernel<T2, TI, false>, shm, &lds_limit);
-    if (rc) return rc;
-    hipLaunchKernelGGL((measure_grouped_kernel<T2, TI, false>), dim3(nblocks), block, shm, st,
-                       args, start, nprob);
-  }
+  static int lds_limit = 0;
+  const int rc = ensure_dyn_lds((const void*)measure_kernel<T2, TI, STAGE>, shm, &lds_limit);
+  if (rc) return rc;
+  hipLaunchKernelGGL((measure_kernel<T2, TI, STAGE>), dim3(W), dim3(64 * meas_waves(T2)), shm,
+                     c->stream, c->d.norb, n, wk, c->trial[s], c->dets, s,
+                     g64 ? c->green[s] : nullptr, gbf ? c->green_bf[s][0] : nullptr,
+                     gbf ? c->green_bf[s][1] : nullptr, c->Np32, tail);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }
@@ -598,31 +629,13 @@ int launch_measure_grouped_t(hipStream_t st, bool stage, const MeasureArgs* args
 template <int T2>
 int launch_measure_t(afqmc_ctx* c, int W, int n, const cplx* wk, int s, bool g64, bool gbf,
                      const WeightsArgs& tail) {
-  // O row tiles per pass: all ceil(n / 16) of them unless the accumulators would spill
-  constexpr int TI = meas_ti(T2);
-  const dim3 block(64 * meas_waves(T2));
-  const int N = c->d.norb;
-  int t2;
-  bool stage;
+  int t2, stage;
   size_t shm;
   measure_plan(c, s, &t2, &stage, &shm);
-  if (stage) {  // small walkers only: staging costs occupancy
-    hipLaunchKernelGGL((measure_kernel<T2, TI, true>), dim3(W), block, shm, c->stream, N,
-                       n, wk, c->trial[s], c->dets, s, g64 ? c->green[s] : nullptr,
-                       gbf ? c->green_bf[s][0] : nullptr, gbf ? c->green_bf[s][1] : nullptr,
-                       c->Np32, tail);
-    LAUNCH_CHECK();
-    return AFQMC_OK;
-  }
-  static int lds_limit = 0;
-  const int rc = ensure_dyn_lds((const void*)measure_kernel<T2, TI, false>, shm, &lds_limit);
-  if (rc) return rc;
-  hipLaunchKernelGGL((measure_kernel<T2, TI, false>), dim3(W), block, shm, c->stream, c->d.norb, n, wk,
-                     c->trial[s], c->dets, s, g64 ? c->green[s] : nullptr,
-                     gbf ? c->green_bf[s][0] : nullptr, gbf ? c->green_bf[s][1] : nullptr, c->Np32,
-                     tail);
-  LAUNCH_CHECK();
-  return AFQMC_OK;
+  if constexpr (T2 <= 3)
+    if (stage == 2) return launch_measure_s<T2, 2>(c, W, n, wk, s, g64, gbf, tail, shm);
+  if (stage != 0) return launch_measure_s<T2, 1>(c, W, n, wk, s, g64, gbf, tail, shm);
+  return launch_measure_s<T2, 0>(c, W, n, wk, s, g64, gbf, tail, shm);
 }
 
 }  // namespace
@@ -651,7 +664,7 @@ int launch_measure(afqmc_ctx* c, int W, const cplx* wup, const cplx* wdn, bool g
 
 // Grouped form: one launch for every (fragment, spin) problem with the same column-tile
 // count t2 and staging choice; shm is the largest LDS need among them.
-int launch_measure_grouped(hipStream_t st, int t2, bool stage, const MeasureArgs* args,
+int launch_measure_grouped(hipStream_t st, int t2, int stage, const MeasureArgs* args,
                            const int* start, int nprob, int nblocks, size_t shm) {
   switch (t2) {
     case 1: return launch_measure_grouped_t<1>(st, stage, args, start, nprob, nblocks, shm);
@@ -665,19 +678,30 @@ int launch_measure_grouped(hipStream_t st, int t2, bool stage, const MeasureArgs
   }
 }
 
-// The kernel launch_measure picks for spin s, as a plan for grouped launches.
-void measure_plan(const afqmc_ctx* c, int s, int* t2, bool* stage, size_t* shm) {
+// The kernel launch_measure picks for spin s, as a plan for grouped launches: column tiles
+// t2 of the Green product, the staging mode and the LDS bytes.
+void measure_plan(const afqmc_ctx* c, int s, int* t2, int* stage, size_t* shm) {
   const int n = c->nocc[s], N = c->d.norb;
   const int t = (2 * n + 15) / 16;
   *t2 = t > 8 ? 8 : t;
   const size_t walker = (size_t)N * n * sizeof(cplx);
-  if (*t2 <= 3) {  // two-wave kernel: O, state and partials alias the staged walker
-    const size_t m_red = gj_bytes(n) + meas_red_bytes(*t2);
-    *stage = walker <= MEAS_STAGE_MAX;
-    *shm = *stage ? std::max(walker, m_red) : m_red;
+  if (*t2 <= 3) {  // two-wave kernel
+    const size_t red = meas_red_bytes(*t2);
+    const size_t kept = walker + gj_bytes(n) + (meas_ti(*t2) == 2 ? 0 : red);
+    const size_t m_red = gj_bytes(n) + red;
+    if (kept <= MEAS_STAGE_MAX) {  // walker kept beside O for the Green product
+      *stage = 2;
+      *shm = kept;
+    } else if (walker <= MEAS_STAGE_MAX) {  // O, state and partials alias the walker
+      *stage = 1;
+      *shm = std::max(walker, m_red);
+    } else {
+      *stage = 0;
+      *shm = m_red;
+    }
     return;
   }
   const size_t base = gj_bytes(n);
-  *stage = base + walker <= 24 * 1024;  // small walkers only: staging costs occupancy
+  *stage = base + walker <= 24 * 1024 ? 1 : 0;  // small walkers only: staging costs occupancy
   *shm = *stage ? base + walker : base;
 }

@@ -8,7 +8,7 @@ of mo_coeff.npz, it returns (ham_data, ham, prop, trial, wave_data, sampler,
 observable, options) ready for driver.afqmc.
 
 ``read_fcidump_chol`` reads the HDF5 file with h5py when that is installed, or an
-.npz holding the same datasets.
+.npz holding the same datasets; ``read_options`` reads an options.bin file.
 """
 
 import numpy as np
@@ -35,6 +35,124 @@ OPTION_DEFAULTS = {
 }
 
 
+def read_options(path):
+    """The options dict of an options.bin file (mpi_jax.py:55-60 reads it with pickle.load;
+    run_afqmc.py:31-32 writes it with pickle.dump).
+
+    Decoded without the pickle module: binary pickle protocols 2-5 holding only dicts,
+    lists, tuples, str, bytes, int, float, bool and None.  A file that needs anything
+    else (a class, a global, a reduce call) raises ValueError; nothing in the file is
+    executed.  A missing file gives {} (mpi_jax.py:57-61 falls back to defaults)."""
+    try:
+        with open(path, "rb") as f:
+            data = f.read()
+    except FileNotFoundError:
+        return {}
+    return _decode_plain_pickle(data)
+
+
+def _decode_plain_pickle(data):
+    import struct
+
+    stack, marks, memo = [], [], {}
+    pos, n = 0, len(data)
+
+    def take(k):
+        nonlocal pos
+        if pos + k > n:
+            raise ValueError("truncated options pickle")
+        b = data[pos:pos + k]
+        pos += k
+        return b
+
+    def pop_mark():
+        m = marks.pop()
+        items = stack[m:]
+        del stack[m:]
+        return items
+
+    while True:
+        op = take(1)[0]
+        if op == 0x80:      # PROTO
+            if take(1)[0] < 2:
+                raise ValueError("options pickle: protocol < 2 is not supported")
+        elif op == 0x95:    # FRAME
+            take(8)
+        elif op == 0x2E:    # STOP
+            if len(stack) != 1:
+                raise ValueError("malformed options pickle")
+            return stack[0]
+        elif op == 0x7D:    # EMPTY_DICT
+            stack.append({})
+        elif op == 0x5D:    # EMPTY_LIST
+            stack.append([])
+        elif op == 0x29:    # EMPTY_TUPLE
+            stack.append(())
+        elif op == 0x28:    # MARK
+            marks.append(len(stack))
+        elif op == 0x75:    # SETITEMS
+            items = pop_mark()
+            d = stack[-1]
+            for i in range(0, len(items), 2):
+                d[items[i]] = items[i + 1]
+        elif op == 0x73:    # SETITEM
+            v = stack.pop()
+            k = stack.pop()
+            stack[-1][k] = v
+        elif op == 0x65:    # APPENDS
+            items = pop_mark()
+            stack[-1].extend(items)
+        elif op == 0x61:    # APPEND
+            v = stack.pop()
+            stack[-1].append(v)
+        elif op == 0x74:    # TUPLE
+            stack.append(tuple(pop_mark()))
+        elif op in (0x85, 0x86, 0x87):  # TUPLE1/2/3
+            k = op - 0x84
+            t = tuple(stack[-k:])
+            del stack[-k:]
+            stack.append(t)
+        elif op == 0x8C:    # SHORT_BINUNICODE
+            stack.append(take(take(1)[0]).decode("utf-8"))
+        elif op == 0x58:    # BINUNICODE
+            stack.append(take(struct.unpack("<I", take(4))[0]).decode("utf-8"))
+        elif op == 0x8D:    # BINUNICODE8
+            stack.append(take(struct.unpack("<Q", take(8))[0]).decode("utf-8"))
+        elif op == 0x43:    # SHORT_BINBYTES
+            stack.append(bytes(take(take(1)[0])))
+        elif op == 0x42:    # BINBYTES
+            stack.append(bytes(take(struct.unpack("<I", take(4))[0])))
+        elif op == 0x4B:    # BININT1
+            stack.append(take(1)[0])
+        elif op == 0x4D:    # BININT2
+            stack.append(struct.unpack("<H", take(2))[0])
+        elif op == 0x4A:    # BININT
+            stack.append(struct.unpack("<i", take(4))[0])
+        elif op == 0x8A:    # LONG1
+            stack.append(int.from_bytes(take(take(1)[0]), "little", signed=True))
+        elif op == 0x47:    # BINFLOAT
+            stack.append(struct.unpack(">d", take(8))[0])
+        elif op == 0x88:    # NEWTRUE
+            stack.append(True)
+        elif op == 0x89:    # NEWFALSE
+            stack.append(False)
+        elif op == 0x4E:    # NONE
+            stack.append(None)
+        elif op == 0x94:    # MEMOIZE
+            memo[len(memo)] = stack[-1]
+        elif op == 0x71:    # BINPUT
+            memo[take(1)[0]] = stack[-1]
+        elif op == 0x72:    # LONG_BINPUT
+            memo[struct.unpack("<I", take(4))[0]] = stack[-1]
+        elif op == 0x68:    # BINGET
+            stack.append(memo[take(1)[0]])
+        elif op == 0x6A:    # LONG_BINGET
+            stack.append(memo[struct.unpack("<I", take(4))[0]])
+        else:
+            raise ValueError(f"options pickle: opcode 0x{op:02x} would construct an object "
+                             "beyond plain values; refusing to decode it")
+
+
 def read_fcidump_chol(path):
     """Datasets of an FCIDUMP_chol file as numpy arrays (mpi_jax.py:39-51)."""
     if str(path).endswith(".npz"):
@@ -47,6 +165,18 @@ def read_fcidump_chol(path):
                           "with the same dataset names") from e
     with h5py.File(path, "r") as f:
         return {k: np.array(f[k]) for k in f.keys()}
+
+
+def prep_afqmc_files(options=None, option_file="options.bin", mo_file="mo_coeff.npz",
+                     chol_file="FCIDUMP_chol"):
+    """mpi_jax.py:36-60 with its file arguments: the integrals of chol_file (HDF5 or .npz),
+    the trial orbitals mo_file["mo_coeff"] and, when options is None, the options of
+    option_file (plain values only, see read_options); then prep_afqmc."""
+    if options is None:
+        options = read_options(option_file)
+    with np.load(mo_file) as z:
+        mo_coeff = np.array(z["mo_coeff"])
+    return prep_afqmc(read_fcidump_chol(chol_file), mo_coeff, options)
 
 
 def prep_afqmc(integrals, mo_coeff, options=None):

@@ -7,13 +7,13 @@ header [nelec, nmo, ms, nchol], energy_core, hcore, chol flat) and the trial orb
 of mo_coeff.npz, it returns (ham_data, ham, prop, trial, wave_data, sampler,
 observable, options) ready for driver.afqmc.
 
-``read_fcidump_chol`` reads the HDF5 file with h5py when that is installed, or an
-.npz holding the same datasets; ``read_options`` reads an options.bin file.
+``read_fcidump_chol`` reads the HDF5 file (its own reader, no h5py needed) or an .npz
+holding the same datasets; ``read_options`` reads an options.bin file.
 """
 
 import numpy as np
 
-from . import hamiltonian, propagation, sampling, wavefunctions
+from . import hamiltonian, hdf5_read, propagation, sampling, wavefunctions
 
 OPTION_DEFAULTS = {
     # mpi_jax.py:62-90
@@ -154,15 +154,19 @@ def _decode_plain_pickle(data):
 
 
 def read_fcidump_chol(path):
-    """Datasets of an FCIDUMP_chol file as numpy arrays (mpi_jax.py:39-51)."""
+    """Datasets of an FCIDUMP_chol file as numpy arrays (mpi_jax.py:39-51): the HDF5 file
+    through the library's own reader (hdf5_read; h5py, when installed, for files beyond
+    the layouts that reader handles), or an .npz holding the same datasets."""
     if str(path).endswith(".npz"):
         with np.load(path) as z:
             return {k: np.array(z[k]) for k in z.files}
     try:
-        import h5py
-    except ImportError as e:
-        raise ImportError("reading HDF5 FCIDUMP_chol needs h5py; convert it to .npz "
-                          "with the same dataset names") from e
+        return hdf5_read.read_datasets(path)
+    except ValueError:
+        try:
+            import h5py
+        except ImportError:
+            raise
     with h5py.File(path, "r") as f:
         return {k: np.array(f[k]) for k in f.keys()}
 

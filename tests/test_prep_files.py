@@ -1,6 +1,8 @@
 """On-disk inputs of the reference's run scripts: options.bin (mpi_jax.py:55-60, written
-by run_afqmc.py:31-32 with pickle.dump) decoded without the pickle module, and the
-file-based set-up (mpi_jax.py:36-60) over the H10 fixture's integrals and orbitals."""
+by run_afqmc.py:31-32 with pickle.dump) decoded without the pickle module; FCIDUMP_chol
+(HDF5, pyscf_interface.py:578-598) read by the library's own HDF5 reader, bit for bit
+against h5py's reading of the reference tests' file; and the file-based set-up
+(mpi_jax.py:36-60)."""
 
 import os
 import pickle
@@ -12,6 +14,8 @@ from cs_afqmc_amd import prep
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 FIXTURE = os.path.join(ROOT, "golden", "h10_chain_r2.0_sto6g.npz")
+H5 = os.path.join(ROOT, "golden", "FCIDUMP_chol_6orb")
+H5_EXPECTED = os.path.join(ROOT, "golden", "FCIDUMP_chol_6orb_expected.npz")
 
 OPTIONS = {
     "n_eql": 3, "n_ene_blocks": 1, "n_sr_blocks": 50, "n_blocks": 10, "n_walkers": 50,
@@ -87,3 +91,32 @@ def test_prep_from_files(tmp_path):
     assert ham_data["chol"].shape == (nchol, nmo * nmo)
     assert opts["n_blocks"] == 50  # mpi_jax.py:62-90 defaults fill the rest
     np.testing.assert_allclose(wave_data["mo_coeff"], z["mo_coeff"][0][:, : nelec // 2])
+
+
+def test_hdf5_fcidump_matches_h5py():
+    ours = prep.read_fcidump_chol(H5)
+    with np.load(H5_EXPECTED) as z:
+        assert sorted(ours) == sorted(z.files)
+        for k in z.files:
+            assert ours[k].dtype == z[k].dtype and ours[k].shape == z[k].shape, k
+            assert np.array_equal(ours[k], z[k]), k
+    nelec, nmo, ms, nchol = (int(x) for x in ours["header"])
+    assert ours["chol"].size == nchol * nmo * nmo and ours["hcore"].size == nmo * nmo
+
+
+def test_hdf5_reader_rejects_other_files(tmp_path):
+    f = tmp_path / "not_hdf5"
+    f.write_bytes(b"\x00" * 128)
+    with pytest.raises(ValueError):
+        prep.hdf5_read.read_datasets(str(f))
+
+
+def test_prep_from_hdf5_file(tmp_path):
+    z = np.load(H5_EXPECTED)
+    mo = tmp_path / "mo_coeff.npz"
+    np.savez(mo, mo_coeff=np.array([z["mo_coeffs_up"], z["mo_coeffs_dn"]]))
+    ham_data, ham, prop, trial, wave_data, smp, obs, opts = prep.prep_afqmc_files(
+        options={"n_walkers": 8}, mo_file=str(mo), chol_file=H5)
+    nelec, nmo, ms, nchol = (int(x) for x in z["header"])
+    assert ham_data["chol"].shape == (nchol, nmo * nmo)
+    assert ham_data["h0"] == float(z["energy_core"])

@@ -620,26 +620,35 @@ def run_h10(precision, n_calls):
 # ------------------------------------------------------------------ config 5: serving
 
 
-def serving_requests(n_requests):
+def serving_draws(n_requests):
+    """(arrival, norb, nocc, W, problem seed) of each request: Poisson arrivals at 200/s,
+    norb ~ U[20, 250], nocc = norb / 5, W from {128, 256, 512, 1024}."""
     import numpy as np
-
-    from cs_afqmc_amd import problems, serving
 
     rng = np.random.default_rng(2024)
     t = 0.0
-    reqs = []
+    out = []
     for i in range(n_requests):
         t += rng.exponential(1.0 / 200.0)
         norb = int(rng.integers(20, 251))
         nocc = max(1, norb // 5)
         W = int(rng.choice([128, 256, 512, 1024]))
-        hd, wd, ne = problems.synthetic_rhf(norb, nocc, 4 * norb, seed=5000 + i)
+        out.append((t, norb, nocc, W, 5000 + i))
+    return out
+
+
+def serving_requests(n_requests):
+    from cs_afqmc_amd import problems, serving
+
+    reqs = []
+    for i, (t, norb, nocc, W, seed) in enumerate(serving_draws(n_requests)):
+        hd, wd, ne = problems.synthetic_rhf(norb, nocc, 4 * norb, seed=seed)
         reqs.append(serving.Request(arrival=t, rid=i, ham_data=hd, wave_data=wd, nelec=ne,
                                     n_walkers=W, n_steps=100, seed=17 + i))
     return reqs
 
 
-def run_serving(reqs):
+def run_serving(reqs, profile=True):
     import numpy as np
     import torch
 
@@ -648,7 +657,7 @@ def run_serving(reqs):
     torch.cuda.set_device(0)
     srv = serving.FragmentServer(precision="bf16x3", n_prop_steps=N_PROP_STEPS,
                                  hbm_budget=200 << 30, max_active=SERVE_MAX_ACTIVE,
-                                 policy=SERVE_POLICY, profile=True)
+                                 policy=SERVE_POLICY, profile=profile)
     t0 = time.perf_counter()
     res = srv.serve(reqs)
     wall = time.perf_counter() - t0
@@ -739,6 +748,11 @@ def main():
             wt, n, o, g, W = SINGLE_WORKLOADS[want]
             cpu["single"] = cpu_baseline([(wt, n, o, g, W, 11 + i) for i in range(16)], 4, 4,
                                          want)
+        if want in ("all", SERVE_WORKLOAD) and not args.no_serving:
+            cpu["serving"] = cpu_baseline(
+                [("rhf", norb, nocc, 4 * norb, W, seed)
+                 for _, norb, nocc, W, seed in serving_draws(args.requests)], 2, 2,
+                SERVE_WORKLOAD)
     import numpy as np
     import torch
     torch.cuda.set_device(0)
@@ -896,6 +910,23 @@ def main():
         sv = run_serving(reqs)
         print(f"# serving: p50 {sv['latency_p50_s']:.3f} s, p99 {sv['latency_p99_s']:.3f} s",
               file=sys.stderr, flush=True)
+        # The Poisson-200/s stream offers an order of magnitude more work than one GPU
+        # serves, so its latencies are queueing.  The same requests again, arrivals
+        # stretched so the offered walker-steps/s is 70% of the rate just served: the
+        # latencies of a sustainable stream.
+        import dataclasses
+        stretch = sv["offered_walker_steps_per_s"] / (0.7 * sv["value"])
+        if stretch > 1.0:
+            slow = [dataclasses.replace(r, arrival=r.arrival * stretch) for r in reqs]
+            sus = run_serving(slow, profile=False)
+            sv["sustainable"] = {k: sus[k] for k in (
+                "value", "latency_p50_s", "latency_p99_s", "latency_mean_s", "latency_max_s",
+                "arrival_window_s", "offered_walker_steps_per_s", "makespan_s",
+                "arrival_to_admit_s_mean", "admit_to_done_s_mean")}
+            sv["sustainable"]["arrival_stretch"] = round(stretch, 3)
+            print(f"# serving at 70% load: p50 {sus['latency_p50_s']:.3f} s, "
+                  f"p99 {sus['latency_p99_s']:.3f} s", file=sys.stderr, flush=True)
+        sv["cpu_baseline"] = cpu.get("serving")
         if want == "all":
             out["serving"] = dict(sv, workload=SERVE_WORKLOAD)
             out["latency_p50_s"] = sv["latency_p50_s"]
@@ -908,7 +939,7 @@ def main():
                         "dtype": DTYPE["bf16x3"], "data": "synthetic fragment requests",
                         "config": {"workload": SERVE_WORKLOAD}})
             out.update(sv)
-            out["cpu_baseline"] = None
+            out["cpu_baseline"] = sv.get("cpu_baseline")
     print(json.dumps(out), flush=True)
 
 

@@ -58,10 +58,21 @@ __device__ inline uint32_t pack_bf16x2(float a, float b) {
 // the floats a packed dword stands for
 __device__ inline float bf16_lo_val(uint32_t u) { return __uint_as_float(u << 16); }
 __device__ inline float bf16_hi_val(uint32_t u) { return __uint_as_float(u & 0xFFFF0000u); }
+// x ~= hi + lo with hi = bf16(float(x)) and lo = bf16(float(x) - hi) (the difference is
+// exact in float32).  The empty asm keeps the double -> float rounding a step of its own:
+// folded into one double -> bf16 conversion it becomes a round-to-odd sequence of about
+// ten instructions.
+__device__ inline uint32_t split_bf16_pair(double x) {
+  float f = (float)x;
+  __asm__("" : "+v"(f));
+  const uint16_t h = f32_to_bf16_rne(f);
+  const uint16_t l = f32_to_bf16_rne(f - __uint_as_float((uint32_t)h << 16));
+  return (uint32_t)h | ((uint32_t)l << 16);
+}
 __device__ inline void split_bf16(double x, uint16_t* hi, uint16_t* lo) {
-  const uint16_t h = f32_to_bf16_rne((float)x);
-  *hi = h;
-  *lo = f32_to_bf16_rne((float)(x - (double)__uint_as_float((uint32_t)h << 16)));
+  const uint32_t hl = split_bf16_pair(x);
+  *hi = (uint16_t)hl;
+  *lo = (uint16_t)(hl >> 16);
 }
 
 // ---------------------------------------------------------------- walker weights

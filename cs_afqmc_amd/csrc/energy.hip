@@ -26,6 +26,32 @@ namespace {
 
 constexpr int ENER_NT = 256;
 
+// (g-tile, walker-tile) of block bx.  Blocks are dealt round-robin to the 8 XCDs (block
+// b runs on XCD b % 8), each with its own 4 MB L2, and each XCD takes a contiguous run of
+// the pairs in this order: chunks of gc g-tiles (as many Cholesky tiles as fill half an
+// L2), within a chunk walker tiles slowest.  The blocks an XCD has in flight then share
+// one or two walker tiles and cycle through Cholesky tiles that stay in its L2, so the
+// Cholesky operand is fetched about once per XCD and chunk, and the Green's-function
+// operand about once per chunk (instead of the whole Cholesky operand once per walker
+// tile with g-tiles fastest).
+__device__ inline void exchange_tile_of(int bx, int g_tiles, int w_tiles, int gc, int* gt,
+                                        int* wt) {
+  const int nblk = g_tiles * w_tiles;
+  const int q = nblk >> 3, r = nblk & 7, x = bx & 7, i = bx >> 3;
+  const int b = x * q + (x < r ? x : r) + i;
+  const int per = gc * w_tiles, c = b / per, g0 = c * gc;
+  const int gcc = g_tiles - g0 < gc ? g_tiles - g0 : gc;
+  const int rem = b - c * per;
+  *wt = rem / gcc;
+  *gt = g0 + rem - (rem / gcc) * gcc;
+}
+
+// g-tiles per chunk of exchange_tile_of: Cholesky tiles of tile_bytes filling half an L2
+inline int exchange_chunk(int g_tiles, int64_t tile_bytes) {
+  const int64_t fit = (int64_t)(2 << 20) / (tile_bytes > 0 ? tile_bytes : 1);
+  return fit < 1 ? 1 : (fit > g_tiles ? g_tiles : (int)fit);
+}
+
 // ---------------------------------------------------------------- fused exchange
 
 // Exchange sums of the (g, w) blocks of an F tile held in LDS (row stride ldf):
@@ -109,8 +135,8 @@ constexpr int EF_LDS = EF_OPS_BYTES > EF_F_PARTS ? EF_OPS_BYTES : EF_F_PARTS;
 
 template <int VEC, bool F32>
 __global__ __launch_bounds__(ENER_NT) void exchange_f64_kernel(
-    int G, int n, int N, int W, int gpt, int wpt, int g_tiles, const double* __restrict__ A,
-    const double* __restrict__ B, cplx* __restrict__ xpart) {
+    int G, int n, int N, int W, int gpt, int wpt, int g_tiles, int gc,
+    const double* __restrict__ A, const double* __restrict__ B, cplx* __restrict__ xpart) {
   typedef typename std::conditional<F32, float, double>::type T;
   typedef typename std::conditional<F32, float4_t, double4_t>::type AccT;
   constexpr int BM = EF_BM, BN = EF_BN, BK = EF_BK;
@@ -123,7 +149,8 @@ __global__ __launch_bounds__(ENER_NT) void exchange_f64_kernel(
   BsT* Bs = reinterpret_cast<BsT*>(lds + 2 * sizeof(AsT));
   T* Fs = reinterpret_cast<T*>(lds);
 
-  const int gt = blockIdx.x % g_tiles, wt = blockIdx.x / g_tiles;
+  int gt, wt;
+  exchange_tile_of(blockIdx.x, g_tiles, (W + wpt - 1) / wpt, gc, &gt, &wt);
   const int g0 = gt * gpt, w0 = wt * wpt;
   const int M = G * n, NB = 2 * W * n;
   const int m0 = g0 * n, n0 = w0 * 2 * n;
@@ -243,7 +270,8 @@ struct EbTile {
 
 template <int WN>
 __global__ __launch_bounds__(128 * WN) void exchange_bf16x3_kernel(
-    int G, int n, int K, int W, int gpt, int wpt, int g_tiles, const uint16_t* __restrict__ Ah,
+    int G, int n, int K, int W, int gpt, int wpt, int g_tiles, int gc,
+    const uint16_t* __restrict__ Ah,
     const uint16_t* __restrict__ Al, const uint16_t* __restrict__ Bh,
     const uint16_t* __restrict__ Bl, cplx* __restrict__ xpart) {
   typedef EbTile<WN> Tile;
@@ -258,7 +286,8 @@ __global__ __launch_bounds__(128 * WN) void exchange_bf16x3_kernel(
   PlaneB* sB = reinterpret_cast<PlaneB*>(lds + 2 * sizeof(PlaneA));
   float* Fs = reinterpret_cast<float*>(lds);
 
-  const int gt = blockIdx.x % g_tiles, wt = blockIdx.x / g_tiles;
+  int gt, wt;
+  exchange_tile_of(blockIdx.x, g_tiles, (W + wpt - 1) / wpt, gc, &gt, &wt);
   const int g0 = gt * gpt, w0 = wt * wpt;
   const int M = G * n, NB = 2 * W * n;
   const int m0 = g0 * n, n0 = w0 * 2 * n;
@@ -456,35 +485,37 @@ int launch_energy(afqmc_ctx* c, int W, cplx* energies) {
       const int gpt = EB_BM / n, wpt = 64 * wn / (2 * n);
       const int g_tiles = (G + gpt - 1) / gpt, w_tiles = (W + wpt - 1) / wpt;
       gts[s] = g_tiles;
+      const int gc = exchange_chunk(g_tiles, (int64_t)gpt * n * c->Np32 * 4);
       const dim3 grid(g_tiles * w_tiles);
       if (wn == 4)
         hipLaunchKernelGGL(exchange_bf16x3_kernel<4>, grid, dim3(EbTile<4>::NT), 0, c->stream,
-                           G, n, c->Np32, W, gpt, wpt, g_tiles, c->rot_chol_bf[s][0],
+                           G, n, c->Np32, W, gpt, wpt, g_tiles, gc, c->rot_chol_bf[s][0],
                            c->rot_chol_bf[s][1], c->green_bf[s][0], c->green_bf[s][1], xp);
       else
         hipLaunchKernelGGL(exchange_bf16x3_kernel<2>, grid, dim3(EbTile<2>::NT), 0, c->stream,
-                           G, n, c->Np32, W, gpt, wpt, g_tiles, c->rot_chol_bf[s][0],
+                           G, n, c->Np32, W, gpt, wpt, g_tiles, gc, c->rot_chol_bf[s][0],
                            c->rot_chol_bf[s][1], c->green_bf[s][0], c->green_bf[s][1], xp);
     } else {
       const int gpt = EF_BM / n, wpt = EF_BN / (2 * n);
       const int g_tiles = (G + gpt - 1) / gpt, w_tiles = (W + wpt - 1) / wpt;
       gts[s] = g_tiles;
       const bool vec2 = (N % 2 == 0);
+      const int gc = exchange_chunk(g_tiles, (int64_t)gpt * n * N * 8);
       const dim3 grid(g_tiles * w_tiles), block(ENER_NT);
       const double* A = c->rot_chol[s];
       const double* B = c->green[s];
       if (c->f32 && vec2)
         hipLaunchKernelGGL((exchange_f64_kernel<2, true>), grid, block, 0, c->stream, G, n, N,
-                           W, gpt, wpt, g_tiles, A, B, xp);
+                           W, gpt, wpt, g_tiles, gc, A, B, xp);
       else if (c->f32)
         hipLaunchKernelGGL((exchange_f64_kernel<1, true>), grid, block, 0, c->stream, G, n, N,
-                           W, gpt, wpt, g_tiles, A, B, xp);
+                           W, gpt, wpt, g_tiles, gc, A, B, xp);
       else if (vec2)
         hipLaunchKernelGGL((exchange_f64_kernel<2, false>), grid, block, 0, c->stream, G, n, N,
-                           W, gpt, wpt, g_tiles, A, B, xp);
+                           W, gpt, wpt, g_tiles, gc, A, B, xp);
       else
         hipLaunchKernelGGL((exchange_f64_kernel<1, false>), grid, block, 0, c->stream, G, n, N,
-                           W, gpt, wpt, g_tiles, A, B, xp);
+                           W, gpt, wpt, g_tiles, gc, A, B, xp);
     }
     LAUNCH_CHECK();
     xp += (int64_t)gts[s] * W;

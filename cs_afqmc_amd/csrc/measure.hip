@@ -269,37 +269,40 @@ __device__ __forceinline__ void overlap_tiles(int N, int n, int tb, const double
   // two operand buffers with compile-time names (a runtime buffer index would put them
   // in scratch memory)
   double a0[MEAS_KC][TI], b0[MEAS_KC][T2], a1[MEAS_KC][TI], b1[MEAS_KC][T2];
+  // Loads come from clamped (always valid) addresses and issue unconditionally.  Rows
+  // i >= n and columns c >= 2n of the product are never stored, so only the k padding
+  // p >= N needs a zero, on one operand; it is applied where the MFMA reads the operand,
+  // so the wait for a chunk's loads falls just before its MFMAs.
   auto load = [&](double (&a)[MEAS_KC][TI], double (&b)[MEAS_KC][T2], int kg0) {
 #pragma unroll
     for (int u = 0; u < MEAS_KC; ++u) {
-      const int p = 4 * (kg0 + u) + s;
-      const bool pok = p < N;
+      const int pc = min(4 * (kg0 + u) + s, N - 1);
 #pragma unroll
-      for (int ti = 0; ti < TI; ++ti) {
-        const int i = (tb + ti) * 16 + col;
-        a[u][ti] = (pok && i < n) ? trial[p * n + i] : 0.0;
-      }
+      for (int ti = 0; ti < TI; ++ti) a[u][ti] = trial[pc * n + min((tb + ti) * 16 + col, n - 1)];
 #pragma unroll
-      for (int tc = 0; tc < T2; ++tc)
-        b[u][tc] = (pok && boff[tc] >= 0) ? wsrc[(int64_t)2 * p * n + boff[tc]] : 0.0;
+      for (int tc = 0; tc < T2; ++tc) b[u][tc] = wsrc[(int64_t)2 * pc * n + max(boff[tc], 0)];
     }
   };
-  auto mma = [&](const double (&a)[MEAS_KC][TI], const double (&b)[MEAS_KC][T2]) {
+  auto mma = [&](const double (&a)[MEAS_KC][TI], const double (&b)[MEAS_KC][T2], int kg0) {
 #pragma unroll
-    for (int u = 0; u < MEAS_KC; ++u)
+    for (int u = 0; u < MEAS_KC; ++u) {
+      const bool pok = 4 * (kg0 + u) + s < N;
 #pragma unroll
-      for (int ti = 0; ti < TI; ++ti)
+      for (int ti = 0; ti < TI; ++ti) {
+        const double av = __builtin_unpredictable(pok) ? a[u][ti] : 0.0;
 #pragma unroll
         for (int tc = 0; tc < T2; ++tc)
-          acc[ti][tc] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][ti], b[u][tc], acc[ti][tc], 0, 0, 0);
+          acc[ti][tc] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[u][tc], acc[ti][tc], 0, 0, 0);
+      }
+    }
   };
   const int nch = (nkg + MEAS_KC - 1) / MEAS_KC;
   if (wv < nch) load(a0, b0, wv * MEAS_KC);
   for (int ch = wv; ch < nch; ch += 2 * NW) {
     if (ch + NW < nch) load(a1, b1, (ch + NW) * MEAS_KC);
-    mma(a0, b0);
+    mma(a0, b0, ch * MEAS_KC);
     if (ch + 2 * NW < nch) load(a0, b0, (ch + 2 * NW) * MEAS_KC);
-    if (ch + NW < nch) mma(a1, b1);
+    if (ch + NW < nch) mma(a1, b1, (ch + NW) * MEAS_KC);
   }
   if constexpr (NW > 1) {
     if (aliased) __syncthreads();
@@ -404,14 +407,11 @@ __device__ __forceinline__ void measure_narrow_body(
     const int k = 4 * kg + s;
     boff[kg] = k < n2 ? 2 * (k < n ? k : k - n) + (k < n ? 0 : 1) : -1;
   }
+  // (clamped, unmasked: columns p >= N are not stored, and A is zero for k >= 2n)
   auto load_b = [&](double (&b)[NKG], int tp) {
-    const int p = tp * 16 + col;
-    const double* rowp = phid + (int64_t)2 * (p < N ? p : 0) * n;
+    const double* rowp = phid + (int64_t)2 * min(tp * 16 + col, N - 1) * n;
 #pragma unroll
-    for (int kg = 0; kg < NKG; ++kg) {
-      const double v = rowp[boff[kg] < 0 ? 0 : boff[kg]];
-      b[kg] = (boff[kg] >= 0 && p < N) ? v : 0.0;
-    }
+    for (int kg = 0; kg < NKG; ++kg) b[kg] = rowp[max(boff[kg], 0)];
   };
   // each wave's first tile loads while the waves invert O
   double b[NKG];
@@ -420,7 +420,19 @@ __device__ __forceinline__ void measure_narrow_body(
   if (threadIdx.x == 0) finish_walker(st->det, dets, w, spin, tail);
   if (!want_green) return;
 
-  // A operand of every tile, from X in LDS, read once
+  // A operand of every tile, from X in LDS, read once: the pivot maps of the lane's rows
+  // and columns first, then every element, each load from a clamped index
+  int xrow[NKG], xcol[T2];
+#pragma unroll
+  for (int kg = 0; kg < NKG; ++kg) {
+    const int k = 4 * kg + s;
+    xrow[kg] = st->rho[min(k < n ? k : k - n, n - 1)] * ld;
+  }
+#pragma unroll
+  for (int tr = 0; tr < T2; ++tr) {
+    const int r = tr * 16 + col;
+    xcol[tr] = st->rhoinv[min(r < n ? r : r - n, n - 1)];
+  }
   double a[NKG][T2];
 #pragma unroll
   for (int kg = 0; kg < NKG; ++kg)
@@ -428,15 +440,10 @@ __device__ __forceinline__ void measure_narrow_body(
     for (int tr = 0; tr < T2; ++tr) {
       const int k = 4 * kg + s;
       const int r = tr * 16 + col;
-      double v = 0.0;
-      if (k < n2 && r < n2) {
-        const int kj = k < n ? k : k - n;
-        const double* x =
-            reinterpret_cast<const double*>(&M[st->rho[kj] * ld + st->rhoinv[r < n ? r : r - n]]);
-        // X[kj][ri]: (r < n, k < n): Xr; (r < n, k >= n): -Xi; (r >= n, k < n): Xi; else Xr
-        v = (r < n) == (k < n) ? x[0] : ((r < n) ? -x[1] : x[1]);
-      }
-      a[kg][tr] = v;
+      const cplx x = M[xrow[kg] + xcol[tr]];
+      // X[kj][ri]: (r < n, k < n): Xr; (r < n, k >= n): -Xi; (r >= n, k < n): Xi; else Xr
+      const double v = (r < n) == (k < n) ? x.re : ((r < n) ? -x.im : x.im);
+      a[kg][tr] = k < n2 ? v : 0.0;  // (rows r >= 2n are not stored)
     }
   double* gw = green ? green + (int64_t)w * n2 * N : nullptr;
   uint16_t* gh = ghi ? ghi + (int64_t)w * n2 * ld16 : nullptr;
@@ -515,29 +522,36 @@ __device__ __forceinline__ void measure_wide_body(
   uint16_t* gl = ghi ? glo + (int64_t)w * n2 * ld16 : nullptr;
   const int tp_n = (N + 15) >> 4;
     const int nkg = (n2 + 3) >> 2;
+    // columns of X in the lane's A rows r = tr * 16 + col (pivot map, clamped index:
+    // rows r >= 2n are not stored)
+    int xcol[T2];
+#pragma unroll
+    for (int tr = 0; tr < T2; ++tr) {
+      const int r = tr * 16 + col;
+      xcol[tr] = st->rhoinv[min(r < n ? r : r - n, n - 1)];
+    }
     for (int tp = wv; tp < tp_n; tp += NW) {
       const int p = tp * 16 + col;  // B column = orbital p
+      const int pc = min(p, N - 1);   // (columns p >= N are not stored)
       double4_t acc[T2];
 #pragma unroll
       for (int tr = 0; tr < T2; ++tr) acc[tr] = double4_t{0.0, 0.0, 0.0, 0.0};
       for (int kg0 = 0; kg0 < nkg; kg0 += MEAS_KC) {
+        // loads from clamped indices, unconditional; A is zero for k >= 2n
         double a[MEAS_KC][T2], b[MEAS_KC];
 #pragma unroll
         for (int u = 0; u < MEAS_KC; ++u) {
           const int k = 4 * (kg0 + u) + s;
-          const bool kok = k < n2;
-          const int kj = k < n ? k : k - n;
-          b[u] = (kok && p < N) ? phid[2 * ((int64_t)p * n + kj) + (k < n ? 0 : 1)] : 0.0;
+          const int kj = min(k < n ? k : k - n, n - 1);
+          b[u] = phid[2 * ((int64_t)pc * n + kj) + (k < n ? 0 : 1)];
+          const int xr = st->rho[kj] * ld;
 #pragma unroll
           for (int tr = 0; tr < T2; ++tr) {
             const int r = tr * 16 + col;  // A row
-            double v = 0.0;
-            if (kok && r < n2) {
-              const cplx x = M[st->rho[kj] * ld + st->rhoinv[r < n ? r : r - n]];  // X[kj][ri]
-              // (r < n, k < n): Xr; (r < n, k >= n): -Xi; (r >= n, k < n): Xi; else Xr
-              v = (r < n) == (k < n) ? x.re : ((r < n) ? -x.im : x.im);
-            }
-            a[u][tr] = v;
+            const cplx x = M[xr + xcol[tr]];  // X[kj][ri]
+            // (r < n, k < n): Xr; (r < n, k >= n): -Xi; (r >= n, k < n): Xi; else Xr
+            const double v = (r < n) == (k < n) ? x.re : ((r < n) ? -x.im : x.im);
+            a[u][tr] = __builtin_unpredictable(k < n2) ? v : 0.0;
           }
         }
 #pragma unroll

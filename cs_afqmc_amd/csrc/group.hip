@@ -57,6 +57,12 @@ struct afqmc_group {
   unsigned char* pinned = nullptr;  // staging of the table upload
   size_t pinned_cap = 0;
   hipEvent_t upload_done = nullptr;  // the last upload has read `pinned`
+  // side streams (default priority) for the kernel instances of one phase, which then
+  // run side by side instead of one after the other; created on first use
+  static constexpr int kSide = 3;
+  hipStream_t side[kSide] = {};
+  hipEvent_t fork = nullptr, join[kSide] = {};
+  bool side_ready = false;
   ProfRec prof{};                    // afqmc_group_prof_*
 };
 
@@ -304,6 +310,15 @@ int build(afqmc_group* g) {
       g->taylor.push_back(L);
     }
   }
+  // largest instances first (work ~ walkers x k blocks x column tiles x threads; for the
+  // measurement ~ walkers x T2^2): they start on the group's stream, the smaller ones
+  // run beside them
+  std::stable_sort(g->taylor.begin(), g->taylor.end(), [](const Launch& a, const Launch& b) {
+    return (int64_t)a.nblocks * a.k1 * a.k2 * a.k3 > (int64_t)b.nblocks * b.k1 * b.k2 * b.k3;
+  });
+  std::stable_sort(g->meas.begin(), g->meas.end(), [](const Launch& a, const Launch& b) {
+    return (int64_t)a.nblocks * a.k0 * a.k0 > (int64_t)b.nblocks * b.k0 * b.k0;
+  });
 
   // ---- weights: one block per fragment
   {
@@ -367,13 +382,45 @@ const int* tstart(const afqmc_group* g, const Launch& L) {
   return reinterpret_cast<const int*>(g->dev_tab + L.start_off);
 }
 
-int run_measure(afqmc_group* g) {
-  for (const Launch& L : g->meas) {
-    const int rc = launch_measure_grouped(g->stream, L.k0, L.k1, targs<MeasureArgs>(g, L),
-                                          tstart(g, L), L.nprob, L.nblocks, L.shm);
-    if (rc) return rc;
+int ensure_side(afqmc_group* g) {
+  if (g->side_ready) return AFQMC_OK;
+  for (int i = 0; i < afqmc_group::kSide; ++i) {
+    HIP_TRY(hipStreamCreateWithFlags(&g->side[i], hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&g->join[i], hipEventDisableTiming));
   }
+  HIP_TRY(hipEventCreateWithFlags(&g->fork, hipEventDisableTiming));
+  g->side_ready = true;
   return AFQMC_OK;
+}
+
+// launch(L, stream) for every launch of one phase (ordered by decreasing work): the
+// first on the group's stream, the others round robin on side streams that wait for the
+// work queued so far and are joined back, so the instances overlap (their tails and the
+// small instances fill the chip while the large ones run).  Works under graph capture:
+// the side streams join the capture through the fork event.
+template <typename F>
+int run_spread(afqmc_group* g, const std::vector<Launch>& ls, F&& launch) {
+  const int n = (int)ls.size();
+  if (n <= 1) return n == 1 ? launch(ls[0], g->stream) : AFQMC_OK;
+  int rc = ensure_side(g);
+  if (rc) return rc;
+  const int used = std::min(n - 1, (int)afqmc_group::kSide);
+  HIP_TRY(hipEventRecord(g->fork, g->stream));
+  for (int i = 0; i < used; ++i) HIP_TRY(hipStreamWaitEvent(g->side[i], g->fork, 0));
+  for (int i = 0; i < n && !rc; ++i)
+    rc = launch(ls[i], i == 0 ? g->stream : g->side[(i - 1) % used]);
+  for (int i = 0; i < used; ++i) {
+    HIP_TRY(hipEventRecord(g->join[i], g->side[i]));
+    HIP_TRY(hipStreamWaitEvent(g->stream, g->join[i], 0));
+  }
+  return rc;
+}
+
+int run_measure(afqmc_group* g) {
+  return run_spread(g, g->meas, [&](const Launch& L, hipStream_t st) {
+    return launch_measure_grouped(st, L.k0, L.k1, targs<MeasureArgs>(g, L), tstart(g, L),
+                                  L.nprob, L.nblocks, L.shm);
+  });
 }
 
 int run_gemm(afqmc_group* g, const Launch& L, int out_f32) {
@@ -416,6 +463,14 @@ extern "C" int afqmc_group_destroy(afqmc_group* g) {
   if (g->dev_tab) (void)hipFree(g->dev_tab);
   if (g->pinned) (void)hipHostFree(g->pinned);
   if (g->upload_done) (void)hipEventDestroy(g->upload_done);
+  if (g->side_ready) {
+    for (int i = 0; i < afqmc_group::kSide; ++i) {
+      (void)hipStreamSynchronize(g->side[i]);
+      (void)hipStreamDestroy(g->side[i]);
+      (void)hipEventDestroy(g->join[i]);
+    }
+    (void)hipEventDestroy(g->fork);
+  }
   prof_free(&g->prof);
   delete g;
   return AFQMC_OK;
@@ -514,11 +569,11 @@ extern "C" int afqmc_group_propagate(afqmc_group* g, int32_t step, int32_t flags
   }
   {
     PhaseTimer t(&g->prof, g->stream, AFQMC_PH_TAYLOR);
-    for (const Launch& L : g->taylor) {
-      rc = launch_taylor_grouped(g->stream, L.k0, L.k1, L.k2, targs<TaylorArgs>(g, L),
-                                 tstart(g, L), L.nprob, L.nblocks, L.k3, L.shm);
-      if (rc) return rc;
-    }
+    rc = run_spread(g, g->taylor, [&](const Launch& L, hipStream_t st) {
+      return launch_taylor_grouped(st, L.k0, L.k1, L.k2, targs<TaylorArgs>(g, L), tstart(g, L),
+                                   L.nprob, L.nblocks, L.k3, L.shm);
+    });
+    if (rc) return rc;
   }
   {
     PhaseTimer t(&g->prof, g->stream, AFQMC_PH_OVERLAP);

@@ -906,6 +906,42 @@ __global__ __launch_bounds__(1024 / bfs_rt(CT)) void taylor_bfs_grouped_kernel(
       blockIdx.x - start[p]);
 }
 
+// Both spins of an unrestricted walker set that share one V (spin-summed Cholesky
+// vectors) in one launch.  Block b = 16 q + 8 s + r takes spin s of walker 8 q + r:
+// blocks are dealt round-robin to the 8 XCDs (block b on XCD b % 8), so the two blocks
+// of a walker run on the same XCD, one dispatch round apart, and the second one's V
+// rows mostly come from that XCD's L2 instead of HBM.
+template <int CT>
+__global__ __launch_bounds__(1024 / bfs_rt(CT)) void taylor_bfs_spins_kernel(
+    int N, int n0, int n1, int n_terms_k, int W, cplx* __restrict__ w0, cplx* __restrict__ w1,
+    const uint16_t* __restrict__ e0h, const uint16_t* __restrict__ e0l,
+    const uint16_t* __restrict__ e1h, const uint16_t* __restrict__ e1l,
+    const float* __restrict__ vhs, int nv) {
+  const int b = blockIdx.x;
+  const int s = (b >> 3) & 1, w = 8 * (b >> 4) + (b & 7);
+  if (w >= W) return;  // (whole workgroup: the tail of the last group of 8 walkers)
+  taylor_bfs_body<CT, bfs_rt(CT)>(N, s ? n1 : n0, n_terms_k, s ? w1 : w0, s ? e1h : e0h,
+                                  s ? e1l : e0l, vhs, nv, w);
+}
+
+template <int CT>
+int launch_bfs_spins(afqmc_ctx* c, int W, cplx* wup, cplx* wdn) {
+  const int N = c->d.norb;
+  if (N > 256 || c->nocc[0] > 16 * CT || c->nocc[1] > 16 * CT || 32 * ((N + 31) / 32) != c->Np32)
+    return afqmc_set_error(AFQMC_ERR_ARG, "taylor bf16x3 streaming: N %d column tiles %d", N, CT);
+  const size_t shm = bfs_shm(N, CT);
+  static int lds_limit = 0;
+  const int rc = ensure_dyn_lds((const void*)taylor_bfs_spins_kernel<CT>, shm, &lds_limit);
+  if (rc) return rc;
+  const int groups = (W + 7) / 8;
+  hipLaunchKernelGGL((taylor_bfs_spins_kernel<CT>), dim3(16 * groups), dim3(bfs_threads(N, CT)),
+                     shm, c->stream, N, c->nocc[0], c->nocc[1], c->d.n_exp_terms - 1, W, wup,
+                     wdn, c->emi_bf[0][0], c->emi_bf[0][1], c->emi_bf[1][0], c->emi_bf[1][1],
+                     vhs_f32_of(c, 0), c->vhs_nv);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
 template <int CT>
 int launch_bfs(afqmc_ctx* c, int W, int n, cplx* wk, int s) {
   const int N = c->d.norb;
@@ -975,6 +1011,17 @@ void taylor_plan(const afqmc_ctx* c, int s, TaylorPlan* pl) {
 
 int launch_taylor(afqmc_ctx* c, int W, cplx* wup, cplx* wdn) {
   const int N = c->d.norb;
+  if (c->taylor_bf && c->nspin == 2 && vhs_f32_of(c, 0) == vhs_f32_of(c, 1) &&
+      !(N <= 128 && c->nocc[0] <= 32) && !(N <= 128 && c->nocc[1] <= 32) &&
+      (c->nocc[0] + 15) / 16 == (c->nocc[1] + 15) / 16) {
+    // both spins streaming with one shared V: one launch, the spins of a walker side by side
+    switch ((c->nocc[0] + 15) / 16) {
+      case 1: return launch_bfs_spins<1>(c, W, wup, wdn);
+      case 2: return launch_bfs_spins<2>(c, W, wup, wdn);
+      case 3: return launch_bfs_spins<3>(c, W, wup, wdn);
+      default: return launch_bfs_spins<4>(c, W, wup, wdn);
+    }
+  }
   if (c->taylor_bf) {
     for (int s = 0; s < c->nspin; ++s) {
       const int n = c->nocc[s];

@@ -52,6 +52,9 @@ EXPORTS = (
     "afqmc_group_prof_phases",
     "afqmc_group_prof_collect",
     "afqmc_group_tune",
+    "afqmc_group_orthonormalize",
+    "afqmc_group_overlap",
+    "afqmc_group_energy",
 )
 
 # phase indices of afqmc_prof_collect (AFQMC_PH_* in the header)
@@ -135,6 +138,9 @@ def _declare(lib):
     lib.afqmc_group_size.argtypes = [_vp]
     lib.afqmc_group_propagate.argtypes = [_vp, _i32, _i32]
     lib.afqmc_group_tune.argtypes = [_vp, _vp]
+    lib.afqmc_group_orthonormalize.argtypes = [_vp]
+    lib.afqmc_group_overlap.argtypes = [_vp]
+    lib.afqmc_group_energy.argtypes = [_vp, _vp]
     lib.afqmc_group_prof_enable.argtypes = [_vp, _i32]
     lib.afqmc_group_prof_phases.argtypes = [_vp, ctypes.c_uint32]
     lib.afqmc_group_prof_collect.argtypes = [_vp, _vp, _vp]

@@ -5,7 +5,9 @@ fragment, each a small problem (norb, nocc, nchol, walkers all different).  Here
 fragment keeps its own device context and resident state, and one grouped step
 (``DeviceGroup.propagate``, afqmc_group_propagate) advances all of them: each phase of
 the step is a single launch over every fragment's walkers, so small fragments fill the
-GPU together.
+GPU together.  At the block end the QR and the overlaps are grouped launches too; each
+fragment's energy, statistics and resampling run on side streams, where they overlap
+one another (faster in the captured block than the grouped energy chain).
 
 A fragment's block is one sampler call of the fragment driver
 (lno_afqmc/ccsd_pt2/run_afqmc_nompi.py:83-90 around sampling.py:319-341 with one
@@ -104,10 +106,20 @@ class Fragment:
         pd = self.prop_data
         self.dev.orthonormalize(pd["walkers"])
         self.dev.overlap(pd["walkers"], out=pd["overlaps"])
-        en = self.dev.energy(pd["walkers"], out=self.energies)
-        self.dev.block_stats(en, pd["weights"], self.scalars, self.slot)
-        self.dev.sr_key(pd["walkers"], pd["weights"], self.key, idx_out=self.idx)
+        self.dev.energy(pd["walkers"], out=self.energies)
+        self.end_block_tail()
         self.dev.overlap(pd["walkers"], out=pd["overlaps"])
+
+    def end_block_energy_tail(self):
+        self.dev.energy(self.prop_data["walkers"], out=self.energies)
+        self.end_block_tail()
+
+    def end_block_tail(self):
+        """The part of end_block between the energies and the last overlaps: block
+        statistics, SR, e_estimate mix (the batch runs the rest as grouped launches)."""
+        pd = self.prop_data
+        self.dev.block_stats(self.energies, pd["weights"], self.scalars, self.slot)
+        self.dev.sr_key(pd["walkers"], pd["weights"], self.key, idx_out=self.idx)
         self.dev.estimate_mix(self.scalars, self.slot)
 
     def record(self, stats_row, sr_idx=None):
@@ -249,7 +261,13 @@ class FragmentBatch:
         self._per_fragment(lambda f: f.begin_block())
         for s in range(self.fragments[0].n_prop_steps):
             self.group.propagate(s, reuse_green=s > 0)
-        self._per_fragment(lambda f: f.end_block())
+        # block ends: QR and overlaps of every fragment as grouped launches, then each
+        # fragment's energy, statistics, SR and estimate on the side streams, and the
+        # overlaps of the resampled walkers grouped again
+        self.group.orthonormalize()
+        self.group.overlap()
+        self._per_fragment(lambda f: f.end_block_energy_tail())
+        self.group.overlap()
 
     def run_block(self):
         """One block of every fragment: fields, n_prop_steps grouped steps, block ends."""

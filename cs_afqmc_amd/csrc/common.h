@@ -435,6 +435,61 @@ struct GemmArgs {
   void* C;
 };
 
+// block ends: QR, overlap products, local energies
+struct QrArgs {
+  int m, n, use_lds;
+  cplx* walkers;
+};
+struct OvcArgs {
+  int W, nspin;
+  const cplx* dets;
+  cplx* out;
+};
+struct ExchArgs {
+  int G, n, K, W, gpt, wpt, g_tiles, gc;
+  const void* A;   // float64, or the bf16 hi plane (half-rotated Cholesky vectors)
+  const void* Al;  // bf16 lo plane (split-bf16 only)
+  const void* B;   // float64 Green's function, or its bf16 hi plane
+  const void* Bl;
+  cplx* xpart;
+};
+struct FinishArgs {
+  int G, N, nspin, mult, n0, n1, nparts, gt0, gt1, W;
+  int64_t part_stride;
+  const double* fb;
+  const cplx* xpart;
+  const double* green0;
+  const double* green1;
+  const double* rot_h1_0;
+  const double* rot_h1_1;
+  double h0;
+  cplx* energies;
+};
+// exchange kernel variants (launch_exchange_grouped)
+enum ExchKind {
+  EXCH_BF16X3_WN2 = 0,
+  EXCH_BF16X3_WN4 = 1,
+  EXCH_F64_V1 = 2,
+  EXCH_F64_V2 = 3,
+  EXCH_F32_V1 = 4,
+  EXCH_F32_V2 = 5
+};
+// the exchange launch of context c, spin s, over W walkers (xpart: c's partial sums of
+// that spin), and the finishing launch's arguments; used by the single and grouped forms
+void exchange_plan(const afqmc_ctx* c, int s, int W, ExchArgs* a, int* kind, int* nblocks);
+void energy_finish_args(const afqmc_ctx* c, int W, cplx* energies, FinishArgs* a);
+int launch_exchange_grouped(hipStream_t st, int kind, const ExchArgs* args, const int* start,
+                            int nprob, int nblocks);
+int launch_energy_finish_grouped(hipStream_t st, const FinishArgs* args, const int* start,
+                                 int nprob, int nblocks);
+// LDS bytes of the QR walker copy (0: factorised in place in global memory)
+size_t qr_lds_bytes(int m, int n);
+int launch_qr_grouped(hipStream_t st, const QrArgs* args, const int* start, int nprob,
+                      int nblocks, size_t shm);
+int launch_overlap_combine_grouped(hipStream_t st, const OvcArgs* args, const int* start,
+                                   int nprob, int nblocks);
+constexpr int OVC_NT = 256;  // threads (walkers) per overlap-combine block
+
 enum TaylorKind {
   TAYLOR_REG_F64 = 0,
   TAYLOR_STREAM_F64 = 1,

@@ -134,9 +134,10 @@ constexpr int EF_F_PARTS = EF_F_BYTES + (ENER_NT / 64) * 16;
 constexpr int EF_LDS = EF_OPS_BYTES > EF_F_PARTS ? EF_OPS_BYTES : EF_F_PARTS;
 
 template <int VEC, bool F32>
-__global__ __launch_bounds__(ENER_NT) void exchange_f64_kernel(
+__device__ __forceinline__ void exchange_f64_body(
     int G, int n, int N, int W, int gpt, int wpt, int g_tiles, int gc,
-    const double* __restrict__ A, const double* __restrict__ B, cplx* __restrict__ xpart) {
+    const double* __restrict__ A, const double* __restrict__ B, cplx* __restrict__ xpart,
+    const int bx) {
   typedef typename std::conditional<F32, float, double>::type T;
   typedef typename std::conditional<F32, float4_t, double4_t>::type AccT;
   constexpr int BM = EF_BM, BN = EF_BN, BK = EF_BK;
@@ -150,7 +151,7 @@ __global__ __launch_bounds__(ENER_NT) void exchange_f64_kernel(
   T* Fs = reinterpret_cast<T*>(lds);
 
   int gt, wt;
-  exchange_tile_of(blockIdx.x, g_tiles, (W + wpt - 1) / wpt, gc, &gt, &wt);
+  exchange_tile_of(bx, g_tiles, (W + wpt - 1) / wpt, gc, &gt, &wt);
   const int g0 = gt * gpt, w0 = wt * wpt;
   const int M = G * n, NB = 2 * W * n;
   const int m0 = g0 * n, n0 = w0 * 2 * n;
@@ -253,6 +254,23 @@ __global__ __launch_bounds__(ENER_NT) void exchange_f64_kernel(
                    reinterpret_cast<cplx*>(lds + EF_F_BYTES));
 }
 
+template <int VEC, bool F32>
+__global__ __launch_bounds__(ENER_NT) void exchange_f64_kernel(
+    int G, int n, int N, int W, int gpt, int wpt, int g_tiles, int gc,
+    const double* __restrict__ A, const double* __restrict__ B, cplx* __restrict__ xpart) {
+  exchange_f64_body<VEC, F32>(G, n, N, W, gpt, wpt, g_tiles, gc, A, B, xpart, blockIdx.x);
+}
+
+template <int VEC, bool F32>
+__global__ __launch_bounds__(ENER_NT) void exchange_f64_grouped_kernel(
+    const ExchArgs* __restrict__ args, const int* __restrict__ start, int nprob) {
+  const int p = group_of(start, nprob, blockIdx.x);
+  const ExchArgs a = args[p];
+  exchange_f64_body<VEC, F32>(a.G, a.n, a.K, a.W, a.gpt, a.wpt, a.g_tiles, a.gc,
+                              static_cast<const double*>(a.A), static_cast<const double*>(a.B),
+                              a.xpart, blockIdx.x - start[p]);
+}
+
 // split-bf16 tile 128 x 64 WN on v_mfma_f32_16x16x32_bf16 (2 x WN waves, each 64 x 64),
 // BK = 32; operand planes have K = Np32 (zero padded), so there is no k tail.  WN = 2
 // (128 columns, 4 waves) or, where it holds more whole walkers (2n columns each) per
@@ -269,11 +287,11 @@ struct EbTile {
 };
 
 template <int WN>
-__global__ __launch_bounds__(128 * WN) void exchange_bf16x3_kernel(
+__device__ __forceinline__ void exchange_bf16x3_body(
     int G, int n, int K, int W, int gpt, int wpt, int g_tiles, int gc,
     const uint16_t* __restrict__ Ah,
     const uint16_t* __restrict__ Al, const uint16_t* __restrict__ Bh,
-    const uint16_t* __restrict__ Bl, cplx* __restrict__ xpart) {
+    const uint16_t* __restrict__ Bl, cplx* __restrict__ xpart, const int bx) {
   typedef EbTile<WN> Tile;
   constexpr int NT = Tile::NT, BM = EB_BM, BN = Tile::BN, LDSW = EB_LDSW;
   constexpr int TM = BM / 2 / 16, TN = BN / WN / 16;
@@ -287,7 +305,7 @@ __global__ __launch_bounds__(128 * WN) void exchange_bf16x3_kernel(
   float* Fs = reinterpret_cast<float*>(lds);
 
   int gt, wt;
-  exchange_tile_of(blockIdx.x, g_tiles, (W + wpt - 1) / wpt, gc, &gt, &wt);
+  exchange_tile_of(bx, g_tiles, (W + wpt - 1) / wpt, gc, &gt, &wt);
   const int g0 = gt * gpt, w0 = wt * wpt;
   const int M = G * n, NB = 2 * W * n;
   const int m0 = g0 * n, n0 = w0 * 2 * n;
@@ -396,16 +414,36 @@ __global__ __launch_bounds__(128 * WN) void exchange_bf16x3_kernel(
                            reinterpret_cast<cplx*>(lds + Tile::F));
 }
 
+template <int WN>
+__global__ __launch_bounds__(128 * WN) void exchange_bf16x3_kernel(
+    int G, int n, int K, int W, int gpt, int wpt, int g_tiles, int gc,
+    const uint16_t* __restrict__ Ah, const uint16_t* __restrict__ Al,
+    const uint16_t* __restrict__ Bh, const uint16_t* __restrict__ Bl, cplx* __restrict__ xpart) {
+  exchange_bf16x3_body<WN>(G, n, K, W, gpt, wpt, g_tiles, gc, Ah, Al, Bh, Bl, xpart,
+                           blockIdx.x);
+}
+
+template <int WN>
+__global__ __launch_bounds__(128 * WN) void exchange_bf16x3_grouped_kernel(
+    const ExchArgs* __restrict__ args, const int* __restrict__ start, int nprob) {
+  const int p = group_of(start, nprob, blockIdx.x);
+  const ExchArgs a = args[p];
+  exchange_bf16x3_body<WN>(a.G, a.n, a.K, a.W, a.gpt, a.wpt, a.g_tiles, a.gc,
+                           static_cast<const uint16_t*>(a.A), static_cast<const uint16_t*>(a.Al),
+                           static_cast<const uint16_t*>(a.B), static_cast<const uint16_t*>(a.Bl),
+                           a.xpart, blockIdx.x - start[p]);
+}
+
 // E_w = sum_g (c_g^2 - x_g) / 2 + sum_s m_s sum G_s rot_h1_s + h0 with c_g the summed
 // force-bias slices and x_g = sum over spins and g-tiles (in order) of m_s * xpart.
-__global__ __launch_bounds__(ENER_NT) void energy_finish_kernel(
+__device__ __forceinline__ void energy_finish_body(
     int G, int N, int nspin, int mult, int n0, int n1, const double* __restrict__ fb, int nparts,
     int64_t part_stride, const cplx* __restrict__ xpart, int gt0, int gt1, int W,
     const double* __restrict__ green0, const double* __restrict__ green1,
     const double* __restrict__ rot_h1_0, const double* __restrict__ rot_h1_1, double h0,
-    cplx* __restrict__ energies) {
+    cplx* __restrict__ energies, const int bx) {
   __shared__ double red[ENER_NT / 64];
-  const int w = blockIdx.x;
+  const int w = bx;
   double c2r = 0.0, c2i = 0.0;
   for (int g = threadIdx.x; g < G; g += ENER_NT) {
     const double* p = fb + (int64_t)w * 2 * G + g;
@@ -451,6 +489,25 @@ __global__ __launch_bounds__(ENER_NT) void energy_finish_kernel(
     energies[w] = cplx{(C2r - Xr) / 2.0 + E1r + h0, (C2i - Xi) / 2.0 + E1i};
 }
 
+__global__ __launch_bounds__(ENER_NT) void energy_finish_kernel(
+    int G, int N, int nspin, int mult, int n0, int n1, const double* __restrict__ fb, int nparts,
+    int64_t part_stride, const cplx* __restrict__ xpart, int gt0, int gt1, int W,
+    const double* __restrict__ green0, const double* __restrict__ green1,
+    const double* __restrict__ rot_h1_0, const double* __restrict__ rot_h1_1, double h0,
+    cplx* __restrict__ energies) {
+  energy_finish_body(G, N, nspin, mult, n0, n1, fb, nparts, part_stride, xpart, gt0, gt1, W,
+                     green0, green1, rot_h1_0, rot_h1_1, h0, energies, blockIdx.x);
+}
+
+__global__ __launch_bounds__(ENER_NT) void energy_finish_grouped_kernel(
+    const FinishArgs* __restrict__ args, const int* __restrict__ start, int nprob) {
+  const int p = group_of(start, nprob, blockIdx.x);
+  const FinishArgs a = args[p];
+  energy_finish_body(a.G, a.N, a.nspin, a.mult, a.n0, a.n1, a.fb, a.nparts, a.part_stride,
+                     a.xpart, a.gt0, a.gt1, a.W, a.green0, a.green1, a.rot_h1_0, a.rot_h1_1,
+                     a.h0, a.energies, blockIdx.x - start[p]);
+}
+
 }  // namespace
 
 // Waves across the split-bf16 exchange tile for n occupied orbitals: 4 (256 columns)
@@ -468,6 +525,104 @@ int energy_g_tiles(const afqmc_ctx* c, int s) {
   return (c->d.nchol + gpt - 1) / gpt;
 }
 
+void exchange_plan(const afqmc_ctx* c, int s, int W, ExchArgs* a, int* kind, int* nblocks) {
+  const int N = c->d.norb, G = c->d.nchol, n = c->nocc[s];
+  cplx* xp = c->energy_x;
+  for (int t = 0; t < s; ++t) xp += (int64_t)energy_g_tiles(c, t) * W;
+  *a = ExchArgs{};
+  a->G = G;
+  a->n = n;
+  a->W = W;
+  a->xpart = xp;
+  if (c->mixed) {
+    const int wn = exchange_wn(n);
+    a->K = c->Np32;
+    a->gpt = EB_BM / n;
+    a->wpt = 64 * wn / (2 * n);
+    a->g_tiles = (G + a->gpt - 1) / a->gpt;
+    a->gc = exchange_chunk(a->g_tiles, (int64_t)a->gpt * n * c->Np32 * 4);
+    a->A = c->rot_chol_bf[s][0];
+    a->Al = c->rot_chol_bf[s][1];
+    a->B = c->green_bf[s][0];
+    a->Bl = c->green_bf[s][1];
+    *kind = wn == 4 ? EXCH_BF16X3_WN4 : EXCH_BF16X3_WN2;
+  } else {
+    const bool vec2 = (N % 2 == 0);
+    a->K = N;
+    a->gpt = EF_BM / n;
+    a->wpt = EF_BN / (2 * n);
+    a->g_tiles = (G + a->gpt - 1) / a->gpt;
+    a->gc = exchange_chunk(a->g_tiles, (int64_t)a->gpt * n * N * 8);
+    a->A = c->rot_chol[s];
+    a->B = c->green[s];
+    *kind = c->f32 ? (vec2 ? EXCH_F32_V2 : EXCH_F32_V1) : (vec2 ? EXCH_F64_V2 : EXCH_F64_V1);
+  }
+  *nblocks = a->g_tiles * ((W + a->wpt - 1) / a->wpt);
+}
+
+void energy_finish_args(const afqmc_ctx* c, int W, cplx* energies, FinishArgs* a) {
+  *a = FinishArgs{};
+  a->G = c->d.nchol;
+  a->N = c->d.norb;
+  a->nspin = c->nspin;
+  a->mult = c->mult;
+  a->n0 = c->nocc[0];
+  a->n1 = c->nocc[1];
+  a->nparts = c->fb_parts;
+  a->gt0 = energy_g_tiles(c, 0);
+  a->gt1 = c->nspin > 1 ? energy_g_tiles(c, 1) : 0;
+  a->W = W;
+  a->part_stride = (int64_t)2 * W * c->d.nchol;
+  a->fb = c->fb;
+  a->xpart = c->energy_x;
+  a->green0 = c->green[0];
+  a->green1 = c->green[1];
+  a->rot_h1_0 = c->rot_h1[0];
+  a->rot_h1_1 = c->rot_h1[1];
+  a->h0 = c->h0;
+  a->energies = energies;
+}
+
+int launch_exchange_grouped(hipStream_t st, int kind, const ExchArgs* args, const int* start,
+                            int nprob, int nblocks) {
+  switch (kind) {
+    case EXCH_BF16X3_WN2:
+      hipLaunchKernelGGL(exchange_bf16x3_grouped_kernel<2>, dim3(nblocks), dim3(EbTile<2>::NT),
+                         0, st, args, start, nprob);
+      break;
+    case EXCH_BF16X3_WN4:
+      hipLaunchKernelGGL(exchange_bf16x3_grouped_kernel<4>, dim3(nblocks), dim3(EbTile<4>::NT),
+                         0, st, args, start, nprob);
+      break;
+    case EXCH_F64_V1:
+      hipLaunchKernelGGL((exchange_f64_grouped_kernel<1, false>), dim3(nblocks), dim3(ENER_NT),
+                         0, st, args, start, nprob);
+      break;
+    case EXCH_F64_V2:
+      hipLaunchKernelGGL((exchange_f64_grouped_kernel<2, false>), dim3(nblocks), dim3(ENER_NT),
+                         0, st, args, start, nprob);
+      break;
+    case EXCH_F32_V1:
+      hipLaunchKernelGGL((exchange_f64_grouped_kernel<1, true>), dim3(nblocks), dim3(ENER_NT),
+                         0, st, args, start, nprob);
+      break;
+    default:
+      hipLaunchKernelGGL((exchange_f64_grouped_kernel<2, true>), dim3(nblocks), dim3(ENER_NT),
+                         0, st, args, start, nprob);
+      break;
+  }
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+int launch_energy_finish_grouped(hipStream_t st, const FinishArgs* args, const int* start,
+                                 int nprob, int nblocks) {
+  hipLaunchKernelGGL(energy_finish_grouped_kernel, dim3(nblocks), dim3(ENER_NT), 0, st, args,
+                     start, nprob);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
 // Needs the float64 Green's function of every spin in c->green (and, in the mixed mode,
 // its split-bf16 planes) and the float64 force bias in c->fb (c->fb_parts slices).
 int launch_energy(afqmc_ctx* c, int W, cplx* energies) {
@@ -475,56 +630,54 @@ int launch_energy(afqmc_ctx* c, int W, cplx* energies) {
                 "a (g, w) block must fit one tile");
   static_assert(AFQMC_MAX_NOCC <= EB_BM && 2 * AFQMC_MAX_NOCC <= EbTile<2>::BN,
                 "a (g, w) block must fit one tile");
-  const int N = c->d.norb, G = c->d.nchol;
-  int gts[2] = {0, 0};
-  cplx* xp = c->energy_x;
   for (int s = 0; s < c->nspin; ++s) {
-    const int n = c->nocc[s];
-    if (c->mixed) {
-      const int wn = exchange_wn(n);
-      const int gpt = EB_BM / n, wpt = 64 * wn / (2 * n);
-      const int g_tiles = (G + gpt - 1) / gpt, w_tiles = (W + wpt - 1) / wpt;
-      gts[s] = g_tiles;
-      const int gc = exchange_chunk(g_tiles, (int64_t)gpt * n * c->Np32 * 4);
-      const dim3 grid(g_tiles * w_tiles);
-      if (wn == 4)
-        hipLaunchKernelGGL(exchange_bf16x3_kernel<4>, grid, dim3(EbTile<4>::NT), 0, c->stream,
-                           G, n, c->Np32, W, gpt, wpt, g_tiles, gc, c->rot_chol_bf[s][0],
-                           c->rot_chol_bf[s][1], c->green_bf[s][0], c->green_bf[s][1], xp);
-      else
-        hipLaunchKernelGGL(exchange_bf16x3_kernel<2>, grid, dim3(EbTile<2>::NT), 0, c->stream,
-                           G, n, c->Np32, W, gpt, wpt, g_tiles, gc, c->rot_chol_bf[s][0],
-                           c->rot_chol_bf[s][1], c->green_bf[s][0], c->green_bf[s][1], xp);
-    } else {
-      const int gpt = EF_BM / n, wpt = EF_BN / (2 * n);
-      const int g_tiles = (G + gpt - 1) / gpt, w_tiles = (W + wpt - 1) / wpt;
-      gts[s] = g_tiles;
-      const bool vec2 = (N % 2 == 0);
-      const int gc = exchange_chunk(g_tiles, (int64_t)gpt * n * N * 8);
-      const dim3 grid(g_tiles * w_tiles), block(ENER_NT);
-      const double* A = c->rot_chol[s];
-      const double* B = c->green[s];
-      if (c->f32 && vec2)
-        hipLaunchKernelGGL((exchange_f64_kernel<2, true>), grid, block, 0, c->stream, G, n, N,
-                           W, gpt, wpt, g_tiles, gc, A, B, xp);
-      else if (c->f32)
-        hipLaunchKernelGGL((exchange_f64_kernel<1, true>), grid, block, 0, c->stream, G, n, N,
-                           W, gpt, wpt, g_tiles, gc, A, B, xp);
-      else if (vec2)
-        hipLaunchKernelGGL((exchange_f64_kernel<2, false>), grid, block, 0, c->stream, G, n, N,
-                           W, gpt, wpt, g_tiles, gc, A, B, xp);
-      else
-        hipLaunchKernelGGL((exchange_f64_kernel<1, false>), grid, block, 0, c->stream, G, n, N,
-                           W, gpt, wpt, g_tiles, gc, A, B, xp);
+    ExchArgs a;
+    int kind, nb;
+    exchange_plan(c, s, W, &a, &kind, &nb);
+    const dim3 grid(nb);
+    switch (kind) {
+      case EXCH_BF16X3_WN4:
+      case EXCH_BF16X3_WN2: {
+        const uint16_t* Ah = static_cast<const uint16_t*>(a.A);
+        const uint16_t* Al = static_cast<const uint16_t*>(a.Al);
+        const uint16_t* Bh = static_cast<const uint16_t*>(a.B);
+        const uint16_t* Bl = static_cast<const uint16_t*>(a.Bl);
+        if (kind == EXCH_BF16X3_WN4)
+          hipLaunchKernelGGL(exchange_bf16x3_kernel<4>, grid, dim3(EbTile<4>::NT), 0, c->stream,
+                             a.G, a.n, a.K, W, a.gpt, a.wpt, a.g_tiles, a.gc, Ah, Al, Bh, Bl,
+                             a.xpart);
+        else
+          hipLaunchKernelGGL(exchange_bf16x3_kernel<2>, grid, dim3(EbTile<2>::NT), 0, c->stream,
+                             a.G, a.n, a.K, W, a.gpt, a.wpt, a.g_tiles, a.gc, Ah, Al, Bh, Bl,
+                             a.xpart);
+        break;
+      }
+      default: {
+        const double* A = static_cast<const double*>(a.A);
+        const double* B = static_cast<const double*>(a.B);
+        const dim3 block(ENER_NT);
+        if (kind == EXCH_F32_V2)
+          hipLaunchKernelGGL((exchange_f64_kernel<2, true>), grid, block, 0, c->stream, a.G,
+                             a.n, a.K, W, a.gpt, a.wpt, a.g_tiles, a.gc, A, B, a.xpart);
+        else if (kind == EXCH_F32_V1)
+          hipLaunchKernelGGL((exchange_f64_kernel<1, true>), grid, block, 0, c->stream, a.G,
+                             a.n, a.K, W, a.gpt, a.wpt, a.g_tiles, a.gc, A, B, a.xpart);
+        else if (kind == EXCH_F64_V2)
+          hipLaunchKernelGGL((exchange_f64_kernel<2, false>), grid, block, 0, c->stream, a.G,
+                             a.n, a.K, W, a.gpt, a.wpt, a.g_tiles, a.gc, A, B, a.xpart);
+        else
+          hipLaunchKernelGGL((exchange_f64_kernel<1, false>), grid, block, 0, c->stream, a.G,
+                             a.n, a.K, W, a.gpt, a.wpt, a.g_tiles, a.gc, A, B, a.xpart);
+        break;
+      }
     }
     LAUNCH_CHECK();
-    xp += (int64_t)gts[s] * W;
   }
-  const int64_t slice = (int64_t)2 * W * G;
-  hipLaunchKernelGGL(energy_finish_kernel, dim3(W), dim3(ENER_NT), 0, c->stream, G, N, c->nspin,
-                     c->mult, c->nocc[0], c->nocc[1], c->fb, c->fb_parts, slice, c->energy_x,
-                     gts[0], gts[1], W, c->green[0], c->green[1], c->rot_h1[0], c->rot_h1[1],
-                     c->h0, energies);
+  FinishArgs f;
+  energy_finish_args(c, W, energies, &f);
+  hipLaunchKernelGGL(energy_finish_kernel, dim3(W), dim3(ENER_NT), 0, c->stream, f.G, f.N,
+                     f.nspin, f.mult, f.n0, f.n1, f.fb, f.nparts, f.part_stride, f.xpart, f.gt0,
+                     f.gt1, W, f.green0, f.green1, f.rot_h1_0, f.rot_h1_1, f.h0, energies);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }

@@ -63,6 +63,12 @@ struct afqmc_group {
   hipStream_t side[kSide] = {};
   hipEvent_t fork = nullptr, join[kSide] = {};
   bool side_ready = false;
+  // block ends (afqmc_group_orthonormalize / _overlap / _energy): tables built with the
+  // step's; en_ptrs are the energies outputs the finishing table was built for
+  Launch qr{}, ovc{}, fb64{}, fin{};
+  std::vector<Launch> meas_ov, meas_en, exch;
+  bool fb64_vec2 = true;
+  std::vector<void*> en_ptrs;
   ProfRec prof{};                    // afqmc_group_prof_*
 };
 
@@ -346,6 +352,198 @@ int build(afqmc_group* g) {
     put_table(tab, g->weights, args, start);
   }
 
+  // ---- block ends
+  {
+    // QR: one block per walker and spin
+    std::vector<QrArgs> qa;
+    std::vector<int> qs;
+    size_t qshm = 0;
+    int b = 0;
+    for (const Member& m : g->mem) {
+      afqmc_ctx* c = m.c;
+      for (int s = 0; s < c->nspin; ++s) {
+        const size_t lb = qr_lds_bytes(c->d.norb, c->nocc[s]);
+        QrArgs a{};
+        a.m = c->d.norb;
+        a.n = c->nocc[s];
+        a.use_lds = lb > 0 ? 1 : 0;
+        a.walkers = s == 0 ? m.wup : m.wdn;
+        qshm = std::max(qshm, lb);
+        qa.push_back(a);
+        qs.push_back(b);
+        b += c->d.n_walkers;
+      }
+    }
+    g->qr = Launch{};
+    g->qr.shm = qshm;
+    g->qr.nprob = (int)qa.size();
+    g->qr.nblocks = b;
+    put_table(tab, g->qr, qa, qs);
+  }
+  {
+    // overlaps (determinants only) and Green's functions for the energy (float64, plus
+    // the split planes in the mixed mode): the step's measurement grouping
+    for (int which = 0; which < 2; ++which) {
+      std::vector<Launch>& out = which == 0 ? g->meas_ov : g->meas_en;
+      out.clear();
+      std::map<std::pair<int, int>, std::vector<std::pair<MeasureArgs, int>>> by_key;
+      std::map<std::pair<int, int>, size_t> shm;
+      for (const Member& m : g->mem) {
+        afqmc_ctx* c = m.c;
+        for (int s = 0; s < c->nspin; ++s) {
+          int t2, stage;
+          size_t sh;
+          measure_plan(c, s, &t2, &stage, &sh);
+          MeasureArgs a{};
+          a.N = c->d.norb;
+          a.n = c->nocc[s];
+          a.spin = s;
+          a.ld16 = c->Np32;
+          a.walkers = s == 0 ? m.wup : m.wdn;
+          a.trial = c->trial[s];
+          a.dets = c->dets;
+          a.green = which == 1 ? c->green[s] : nullptr;
+          a.ghi = (which == 1 && c->mixed) ? c->green_bf[s][0] : nullptr;
+          a.glo = (which == 1 && c->mixed) ? c->green_bf[s][1] : nullptr;
+          const auto key = std::make_pair(t2, stage);
+          by_key[key].push_back({a, c->d.n_walkers});
+          shm[key] = std::max(shm[key], sh);
+        }
+      }
+      for (auto& kv : by_key) {
+        Launch L{};
+        L.k0 = kv.first.first;
+        L.k1 = kv.first.second;
+        L.shm = shm[kv.first];
+        std::vector<MeasureArgs> args;
+        std::vector<int> start;
+        int b = 0;
+        for (auto& pa : kv.second) {
+          args.push_back(pa.first);
+          start.push_back(b);
+          b += pa.second;
+        }
+        L.nprob = (int)args.size();
+        L.nblocks = b;
+        put_table(tab, L, args, start);
+        out.push_back(L);
+      }
+      std::stable_sort(out.begin(), out.end(), [](const Launch& x, const Launch& y) {
+        return (int64_t)x.nblocks * x.k0 * x.k0 > (int64_t)y.nblocks * y.k0 * y.k0;
+      });
+    }
+  }
+  {
+    // overlap = product of the spins' determinants
+    std::vector<OvcArgs> oa;
+    std::vector<int> os;
+    int b = 0;
+    for (const Member& m : g->mem) {
+      OvcArgs a{};
+      a.W = m.c->d.n_walkers;
+      a.nspin = m.c->nspin;
+      a.dets = m.c->dets;
+      a.out = m.overlaps;
+      oa.push_back(a);
+      os.push_back(b);
+      b += (a.W + OVC_NT - 1) / OVC_NT;
+    }
+    g->ovc = Launch{};
+    g->ovc.nprob = (int)oa.size();
+    g->ovc.nblocks = b;
+    put_table(tab, g->ovc, oa, os);
+  }
+  {
+    // c_g of the energy: float64 force-bias products, part s of fb from spin s
+    std::vector<GemmArgs> args;
+    std::vector<int> start;
+    int64_t t128 = 0;
+    for (const Member& m : g->mem)
+      for (int s = 0; s < m.c->nspin; ++s)
+        t128 += (int64_t)((2 * m.c->d.n_walkers + 127) / 128) * ((m.c->d.nchol + 127) / 128);
+    const int bm = heuristic_bm(t128);
+    g->fb64_vec2 = true;
+    int b = 0;
+    for (const Member& m : g->mem) {
+      afqmc_ctx* c = m.c;
+      const int W = c->d.n_walkers, G = c->d.nchol;
+      for (int s = 0; s < c->nspin; ++s) {
+        GemmArgs a{};
+        a.M = 2 * W;
+        a.N = G;
+        a.K = c->kdim[s];
+        a.alpha = (float)c->mult;
+        a.A = c->green[s];
+        a.B = c->rot_chol[s];
+        a.C = c->fb + (int64_t)s * 2 * W * G;
+        a.lda = a.ldb = a.K;
+        a.ldc = G;
+        if (a.K % 2) g->fb64_vec2 = false;
+        a.tiles_m = (a.M + bm - 1) / bm;
+        a.tiles_n = (a.N + bm - 1) / bm;
+        args.push_back(a);
+        start.push_back(b);
+        b += a.tiles_m * a.tiles_n;
+      }
+    }
+    g->fb64 = Launch{};
+    g->fb64.k0 = bm;
+    g->fb64.nprob = (int)args.size();
+    g->fb64.nblocks = b;
+    put_table(tab, g->fb64, args, start);
+  }
+  {
+    // exchange sums, one launch per kernel variant
+    g->exch.clear();
+    std::map<int, std::vector<std::pair<ExchArgs, int>>> by_kind;
+    for (const Member& m : g->mem)
+      for (int s = 0; s < m.c->nspin; ++s) {
+        ExchArgs a;
+        int kind, nb;
+        exchange_plan(m.c, s, m.c->d.n_walkers, &a, &kind, &nb);
+        by_kind[kind].push_back({a, nb});
+      }
+    for (auto& kv : by_kind) {
+      Launch L{};
+      L.k0 = kv.first;
+      std::vector<ExchArgs> args;
+      std::vector<int> start;
+      int b = 0;
+      for (auto& pa : kv.second) {
+        args.push_back(pa.first);
+        start.push_back(b);
+        b += pa.second;
+      }
+      L.nprob = (int)args.size();
+      L.nblocks = b;
+      put_table(tab, L, args, start);
+      g->exch.push_back(L);
+    }
+    std::stable_sort(g->exch.begin(), g->exch.end(),
+                     [](const Launch& x, const Launch& y) { return x.nblocks > y.nblocks; });
+  }
+  {
+    // energies: one block per walker (the force-bias parts are the spins' products)
+    std::vector<FinishArgs> fa;
+    std::vector<int> fs;
+    int b = 0;
+    for (size_t i = 0; i < g->mem.size(); ++i) {
+      afqmc_ctx* c = g->mem[i].c;
+      FinishArgs a;
+      energy_finish_args(c, c->d.n_walkers,
+                         i < g->en_ptrs.size() ? static_cast<cplx*>(g->en_ptrs[i]) : nullptr,
+                         &a);
+      a.nparts = c->nspin;
+      fa.push_back(a);
+      fs.push_back(b);
+      b += c->d.n_walkers;
+    }
+    g->fin = Launch{};
+    g->fin.nprob = (int)fa.size();
+    g->fin.nblocks = b;
+    put_table(tab, g->fin, fa, fs);
+  }
+
   // Upload without draining the stream: launches already queued read the old table
   // before this stream-ordered copy overwrites it; the pinned staging buffer is reused
   // once the previous copy has read it.
@@ -585,6 +783,78 @@ extern "C" int afqmc_group_propagate(afqmc_group* g, int32_t step, int32_t flags
   }
   PhaseTimer t(&g->prof, g->stream, AFQMC_PH_WEIGHTS);
   return launch_weights_grouped(g->stream, targs<WeightsArgs>(g, g->weights), g->weights.nprob);
+}
+
+// ---- block ends of every member (sampling.py:74-91 around each fragment's block):
+// the per-context afqmc_orthonormalize, afqmc_overlap and afqmc_energy as grouped
+// launches, each fragment's results exactly those of its own context's kernels
+
+namespace {
+int group_ready(afqmc_group* g) {
+  int rc = set_group_device(g);
+  if (rc) return rc;
+  if (g->dirty && (rc = build(g))) return rc;
+  for (const Member& m : g->mem) m.c->green_src = nullptr;
+  return AFQMC_OK;
+}
+}  // namespace
+
+extern "C" int afqmc_group_orthonormalize(afqmc_group* g) {
+  if (!g) return afqmc_set_error(AFQMC_ERR_ARG, "null group");
+  if (g->mem.empty()) return AFQMC_OK;
+  int rc = group_ready(g);
+  if (rc) return rc;
+  PhaseTimer t(&g->prof, g->stream, AFQMC_PH_QR);
+  return launch_qr_grouped(g->stream, targs<QrArgs>(g, g->qr), tstart(g, g->qr), g->qr.nprob,
+                           g->qr.nblocks, g->qr.shm);
+}
+
+extern "C" int afqmc_group_overlap(afqmc_group* g) {
+  if (!g) return afqmc_set_error(AFQMC_ERR_ARG, "null group");
+  if (g->mem.empty()) return AFQMC_OK;
+  int rc = group_ready(g);
+  if (rc) return rc;
+  rc = run_spread(g, g->meas_ov, [&](const Launch& L, hipStream_t st) {
+    return launch_measure_grouped(st, L.k0, L.k1, targs<MeasureArgs>(g, L), tstart(g, L),
+                                  L.nprob, L.nblocks, L.shm);
+  });
+  if (rc) return rc;
+  return launch_overlap_combine_grouped(g->stream, targs<OvcArgs>(g, g->ovc), tstart(g, g->ovc),
+                                        g->ovc.nprob, g->ovc.nblocks);
+}
+
+extern "C" int afqmc_group_energy(afqmc_group* g, void* const* energies) {
+  if (!g) return afqmc_set_error(AFQMC_ERR_ARG, "null group");
+  if (g->mem.empty()) return AFQMC_OK;
+  if (!energies) return afqmc_set_error(AFQMC_ERR_ARG, "null energies");
+  const size_t nm = g->mem.size();
+  for (size_t i = 0; i < nm; ++i)
+    if (!energies[i]) return afqmc_set_error(AFQMC_ERR_ARG, "null energies of member %d", (int)i);
+  if (g->en_ptrs.size() != nm || !std::equal(g->en_ptrs.begin(), g->en_ptrs.end(), energies)) {
+    g->en_ptrs.assign(energies, energies + nm);
+    g->dirty = true;
+  }
+  int rc = group_ready(g);
+  if (rc) return rc;
+  rc = run_spread(g, g->meas_en, [&](const Launch& L, hipStream_t st) {
+    return launch_measure_grouped(st, L.k0, L.k1, targs<MeasureArgs>(g, L), tstart(g, L),
+                                  L.nprob, L.nblocks, L.shm);
+  });
+  if (rc) return rc;
+  PhaseTimer t(&g->prof, g->stream, AFQMC_PH_ENERGY);
+  // c_g = force bias, contracted in float64 as the single context's energy does
+  if ((rc = launch_gemm_f64_grouped(g->stream, g->fb64.k0, g->fb64_vec2,
+                                    targs<GemmArgs>(g, g->fb64), tstart(g, g->fb64),
+                                    g->fb64.nprob, g->fb64.nblocks)))
+    return rc;
+  for (const Member& m : g->mem) m.c->fb_parts = m.c->nspin;
+  rc = run_spread(g, g->exch, [&](const Launch& L, hipStream_t st) {
+    return launch_exchange_grouped(st, L.k0, targs<ExchArgs>(g, L), tstart(g, L), L.nprob,
+                                   L.nblocks);
+  });
+  if (rc) return rc;
+  return launch_energy_finish_grouped(g->stream, targs<FinishArgs>(g, g->fin),
+                                      tstart(g, g->fin), g->fin.nprob, g->fin.nblocks);
 }
 
 extern "C" int afqmc_group_prof_enable(afqmc_group* g, int32_t on) {

@@ -127,11 +127,9 @@ __device__ void qr_body(cplx* A, int ldr, int ldc, int m, int n, cplx* tau, bool
   __syncthreads();
 }
 
-__global__ __launch_bounds__(QR_NT) void qr_kernel(int m, int n, cplx* __restrict__ walkers,
-                                                   int use_lds) {
-  extern __shared__ double smem[];
-  __shared__ cplx tau[AFQMC_MAX_NOCC];
-  cplx* g = walkers + (int64_t)blockIdx.x * m * n;
+// one walker (walker pointer g, m x n): LDS copy when use_lds, else in place
+__device__ __forceinline__ void qr_walker(int m, int n, cplx* __restrict__ g, int use_lds,
+                                          double* smem, cplx* tau) {
   if (use_lds) {
     // column-major copy: element (r, j) at A[j * ldc + r]
     const int ldc = qr_ldc(m);
@@ -151,7 +149,40 @@ __global__ __launch_bounds__(QR_NT) void qr_kernel(int m, int n, cplx* __restric
   }
 }
 
+__global__ __launch_bounds__(QR_NT) void qr_kernel(int m, int n, cplx* __restrict__ walkers,
+                                                   int use_lds) {
+  extern __shared__ double smem[];
+  __shared__ cplx tau[AFQMC_MAX_NOCC];
+  qr_walker(m, n, walkers + (int64_t)blockIdx.x * m * n, use_lds, smem, tau);
+}
+
+__global__ __launch_bounds__(QR_NT) void qr_grouped_kernel(const QrArgs* __restrict__ args,
+                                                           const int* __restrict__ start,
+                                                           int nprob) {
+  extern __shared__ double smem[];
+  __shared__ cplx tau[AFQMC_MAX_NOCC];
+  const int p = group_of(start, nprob, blockIdx.x);
+  const QrArgs a = args[p];
+  qr_walker(a.m, a.n, a.walkers + (int64_t)(blockIdx.x - start[p]) * a.m * a.n, a.use_lds,
+            smem, tau);
+}
+
 }  // namespace
+
+size_t qr_lds_bytes(int m, int n) {
+  const size_t bytes = (size_t)qr_ldc(m) * n * sizeof(cplx);
+  return bytes <= QR_LDS_LIMIT ? bytes : 0;
+}
+
+int launch_qr_grouped(hipStream_t st, const QrArgs* args, const int* start, int nprob,
+                      int nblocks, size_t shm) {
+  static int lds_limit = 0;
+  const int rc = ensure_dyn_lds((const void*)qr_grouped_kernel, shm, &lds_limit);
+  if (rc) return rc;
+  hipLaunchKernelGGL(qr_grouped_kernel, dim3(nblocks), dim3(QR_NT), shm, st, args, start, nprob);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
 
 int launch_qr(afqmc_ctx* c, int W, cplx* wup, cplx* wdn) {
   const int N = c->d.norb;

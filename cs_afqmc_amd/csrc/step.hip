@@ -241,6 +241,14 @@ __global__ void overlap_combine_kernel(int W, int nspin, const cplx* __restrict_
   if (w < W) overlaps[w] = overlap_of(dets, w, nspin);
 }
 
+__global__ __launch_bounds__(OVC_NT) void overlap_combine_grouped_kernel(
+    const OvcArgs* __restrict__ args, const int* __restrict__ start, int nprob) {
+  const int p = group_of(start, nprob, blockIdx.x);
+  const OvcArgs a = args[p];
+  const int w = (blockIdx.x - start[p]) * OVC_NT + threadIdx.x;
+  if (w < a.W) a.out[w] = overlap_of(a.dets, w, a.nspin);
+}
+
 __global__ void fb_out_kernel(int W, int G, const double* __restrict__ fb, int nparts,
                               int64_t part_stride, cplx* __restrict__ out) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -286,6 +294,14 @@ int launch_estimate_mix(afqmc_ctx* c, int nblocks, int stride, const double* sta
                         double* scalars) {
   hipLaunchKernelGGL(estimate_mix_kernel, dim3(1), dim3(64), 0, c->stream, nblocks, stride, stats,
                      scalars);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+int launch_overlap_combine_grouped(hipStream_t st, const OvcArgs* args, const int* start,
+                                   int nprob, int nblocks) {
+  hipLaunchKernelGGL(overlap_combine_grouped_kernel, dim3(nblocks), dim3(OVC_NT), 0, st, args,
+                     start, nprob);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }

@@ -417,6 +417,27 @@ class DeviceGroup:
         flags = _lib.AFQMC_STEP_REUSE_GREEN if reuse_green else 0
         _lib.check(self._lib.afqmc_group_propagate(self._h, int(step), flags))
 
+    def orthonormalize(self):
+        """QR of every member's walkers (afqmc_group_orthonormalize)."""
+        self._prep()
+        _lib.check(self._lib.afqmc_group_orthonormalize(self._h))
+
+    def overlap(self):
+        """Every member's overlaps into its bound overlaps tensor (afqmc_group_overlap)."""
+        self._prep()
+        _lib.check(self._lib.afqmc_group_overlap(self._h))
+
+    def energy(self, energies):
+        """Local energies of every member's walkers into energies[i] (complex128, W_i),
+        in the order members were added (afqmc_group_energy)."""
+        if len(energies) != len(self._members):
+            raise ValueError("one energies tensor per member")
+        for (dev, *_), e in zip(self._members, energies):
+            dev._vec(e, dev.n_walkers, torch.complex128, "energies")
+        self._prep()
+        self._en_ptrs = (ctypes.c_void_p * len(energies))(*[e.data_ptr() for e in energies])
+        _lib.check(self._lib.afqmc_group_energy(self._h, self._en_ptrs))
+
     def prof_enable(self, on=True, phases=None):
         """Event pairs around each phase of the grouped step (all grouped launches of a
         phase inside one pair); `phases` (names) restricts them, None = all."""

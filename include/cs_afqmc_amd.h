@@ -282,6 +282,23 @@ AFQMC_API int afqmc_group_add(afqmc_group* g, afqmc_ctx* ctx, void* walkers_up,
 AFQMC_API int afqmc_group_size(const afqmc_group* g);
 /* One phaseless step of every member (flags: 0 or AFQMC_STEP_REUSE_GREEN). */
 AFQMC_API int afqmc_group_propagate(afqmc_group* g, int32_t step, int32_t flags);
+/* Block ends of every member in grouped launches; a member's result does not depend on
+ * the other members:
+ *   afqmc_group_orthonormalize: afqmc_orthonormalize of each member's walkers
+ *     (propagation.py:251-253, linalg_utils.py:51-65 per fragment);
+ *   afqmc_group_overlap: afqmc_overlap into each member's bound overlaps
+ *     (wavefunctions.py:478-486 / :658-670, trial.calc_overlap per fragment);
+ *   afqmc_group_energy: afqmc_energy of each member's walkers into energies[i]
+ *     (dev, W complex128, i in the order of afqmc_group_add; wavefunctions.py:529-541 /
+ *     :711-743, trial.calc_energy per fragment).
+ * The first two give exactly the per-context results.  The energy's force-bias part c_g
+ * is contracted in float64 without split-K, so it may differ from afqmc_energy by
+ * rounding.
+ * They replace one launch sequence per fragment at the end of every block
+ * (sampling.py:74-91). */
+AFQMC_API int afqmc_group_orthonormalize(afqmc_group* g);
+AFQMC_API int afqmc_group_overlap(afqmc_group* g);
+AFQMC_API int afqmc_group_energy(afqmc_group* g, void* const* energies);
 /* Per-phase event timing of the grouped step, as afqmc_prof_enable / _phases / _collect
  * for one context (phases AFQMC_PH_GREEN .. AFQMC_PH_WEIGHTS, each phase covering all
  * of its grouped launches).  No counterpart in the reference: measurement only. */

@@ -128,3 +128,36 @@ def test_graph_replay_matches_eager(gpu):
         assert np.array_equal(fe.block_energies()[0], fg.block_energies()[0])
         for a, r in zip(walkers_np(fe.prop_data), walkers_np(fg.prop_data)):
             assert np.array_equal(a, r)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_grouped_block_ends_match_each_context(gpu, precision):
+    """afqmc_group_orthonormalize / _overlap / _energy against each member context's own
+    afqmc_orthonormalize / _overlap / _energy on the same walkers: QR and overlaps bit for
+    bit, energies to 1e-12 relative (the grouped force-bias part c_g is one float64
+    product per spin, the context's splits it over K)."""
+    frags = [fragment(s, precision, 4) for s in SPECS]
+    b = batch.FragmentBatch(frags)
+    b.run_block()  # walkers away from the trial
+    torch.cuda.synchronize()
+    copies = [[w.clone() for w in (f.prop_data["walkers"]
+                                    if isinstance(f.prop_data["walkers"], list)
+                                    else [f.prop_data["walkers"]])] for f in frags]
+    with torch.cuda.stream(b.stream):
+        b.group.orthonormalize()
+        b.group.overlap()
+        en = [torch.empty_like(f.energies) for f in frags]
+        b.group.energy(en)
+    torch.cuda.synchronize()
+    for f, ws, e in zip(frags, copies, en):
+        ref = ws if len(ws) > 1 else ws[0]
+        f.dev.orthonormalize(ref)
+        ov = f.dev.overlap(ref)
+        er = f.dev.energy(ref)
+        torch.cuda.synchronize()
+        for a, r in zip(walkers_np(f.prop_data), [w.cpu().numpy() for w in ws]):
+            assert np.array_equal(a, r)
+        assert torch.equal(f.prop_data["overlaps"], ov)
+        e_np, r_np = e.cpu().numpy(), er.cpu().numpy()
+        assert np.all(np.isfinite(e_np))
+        assert np.max(np.abs(e_np - r_np)) <= 1e-12 * np.max(np.abs(r_np))

@@ -123,6 +123,10 @@ int force_bias_gemm(afqmc_ctx* c, int W, int mode) {
       // split-bf16: rows (w, re/im) of the padded Green planes, K = n_s * Np32
       const int kd = c->nocc[s] * c->Np32;
       sp = gemm_f64_splits(2 * W, G, kd);
+      // a 64-wide grid that already fills the chip on its own but a 128-wide one that does
+      // not (large W and G): split K instead, so the product runs on 128 x 128 tiles
+      const int64_t big = (int64_t)((2 * W + 127) / 128) * ((G + 127) / 128);
+      if (sp == 1 && big < 512 && kd >= 4096) sp = (int)std::min<int64_t>(4, (511 + big) / big);
       if (part + sp > c->fb_cap)
         return afqmc_set_error(AFQMC_ERR_STATE, "force-bias split slices exceed workspace");
       rc = launch_gemm_nt_bf16x3_f64(c->stream, 2 * W, G, kd, (float)c->mult, c->green_bf[s][0],

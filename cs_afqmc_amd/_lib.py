@@ -55,6 +55,7 @@ EXPORTS = (
     "afqmc_group_orthonormalize",
     "afqmc_group_overlap",
     "afqmc_group_energy",
+    "afqmc_group_fields_normal_key",
 )
 
 # phase indices of afqmc_prof_collect (AFQMC_PH_* in the header)
@@ -141,6 +142,7 @@ def _declare(lib):
     lib.afqmc_group_orthonormalize.argtypes = [_vp]
     lib.afqmc_group_overlap.argtypes = [_vp]
     lib.afqmc_group_energy.argtypes = [_vp, _vp]
+    lib.afqmc_group_fields_normal_key.argtypes = [_vp, _vp]
     lib.afqmc_group_prof_enable.argtypes = [_vp, _i32]
     lib.afqmc_group_prof_phases.argtypes = [_vp, ctypes.c_uint32]
     lib.afqmc_group_prof_collect.argtypes = [_vp, _vp, _vp]

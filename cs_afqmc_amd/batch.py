@@ -258,7 +258,7 @@ class FragmentBatch:
             main.wait_event(join)
 
     def _block(self):
-        self._per_fragment(lambda f: f.begin_block())
+        self.group.fields_normal_key([f.key for f in self.fragments])
         for s in range(self.fragments[0].n_prop_steps):
             self.group.propagate(s, reuse_green=s > 0)
         # block ends: QR and overlaps of every fragment as grouped launches, then each

@@ -489,6 +489,14 @@ int launch_qr_grouped(hipStream_t st, const QrArgs* args, const int* start, int 
 int launch_overlap_combine_grouped(hipStream_t st, const OvcArgs* args, const int* start,
                                    int nprob, int nblocks);
 constexpr int OVC_NT = 256;  // threads (walkers) per overlap-combine block
+// a block's fields draw from a device-resident key {seed, counter}
+struct DrawArgs {
+  uint64_t* key;
+  double* out;
+  int64_t count;
+};
+int launch_fields_normal_key_grouped(hipStream_t st, const DrawArgs* args, const int* start,
+                                     int nprob, int nblocks);
 
 enum TaylorKind {
   TAYLOR_REG_F64 = 0,

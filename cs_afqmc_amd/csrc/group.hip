@@ -69,6 +69,9 @@ struct afqmc_group {
   std::vector<Launch> meas_ov, meas_en, exch;
   bool fb64_vec2 = true;
   std::vector<void*> en_ptrs;
+  // block fields draws (afqmc_group_fields_normal_key), keys the table was built for
+  Launch draw{};
+  std::vector<void*> key_ptrs;
   ProfRec prof{};                    // afqmc_group_prof_*
 };
 
@@ -523,6 +526,26 @@ int build(afqmc_group* g) {
                      [](const Launch& x, const Launch& y) { return x.nblocks > y.nblocks; });
   }
   {
+    // fields blocks: member i draws n_steps x W x nchol normals from its key
+    std::vector<DrawArgs> da;
+    std::vector<int> ds;
+    int b = 0;
+    for (size_t i = 0; i < g->mem.size(); ++i) {
+      const Member& m = g->mem[i];
+      DrawArgs a{};
+      a.key = i < g->key_ptrs.size() ? static_cast<uint64_t*>(g->key_ptrs[i]) : nullptr;
+      a.out = const_cast<double*>(m.fields);
+      a.count = (int64_t)m.n_steps * m.c->d.n_walkers * m.c->d.nchol;
+      da.push_back(a);
+      ds.push_back(b);
+      b += (int)(((a.count + 1) / 2 + 255) / 256);
+    }
+    g->draw = Launch{};
+    g->draw.nprob = (int)da.size();
+    g->draw.nblocks = b;
+    put_table(tab, g->draw, da, ds);
+  }
+  {
     // energies: one block per walker (the force-bias parts are the spins' products)
     std::vector<FinishArgs> fa;
     std::vector<int> fs;
@@ -855,6 +878,25 @@ extern "C" int afqmc_group_energy(afqmc_group* g, void* const* energies) {
   if (rc) return rc;
   return launch_energy_finish_grouped(g->stream, targs<FinishArgs>(g, g->fin),
                                       tstart(g, g->fin), g->fin.nprob, g->fin.nblocks);
+}
+
+extern "C" int afqmc_group_fields_normal_key(afqmc_group* g, void* const* keys) {
+  if (!g) return afqmc_set_error(AFQMC_ERR_ARG, "null group");
+  if (g->mem.empty()) return AFQMC_OK;
+  if (!keys) return afqmc_set_error(AFQMC_ERR_ARG, "null keys");
+  const size_t nm = g->mem.size();
+  for (size_t i = 0; i < nm; ++i)
+    if (!keys[i]) return afqmc_set_error(AFQMC_ERR_ARG, "null key of member %d", (int)i);
+  if (g->key_ptrs.size() != nm || !std::equal(g->key_ptrs.begin(), g->key_ptrs.end(), keys)) {
+    g->key_ptrs.assign(keys, keys + nm);
+    g->dirty = true;
+  }
+  int rc = set_group_device(g);
+  if (rc) return rc;
+  if (g->dirty && (rc = build(g))) return rc;
+  PhaseTimer t(&g->prof, g->stream, AFQMC_PH_RNG);
+  return launch_fields_normal_key_grouped(g->stream, targs<DrawArgs>(g, g->draw),
+                                          tstart(g, g->draw), g->draw.nprob, g->draw.nblocks);
 }
 
 extern "C" int afqmc_group_prof_enable(afqmc_group* g, int32_t on) {

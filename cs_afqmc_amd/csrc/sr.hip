@@ -116,10 +116,11 @@ __global__ void gather_kernel(int W, int elems, const int32_t* __restrict__ idx,
     d[e] = s[e];
 }
 
-__global__ void normal_kernel(uint64_t seed, uint64_t counter, double* __restrict__ out,
-                              int64_t count) {
-  const int64_t pair = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (2 * pair >= count) return;
+// Normals 2 pair and 2 pair + 1 of the stream (seed, counter): one Philox4x32-10 block
+// at counter (pair, counter) gives two 53-bit uniforms u1 in (0, 1], u2 in [0, 1), and
+// Box-Muller turns them into rad cos and rad sin.
+__device__ inline void normal_pair(uint64_t seed, uint64_t counter, int64_t pair,
+                                   double* __restrict__ out, int64_t count) {
   const u32x4 c = u32x4{(uint32_t)pair, (uint32_t)((uint64_t)pair >> 32), (uint32_t)counter,
                         (uint32_t)(counter >> 32)};
   const u32x4 r = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
@@ -134,29 +135,37 @@ __global__ void normal_kernel(uint64_t seed, uint64_t counter, double* __restric
   if (2 * pair + 1 < count) out[2 * pair + 1] = rad * s;
 }
 
+__global__ void normal_kernel(uint64_t seed, uint64_t counter, double* __restrict__ out,
+                              int64_t count) {
+  const int64_t pair = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (2 * pair < count) normal_pair(seed, counter, pair, out, count);
+}
+
 // The same draws keyed by a device-resident {seed, counter} (read, not modified: the
 // counter advances in key_advance_kernel, stream-ordered after this launch).
 __global__ void normal_key_kernel(const uint64_t* __restrict__ key, double* __restrict__ out,
                                   int64_t count) {
   const int64_t pair = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (2 * pair >= count) return;
-  const uint64_t seed = key[0], counter = key[1];
-  const u32x4 c = u32x4{(uint32_t)pair, (uint32_t)((uint64_t)pair >> 32), (uint32_t)counter,
-                        (uint32_t)(counter >> 32)};
-  const u32x4 r = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-  const uint64_t a = ((uint64_t)r.y << 32) | r.x;
-  const uint64_t b = ((uint64_t)r.w << 32) | r.z;
-  const double u1 = ((double)(a >> 11) + 1.0) * 0x1.0p-53;
-  const double u2 = (double)(b >> 11) * 0x1.0p-53;
-  const double rad = sqrt(-2.0 * log(u1));
-  double s, co;
-  sincos(2.0 * M_PI * u2, &s, &co);
-  out[2 * pair] = rad * co;
-  if (2 * pair + 1 < count) out[2 * pair + 1] = rad * s;
+  if (2 * pair < count) normal_pair(key[0], key[1], pair, out, count);
 }
 
 __global__ void key_advance_kernel(uint64_t* __restrict__ key) {
   if (threadIdx.x == 0) key[1] += 1;
+}
+
+// grouped form: member p draws its count normals from its own key (blocks
+// [start[p], start[p + 1]) of 256 pairs), then every key advances
+__global__ __launch_bounds__(256) void normal_key_grouped_kernel(
+    const DrawArgs* __restrict__ args, const int* __restrict__ start, int nprob) {
+  const int p = group_of(start, nprob, blockIdx.x);
+  const DrawArgs a = args[p];
+  const int64_t pair = (int64_t)(blockIdx.x - start[p]) * 256 + threadIdx.x;
+  if (2 * pair < a.count) normal_pair(a.key[0], a.key[1], pair, a.out, a.count);
+}
+
+__global__ void key_advance_grouped_kernel(const DrawArgs* __restrict__ args, int nprob) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < nprob) args[p].key[1] += 1;
 }
 
 }  // namespace
@@ -185,6 +194,17 @@ int launch_fields_normal(afqmc_ctx* c, uint64_t seed, uint64_t counter, double* 
   const int64_t pairs = (count + 1) / 2;
   hipLaunchKernelGGL(normal_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0,
                      c->stream, seed, counter, out, count);
+  LAUNCH_CHECK();
+  return AFQMC_OK;
+}
+
+int launch_fields_normal_key_grouped(hipStream_t st, const DrawArgs* args, const int* start,
+                                     int nprob, int nblocks) {
+  hipLaunchKernelGGL(normal_key_grouped_kernel, dim3(nblocks), dim3(256), 0, st, args, start,
+                     nprob);
+  LAUNCH_CHECK();
+  hipLaunchKernelGGL(key_advance_grouped_kernel, dim3((nprob + 255) / 256), dim3(256), 0, st,
+                     args, nprob);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }

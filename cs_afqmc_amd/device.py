@@ -438,6 +438,17 @@ class DeviceGroup:
         self._en_ptrs = (ctypes.c_void_p * len(energies))(*[e.data_ptr() for e in energies])
         _lib.check(self._lib.afqmc_group_energy(self._h, self._en_ptrs))
 
+    def fields_normal_key(self, keys):
+        """Every member's bound fields block drawn from its device key (keys[i], int64[2]
+        {seed, counter}), then each counter += 1 (afqmc_group_fields_normal_key)."""
+        if len(keys) != len(self._members):
+            raise ValueError("one key per member")
+        for (dev, *_), k in zip(self._members, keys):
+            dev._check_key(k)
+        self._prep()
+        self._key_ptrs = (ctypes.c_void_p * len(keys))(*[k.data_ptr() for k in keys])
+        _lib.check(self._lib.afqmc_group_fields_normal_key(self._h, self._key_ptrs))
+
     def prof_enable(self, on=True, phases=None):
         """Event pairs around each phase of the grouped step (all grouped launches of a
         phase inside one pair); `phases` (names) restricts them, None = all."""

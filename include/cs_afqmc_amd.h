@@ -299,6 +299,11 @@ AFQMC_API int afqmc_group_propagate(afqmc_group* g, int32_t step, int32_t flags)
 AFQMC_API int afqmc_group_orthonormalize(afqmc_group* g);
 AFQMC_API int afqmc_group_overlap(afqmc_group* g);
 AFQMC_API int afqmc_group_energy(afqmc_group* g, void* const* energies);
+/* Every member's fields block from its own device key: keys[i] (dev uint64[2] {seed,
+ * counter}) draws n_steps x W x nchol normals into member i's bound fields_block, then
+ * counter += 1 -- afqmc_fields_normal_key per member (the sampler's per-block draw,
+ * sampling.py:60-69), bit for bit, in two launches. */
+AFQMC_API int afqmc_group_fields_normal_key(afqmc_group* g, void* const* keys);
 /* Per-phase event timing of the grouped step, as afqmc_prof_enable / _phases / _collect
  * for one context (phases AFQMC_PH_GREEN .. AFQMC_PH_WEIGHTS, each phase covering all
  * of its grouped launches).  No counterpart in the reference: measurement only. */

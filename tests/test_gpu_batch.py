@@ -161,3 +161,20 @@ def test_grouped_block_ends_match_each_context(gpu, precision):
         e_np, r_np = e.cpu().numpy(), er.cpu().numpy()
         assert np.all(np.isfinite(e_np))
         assert np.max(np.abs(e_np - r_np)) <= 1e-12 * np.max(np.abs(r_np))
+
+
+def test_grouped_fields_draw_matches_each_context(gpu):
+    """afqmc_group_fields_normal_key against each member's afqmc_fields_normal_key from a
+    copy of its key: the fields blocks bit for bit, the keys advanced alike."""
+    frags = [fragment(s, "bf16x3", 3) for s in SPECS]
+    b = batch.FragmentBatch(frags)
+    keys0 = [f.key.clone() for f in frags]
+    with torch.cuda.stream(b.stream):
+        b.group.fields_normal_key([f.key for f in frags])
+    torch.cuda.synchronize()
+    for f, k in zip(frags, keys0):
+        ref = torch.empty_like(f.fields)
+        f.dev.fields_normal_key(k, ref)
+        torch.cuda.synchronize()
+        assert torch.equal(f.fields, ref)
+        assert torch.equal(f.key, k)

@@ -276,7 +276,10 @@ __global__ __launch_bounds__(ENER_NT) void exchange_f64_grouped_kernel(
 // (128 columns, 4 waves) or, where it holds more whole walkers (2n columns each) per
 // tile, WN = 4 (256 columns, 8 waves): the Cholesky tile is then read once per more
 // walkers.
-constexpr int EB_BM = 128, EB_BK = 32, EB_LDSW = EB_BK + 8;
+// operand rows of 32 bf16 (64 B) with the chunk order c ^ ((r >> 2) & 3), as in the
+// split-bf16 GEMM: operand reads and tile stores cover the 64 banks without conflicts
+constexpr int EB_BM = 128, EB_BK = 32, EB_LDSW = EB_BK;
+__device__ inline int eb_chunk(int r, int c) { return (c ^ ((r >> 2) & 3)) << 3; }
 template <int WN>
 struct EbTile {
   static constexpr int NT = 128 * WN, BN = 64 * WN, LDF = BN + 1;
@@ -345,14 +348,14 @@ __device__ __forceinline__ void exchange_bf16x3_body(
       const int q = t + NT * j, row = q >> 2, ch = q & 3;
 #pragma unroll
       for (int h = 0; h < 2; ++h)
-        *reinterpret_cast<u32x4_t*>(&sA[buf][h][row * LDSW + ch * 8]) = ra[h][j];
+        *reinterpret_cast<u32x4_t*>(&sA[buf][h][row * LDSW + eb_chunk(row, ch)]) = ra[h][j];
     }
 #pragma unroll
     for (int j = 0; j < B_LD; ++j) {
       const int q = t + NT * j, row = q >> 2, ch = q & 3;
 #pragma unroll
       for (int h = 0; h < 2; ++h)
-        *reinterpret_cast<u32x4_t*>(&sB[buf][h][row * LDSW + ch * 8]) = rb[h][j];
+        *reinterpret_cast<u32x4_t*>(&sB[buf][h][row * LDSW + eb_chunk(row, ch)]) = rb[h][j];
     }
   };
 
@@ -368,7 +371,8 @@ __device__ __forceinline__ void exchange_bf16x3_body(
   __syncthreads();
   const int arow = wm * (BM / 2) + (lane & 15);
   const int brow = wn * (BN / WN) + (lane & 15);
-  const int koff = (lane >> 4) * 8;
+  // k chunk lane >> 4 of rows whose bits 2-3 are those of lane (row = 16 t + lane % 16)
+  const int koff = eb_chunk(lane & 15, lane >> 4);
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nk) load_tile((kt + 1) * EB_BK);

@@ -20,7 +20,11 @@ typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 namespace {
 
 constexpr int BK = 32;
-constexpr int LDSW = BK + 8;  // LDS row stride in bf16 (80 B): 16 rows x 16 B are conflict free
+// LDS rows of BK bf16 (64 B), the four 16-byte chunks of row r stored in the order
+// c ^ ((r >> 2) & 3): the 16 rows x 16 B an MFMA operand read takes at one k chunk, and
+// the 4 rows x 64 B a tile store writes, each cover all 64 banks once
+constexpr int LDSW = BK;
+__device__ inline int lds_chunk(int r, int c) { return (c ^ ((r >> 2) & 3)) << 3; }
 
 __device__ inline uint16_t f32_to_bf16(float f) {
   uint32_t u = __float_as_uint(f);
@@ -153,14 +157,14 @@ __device__ __forceinline__ void gemm_nt_bf16x3_body(
       const int q = t + 256 * j, row = q >> 2, ch = q & 3;
 #pragma unroll
       for (int h = 0; h < 2; ++h)
-        *reinterpret_cast<u32x4_t*>(&sA[buf][h][row * LDSW + ch * 8]) = ra[h][j];
+        *reinterpret_cast<u32x4_t*>(&sA[buf][h][row * LDSW + lds_chunk(row, ch)]) = ra[h][j];
     }
 #pragma unroll
     for (int j = 0; j < B_LD; ++j) {
       const int q = t + 256 * j, row = q >> 2, ch = q & 3;
 #pragma unroll
       for (int h = 0; h < 2; ++h)
-        *reinterpret_cast<u32x4_t*>(&sB[buf][h][row * LDSW + ch * 8]) = rb[h][j];
+        *reinterpret_cast<u32x4_t*>(&sB[buf][h][row * LDSW + lds_chunk(row, ch)]) = rb[h][j];
     }
   };
 
@@ -178,7 +182,8 @@ __device__ __forceinline__ void gemm_nt_bf16x3_body(
   __syncthreads();
   const int arow = wm * (BM / 2) + (lane & 15);
   const int brow = wn * (BN / 2) + (lane & 15);
-  const int koff = (lane >> 4) * 8;
+  // k chunk lane >> 4 of rows whose bits 2-3 are those of lane (row = 16 t + lane % 16)
+  const int koff = lds_chunk(lane & 15, lane >> 4);
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nk) load_tile(kbeg + (kt + 1) * BK);

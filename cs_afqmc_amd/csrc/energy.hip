@@ -276,10 +276,10 @@ __global__ __launch_bounds__(ENER_NT) void exchange_f64_grouped_kernel(
 // (128 columns, 4 waves) or, where it holds more whole walkers (2n columns each) per
 // tile, WN = 4 (256 columns, 8 waves): the Cholesky tile is then read once per more
 // walkers.
-// operand rows of 32 bf16 (64 B) with the chunk order c ^ ((r >> 2) & 3), as in the
-// split-bf16 GEMM: operand reads and tile stores cover the 64 banks without conflicts
+// operand rows of 32 bf16 (64 B) with the chunk order c ^ ((r >> 1) & 3), as in the
+// split-bf16 GEMM, under which the 16-row operand reads meet no LDS bank conflicts
 constexpr int EB_BM = 128, EB_BK = 32, EB_LDSW = EB_BK;
-__device__ inline int eb_chunk(int r, int c) { return (c ^ ((r >> 2) & 3)) << 3; }
+__device__ inline int eb_chunk(int r, int c) { return (c ^ ((r >> 1) & 3)) << 3; }
 template <int WN>
 struct EbTile {
   static constexpr int NT = 128 * WN, BN = 64 * WN, LDF = BN + 1;

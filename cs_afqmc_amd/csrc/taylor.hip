@@ -349,11 +349,22 @@ int launch_reg(afqmc_ctx* c, int W, int n, cplx* wk, const double* E, const doub
 // relative error stays far below float32 rounding of E y itself.  The Horner recursion
 // y <- t0 + V y / k and all additions run in float32; the walker is read and written in
 // float64.
+// LDS rows (one column of y) are 32 KB + 16 bf16 long, 32 bytes past a multiple of 64,
+// and 16-byte chunk j of column c is stored at chunk j ^ ((c >> 2) & 1).  With that, the
+// sixteen columns a wave's B operand reads (8 consecutive k each) meet no bank conflict,
+// and the D-layout stores of put (4 consecutive k) cost no more than with unswizzled
+// rows 16 bytes past a multiple of 32, which conflict on the reads.
 template <int KB>
 struct BfTile {
-  static constexpr int LDK = 32 * KB + 8;  // LDS row (one column of y) in bf16 elements
-  static constexpr int KH = (KB + 1) / 2;  // 32-wide k blocks of E - I: ceil(N / 32)
+  static constexpr int LDK = 32 * KB + 16;  // LDS row (one column of y) in bf16 elements
+  static constexpr int KH = (KB + 1) / 2;   // 32-wide k blocks of E - I: ceil(N / 32)
 };
+inline int bf_ldk(int kb) { return 32 * kb + 16; }  // BfTile<kb>::LDK for a runtime kb
+// element offset of k (a multiple of 4) in LDS column c
+__device__ inline int bf_koff(int c, int k) { return (((k >> 3) ^ ((c >> 2) & 1)) << 3) | (k & 7); }
+// y's real part spans k < Nh, its imaginary part k in [Nh, 2 Nh); Nh = N rounded up to 8
+// keeps both halves' 4-row groups in the same chunk halves
+__host__ __device__ inline int bf_nh(int N) { return (N + 7) & ~7; }
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
@@ -384,8 +395,9 @@ __device__ __forceinline__ void taylor_bf_body(int N, int n, int n_terms_k,
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rb = wave * 16;
   const int np32 = 32 * KH;
-  const int Nh = (N + 3) & ~3;
+  const int Nh = bf_nh(N);
   const int q4 = lane >> 4, col = lane & 15;
+  const int q4s = q4 ^ ((col >> 2) & 1);  // this lane's B chunk within a 32-wide k block
   const int p = rb + col;  // A row of this lane
   const bool prow = p < N;
   const int n2 = 2 * n;
@@ -440,18 +452,20 @@ __device__ __forceinline__ void taylor_bf_body(int N, int n, int n_terms_k,
         const uint32_t h23 = pack_bf16x2(y[ct][2], y[ct][3]);
         const uint32_t l01 = pack_bf16x2(y[ct][0] - bf16_lo_val(h01), y[ct][1] - bf16_hi_val(h01));
         const uint32_t l23 = pack_bf16x2(y[ct][2] - bf16_lo_val(h23), y[ct][3] - bf16_hi_val(h23));
-        *reinterpret_cast<u32x2_t*>(Yh + c * LDK + row0) = u32x2_t{h01, h23};
-        *reinterpret_cast<u32x2_t*>(Yl + c * LDK + row0) = u32x2_t{l01, l23};
+        const int ot = c * LDK + bf_koff(c, row0);
+        *reinterpret_cast<u32x2_t*>(Yh + ot) = u32x2_t{h01, h23};
+        *reinterpret_cast<u32x2_t*>(Yl + ot) = u32x2_t{l01, l23};
         const int cb = c < n ? c + n : c - n;
         const uint32_t sg = c < n ? 0u : 0x80008000u;  // bottom half: +yr, -yi
-        *reinterpret_cast<u32x2_t*>(Yh + cb * LDK + Nh + row0) = u32x2_t{h01 ^ sg, h23 ^ sg};
-        *reinterpret_cast<u32x2_t*>(Yl + cb * LDK + Nh + row0) = u32x2_t{l01 ^ sg, l23 ^ sg};
+        const int ob = cb * LDK + bf_koff(cb, Nh + row0);
+        *reinterpret_cast<u32x2_t*>(Yh + ob) = u32x2_t{h01 ^ sg, h23 ^ sg};
+        *reinterpret_cast<u32x2_t*>(Yl + ob) = u32x2_t{l01 ^ sg, l23 ^ sg};
       }
     }
   };
   auto bload = [&](const uint16_t* plane, int ct, int kb) {
     return __builtin_bit_cast(
-        bf16x8_t, *reinterpret_cast<const u32x4_t*>(plane + (ct * 16 + col) * LDK + kb * 32 + 8 * q4));
+        bf16x8_t, *reinterpret_cast<const u32x4_t*>(plane + (ct * 16 + col) * LDK + kb * 32 + 8 * q4s));
   };
   // acc = (E - I) y: A (rows of E - I) streams from L2; k < Np32 reads the top half of
   // each column and, past Nh, bottom-half entries that meet zero columns of E - I
@@ -559,7 +573,7 @@ __global__ __launch_bounds__(512, bf_waves_per_simd(KB, CTP)) void taylor_bf_gro
 }
 
 // k blocks of the packed [Vr | Vi] axis
-inline int bf_kblocks(int N) { return (2 * ((N + 3) & ~3) + 31) / 32; }
+inline int bf_kblocks(int N) { return (2 * bf_nh(N) + 31) / 32; }
 
 template <int KB, int CTP>
 int launch_bf(afqmc_ctx* c, int W, int n, cplx* wk, int s) {
@@ -610,7 +624,7 @@ int launch_bf_ct(afqmc_ctx* c, int W, int n, cplx* wk, int s) {
 // registers around the Horner steps (UHF N = 200: 2.21 -> 2.04 ms per step).
 typedef float f4u_t __attribute__((ext_vector_type(4), aligned(4)));  // dword-aligned rows
 
-inline int bfs_ldk(int N) { return 32 * ((N + 31) / 32) + 8; }
+inline int bfs_ldk(int N) { return 32 * ((N + 31) / 32) + 16; }
 inline size_t bfs_shm(int N, int ct) {
   return (size_t)2 * 32 * ct * bfs_ldk(N) * sizeof(uint16_t);
 }
@@ -629,7 +643,7 @@ __device__ __forceinline__ void taylor_bfs_body(int N, int n, int n_terms_k,
                                                 const int w) {
   constexpr int NC = 32 * CT;  // LDS columns: CT tiles of yr, then CT tiles of yi
   const int KBk = (N + 31) >> 5;
-  const int LDK = 32 * KBk + 8;
+  const int LDK = 32 * KBk + 16;  // rows and chunk swizzle as in the register form
   const int np32 = 32 * KBk;
   extern __shared__ uint16_t YB[];  // [2 planes][NC][LDK]
   uint16_t* Yh = YB;
@@ -637,6 +651,8 @@ __device__ __forceinline__ void taylor_bfs_body(int N, int n, int n_terms_k,
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rb = wave * 16 * RT;
   const int q4 = lane >> 4, col = lane & 15;
+  const int sw = (col >> 2) & 1;  // chunk swizzle of this lane's LDS columns
+  const int q4s = q4 ^ sw;
   // uniform bases with 32-bit lane offsets (saddr + voffset addressing)
   cplx* phi = walkers + (int64_t)w * N * n;
   // V planes: all N^2 entries row-major, or (nv = N(N+1)/2, symmetric V) the upper
@@ -674,7 +690,7 @@ __device__ __forceinline__ void taylor_bfs_body(int N, int n, int n_terms_k,
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct) {
           const float4_t v = h ? yi[t][ct] : yr[t][ct];
-          const int off = (h * 16 * CT + ct * 16 + col) * LDK + rb + 16 * t + 4 * q4;
+          const int off = (h * 16 * CT + ct * 16 + col) * LDK + rb + 16 * t + 8 * ((q4 >> 1) ^ sw) + 4 * (q4 & 1);
           const uint32_t h01 = pack_bf16x2(v[0], v[1]);
           const uint32_t h23 = pack_bf16x2(v[2], v[3]);
           const uint32_t l01 = pack_bf16x2(v[0] - bf16_lo_val(h01), v[1] - bf16_hi_val(h01));
@@ -685,7 +701,7 @@ __device__ __forceinline__ void taylor_bfs_body(int N, int n, int n_terms_k,
   };
   auto bload = [&](const uint16_t* plane, int c0, int kb) {
     return __builtin_bit_cast(
-        bf16x8_t, *reinterpret_cast<const u32x4_t*>(plane + (c0 + col) * LDK + kb * 32 + 8 * q4));
+        bf16x8_t, *reinterpret_cast<const u32x4_t*>(plane + (c0 + col) * LDK + kb * 32 + 8 * q4s));
   };
   auto zero = [&](float4_t (*a)[CT]) {
 #pragma unroll
@@ -989,7 +1005,7 @@ void taylor_plan(const afqmc_ctx* c, int s, TaylorPlan* pl) {
       pl->kind = TAYLOR_REG_BF16X3;
       pl->nkgv = kb;
       pl->ct = ctp;
-      pl->shm = (size_t)2 * ctp * 16 * (32 * kb + 8) * sizeof(uint16_t);
+      pl->shm = (size_t)2 * ctp * 16 * bf_ldk(kb) * sizeof(uint16_t);
     } else {  // N <= AFQMC_MAX_NORB = 256, n <= AFQMC_MAX_NOCC = 64
       pl->kind = TAYLOR_STREAM_BF16X3;
       pl->ct = (n + 15) / 16;

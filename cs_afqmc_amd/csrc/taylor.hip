@@ -408,6 +408,25 @@ __device__ __forceinline__ void taylor_bf_body(int N, int n, int n_terms_k,
   // triangle row by row: entry (a, b), a <= b, at a N - a (a - 1) / 2 + b - a
   const bool packed = nv != N * N;
 
+  // walker -> registers (D layout), issued first so that its round trip overlaps the V
+  // loads'; clamped into the walker and masked after, like the V loads below
+  float4_t y[CTP], t0[CTP];
+  {
+    const double* wz = reinterpret_cast<const double*>(phi);
+#pragma unroll
+    for (int ct = 0; ct < CTP; ++ct)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rb + 4 * q4 + r;
+        const int c = ct * 16 + col;
+        const bool ok = (row < N) & (c < n2);
+        const int rc = min(row, N - 1);
+        const int cc = c < n ? c : min(c - n, n - 1);
+        const double z = wz[2 * ((int64_t)rc * n + cc) + (c < n ? 0 : 1)];
+        y[ct][r] = ok ? (float)z : 0.f;
+      }
+  }
+
   // V rows in registers, split once; loads are clamped into the matrix and masked after,
   // so they issue back to back without branches
   bf16x8_t vh[KB], vl[KB];
@@ -490,21 +509,6 @@ __device__ __forceinline__ void taylor_bf_body(int N, int n, int n_terms_k,
     }
   };
 
-  // walker -> registers (D layout) and LDS
-  float4_t y[CTP], t0[CTP];
-#pragma unroll
-  for (int ct = 0; ct < CTP; ++ct)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = rb + 4 * q4 + r;
-      const int c = ct * 16 + col;
-      float v = 0.f;
-      if (row < N && c < n2) {
-        const double* z = reinterpret_cast<const double*>(phi + (int64_t)row * n + (c < n ? c : c - n));
-        v = (float)z[c < n ? 0 : 1];
-      }
-      y[ct][r] = v;
-    }
   put(y);
   __syncthreads();
   e_product(t0);

@@ -348,6 +348,126 @@ __device__ inline void finish_walker(cplx det, cplx* dets, int w, int spin,
   }
 }
 
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+// 8 floats -> hi and lo bf16 planes, x ~= hi + lo
+__device__ inline void split8_bf(const float (&v)[8], bf16x8_t& h8, bf16x8_t& l8) {
+  u32x4_t ph, pl;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t h = pack_bf16x2(v[2 * j], v[2 * j + 1]);
+    ph[j] = h;
+    pl[j] = pack_bf16x2(v[2 * j] - bf16_lo_val(h), v[2 * j + 1] - bf16_hi_val(h));
+  }
+  h8 = __builtin_bit_cast(bf16x8_t, ph);
+  l8 = __builtin_bit_cast(bf16x8_t, pl);
+}
+
+// The mixed mode's Green product, where only the split-bf16 planes are wanted (they are
+// the A operand of the split-bf16 force-bias GEMM): [Gr ; Gi] = X' [wr^T ; wi^T] with
+// X' = [[Xr^T, -Xi^T], [Xi^T, Xr^T]] (2n x 2n), on v_mfma_f32_16x16x32_bf16 with both
+// operands split into hi + lo bf16 planes (al bh + ah bl + ah bh, float32 accumulation)
+// rather than on float64 MFMA.  K = 2n runs in KB blocks of 32, zero beyond 2n in A.
+//   A (lane l): row r = 16 tr + l % 16, k = 32 kb + 8 (l / 16) + e, read once from X in LDS
+//   B (lane l): k = 32 kb + 8 (l / 16) + e, column p = 16 tp + l % 16, from the walker
+//   D (lane l, element j): row 16 tr + 4 (l / 16) + j, column 16 tp + l % 16
+// bpre holds column tile wv's walker operand, loaded by the caller (it is in flight
+// during the inversion).  The waves take alternate column tiles.
+template <int T2>
+struct GreenBf {
+  static constexpr int KB = (T2 + 1) / 2;  // 2n <= 16 T2
+};
+template <int T2>
+__device__ __forceinline__ void green_bf_load_b(const double* __restrict__ phid, int N, int n,
+                                                int tp, int lane, double (&b)[GreenBf<T2>::KB][8]) {
+  const int s = lane >> 4, col = lane & 15;
+  const double* rowp = phid + (int64_t)2 * min(tp * 16 + col, N - 1) * n;
+#pragma unroll
+  for (int kb = 0; kb < GreenBf<T2>::KB; ++kb)
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = kb * 32 + 8 * s + e;  // k >= 2n meets zeros of A: any valid entry
+      const int kj = min(k < n ? k : k - n, n - 1);
+      b[kb][e] = rowp[2 * kj + (k < n ? 0 : 1)];
+    }
+}
+template <int T2, int NW>
+__device__ __forceinline__ void green_bf_tiles(int N, int n, const double* __restrict__ phid,
+                                               const cplx* M, int ld, const GjState* st,
+                                               uint16_t* __restrict__ gh,
+                                               uint16_t* __restrict__ gl, int ld16, int lane,
+                                               int wv, double (&bpre)[GreenBf<T2>::KB][8]) {
+  constexpr int KB = GreenBf<T2>::KB;
+  const int n2 = 2 * n;
+  const int s = lane >> 4, col = lane & 15;
+  const int tp_n = (N + 15) >> 4;
+  bf16x8_t ah[T2][KB], al[T2][KB];
+  {
+    int xcol[T2];
+#pragma unroll
+    for (int tr = 0; tr < T2; ++tr) {
+      const int r = tr * 16 + col;  // (rows r >= 2n are not stored)
+      xcol[tr] = st->rhoinv[min(r < n ? r : r - n, n - 1)];
+    }
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      float v[T2][8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int k = kb * 32 + 8 * s + e;
+        const int xr = st->rho[min(k < n ? k : k - n, n - 1)] * ld;
+#pragma unroll
+        for (int tr = 0; tr < T2; ++tr) {
+          const int r = tr * 16 + col;
+          const cplx x = M[xr + xcol[tr]];  // X[kj][ri]
+          // (r < n, k < n): Xr; (r < n, k >= n): -Xi; (r >= n, k < n): Xi; else Xr
+          const double d = (r < n) == (k < n) ? x.re : ((r < n) ? -x.im : x.im);
+          v[tr][e] = k < n2 ? (float)d : 0.f;
+        }
+      }
+#pragma unroll
+      for (int tr = 0; tr < T2; ++tr) split8_bf(v[tr], ah[tr][kb], al[tr][kb]);
+    }
+  }
+  for (int tp = wv; tp < tp_n; tp += NW) {
+    bf16x8_t bh[KB], bl[KB];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      float f[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) f[e] = (float)bpre[kb][e];
+      split8_bf(f, bh[kb], bl[kb]);
+    }
+    if (tp + NW < tp_n) green_bf_load_b<T2>(phid, N, n, tp + NW, lane, bpre);
+    float4_t acc[T2];
+#pragma unroll
+    for (int tr = 0; tr < T2; ++tr) {
+      acc[tr] = float4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        acc[tr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[tr][kb], bh[kb], acc[tr], 0, 0, 0);
+        acc[tr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[tr][kb], bl[kb], acc[tr], 0, 0, 0);
+        acc[tr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[tr][kb], bh[kb], acc[tr], 0, 0, 0);
+      }
+    }
+    const int p = tp * 16 + col;
+    if (p >= N) continue;
+#pragma unroll
+    for (int tr = 0; tr < T2; ++tr)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = tr * 16 + 4 * s + j;
+        if (r < n2) {
+          const float v = acc[tr][j];
+          const uint16_t h = f32_to_bf16_rne(v);
+          gh[(int64_t)r * ld16 + p] = h;
+          gl[(int64_t)r * ld16 + p] = f32_to_bf16_rne(v - __uint_as_float((uint32_t)h << 16));
+        }
+      }
+  }
+}
+
 // n <= 24 (T2 <= 3): two waves per walker.  STAGE 2: the walker is staged in LDS (one
 // round of loads) and stays there for the Green product, with O and the inversion's
 // state after it; with two row tiles of O each wave then forms one of them over the whole
@@ -415,10 +535,17 @@ __device__ __forceinline__ void measure_narrow_body(
   };
   // each wave's first tile loads while the waves invert O
   double b[NKG];
-  if (want_green && wv < tp_n) load_b(b, wv);
+  double bbf[GreenBf<T2>::KB][8];
+  if (green && wv < tp_n) load_b(b, wv);
+  if (!green && ghi && wv < tp_n) green_bf_load_b<T2>(phid, N, n, wv, lane, bbf);
   gj_invert<NW, 2>(M, ld, n, st, lane, wv);
   if (threadIdx.x == 0) finish_walker(st->det, dets, w, spin, tail);
   if (!want_green) return;
+  if (!green) {  // split-bf16 planes only (mixed mode's step)
+    green_bf_tiles<T2, NW>(N, n, phid, M, ld, st, ghi + (int64_t)w * n2 * ld16,
+                           glo + (int64_t)w * n2 * ld16, ld16, lane, wv, bbf);
+    return;
+  }
 
   // A operand of every tile, from X in LDS, read once: the pivot maps of the lane's rows
   // and columns first, then every element, each load from a clamped index

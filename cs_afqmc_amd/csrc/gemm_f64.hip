@@ -18,6 +18,14 @@ namespace {
 
 constexpr int GEMM_BK = 16;
 constexpr int GEMM_PAD = 16;  // row stride == 16 (mod 32) doubles: k and k+1 rows hit disjoint bank halves
+// Element (k, m) of a k-major LDS tile sits at column m ^ gemm_swz<VEC>(k).  The MFMA
+// operand reads (16 consecutive m at k and k + 1 per 32 lanes) see the same banks as
+// without it: k and k + 1 share the swizzle, which keeps 16-aligned groups of m
+// together.  The staging stores (8 or 16 k rows x 4 or 2 m per 32 lanes) without it put
+// the rows of one k parity on one bank pair, an 8-way conflict; with it the 32 lanes
+// meet 32 bank pairs.
+template <int VEC>
+__device__ inline int gemm_swz(int k) { return ((k >> 1) & 7) << (VEC == 2 ? 2 : 1); }
 
 template <int BM, int BN, int WM, int WN, int VEC>
 __device__ __forceinline__ void gemm_nt_f64_body(
@@ -88,11 +96,13 @@ __device__ __forceinline__ void gemm_nt_f64_body(
 #pragma unroll
     for (int i = 0; i < A_LD; ++i)
 #pragma unroll
-      for (int v = 0; v < VEC; ++v) As[buf][lk + v][lr + i * RSTEP] = ra[i][v];
+      for (int v = 0; v < VEC; ++v)
+        As[buf][lk + v][(lr + i * RSTEP) ^ gemm_swz<VEC>(lk + v)] = ra[i][v];
 #pragma unroll
     for (int i = 0; i < B_LD; ++i)
 #pragma unroll
-      for (int v = 0; v < VEC; ++v) Bs[buf][lk + v][lr + i * RSTEP] = rb[i][v];
+      for (int v = 0; v < VEC; ++v)
+        Bs[buf][lk + v][(lr + i * RSTEP) ^ gemm_swz<VEC>(lk + v)] = rb[i][v];
   };
 
   double4_t acc[TM][TN];
@@ -116,10 +126,11 @@ __device__ __forceinline__ void gemm_nt_f64_body(
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       double af[TM], bf[TN];
+      const int sw = gemm_swz<VEC>(ks * 4 + kq);
 #pragma unroll
-      for (int i = 0; i < TM; ++i) af[i] = As[cur][ks * 4 + kq][am + i * 16];
+      for (int i = 0; i < TM; ++i) af[i] = As[cur][ks * 4 + kq][(am + i * 16) ^ sw];
 #pragma unroll
-      for (int j = 0; j < TN; ++j) bf[j] = Bs[cur][ks * 4 + kq][bn + j * 16];
+      for (int j = 0; j < TN; ++j) bf[j] = Bs[cur][ks * 4 + kq][(bn + j * 16) ^ sw];
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll

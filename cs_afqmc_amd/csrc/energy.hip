@@ -127,6 +127,14 @@ __device__ inline void exchange_tile(const T* __restrict__ Fs, int ldf, int n, i
 // accumulation and F tile (its D rows are 4 (l / 16) + r instead of l / 16 + 4 r); the
 // exchange sums still accumulate in float64.
 constexpr int EF_BM = 64, EF_BN = 128, EF_BK = 16, EF_PAD = 16;
+// Element (k, m) of a k-major operand tile sits at column m ^ ef_swz<F32, VEC>(k): the
+// MFMA operand reads see the same banks as without it, and the staging stores (lanes
+// along k) meet distinct banks instead of 8 (float64) or 4 (float32) k rows on one
+template <bool F32, int VEC>
+__device__ inline int ef_swz(int k) {
+  if constexpr (F32) return ((k >> 2) & 3) << (VEC == 2 ? 3 : 2);
+  return ((k >> 1) & 7) << (VEC == 2 ? 2 : 1);
+}
 constexpr int EF_LDF = 129;  // F tile row stride (odd: transposed reads spread over banks)
 constexpr int EF_OPS_BYTES = 2 * EF_BK * (EF_BM + EF_PAD) * 8 + 2 * EF_BK * (EF_BN + EF_PAD) * 8;
 constexpr int EF_F_BYTES = EF_BM * EF_LDF * 8;
@@ -192,11 +200,13 @@ __device__ __forceinline__ void exchange_f64_body(
 #pragma unroll
     for (int i = 0; i < A_LD; ++i)
 #pragma unroll
-      for (int v = 0; v < VEC; ++v) As[buf][lk + v][lr + i * RSTEP] = ra[i][v];
+      for (int v = 0; v < VEC; ++v)
+        As[buf][lk + v][(lr + i * RSTEP) ^ ef_swz<F32, VEC>(lk + v)] = ra[i][v];
 #pragma unroll
     for (int i = 0; i < B_LD; ++i)
 #pragma unroll
-      for (int v = 0; v < VEC; ++v) Bs[buf][lk + v][lr + i * RSTEP] = rb[i][v];
+      for (int v = 0; v < VEC; ++v)
+        Bs[buf][lk + v][(lr + i * RSTEP) ^ ef_swz<F32, VEC>(lk + v)] = rb[i][v];
   };
 
   AccT acc[TM][TN];
@@ -218,10 +228,11 @@ __device__ __forceinline__ void exchange_f64_body(
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       T af[TM], bf[TN];
+      const int sw = ef_swz<F32, VEC>(ks * 4 + kq);
 #pragma unroll
-      for (int i = 0; i < TM; ++i) af[i] = As[cur][ks * 4 + kq][am + i * 16];
+      for (int i = 0; i < TM; ++i) af[i] = As[cur][ks * 4 + kq][(am + i * 16) ^ sw];
 #pragma unroll
-      for (int j = 0; j < TN; ++j) bf[j] = Bs[cur][ks * 4 + kq][bn + j * 16];
+      for (int j = 0; j < TN; ++j) bf[j] = Bs[cur][ks * 4 + kq][(bn + j * 16) ^ sw];
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll

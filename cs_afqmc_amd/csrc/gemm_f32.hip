@@ -22,6 +22,13 @@ namespace {
 
 constexpr int F32_BK = 16;
 constexpr int F32_PAD = 16;  // row stride == 16 (mod 32) floats: rows k, k+1 use disjoint banks
+// Element (k, m) of a k-major LDS tile sits at column m ^ f32_swz<VEC>(k).  The MFMA
+// operand reads (16 consecutive m at k = 4 s .. 4 s + 3 per wave) share the swizzle and
+// see the same banks as without it; the staging stores (16 or 8 k rows x 4 or 8 m per
+// wave) without it put four k rows on one bank, a 4-way conflict, and with it the 64
+// lanes meet 64 banks.
+template <int VEC>
+__device__ inline int f32_swz(int k) { return ((k >> 2) & 3) << (VEC == 2 ? 3 : 2); }
 
 template <int BM, int BN, int WM, int WN, int VEC>
 __device__ __forceinline__ void gemm_nt_f32_body(
@@ -90,11 +97,13 @@ __device__ __forceinline__ void gemm_nt_f32_body(
 #pragma unroll
     for (int i = 0; i < A_LD; ++i)
 #pragma unroll
-      for (int v = 0; v < VEC; ++v) As[buf][lk + v][lr + i * RSTEP] = ra[i][v];
+      for (int v = 0; v < VEC; ++v)
+        As[buf][lk + v][(lr + i * RSTEP) ^ f32_swz<VEC>(lk + v)] = ra[i][v];
 #pragma unroll
     for (int i = 0; i < B_LD; ++i)
 #pragma unroll
-      for (int v = 0; v < VEC; ++v) Bs[buf][lk + v][lr + i * RSTEP] = rb[i][v];
+      for (int v = 0; v < VEC; ++v)
+        Bs[buf][lk + v][(lr + i * RSTEP) ^ f32_swz<VEC>(lk + v)] = rb[i][v];
   };
 
   float4_t acc[TM][TN];
@@ -118,10 +127,11 @@ __device__ __forceinline__ void gemm_nt_f32_body(
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       float af[TM], bf[TN];
+      const int sw = f32_swz<VEC>(ks * 4 + kq);
 #pragma unroll
-      for (int i = 0; i < TM; ++i) af[i] = As[cur][ks * 4 + kq][am + i * 16];
+      for (int i = 0; i < TM; ++i) af[i] = As[cur][ks * 4 + kq][(am + i * 16) ^ sw];
 #pragma unroll
-      for (int j = 0; j < TN; ++j) bf[j] = Bs[cur][ks * 4 + kq][bn + j * 16];
+      for (int j = 0; j < TN; ++j) bf[j] = Bs[cur][ks * 4 + kq][(bn + j * 16) ^ sw];
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll

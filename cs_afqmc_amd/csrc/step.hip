@@ -55,66 +55,114 @@ __device__ inline void block_sum4_first(double (&v)[4], double* red) {
   }
 }
 
+// Fields of walker w by one workgroup of NT threads, NT = 256 or 64.  The sums st and ft
+// always run in the order of 256 threads: virtual thread v adds its g = v, v + 256, ...;
+// then each virtual wave of 64 is reduced by the wave tree and the four wave sums as
+// (w0 + w2) + (w1 + w3) -- block_sum4_first's order.  With NT = 64 one wave carries the
+// four virtual waves (no LDS, no barrier), so both forms give bit-identical results and
+// the launch can take either.
+template <int NT>
 __device__ __forceinline__ void fields_body(
     int G, double sqdt, const double* __restrict__ fb, int nparts, int64_t part_stride,
     const cplx* __restrict__ mf, const double* __restrict__ fields, double* __restrict__ xs,
     uint16_t* __restrict__ xs_hi, uint16_t* __restrict__ xs_lo, int Gp,
     cplx* __restrict__ terms, cplx* __restrict__ fb_out, const int w) {
-  __shared__ double red[4 * (FIELDS_NT / 64)];
+  constexpr int VT = FIELDS_NT;  // threads whose summation order is kept
+  constexpr int R = VT / NT;     // virtual threads per thread
+  static_assert(VT % NT == 0 && NT % 64 == 0, "whole waves");
   const double* x = fields + (int64_t)w * G;
   double* xr = xs + (int64_t)w * 2 * G;
   double* xi = xr + G;
-  cplx st = cplx{0.0, 0.0}, ft = cplx{0.0, 0.0};
-  for (int g = threadIdx.x; g < G; g += FIELDS_NT) {
-    const cplx f = fb_sum(fb, nparts, part_stride, w, G, g);
-    const cplx m = mf[g];
-    // field_shifts = -sqrt(dt) * (1j * fb - mf_shifts)
-    const cplx ifb_m = cplx{-f.im - m.re, f.re - m.im};
-    const cplx fs = cplx{-sqdt * ifb_m.re, -sqdt * ifb_m.im};
-    const double xv = x[g];
-    // shifted_fields = fields - field_shifts
-    const cplx xt = cplx{xv - fs.re, -fs.im};
-    st = cadd(st, cmul(xt, m));
-    // fields * field_shifts - field_shifts * field_shifts / 2
-    const cplx fs2 = cmul(fs, fs);
-    ft = cadd(ft, cplx{xv * fs.re - fs2.re / 2.0, xv * fs.im - fs2.im / 2.0});
-    // VHS operand: 1j * sqrt(dt) * shifted_fields (float64, or split-bf16 planes)
-    if (xs_hi) {
-      const int64_t o = (int64_t)w * 2 * Gp + g;
-      split_bf16(-sqdt * xt.im, &xs_hi[o], &xs_lo[o]);
-      split_bf16(sqdt * xt.re, &xs_hi[o + Gp], &xs_lo[o + Gp]);
-    } else {
-      xr[g] = -sqdt * xt.im;
-      xi[g] = sqdt * xt.re;
+  cplx st[R], ft[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) st[r] = ft[r] = cplx{0.0, 0.0};
+  for (int g0 = 0; g0 < G; g0 += VT) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int g = g0 + (int)threadIdx.x + NT * r;
+      if (g >= G) continue;
+      const cplx f = fb_sum(fb, nparts, part_stride, w, G, g);
+      const cplx m = mf[g];
+      // field_shifts = -sqrt(dt) * (1j * fb - mf_shifts)
+      const cplx ifb_m = cplx{-f.im - m.re, f.re - m.im};
+      const cplx fs = cplx{-sqdt * ifb_m.re, -sqdt * ifb_m.im};
+      const double xv = x[g];
+      // shifted_fields = fields - field_shifts
+      const cplx xt = cplx{xv - fs.re, -fs.im};
+      st[r] = cadd(st[r], cmul(xt, m));
+      // fields * field_shifts - field_shifts * field_shifts / 2
+      const cplx fs2 = cmul(fs, fs);
+      ft[r] = cadd(ft[r], cplx{xv * fs.re - fs2.re / 2.0, xv * fs.im - fs2.im / 2.0});
+      // VHS operand: 1j * sqrt(dt) * shifted_fields (float64, or split-bf16 planes)
+      if (xs_hi) {
+        const int64_t o = (int64_t)w * 2 * Gp + g;
+        split_bf16(-sqdt * xt.im, &xs_hi[o], &xs_lo[o]);
+        split_bf16(sqdt * xt.re, &xs_hi[o + Gp], &xs_lo[o + Gp]);
+      } else {
+        xr[g] = -sqdt * xt.im;
+        xi[g] = sqdt * xt.re;
+      }
+      if (fb_out) fb_out[(int64_t)w * G + g] = f;
     }
-    if (fb_out) fb_out[(int64_t)w * G + g] = f;
   }
-  double v[4] = {st.re, st.im, ft.re, ft.im};
-  block_sum4_first<FIELDS_NT>(v, red);
+  double v[4];
+  if constexpr (R == 1) {
+    __shared__ double red[4 * (VT / 64)];
+    v[0] = st[0].re;
+    v[1] = st[0].im;
+    v[2] = ft[0].re;
+    v[3] = ft[0].im;
+    block_sum4_first<VT>(v, red);
+  } else {
+    static_assert(R == 4, "four virtual waves");
+    double ws[R][4];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      ws[r][0] = st[r].re;
+      ws[r][1] = st[r].im;
+      ws[r][2] = ft[r].re;
+      ws[r][3] = ft[r].im;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ws[r][c] += __shfl_down(ws[r][c], off, 64);
+    }
+    // block_sum4_first's second stage over the wave sums at lanes 0..3 of one wave:
+    // each sum first meets zeros (x + 0), then (w0 + w2) + (w1 + w3)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      v[c] = ((ws[0][c] + 0.0) + (ws[2][c] + 0.0)) + ((ws[1][c] + 0.0) + (ws[3][c] + 0.0));
+  }
   if (threadIdx.x == 0) {
     terms[2 * w] = cplx{v[0], v[1]};
     terms[2 * w + 1] = cplx{v[2], v[3]};
   }
 }
 
-__global__ __launch_bounds__(FIELDS_NT) void fields_kernel(
+template <int NT>
+__global__ __launch_bounds__(NT) void fields_kernel(
     int G, double sqdt, const double* __restrict__ fb, int nparts, int64_t part_stride,
     const cplx* __restrict__ mf, const double* __restrict__ fields, double* __restrict__ xs,
     uint16_t* __restrict__ xs_hi, uint16_t* __restrict__ xs_lo, int Gp,
     cplx* __restrict__ terms, cplx* __restrict__ fb_out) {
-  fields_body(G, sqdt, fb, nparts, part_stride, mf, fields, xs, xs_hi, xs_lo, Gp, terms, fb_out,
-              blockIdx.x);
+  fields_body<NT>(G, sqdt, fb, nparts, part_stride, mf, fields, xs, xs_hi, xs_lo, Gp, terms,
+                  fb_out, blockIdx.x);
 }
 
 // one block per walker of every fragment; step selects the fields of this step
-__global__ __launch_bounds__(FIELDS_NT) void fields_grouped_kernel(
+template <int NT>
+__global__ __launch_bounds__(NT) void fields_grouped_kernel(
     const FieldsArgs* __restrict__ args, const int* __restrict__ start, int nprob, int step) {
   const int p = group_of(start, nprob, blockIdx.x);
   const FieldsArgs a = args[p];
-  fields_body(a.G, a.sqdt, a.fb, a.nparts, a.part_stride, a.mf,
-              a.fields + (int64_t)step * a.W * a.G, a.xs, a.xs_hi, a.xs_lo, a.Gp, a.terms,
-              nullptr, blockIdx.x - start[p]);
+  fields_body<NT>(a.G, a.sqdt, a.fb, a.nparts, a.part_stride, a.mf,
+                  a.fields + (int64_t)step * a.W * a.G, a.xs, a.xs_hi, a.xs_lo, a.Gp, a.terms,
+                  nullptr, blockIdx.x - start[p]);
 }
+
+// threads per walker of the fields kernels: one wave when there are walkers enough to
+// fill the chip with waves, else four (the results do not depend on the choice)
+inline int fields_nt(int64_t walkers) { return walkers >= 8192 ? 64 : FIELDS_NT; }
 
 // single workgroup over all walkers (the weight sum needs every walker)
 __device__ __forceinline__ void weights_body(
@@ -261,7 +309,8 @@ __global__ void fb_out_kernel(int W, int G, const double* __restrict__ fb, int n
 }  // namespace
 
 int launch_fields(afqmc_ctx* c, int W, const double* fields, cplx* fb_out) {
-  hipLaunchKernelGGL(fields_kernel, dim3(W), dim3(FIELDS_NT), 0, c->stream, c->d.nchol,
+  const auto kern = fields_nt(W) == 64 ? fields_kernel<64> : fields_kernel<FIELDS_NT>;
+  hipLaunchKernelGGL(kern, dim3(W), dim3(fields_nt(W)), 0, c->stream, c->d.nchol,
                      sqrt(c->d.dt), c->fb, c->fb_parts, (int64_t)2 * W * c->d.nchol, c->mf_shifts,
                      fields, c->xs, c->mixed ? c->xs_bf[0] : nullptr,
                      c->mixed ? c->xs_bf[1] : nullptr, c->Gp, c->terms, fb_out);
@@ -324,8 +373,10 @@ int launch_fb_out(afqmc_ctx* c, int W, cplx* fb_out) {
 
 int launch_fields_grouped(hipStream_t st, const FieldsArgs* args, const int* start, int nprob,
                           int nblocks, int step) {
-  hipLaunchKernelGGL(fields_grouped_kernel, dim3(nblocks), dim3(FIELDS_NT), 0, st, args, start,
-                     nprob, step);
+  const auto kern = fields_nt(nblocks) == 64 ? fields_grouped_kernel<64>
+                                               : fields_grouped_kernel<FIELDS_NT>;
+  hipLaunchKernelGGL(kern, dim3(nblocks), dim3(fields_nt(nblocks)), 0, st, args, start, nprob,
+                     step);
   LAUNCH_CHECK();
   return AFQMC_OK;
 }

@@ -578,18 +578,25 @@ __global__ __launch_bounds__(512, bf_waves_per_simd(KB, CTP)) void taylor_bf_gro
 
 // k blocks of the packed [Vr | Vi] axis
 inline int bf_kblocks(int N) { return (2 * bf_nh(N) + 31) / 32; }
+// threads of a register-form workgroup: a wave per 16 rows, with 6 or 7 waves rounded up
+// to 8 (the extra waves own rows >= N: zero work, but every SIMD then holds two waves of
+// the group at its barriers, and at four waves per SIMD two such groups fill a CU either
+// way); 5 waves stay 5, which lets three groups share a CU
+inline int bf_threads(int N) {
+  const int w = (N + 15) / 16;
+  return 64 * ((w == 6 || w == 7) ? 8 : w);
+}
 
 template <int KB, int CTP>
 int launch_bf(afqmc_ctx* c, int W, int n, cplx* wk, int s) {
   const int N = c->d.norb;
   if (BfTile<KB>::KH * 32 != c->Np32 || bf_kblocks(N) != KB)
     return afqmc_set_error(AFQMC_ERR_ARG, "taylor bf16x3: k blocks %d do not fit N %d", KB, N);
-  const int nwaves = (N + 15) / 16;
   const size_t shm = (size_t)2 * CTP * 16 * BfTile<KB>::LDK * sizeof(uint16_t);
   static int lds_limit = 0;
   const int rc = ensure_dyn_lds((const void*)taylor_bf_kernel<KB, CTP>, shm, &lds_limit);
   if (rc) return rc;
-  hipLaunchKernelGGL((taylor_bf_kernel<KB, CTP>), dim3(W), dim3(nwaves * 64), shm, c->stream, N,
+  hipLaunchKernelGGL((taylor_bf_kernel<KB, CTP>), dim3(W), dim3(bf_threads(N)), shm, c->stream, N,
                      n, c->d.n_exp_terms - 1, wk, c->emi_bf[s][0], c->emi_bf[s][1],
                      vhs_f32_of(c, s), c->vhs_nv);
   LAUNCH_CHECK();
@@ -1007,6 +1014,7 @@ void taylor_plan(const afqmc_ctx* c, int s, TaylorPlan* pl) {
     if (N <= 128 && n <= 32) {
       const int kb = bf_kblocks(N);
       pl->kind = TAYLOR_REG_BF16X3;
+      pl->nthreads = bf_threads(N);
       pl->nkgv = kb;
       pl->ct = ctp;
       pl->shm = (size_t)2 * ctp * 16 * bf_ldk(kb) * sizeof(uint16_t);

@@ -369,7 +369,7 @@ def all_rooflines(precision, phases, fl, by, kernel_of):
     return out
 
 
-# ------------------------------------------------------------------ config 4: fragments
+# ------------------------------------------------------------------ 64 fragments, N 20-120, W 256
 
 
 def _make_fragments(specs, precision):
@@ -479,7 +479,7 @@ def run_fragments(args, modes):
     return specs, runs
 
 
-# ------------------------------------------------------------------ config 2/3: one problem
+# ------------------------------------------------------------------ one problem (RHF N100 / UHF N200)
 
 
 def run_single(workload, precision, warm_blocks, timed_blocks, n_ene_blocks=4):
@@ -578,7 +578,7 @@ def run_single(workload, precision, warm_blocks, timed_blocks, n_ene_blocks=4):
     return res
 
 
-# ------------------------------------------------------------------ H10 (config 1)
+# ------------------------------------------------------------------ H10 chain, 1.6 bohr, STO-6G
 
 
 def run_h10(precision, n_calls):
@@ -617,7 +617,7 @@ def run_h10(precision, n_calls):
             "value": 200 * 50 * (n_calls - 2) / (ms / 1e3)}
 
 
-# ------------------------------------------------------------------ config 5: serving
+# ------------------------------------------------------------------ serving: Poisson 200 req/s, N 20-250
 
 
 def serving_draws(n_requests):

@@ -47,6 +47,7 @@ struct afqmc_group {
   bool mixed = false;
   bool f32 = false;           // FP32 mode: the float64 operands on float32 MFMA
   int fb_bm = 0, vhs_bm = 0;  // GEMM tile edge (0: heuristic)
+  int cg_bm = 0;              // tile edge of the energy's float64 c_g product (0: heuristic)
   bool f64_vec2 = true;
   // built tables
   std::vector<Launch> meas, taylor;
@@ -464,7 +465,7 @@ int build(afqmc_group* g) {
     for (const Member& m : g->mem)
       for (int s = 0; s < m.c->nspin; ++s)
         t128 += (int64_t)((2 * m.c->d.n_walkers + 127) / 128) * ((m.c->d.nchol + 127) / 128);
-    const int bm = heuristic_bm(t128);
+    const int bm = g->cg_bm ? g->cg_bm : heuristic_bm(t128);
     g->fb64_vec2 = true;
     int b = 0;
     for (const Member& m : g->mem) {
@@ -915,9 +916,13 @@ extern "C" int afqmc_group_prof_collect(afqmc_group* g, double* ms, int64_t* cou
   return prof_collect(&g->prof, ms, counts);
 }
 
-// Times both GEMM tile edges (64, 128) for the force-bias and VHS launches of the current
-// member set and keeps the faster one.  Runs the GEMMs on the bound buffers (their
-// outputs are workspace that every step rewrites).  Synchronises.
+// Times both GEMM tile edges (64, 128) for every grouped GEMM of the current member set --
+// the step's force-bias and VHS products and the energy's float64 c_g product -- and keeps
+// the faster edge for each.  Runs the GEMMs on the bound buffers: their outputs (force
+// bias, VHS) are workspace that the next step or energy call rewrites before reading, and
+// the float64 Green's functions and rotated vectors the c_g product reads are allocated
+// with the context.  The k order of every output is the same for both edges, so the choice
+// does not change results.  Synchronises.
 extern "C" int afqmc_group_tune(afqmc_group* g, double* ms_out) {
   if (!g) return afqmc_set_error(AFQMC_ERR_ARG, "null group");
   if (g->mem.empty()) return AFQMC_OK;
@@ -926,16 +931,22 @@ extern "C" int afqmc_group_tune(afqmc_group* g, double* ms_out) {
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
-  float best[2][2];  // [fb / vhs][64 / 128]
+  float best[3][2];  // [fb / vhs / c_g][64 / 128]
+  auto run = [&](int which) {
+    if (which == 2)
+      return launch_gemm_f64_grouped(g->stream, g->fb64.k0, g->fb64_vec2,
+                                     targs<GemmArgs>(g, g->fb64), tstart(g, g->fb64),
+                                     g->fb64.nprob, g->fb64.nblocks);
+    return run_gemm(g, which ? g->vhs : g->fb, which);
+  };
   for (int bi = 0; bi < 2; ++bi) {
-    g->fb_bm = g->vhs_bm = bi ? 128 : 64;
+    g->fb_bm = g->vhs_bm = g->cg_bm = bi ? 128 : 64;
     if ((rc = build(g))) break;
-    for (int which = 0; which < 2 && !rc; ++which) {
-      const Launch& L = which ? g->vhs : g->fb;
-      rc = run_gemm(g, L, which);  // warm-up
+    for (int which = 0; which < 3 && !rc; ++which) {
+      rc = run(which);  // warm-up
       if (rc) break;
       (void)hipEventRecord(e0, g->stream);
-      for (int r = 0; r < 3 && !rc; ++r) rc = run_gemm(g, L, which);
+      for (int r = 0; r < 3 && !rc; ++r) rc = run(which);
       (void)hipEventRecord(e1, g->stream);
       (void)hipEventSynchronize(e1);
       float ms = 0.f;
@@ -949,12 +960,12 @@ extern "C" int afqmc_group_tune(afqmc_group* g, double* ms_out) {
   if (rc) return rc;
   g->fb_bm = best[0][1] < best[0][0] ? 128 : 64;
   g->vhs_bm = best[1][1] < best[1][0] ? 128 : 64;
-  if (ms_out) {
-    ms_out[0] = best[0][0];
-    ms_out[1] = best[0][1];
-    ms_out[2] = best[1][0];
-    ms_out[3] = best[1][1];
-  }
+  g->cg_bm = best[2][1] < best[2][0] ? 128 : 64;
+  if (ms_out)
+    for (int w = 0; w < 3; ++w) {
+      ms_out[2 * w] = best[w][0];
+      ms_out[2 * w + 1] = best[w][1];
+    }
   g->dirty = true;
   return AFQMC_OK;
 }

@@ -467,8 +467,9 @@ class DeviceGroup:
         return {p: (float(ms[i]), int(cnt[i])) for i, p in enumerate(_lib.PHASES)}
 
     def tune(self):
-        """Pick GEMM tiles by timing; returns {fb: (ms64, ms128), vhs: (ms64, ms128)}."""
+        """Pick GEMM tiles by timing; returns {fb, vhs, cg: (ms64, ms128)} (cg: the
+        energy's float64 force-bias product)."""
         self._prep()
-        ms = np.zeros(4, dtype=np.float64)
+        ms = np.zeros(6, dtype=np.float64)
         _lib.check(self._lib.afqmc_group_tune(self._h, ms.ctypes.data_as(ctypes.c_void_p)))
-        return {"fb": (ms[0], ms[1]), "vhs": (ms[2], ms[3])}
+        return {"fb": (ms[0], ms[1]), "vhs": (ms[2], ms[3]), "cg": (ms[4], ms[5])}

@@ -310,9 +310,10 @@ AFQMC_API int afqmc_group_fields_normal_key(afqmc_group* g, void* const* keys);
 AFQMC_API int afqmc_group_prof_enable(afqmc_group* g, int32_t on);
 AFQMC_API int afqmc_group_prof_phases(afqmc_group* g, uint32_t mask);
 AFQMC_API int afqmc_group_prof_collect(afqmc_group* g, double* ms, int64_t* counts);
-/* Picks the GEMM tile edge (64 or 128) of the force-bias and VHS launches by timing
- * both on the current members; ms_out (4 floats64, may be NULL) receives the times
- * [fb 64, fb 128, vhs 64, vhs 128].  Synchronises. */
+/* Picks the GEMM tile edge (64 or 128) of the force-bias and VHS launches and of the
+ * energy's float64 c_g product by timing both edges on the current members; ms_out
+ * (6 float64, may be NULL) receives the times [fb 64, fb 128, vhs 64, vhs 128, c_g 64,
+ * c_g 128].  The edge does not change results.  Synchronises. */
 AFQMC_API int afqmc_group_tune(afqmc_group* g, double* ms_out);
 
 #ifdef __cplusplus

@@ -163,6 +163,37 @@ def test_grouped_block_ends_match_each_context(gpu, precision):
         assert np.max(np.abs(e_np - r_np)) <= 1e-12 * np.max(np.abs(r_np))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_tuned_group_energy_matches_each_context(gpu, precision):
+    """afqmc_group_tune times both tile edges of every grouped GEMM, the energy's float64
+    c_g product included, on the bound buffers.  Afterwards the grouped energy (with the
+    chosen c_g tile) still matches each context's own afqmc_energy to 1e-12 relative, and
+    the grouped energy computed before tuning bit for bit (the k order of a GEMM output
+    does not depend on the tile edge)."""
+    frags = [fragment(s, precision, 4) for s in SPECS]
+    b = batch.FragmentBatch(frags)
+    b.run_block()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(b.stream):
+        before = [torch.empty_like(f.energies) for f in frags]
+        b.group.energy(before)
+    torch.cuda.synchronize()
+    tuned = b.tune()
+    assert set(tuned) == {"fb", "vhs", "cg"}
+    assert all(t > 0 for pair in tuned.values() for t in pair)
+    with torch.cuda.stream(b.stream):
+        after = [torch.empty_like(f.energies) for f in frags]
+        b.group.energy(after)
+    torch.cuda.synchronize()
+    for f, e0, e in zip(frags, before, after):
+        assert torch.equal(e0, e)
+        er = f.dev.energy(f.prop_data["walkers"])
+        torch.cuda.synchronize()
+        e_np, r_np = e.cpu().numpy(), er.cpu().numpy()
+        assert np.all(np.isfinite(e_np))
+        assert np.max(np.abs(e_np - r_np)) <= 1e-12 * np.max(np.abs(r_np))
+
+
 def test_grouped_fields_draw_matches_each_context(gpu):
     """afqmc_group_fields_normal_key against each member's afqmc_fields_normal_key from a
     copy of its key: the fields blocks bit for bit, the keys advanced alike."""

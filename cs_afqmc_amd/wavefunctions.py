@@ -205,34 +205,43 @@ def _eigh_vectors(a):
         return np.linalg.eigh(a)[1]
 
 
-def _init_walkers(trial, wave_data, n_walkers, restricted):
-    """wavefunctions.py:298-355: natural orbitals (descending occupation) of each spin
-    rdm1, copied n_walkers times, as cuda complex128 tensors."""
-    rdm1 = trial.get_rdm1(wave_data)
+def init_orbitals(rdm1, nelec, restricted):
+    """wavefunctions.py:298-355: the starting walker orbitals as numpy arrays -- natural
+    orbitals (descending occupation) of each spin rdm1; restricted walkers take the
+    spin-up ones when |det(up^T dn)| > 1e-3, else the QR-orthonormalised column sums
+    up_i + sign(up_i . dn_i) dn_i, if those overlap both spins' orbitals by more than
+    1e-3 in |det(Q^T up) det(Q^T dn)|.  Returns one (N, n) array when restricted, else
+    [up (N, n_up), dn (N, n_dn)]."""
     vec_up = _eigh_vectors(rdm1[0])
     # identical spin densities (closed shells) share one decomposition
     vec_dn = vec_up if np.array_equal(rdm1[0], rdm1[1]) else _eigh_vectors(rdm1[1])
-    natorbs_up = vec_up[:, ::-1][:, : trial.nelec[0]]
-    natorbs_dn = vec_dn[:, ::-1][:, : trial.nelec[1]]
+    natorbs_up = vec_up[:, ::-1][:, : nelec[0]]
+    natorbs_dn = vec_dn[:, ::-1][:, : nelec[1]]
+    if not restricted:
+        return [natorbs_up, natorbs_dn]
+    if nelec[0] != nelec[1]:
+        raise NotImplementedError("restricted walkers need nelec[0] == nelec[1]")
+    det_overlap = np.linalg.det(natorbs_up.T @ natorbs_dn)
+    if np.abs(det_overlap) > 1e-3:
+        return natorbs_up
+    overlaps = np.array([natorbs_up[:, i].T @ natorbs_dn[:, i] for i in range(nelec[0])])
+    new_vecs = natorbs_up + np.einsum("ij,j->ij", natorbs_dn, np.sign(overlaps))
+    new_vecs = np.linalg.qr(new_vecs)[0]
+    det_overlap = np.linalg.det(new_vecs.T @ natorbs_up) * np.linalg.det(
+        new_vecs.T @ natorbs_dn)
+    if np.abs(det_overlap) > 1e-3:
+        return new_vecs
+    raise ValueError("Cannot find a set of RHF orbitals with good trial overlap.")
+
+
+def _init_walkers(trial, wave_data, n_walkers, restricted):
+    """init_orbitals of the trial's rdm1, copied n_walkers times, as cuda complex128
+    tensors."""
+    orbs = init_orbitals(trial.get_rdm1(wave_data), trial.nelec, restricted)
     dev = torch.device("cuda", torch.cuda.current_device())
 
     def rep(m):
         t = torch.as_tensor(np.ascontiguousarray(m + 0.0j), device=dev)
         return t.unsqueeze(0).repeat(n_walkers, 1, 1).contiguous()
 
-    if restricted:
-        if trial.nelec[0] != trial.nelec[1]:
-            raise NotImplementedError("restricted walkers need nelec[0] == nelec[1]")
-        det_overlap = np.linalg.det(natorbs_up.T @ natorbs_dn)
-        if np.abs(det_overlap) > 1e-3:
-            return rep(natorbs_up)
-        overlaps = np.array(
-            [natorbs_up[:, i].T @ natorbs_dn[:, i] for i in range(trial.nelec[0])])
-        new_vecs = natorbs_up + np.einsum("ij,j->ij", natorbs_dn, np.sign(overlaps))
-        new_vecs = np.linalg.qr(new_vecs)[0]
-        det_overlap = np.linalg.det(new_vecs.T @ natorbs_up) * np.linalg.det(
-            new_vecs.T @ natorbs_dn)
-        if np.abs(det_overlap) > 1e-3:
-            return rep(new_vecs)
-        raise ValueError("Cannot find a set of RHF orbitals with good trial overlap.")
-    return [rep(natorbs_up), rep(natorbs_dn)]
+    return rep(orbs) if restricted else [rep(m) for m in orbs]

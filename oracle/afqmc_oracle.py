@@ -26,6 +26,9 @@ Pinning (what checks each function besides its own restatement; see tests/):
     expm(-dt h/2) (scipy); chol == 0 gives expm(-dt h1_mod) exactly.
   * propagate: field shift vanishes at walker == trial; closed-form single-walker
     weight update with chol == 0; clipping branches (NaN, < 1e-3, > 100, w > 100).
+  * get_init_walkers: restricted fallback (|det(up^T dn)| <= 1e-3) against closed
+    forms for orthogonal column sums (columns, det product), the ValueError case, and
+    the spin-up branch above the threshold.
   * orthonormalize: numpy.linalg.qr is LAPACK geqrf + orgqr/ungqr, the routine
     jnp.linalg.qr calls on CPU; invariants Q^H Q = I, Q R = walker, real diag(R) with
     the zlarfg sign rule.
@@ -212,18 +215,25 @@ def build_measurement_intermediates(trial, ham_data, wave_data, norb):
 
 
 def get_init_walkers(rdm1, nelec, n_walkers, restricted):
-    """wavefunctions.py:298-355 (wave_function.get_init_walkers), common branches:
-    natural orbitals (descending occupation) of each spin rdm1, copied n_walkers
-    times; the restricted equal-spin case requires |det(up^T dn)| > 1e-3."""
+    """wavefunctions.py:298-355 (wave_function.get_init_walkers): natural orbitals
+    (descending occupation) of each spin rdm1, copied n_walkers times.  Restricted
+    walkers (equal spin counts) take the spin-up orbitals when |det(up^T dn)| > 1e-3
+    (:316-324); otherwise (:325-342) the columns up_i + sign(up_i . dn_i) dn_i,
+    orthonormalised by QR, if |det(Q^T up) det(Q^T dn)| > 1e-3, else ValueError."""
     natorbs_up = np.linalg.eigh(rdm1[0])[1][:, ::-1][:, : nelec[0]]
     natorbs_dn = np.linalg.eigh(rdm1[1])[1][:, ::-1][:, : nelec[1]]
     if restricted:
         if nelec[0] != nelec[1]:
             raise NotImplementedError("restricted walkers need nelec[0] == nelec[1]")
         det_overlap = np.linalg.det(natorbs_up.T @ natorbs_dn)
-        if np.abs(det_overlap) <= 1e-3:
-            raise NotImplementedError("spin-projected RHF start not restated")
-        return np.array([natorbs_up + 0.0j] * n_walkers)
+        if np.abs(det_overlap) > 1e-3:
+            return np.array([natorbs_up + 0.0j] * n_walkers)
+        signs = np.sign(np.sum(natorbs_up * natorbs_dn, axis=0))
+        q = np.linalg.qr(natorbs_up + natorbs_dn * signs[None, :])[0]
+        det_overlap = np.linalg.det(q.T @ natorbs_up) * np.linalg.det(q.T @ natorbs_dn)
+        if np.abs(det_overlap) > 1e-3:
+            return np.array([q + 0.0j] * n_walkers)
+        raise ValueError("no RHF orbitals with a trial overlap above 1e-3")
     return [
         np.array([natorbs_up + 0.0j] * n_walkers),
         np.array([natorbs_dn + 0.0j] * n_walkers),

@@ -295,3 +295,85 @@ def test_corr_outlier_rule_closed_form():
     assert np.array_equal(a[keep], w1[keep]) and np.array_equal(b[keep], w2[keep])
     c, d = o.corr_outlier_rm(np.ones(8), np.ones(8))
     assert np.array_equal(c, np.ones(8)) and np.array_equal(d, np.ones(8))
+
+
+# ------------------------------------------------------ restricted start-orbital fallback
+
+
+def _rdm1(vecs, occs):
+    """sum_i occ_i v_i v_i^T: natural orbitals v_i with distinct occupations, so the
+    eigenvectors are fixed up to sign."""
+    return sum(oc * np.outer(v, v) for v, oc in zip(vecs, occs))
+
+
+def _unit(v):
+    return v / np.linalg.norm(v)
+
+
+def _fallback_case(n, eps):
+    """Spin-up natural orbitals e_0 .. e_{n-1}, spin-down d_i = unit(e_{n+i} + eps e_i):
+    det(up^T dn) = (eps / sqrt(1 + eps^2))^n, and the columns e_i + d_i are mutually
+    orthogonal, so the fallback's QR leaves them as they are, normalised (up to sign)."""
+    norb = 2 * n + 2
+    e = np.eye(norb)
+    up = [e[i] for i in range(n)]
+    dn = [_unit(e[n + i] + eps * e[i]) for i in range(n)]
+    occs = [0.95 - 0.1 * i for i in range(n)]
+    rdm1 = np.array([_rdm1(up, occs), _rdm1(dn, occs)])
+    expect = [_unit(u + d) for u, d in zip(up, dn)]
+    return rdm1, (n, n), expect, up, dn
+
+
+@pytest.mark.parametrize("n,eps", [(1, 5e-4), (2, 1e-2), (3, 0.05)])
+def test_restricted_fallback_closed_form(n, eps):
+    """wavefunctions.py:325-342: with |det(up^T dn)| <= 1e-3 the restricted start is
+    QR(up_i + sign(up_i . dn_i) dn_i).  Closed form for orthogonal column sums: the
+    columns unit(e_i + d_i) up to sign, and |det(Q^T up) det(Q^T dn)| =
+    ((1 + c) / sqrt(2 + 2 c))^(2n) with c = eps / sqrt(1 + eps^2)."""
+    rdm1, nelec, expect, up, dn = _fallback_case(n, eps)
+    c = eps / np.sqrt(1 + eps**2)
+    assert c**n <= 1e-3  # the fallback branch is taken
+    w = o.get_init_walkers(rdm1, nelec, 3, restricted=True)
+    assert w.shape == (3, rdm1.shape[1], n)
+    q = w[0].real
+    assert np.allclose(q.T @ q, np.eye(n), atol=1e-13)
+    for j in range(n):
+        assert abs(abs(q[:, j] @ expect[j]) - 1.0) < 1e-12
+    det = np.linalg.det(q.T @ np.array(up).T) * np.linalg.det(q.T @ np.array(dn).T)
+    assert abs(abs(det) - ((1 + c) / np.sqrt(2 + 2 * c)) ** (2 * n)) < 1e-12
+
+
+def test_restricted_fallback_raises_without_overlap():
+    """Orthogonal spin orbitals: sign(0) = 0 drops the spin-down columns, the QR gives the
+    spin-up orbitals, det(Q^T dn) = 0, and the reference raises ValueError (:339-342)."""
+    e = np.eye(4)
+    rdm1 = np.array([np.outer(e[0], e[0]), np.outer(e[1], e[1])])
+    with pytest.raises(ValueError):
+        o.get_init_walkers(rdm1, (1, 1), 1, restricted=True)
+
+
+def test_restricted_start_above_threshold_takes_spin_up():
+    """|det(up^T dn)| > 1e-3 keeps the spin-up natural orbitals (:316-324)."""
+    rdm1, nelec, _, up, _ = _fallback_case(1, 0.1)
+    w = o.get_init_walkers(rdm1, nelec, 1, restricted=True)
+    assert abs(abs(w[0][:, 0].real @ up[0]) - 1.0) < 1e-14
+
+
+@pytest.mark.parametrize("n,eps", [(1, 5e-4), (2, 1e-2), (3, 0.05), (2, 0.5)])
+def test_product_start_orbitals_match_oracle(n, eps):
+    """cs_afqmc_amd.wavefunctions.init_orbitals (the host part of get_init_walkers) equals
+    the oracle on the same rdm1, in both branches, and raises where the oracle does."""
+    from cs_afqmc_amd.wavefunctions import init_orbitals
+
+    rdm1, nelec, *_ = _fallback_case(n, eps)
+    got = init_orbitals(rdm1, nelec, True)
+    ref = o.get_init_walkers(rdm1, nelec, 1, restricted=True)[0]
+    assert np.allclose(got, ref.real, atol=1e-14)
+    up, dn = init_orbitals(rdm1, nelec, False)
+    ref_up, ref_dn = o.get_init_walkers(rdm1, nelec, 1, restricted=False)
+    assert np.allclose(up, ref_up[0].real, atol=1e-14)
+    assert np.allclose(dn, ref_dn[0].real, atol=1e-14)
+    e = np.eye(4)
+    bad = np.array([np.outer(e[0], e[0]), np.outer(e[1], e[1])])
+    with pytest.raises(ValueError):
+        init_orbitals(bad, (1, 1), True)

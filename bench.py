@@ -612,8 +612,25 @@ def run_h10(precision, n_calls):
     e1.record(stream)
     e1.synchronize()
     ms = e0.elapsed_time(e1)
+    # roofline of the dominant step phase: one eager 5-step block with every phase's event
+    # pair, then one with the dominant phase's only (after the timed calls)
+    dev, st, prop, trial = pd["_afqmc_device"], pd["_block_state"], setup["prop"], setup["trial"]
+    norb, nelec = trial.norb, tuple(trial.nelec)
+    fl, by = phase_work(precision, [("rhf", norb, nelec, setup["nchol"], prop.n_walkers,
+                                     dev.vhs_entries(), prop.n_exp_terms)])
+    kern = lambda p: phase_kernel(precision, p, norb, nelec)  # noqa: E731
+    dev.prof_collect()
+    dev.prof_enable(True)
+    st._block()
+    phases = dev.prof_collect()
+    dom, _ = dominant_roofline(precision, phases, fl, by, kern, H10_WORKLOAD)
+    dev.prof_enable(True, phases=[dom])
+    st._block()
+    timed_ph = dev.prof_collect()
+    dev.prof_enable(False, phases=None)
+    _, roof = dominant_roofline(precision, {dom: timed_ph[dom]}, fl, by, kern, H10_WORKLOAD)
     return {"stats": np.concatenate(rows), "sr_idx": idx, "nchol": setup["nchol"],
-            "e_hf": setup["e_hf"],
+            "e_hf": setup["e_hf"], "roofline": roof,
             "value": 200 * 50 * (n_calls - 2) / (ms / 1e3)}
 
 
@@ -878,7 +895,8 @@ def main():
                                              idx_b=h["fp64"]["sr_idx"], sr_every=10)])
         h10 = {"workload": H10_WORKLOAD, "nchol": h["fp64"]["nchol"],
                "e_rhf": h["fp64"]["e_hf"], "calls": H10_CALLS, "blocks_per_call": 10,
-               "value": round(h[mixed]["value"], 1), "fp64_value": round(h["fp64"]["value"], 1)}
+               "value": round(h[mixed]["value"], 1), "fp64_value": round(h["fp64"]["value"], 1),
+               "roofline": h[mixed]["roofline"], "fp64_roofline": h["fp64"]["roofline"]}
         for p, tag in (("fp64", "fp64"), (mixed, "mixed")):
             st = h[p]["stats"][10 * 2:]  # the first two calls equilibrate
             e, err = stat_utils.blocking_analysis(st[:, 1], st[:, 0], block_dtype=np.float64)
